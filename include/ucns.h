@@ -1,0 +1,156 @@
+/*
+ * ucns.h -- C-ABI of the MI355X residual + SSP-RK3 library (libucnsgpu.so).
+ *
+ * This is the drop-in boundary for UCNS3D's explicit spatial residual and
+ * time update.  The reference has no FFI layer: the seams replaced here are
+ * plain Fortran calls on module state, namely
+ *   - the residual body of CALL_FLUX_SUBROUTINES_3D   (src/time_p.f90:2253-2497)
+ *     = ARBITRARY_ORDER (src/reconstruct_p.f90:2954-3015)
+ *     + CALCULATE_FLUXESHI_CONVECTIVE (src/flux_p.f90:478-1030)
+ *   - the three stage loops of RUNGE_KUTTA3             (src/time_p.f90:885-1027)
+ *   - the dt loop of CALCULATE_CFL + its MPI min        (src/time_p.f90:27-191, 4131)
+ * A Fortran host binds these with ISO_C_BINDING; Python
+ * binds them with ctypes (ucns3d_amd/native.py).
+ *
+ * Conventions
+ *   - every array argument is caller-owned and only read/written during the
+ *     call; the library copies what it needs into device memory it owns;
+ *   - per-face arrays are [ncell][nface_max] row-major (face fastest);
+ *   - index arrays carry an explicit index_base (1 for Fortran arrays);
+ *   - matrices keep the reference's column-major element order, e.g.
+ *     INVMAT_STENCILT(ideg,imax,s) is stored with ideg fastest;
+ *   - every call returns 0 or a negative UCNS_E* code; ucns_last_error()
+ *     gives the message.  Nothing throws across the ABI, nothing exits;
+ *   - a context is not re-entrant: call it from one host thread (the
+ *     reference's !$OMP MASTER discipline, src/communications.f90:1137-1242).
+ */
+#ifndef UCNS_H
+#define UCNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UCNS_ABI_VERSION 1
+
+/* return codes */
+#define UCNS_OK             0
+#define UCNS_EBADARG       -1
+#define UCNS_EUNSUPPORTED  -2
+#define UCNS_ENOMEM        -3
+#define UCNS_EHIP          -4
+#define UCNS_ENONFINITE    -5
+#define UCNS_ESTATE        -6
+
+/* precision modes */
+#define UCNS_FP64   0   /* everything in FP64 */
+#define UCNS_MIXED  1   /* reconstruction, WENO weights, HLLC in FP32; dU
+                           formation, residual accumulation, RK update and
+                           conserved->primitive recovery in FP64 */
+
+/* state layouts accepted by set/get */
+#define UCNS_LAYOUT_AOS  0   /* u[i*5+v]     = U_C(i)%VAL(1,v)   */
+#define UCNS_LAYOUT_SOA  1   /* u[v*ncell+i] = U_C(i)%VAL(1,v)   */
+
+/* Mesh topology and geometry, one partition (the whole mesh).
+ * Mirrors IELEM(N,i)%{IFCA,INEIGH,INEIGHN,IBOUNDS->ICODE,TYPES_FACES,
+ * FACEANGLEX,FACEANGLEY,SURF,TOTVOLUME,MINEDGE} (src/declarations.f90:644-712). */
+typedef struct ucns_mesh_v1 {
+    int32_t abi_version;        /* UCNS_ABI_VERSION */
+    int32_t ncell;              /* KMAXE */
+    int32_t index_base;         /* 0 or 1 */
+    int32_t nface_max;          /* leading dimension of the per-face arrays */
+    const int32_t *ifca;        /* [ncell] faces per cell (4 tet, 5 prism/pyramid, 6 hex) */
+    const int32_t *ineigh;      /* [ncell*nface_max] cell across face L (interior or periodic) */
+    const int32_t *ineighn;     /* [ncell*nface_max] face slot of L inside that neighbour */
+    const int32_t *bcode;       /* [ncell*nface_max] 0 interior, else boundary code (5 periodic) */
+    const int32_t *types_faces; /* [ncell*nface_max] 5 quadrilateral, 6 triangle */
+    const double *faceanglex;   /* [ncell*nface_max] outward normal angle a1 */
+    const double *faceangley;   /* [ncell*nface_max] outward normal angle a2 */
+    const double *surf;         /* [ncell*nface_max] face area */
+    const double *totvolume;    /* [ncell] */
+    const double *minedge;      /* [ncell] 3V/sum(S), src/grid_t.f90:1122 */
+} ucns_mesh_v1;
+
+/* Scheme switches (the UCNS3D.DAT fields on the path, src/parameters.f90:215-287). */
+typedef struct ucns_scheme_v1 {
+    int32_t abi_version;
+    int32_t precision;          /* UCNS_FP64 | UCNS_MIXED */
+    int32_t iweno;              /* 1 = WENO (spatiladiscret 3), -1 = MUSCL (2) */
+    int32_t iorder;             /* IORDER = spatialorder-1 */
+    int32_t idegfree;           /* coefficients without the mean */
+    int32_t inumneighbours;     /* stencil size including the cell */
+    int32_t typesten;           /* stencil slots per cell (1 MUSCL, 7 WENO 3D) */
+    int32_t ees;                /* 0 (WENO) ; 5 (CWENO) not supported yet */
+    int32_t wenoz;              /* 0 */
+    int32_t wenwrt;             /* 1 conserved-variable reconstruction */
+    int32_t limiter;            /* 1 Barth-Jespersen */
+    int32_t riemann;            /* 1 HLLC */
+    int32_t qp_quad;            /* Gauss points per quadrilateral face */
+    int32_t qp_triangle;        /* Gauss points per triangular face */
+    double gamma;
+    double lwci1;               /* wenocentralweight */
+    double weno_eps;            /* 10E-12 = 1e-11 in the reference */
+    const double *weights_quad; /* [qp_quad], sum 1 */
+    const double *weights_tri;  /* [qp_triangle], sum 1 */
+} ucns_scheme_v1;
+
+/* Per-cell precomputed reconstruction data (PRESTORE_RECONSTRUCTION3 and
+ * MEMORY_FAST outputs, src/prestore.f90:167-815, src/reconstruct_p.f90:45-302).
+ * nqp_max = max(qp_quad, qp_triangle); imax = inumneighbours-1. */
+typedef struct ucns_prestore_v1 {
+    const int32_t *admis;       /* [ncell] IELEM%ADMIS (complete stencils) */
+    const int32_t *full;        /* [ncell] IELEM%FULL */
+    const int32_t *ihexl;       /* [ncell][typesten][inumneighbours] IHEXL(s,1:nnb), first = cell itself */
+    const double *invmat;       /* [ncell][typesten][imax][idegfree] INVMAT_STENCILT(ideg,imax,s) */
+    const double *indicator;    /* [ncell][idegfree][idegfree] INDICATOR (WENO only, may be NULL for MUSCL) */
+    const double *integ_basis;  /* [ncell][idegfree] INTEG_BASIS%VALUE */
+    const double *volume11;     /* [ncell] ILOCAL_RECON3%VOLUME(1,1) */
+    const double *qpoints;      /* [ncell][nface_max][nqp_max][3] QPOINTS(L,NGP,1:3), local frame */
+} ucns_prestore_v1;
+
+/* Accumulated device-side timings and traffic model (ucns_stats). */
+typedef struct ucns_stats_v1 {
+    int64_t n_recon, n_flux, n_update, n_dt;      /* launches */
+    double ms_recon, ms_flux, ms_update, ms_dt;   /* summed kernel time (hipEvents on the library stream) */
+    double bytes_recon_per_cell;                  /* algorithmic bytes per cell of the reconstruction kernel */
+    int64_t n_reduced_last;                       /* cells sent to MUSCL by CHECKSOL in the last residual */
+    int64_t n_firstorder_last;                    /* cells sent to first order by CHECKSOLX */
+} ucns_stats_v1;
+
+typedef struct ucns_ctx ucns_ctx;
+
+/* Set up one partition on the current HIP device (topology, faces, scheme). */
+int  ucns_create(const ucns_mesh_v1 *mesh, const ucns_scheme_v1 *scheme, ucns_ctx **out);
+/* Upload PRESTORE data (pseudo-inverses, indicators, qp coordinates ...). */
+int  ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre);
+/* Cell averages U_C(i)%VAL(1,1:5) in / out. */
+int  ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout);
+int  ucns_get_state(ucns_ctx *ctx, double *u, int32_t layout);
+/* Same, with u a device pointer (no host staging). */
+int  ucns_set_state_device(ucns_ctx *ctx, const double *u_dev, int32_t layout);
+int  ucns_get_state_device(ucns_ctx *ctx, double *u_dev, int32_t layout);
+/* CALCULATE_CFL: dt = (CFL/3) min_i MINEDGE_i / (max(|u|,|v|,|w|)+c). */
+int  ucns_compute_dt(ucns_ctx *ctx, double cfl, double *dt_out);
+/* CALL_FLUX_SUBROUTINES_3D on the current state: RHS(i)%VAL(1:5) (face-flux
+ * sums, not divided by the volume).  rhs_out may be NULL (result stays on device). */
+int  ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout);
+/* ARBITRARY_ORDER only: ILOCAL_RECON3(i)%ULEFT(1:5,L,NGP) as [ncell][nface_max][nqp_max][5]. */
+int  ucns_reconstruct(ucns_ctx *ctx, double *uleft_out);
+/* TIME_MARCHING's RK3 loop for nsteps steps.  If dt_fixed > 0 it is used for
+ * every step, else dt is recomputed each step from cfl.  *t_io is advanced. */
+int  ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, double *t_io);
+int  ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out);
+int  ucns_reset_stats(ucns_ctx *ctx);
+const char *ucns_last_error(const ucns_ctx *ctx);
+void ucns_destroy(ucns_ctx *ctx);
+/* ABI version compiled into the library. */
+int  ucns_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UCNS_H */

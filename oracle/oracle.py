@@ -1,0 +1,133 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return os.path.join(HERE, "liboracle.so")
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        f64 = C.POINTER(C.c_double)
+        i64 = C.POINTER(C.c_int64)
+        L.oracle_residual.argtypes = [vp, vp, vp, f64, f64, i64]
+        L.oracle_reconstruct.argtypes = [vp, vp, vp, f64, f64, i64]
+        L.oracle_fluxes.argtypes = [vp, vp, vp, f64, f64]
+        L.oracle_dt.argtypes = [vp, vp, f64, C.c_double, f64]
+        L.oracle_rk3_steps.argtypes = [vp, vp, vp, f64, C.c_int32, C.c_double, C.c_double, f64]
+        L.oracle_hllc.argtypes = [C.c_double, f64, f64, f64]
+        L.oracle_hllc.restype = None
+        L.oracle_rotatef.argtypes = [f64, f64, C.c_double, C.c_double]
+        L.oracle_rotatef.restype = None
+        L.oracle_rotateb.argtypes = [f64, f64, C.c_double, C.c_double]
+        L.oracle_rotateb.restype = None
+        L.oracle_weno_weights.argtypes = [C.c_int, f64, C.c_double, C.c_double, f64]
+        L.oracle_weno_weights.restype = None
+        L.oracle_barth.argtypes = [C.c_double] * 4
+        L.oracle_barth.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """Oracle bound to one case's ucns_*_v1 structs."""
+
+    def __init__(self, case, precision=0):
+        self.case = case
+        self.m, self.s, self.p = case.structs(precision)
+        self.L = lib()
+        nqm = max(case.qp_quad, case.qp_tri)
+        self.uleft_shape = (case.ncell, case.nface_max, nqm, 5)
+
+    def _refs(self):
+        return C.byref(self.m), C.byref(self.s), C.byref(self.p)
+
+    def reconstruct(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        ul = np.zeros(self.uleft_shape)
+        cnt = np.zeros(2, dtype=np.int64)
+        m, s, p = self._refs()
+        rc = self.L.oracle_reconstruct(m, s, p, _p(u), _p(ul), cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        assert rc == 0
+        return ul, cnt
+
+    def residual(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        rhs = np.zeros_like(u)
+        cnt = np.zeros(2, dtype=np.int64)
+        m, s, p = self._refs()
+        rc = self.L.oracle_residual(m, s, p, _p(u), _p(rhs), cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        assert rc == 0
+        return rhs, cnt
+
+    def dt(self, u, cfl):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = C.c_double()
+        m, s, _ = self._refs()
+        self.L.oracle_dt(m, s, _p(u), cfl, C.byref(out))
+        return out.value
+
+    def rk3(self, u, nsteps, cfl=0.5, dt_fixed=0.0, t0=0.0):
+        u = np.array(u, dtype=np.float64, copy=True)
+        t = C.c_double(t0)
+        m, s, p = self._refs()
+        rc = self.L.oracle_rk3_steps(m, s, p, _p(u), nsteps, cfl, dt_fixed, C.byref(t))
+        assert rc == 0
+        return u, t.value
+
+
+def hllc(gamma, cl, cr):
+    cl = np.ascontiguousarray(cl, dtype=np.float64)
+    cr = np.ascontiguousarray(cr, dtype=np.float64)
+    out = np.zeros(5)
+    lib().oracle_hllc(gamma, _p(cl), _p(cr), _p(out))
+    return out
+
+
+def rotatef(vec, a1, a2):
+    vec = np.ascontiguousarray(vec, dtype=np.float64)
+    out = np.zeros(5)
+    lib().oracle_rotatef(_p(out), _p(vec), a1, a2)
+    return out
+
+
+def rotateb(vec, a1, a2):
+    vec = np.ascontiguousarray(vec, dtype=np.float64)
+    out = np.zeros(5)
+    lib().oracle_rotateb(_p(out), _p(vec), a1, a2)
+    return out
+
+
+def weno_weights(beta, lwci1, eps=1e-11):
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    out = np.zeros(len(beta))
+    lib().oracle_weno_weights(len(beta), _p(beta), lwci1, eps, _p(out))
+    return out
+
+
+def barth(ui, umin, umax, usol):
+    return lib().oracle_barth(ui, umin, umax, usol)

@@ -1,0 +1,185 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle.
+
+Bar (FP64 path): relative L2 per conserved variable <= 1e-12 for the
+reconstruction, the residual and one SSP-RK3 step, <= 1e-9 after 100 steps.
+Mixed path: <= 1e-5 against the FP64 oracle after one step.
+Sizes are reduced versions of the benchmark meshes (same construction) so
+the oracle finishes in seconds; config 1 runs at its real size.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from ucns3d_amd.case import PRESETS, Case, reduced
+from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+from ucns3d_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "hex-muscl-8": lambda: reduced("c1", 8),
+    "hex-weno4-12": lambda: reduced("c3", 12),
+    "tet-weno4-6": lambda: reduced("c2", 6),
+}
+
+_cache = {}
+
+
+def get(name):
+    if name not in _cache:
+        spec = CASES[name]()
+        case = Case(spec)
+        _cache[name] = (case, Oracle(case), case.init_state())
+    return _cache[name]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).reshape(-1, 5)
+    b = np.asarray(b).reshape(-1, 5)
+    num = np.sqrt(((a - b) ** 2).sum(axis=0))
+    den = np.sqrt((b ** 2).sum(axis=0))
+    return num / np.where(den > 0, den, 1.0)
+
+
+def valid_uleft_mask(case):
+    """face points that exist (triangles use fewer than the padded count)"""
+    nqm = max(case.qp_quad, case.qp_tri)
+    mask = np.zeros((case.ncell, case.nface_max, nqm), dtype=bool)
+    faces = 6 if case.shape == 1 else 4
+    nq = case.qp_quad if case.shape == 1 else case.qp_tri
+    mask[:, :faces, :nq] = True
+    return mask
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_reconstruction_fp64(name):
+    case, orc, u = get(name)
+    print(f"[recon] {name} ncell={case.ncell}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    ul_gpu = s.reconstruct()
+    ul_ref, cnt = orc.reconstruct(u)
+    m = valid_uleft_mask(case)
+    err = rel_l2(ul_gpu[m], ul_ref[m])
+    st = s.stats()
+    print("  rel L2", err, "muscl cells gpu/oracle", st["n_reduced_last"], cnt[0])
+    assert st["n_reduced_last"] == cnt[0]
+    assert st["n_firstorder_last"] == cnt[1]
+    assert np.all(err <= 1e-12), err
+    s.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_residual_fp64(name):
+    case, orc, u = get(name)
+    print(f"[residual] {name}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    r_gpu = s.residual()
+    r_ref, _ = orc.residual(u)
+    # the residual is a sum of opposite face fluxes: measure against the
+    # flux scale (sum of |face flux| ~ |r|) via the state-weighted norm
+    err = rel_l2(r_gpu, r_ref)
+    print("  rel L2", err)
+    assert np.all(err <= 1e-11), err
+    s.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_one_step_fp64(name):
+    case, orc, u = get(name)
+    print(f"[rk3 x1] {name}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    dt_gpu = s.compute_dt(case.spec.cfl)
+    dt_ref = orc.dt(u, case.spec.cfl)
+    assert abs(dt_gpu - dt_ref) <= 1e-15 * dt_ref
+    t = s.rk3_steps(1, cfl=case.spec.cfl)
+    u1 = s.get_state()
+    u1_ref, t_ref = orc.rk3(u, 1, cfl=case.spec.cfl)
+    assert abs(t - t_ref) <= 1e-14 * t_ref
+    err = rel_l2(u1, u1_ref)
+    print("  rel L2", err)
+    assert np.all(err <= 1e-12), err
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["hex-muscl-8", "hex-weno4-12"])
+def test_100_steps_fp64(name):
+    case, orc, u = get(name)
+    print(f"[rk3 x100] {name}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    t = s.rk3_steps(100, cfl=case.spec.cfl)
+    u100 = s.get_state()
+    u100_ref, t_ref = orc.rk3(u, 100, cfl=case.spec.cfl)
+    err = rel_l2(u100, u100_ref)
+    print("  rel L2", err, "t", t, t_ref)
+    assert abs(t - t_ref) <= 1e-12 * t_ref
+    assert np.all(err <= 1e-9), err
+    s.close()
+
+
+def test_config1_full_size_100_steps():
+    """config 1 at its real size: 24^3 hexes, MUSCL+Barth, CFL 0.5, 100 steps"""
+    spec = PRESETS["c1"]
+    case = Case(spec)
+    orc = Oracle(case)
+    u = case.init_state()
+    print(f"[c1 full] ncell={case.ncell}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    s.rk3_steps(1, cfl=spec.cfl)
+    u1 = s.get_state()
+    u1_ref, _ = orc.rk3(u, 1, cfl=spec.cfl)
+    e1 = rel_l2(u1, u1_ref)
+    s.rk3_steps(99, cfl=spec.cfl)
+    u100 = s.get_state()
+    u100_ref, _ = orc.rk3(u1_ref, 99, cfl=spec.cfl)
+    e100 = rel_l2(u100, u100_ref)
+    print("  rel L2 step1", e1, "step100", e100)
+    assert np.all(e1 <= 1e-12) and np.all(e100 <= 1e-9)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["hex-weno4-12", "tet-weno4-6"])
+def test_mixed_vs_fp64(name):
+    case, orc, u = get(name)
+    print(f"[mixed] {name}", flush=True)
+    s = Solver(case, UCNS_MIXED)
+    s.set_state(u)
+    s.rk3_steps(1, cfl=case.spec.cfl)
+    u1 = s.get_state()
+    u1_ref, _ = orc.rk3(u, 1, cfl=case.spec.cfl)
+    err = rel_l2(u1, u1_ref)
+    print("  rel L2 mixed vs FP64 oracle", err)
+    assert np.all(err <= 1e-5), err
+    s.close()
+
+
+def test_conservation_and_freestream():
+    case, orc, u = get("tet-weno4-6")
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    r = s.residual()
+    # discrete conservation on a periodic mesh: fluxes cancel pairwise
+    scale = np.abs(r).sum(axis=0)
+    assert np.all(np.abs(r.sum(axis=0)) <= 1e-12 * scale)
+    # free stream: a uniform state has zero residual
+    uf = np.tile(np.array([1.2, 0.3, -0.2, 0.1, 3.0]), (case.ncell, 1))
+    s.set_state(uf)
+    rf = s.residual()
+    assert np.abs(rf).max() <= 1e-12 * 3.0
+    s.close()
+
+
+def test_deterministic():
+    case, orc, u = get("hex-weno4-12")
+    outs = []
+    for _ in range(2):
+        s = Solver(case, UCNS_FP64)
+        s.set_state(u)
+        s.rk3_steps(3, cfl=0.5)
+        outs.append(s.get_state())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])

@@ -1,0 +1,70 @@
+"""Build the native libraries in-tree.
+
+libucnsprep.so  host FP64 preprocessor (g++ -fopenmp)
+libucnsgpu.so   HIP residual/RK3 library for gfx950 (hipcc)
+
+The built .so files live in ucns3d_amd/lib/ so they travel with the source
+tree; nothing is installed into site-packages.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIBDIR = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+# packed-FP32 instructions are disabled for every HIP translation unit
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+
+
+def build_prep() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libucnsprep.so")
+    src = os.path.join(CSRC, "prep", "ucns_prep.cpp")
+    _run([CXX, "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-I" + INCLUDE, src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_gpu() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libucnsgpu.so")
+    src = os.path.join(CSRC, "ucns_gpu.hip")
+    _run([HIPCC, *HIP_FLAGS, "-I" + INCLUDE, src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all() -> list[str]:
+    return [build_prep(), build_gpu()]
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        if w == "prep":
+            print(build_prep())
+        elif w == "gpu":
+            print(build_gpu())
+        else:
+            for p in build_all():
+                print(p)

@@ -1,0 +1,140 @@
+"""Synthetic cases: mesh + scheme + PRESTORE arrays built by the host preprocessor.
+
+A Case owns the native preprocessor object (libucnsprep.so) and hands out
+the ucns_*_v1 structs of include/ucns.h that point into its storage, so the
+GPU library (and any other consumer of the C-ABI) receives exactly what a
+Fortran host would pass.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native
+
+HEX = 1
+TET = 2
+
+
+@dataclass(frozen=True)
+class CaseSpec:
+    """Mesh and scheme choices of one configuration."""
+    name: str
+    shape: int            # HEX or TET (6 Kuhn tets per hex)
+    n: int                # hexes per direction
+    scheme: str           # "muscl" (2nd order, Barth) or "weno4" (IORDER 3)
+    jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
+    seed: int = 1
+    noise: float = 0.0    # N(0,1) density perturbation amplitude per cell
+    cfl: float = 0.5
+    steps: int = 100
+    length: float = 2.0 * math.pi
+    lwci1: float = 10000.0   # wenocentralweight of tests/taylor_green_vortex/UCNS3D.DAT
+    gamma: float = 1.4
+
+    @property
+    def iorder(self) -> int:
+        return 1 if self.scheme == "muscl" else 3
+
+    @property
+    def iweno(self) -> int:
+        return -1 if self.scheme == "muscl" else 1
+
+    @property
+    def extf(self) -> int:
+        # CODE_PROFILE 0 (extf=3) for the MUSCL case, profile 8 (extf=2) for WENO
+        return 3 if self.scheme == "muscl" else 2
+
+    @property
+    def ncell(self) -> int:
+        return self.n ** 3 * (6 if self.shape == TET else 1)
+
+
+# The benchmark configurations (sizes as specified) and reduced versions of
+# the same construction used by the parity tests.
+PRESETS = {
+    "c1": CaseSpec("c1: 24^3 hex, MUSCL+Barth, HLLC, SSP-RK3", HEX, 24, "muscl", seed=1, steps=100),
+    "c2": CaseSpec("c2: 64^3x6 jittered tets, WENO-4, HLLC, SSP-RK3, FP64", TET, 64, "weno4", jitter=0.1, seed=2,
+                   steps=200),
+    "c3": CaseSpec("c3: 128^3 hex + 1e-3 density noise, WENO-4, HLLC, SSP-RK3, mixed", HEX, 128, "weno4", seed=3,
+                   noise=1e-3, steps=500),
+}
+
+
+def reduced(key: str, n: int) -> CaseSpec:
+    s = PRESETS[key]
+    return CaseSpec(f"{key}-reduced-{n}", s.shape, n, s.scheme, s.jitter, s.seed, s.noise, s.cfl, s.steps,
+                    s.length, s.lwci1, s.gamma)
+
+
+class Case:
+    def __init__(self, spec: CaseSpec, nthreads: int = 0):
+        self.spec = spec
+        lib = native.prep_lib()
+        o = native.PrepOpts(shape=spec.shape, nx=spec.n, ny=spec.n, nz=spec.n,
+                            lx=spec.length, ly=spec.length, lz=spec.length, jitter=spec.jitter, seed=spec.seed,
+                            iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads)
+        h = C.c_void_p()
+        rc = lib.ucnsp_build(C.byref(o), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"ucnsp_build failed ({rc}): {lib.ucnsp_last_error().decode()}")
+        self._h = h
+        self._lib = lib
+        ints = (C.c_int64 * 16)()
+        dbls = (C.c_double * 8)()
+        lib.ucnsp_info(h, ints, dbls)
+        (self.ncell, self.nnode, self.nface_max, self.iorder, self.ideg, self.nnb, self.imax, self.typesten,
+         self.iselem, self.igq, self.qp_quad, self.qp_tri, self.nqpmax, self.shape, self.iweno, _) = list(ints)
+        self.period = (dbls[0], dbls[1], dbls[2])
+
+    # -- raw arrays --------------------------------------------------------
+    def array(self, name: str) -> np.ndarray:
+        ptr = C.c_void_p()
+        cnt = C.c_int64()
+        dt = C.c_int32()
+        rc = self._lib.ucnsp_array(self._h, name.encode(), C.byref(ptr), C.byref(cnt), C.byref(dt))
+        if rc != 0:
+            raise KeyError(self._lib.ucnsp_last_error().decode())
+        if cnt.value == 0:
+            return np.zeros(0, dtype=np.int32 if dt.value == 0 else np.float64)
+        ctype = C.c_int32 if dt.value == 0 else C.c_double
+        buf = (ctype * cnt.value).from_address(ptr.value)
+        return np.ctypeslib.as_array(buf)
+
+    def structs(self, precision: int = native.UCNS_FP64):
+        m, s, p = native.MeshV1(), native.SchemeV1(), native.PrestoreV1()
+        rc = self._lib.ucnsp_fill(self._h, precision, self.spec.gamma, self.spec.lwci1, C.byref(m), C.byref(s),
+                                  C.byref(p))
+        if rc != 0:
+            raise RuntimeError(self._lib.ucnsp_last_error().decode())
+        return m, s, p
+
+    def init_state(self) -> np.ndarray:
+        u = np.empty((self.ncell, 5), dtype=np.float64)
+        rc = self._lib.ucnsp_init_vortex(self._h, self.spec.gamma, self.spec.noise, self.spec.seed,
+                                         native.f64p(u))
+        if rc != 0:
+            raise RuntimeError(self._lib.ucnsp_last_error().decode())
+        return u
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ucnsp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def face_rule(igq: int, tri: bool):
+    lib = native.prep_lib()
+    pts = np.zeros((64, 3))
+    w = np.zeros(64)
+    n = C.c_int32()
+    rc = lib.ucnsp_face_rule(igq, 1 if tri else 0, native.f64p(pts), native.f64p(w), C.byref(n))
+    if rc != 0:
+        raise RuntimeError(lib.ucnsp_last_error().decode())
+    return pts[: n.value].copy(), w[: n.value].copy()

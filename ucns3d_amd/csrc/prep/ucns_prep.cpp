@@ -1,0 +1,1328 @@
+// Host-side FP64 preprocessor for the residual path: synthetic periodic
+// meshes, connectivity, geometry, stencils and PRESTORE arrays.
+// Every stage cites the reference routine it restates (src/... file:line).
+// All floating point work is FP64; loops over cells run under OpenMP.
+
+#include "ucns_prep.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+// ----------------------------------------------------------------------------
+// Reference element tables (0-based local node numbers)
+// Face node lists: src/grid_p.f90:1950-2063 (hex, tet), 1896-1925 (prism).
+const int HEX_FACES[6][4] = {{1, 0, 3, 2}, {4, 5, 6, 7}, {5, 1, 2, 6},
+                             {7, 3, 0, 4}, {6, 2, 3, 7}, {4, 0, 1, 5}};
+const int TET_FACES[4][3] = {{1, 3, 0}, {1, 0, 2}, {1, 2, 3}, {2, 0, 3}};
+// Sub-tetrahedra of DECOMPOSE3, src/grid_t.f90:1027-1074.
+const int HEX_TETS[6][4] = {{0, 5, 7, 4}, {0, 1, 7, 5}, {1, 6, 7, 5},
+                            {0, 7, 2, 3}, {0, 7, 1, 2}, {1, 7, 6, 2}};
+const int TET_TETS[1][4] = {{0, 1, 2, 3}};
+
+inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : 4; }
+inline int nnodes_of(int shape) { return shape == UCNSP_HEX ? 8 : 4; }
+inline int face_nv(int shape, int) { return shape == UCNSP_HEX ? 4 : 3; }
+inline int face_node(int shape, int f, int k) {
+    return shape == UCNSP_HEX ? HEX_FACES[f][k] : TET_FACES[f][k];
+}
+inline int nsubtets(int shape) { return shape == UCNSP_HEX ? 6 : 1; }
+inline const int *subtet(int shape, int t) {
+    return shape == UCNSP_HEX ? HEX_TETS[t] : TET_TETS[t];
+}
+
+// ----------------------------------------------------------------------------
+// Monomial basis of BASIS_REC (POLY=1), src/basis.f90:11-102, up to degree 4.
+const int MONO[34][3] = {
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+    {2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2},
+    {3, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 3, 0}, {2, 0, 1}, {1, 1, 1}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {0, 0, 3},
+    {4, 0, 0}, {3, 1, 0}, {2, 2, 0}, {1, 3, 0}, {0, 4, 0}, {3, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 3, 1},
+    {2, 0, 2}, {1, 1, 2}, {0, 2, 2}, {1, 0, 3}, {0, 1, 3}, {0, 0, 4}};
+
+inline int ndeg_of(int iorder) { return (iorder + 1) * (iorder + 2) * (iorder + 3) / 6 - 1; }
+
+inline void monomials(double x, double y, double z, int ideg, double *m) {
+    double px[5], py[5], pz[5];
+    px[0] = py[0] = pz[0] = 1.0;
+    for (int k = 1; k < 5; ++k) {
+        px[k] = px[k - 1] * x;
+        py[k] = py[k - 1] * y;
+        pz[k] = pz[k - 1] * z;
+    }
+    for (int d = 0; d < ideg; ++d) m[d] = px[MONO[d][0]] * py[MONO[d][1]] * pz[MONO[d][2]];
+}
+
+// ----------------------------------------------------------------------------
+// Quadrature
+// Gauss-Legendre nodes on [-1,1] by Newton iteration on P_n.
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const double PI = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        double t = std::cos(PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) { p1 = t; p0 = 1.0; }
+            double dp = n * (t * p1 - p0) / (t * t - 1.0);
+            double dt = p1 / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-17) break;
+        }
+        double p0 = 1.0, p1 = t;
+        for (int k = 2; k <= n; ++k) {
+            double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        double dp = (n == 1) ? 1.0 : n * (t * p1 - p0) / (t * t - 1.0);
+        x[i] = t;
+        w[i] = 2.0 / ((1.0 - t * t) * dp * dp);
+    }
+    if (n == 1) { x[0] = 0.0; w[0] = 2.0; }
+    // sort ascending
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return x[a] < x[b]; });
+    std::vector<double> xs(n), ws(n);
+    for (int i = 0; i < n; ++i) { xs[i] = x[idx[i]]; ws[i] = w[idx[i]]; }
+    x = xs;
+    w = ws;
+}
+
+// Quadrilateral face rule in the reference's point order
+// (QUADRATUREQUAD3D, src/grid_t.f90:2834-3035): 1-D nodes ordered
+// n=1 {0}; n=2 {-g,+g}; n=3 {0,-g,+g}; n=4 {-g1,+g1,-g2,+g2}; tensor loop
+// with R outer, S inner; weights scaled by 1/4 so they sum to 1.
+int quad_rule(int n, std::vector<double> &rs, std::vector<double> &w) {
+    if (n < 1 || n > 4) return -1;
+    std::vector<double> x, wx;
+    gauss_legendre(n, x, wx);  // ascending
+    std::vector<double> nx, nw;
+    if (n == 1) { nx = {0.0}; nw = {2.0}; }
+    if (n == 2) { nx = {x[0], x[1]}; nw = {wx[0], wx[1]}; }
+    if (n == 3) { nx = {0.0, x[0], x[2]}; nw = {wx[1], wx[0], wx[2]}; }
+    if (n == 4) { nx = {x[1], x[2], x[0], x[3]}; nw = {wx[1], wx[2], wx[0], wx[3]}; }
+    rs.clear();
+    w.clear();
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            rs.push_back(nx[a]);
+            rs.push_back(nx[b]);
+            w.push_back(nw[a] * nw[b] * 0.25);
+        }
+    return n * n;
+}
+
+// Triangle face rule (QUADRATURETRIANG, src/grid_t.f90:2054-2348): the
+// symmetric rules with 1, 3 and 6 points (degrees 1, 2, 4) in the reference's
+// point order.  The 6-point orbit parameters are solved here to full double
+// precision from the moment equations of the S3-invariants 1, e2, e3, e2^2
+// (means 1, 1/4, 1/60, 1/15 over the triangle) instead of copying 15-digit
+// constants.
+int tri_rule(int n, std::vector<double> &bary, std::vector<double> &w) {
+    bary.clear();
+    w.clear();
+    if (n == 1) {
+        bary = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};
+        w = {1.0};
+        return 1;
+    }
+    if (n == 2) {
+        const double a = 2.0 / 3.0, b = 1.0 / 6.0;
+        bary = {a, b, b, b, a, b, b, b, a};
+        w = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};
+        return 3;
+    }
+    if (n == 3) {
+        // unknowns: a (small orbit), b (large orbit), wa, wb (per point)
+        double v[4] = {0.0915762135, 0.4459484909, 0.1099517437, 0.2233815897};
+        auto F = [](const double *p, double *f) {
+            double a = p[0], b = p[1], wa = p[2], wb = p[3];
+            double e2a = 2 * a - 3 * a * a, e2b = 2 * b - 3 * b * b;
+            double e3a = a * a * (1 - 2 * a), e3b = b * b * (1 - 2 * b);
+            f[0] = 3 * wa + 3 * wb - 1.0;
+            f[1] = 3 * wa * e2a + 3 * wb * e2b - 0.25;
+            f[2] = 3 * wa * e3a + 3 * wb * e3b - 1.0 / 60.0;
+            f[3] = 3 * wa * e2a * e2a + 3 * wb * e2b * e2b - 1.0 / 15.0;
+        };
+        for (int it = 0; it < 60; ++it) {
+            double f0[4];
+            F(v, f0);
+            double J[4][4];
+            for (int j = 0; j < 4; ++j) {
+                double vp[4] = {v[0], v[1], v[2], v[3]};
+                double h = 1e-7 * std::max(1.0, std::fabs(v[j]));
+                vp[j] += h;
+                double f1[4];
+                F(vp, f1);
+                for (int i = 0; i < 4; ++i) J[i][j] = (f1[i] - f0[i]) / h;
+            }
+            // solve J dx = -f0 (Gaussian elimination with pivoting)
+            double M[4][5];
+            for (int i = 0; i < 4; ++i) {
+                for (int j = 0; j < 4; ++j) M[i][j] = J[i][j];
+                M[i][4] = -f0[i];
+            }
+            for (int c = 0; c < 4; ++c) {
+                int p = c;
+                for (int r = c + 1; r < 4; ++r)
+                    if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+                for (int j = 0; j < 5; ++j) std::swap(M[c][j], M[p][j]);
+                for (int r = c + 1; r < 4; ++r) {
+                    double m = M[r][c] / M[c][c];
+                    for (int j = c; j < 5; ++j) M[r][j] -= m * M[c][j];
+                }
+            }
+            double dx[4];
+            for (int r = 3; r >= 0; --r) {
+                double s = M[r][4];
+                for (int j = r + 1; j < 4; ++j) s -= M[r][j] * dx[j];
+                dx[r] = s / M[r][r];
+            }
+            double nrm = 0;
+            for (int j = 0; j < 4; ++j) { v[j] += dx[j]; nrm += std::fabs(dx[j]); }
+            if (nrm < 1e-17) break;
+        }
+        const double a = v[0], b = v[1], A = 1.0 - 2.0 * a, B = 1.0 - 2.0 * b;
+        bary = {A, a, a, a, A, a, a, a, A, b, b, B, b, B, b, B, b, b};
+        w = {v[2], v[2], v[2], v[3], v[3], v[3]};
+        return 6;
+    }
+    return -1;
+}
+
+// Volume rules.  Stroud T3:3-1 (degree 3, 5 points) on a tetrahedron in
+// barycentric form; collapsed (Duffy) Gauss-Legendre product rule of any
+// degree; tensor Gauss-Legendre on a parallelepiped.  All return points in
+// physical coordinates and weights that include the volume.
+struct QPts {
+    std::vector<double> x;  // n*3
+    std::vector<double> w;
+};
+
+void tet_stroud3(const double v[4][3], QPts &q) {
+    double e1[3], e2[3], e3[3];
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[2][d] - v[0][d];
+        e3[d] = v[3][d] - v[0][d];
+    }
+    double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                 e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    double vol = std::fabs(det) / 6.0;
+    const double L[5][4] = {{0.25, 0.25, 0.25, 0.25},
+                            {0.5, 1.0 / 6, 1.0 / 6, 1.0 / 6},
+                            {1.0 / 6, 0.5, 1.0 / 6, 1.0 / 6},
+                            {1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6},
+                            {1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5}};
+    const double W[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+    for (int p = 0; p < 5; ++p) {
+        for (int d = 0; d < 3; ++d)
+            q.x.push_back(L[p][0] * v[0][d] + L[p][1] * v[1][d] + L[p][2] * v[2][d] + L[p][3] * v[3][d]);
+        q.w.push_back(W[p] * vol);
+    }
+}
+
+void tet_collapsed(const double v[4][3], int n, QPts &q) {
+    std::vector<double> g, gw;
+    gauss_legendre(n, g, gw);
+    double e1[3], e2[3], e3[3];
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[2][d] - v[0][d];
+        e3[d] = v[3][d] - v[0][d];
+    }
+    double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                 e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    double ad = std::fabs(det);
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b)
+            for (int c = 0; c < n; ++c) {
+                double u = 0.5 * (g[a] + 1), s = 0.5 * (g[b] + 1), t = 0.5 * (g[c] + 1);
+                // reference simplex point: (xi,eta,zeta) with xi=u, eta=(1-u)s, zeta=(1-u)(1-s)t
+                double xi = u, eta = (1 - u) * s, zeta = (1 - u) * (1 - s) * t;
+                double jac = (1 - u) * (1 - u) * (1 - s);
+                double wt = 0.125 * gw[a] * gw[b] * gw[c] * jac * ad;
+                for (int d = 0; d < 3; ++d) q.x.push_back(v[0][d] + xi * e1[d] + eta * e2[d] + zeta * e3[d]);
+                q.w.push_back(wt);
+            }
+}
+
+void hex_tensor(const double v[8][3], int n, QPts &q) {
+    // parallelepiped spanned at node 1 by (2-1), (4-1), (5-1)
+    std::vector<double> g, gw;
+    gauss_legendre(n, g, gw);
+    double e1[3], e2[3], e3[3];
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[3][d] - v[0][d];
+        e3[d] = v[4][d] - v[0][d];
+    }
+    double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                 e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    double ad = std::fabs(det);
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b)
+            for (int c = 0; c < n; ++c) {
+                double r = 0.5 * (g[a] + 1), s = 0.5 * (g[b] + 1), t = 0.5 * (g[c] + 1);
+                for (int d = 0; d < 3; ++d) q.x.push_back(v[0][d] + r * e1[d] + s * e2[d] + t * e3[d]);
+                q.w.push_back(0.125 * gw[a] * gw[b] * gw[c] * ad);
+            }
+}
+
+bool hex_is_affine(const double v[8][3]) {
+    double scale = 0;
+    for (int d = 0; d < 3; ++d) scale = std::max(scale, std::fabs(v[6][d] - v[0][d]));
+    const double tol = 1e-12 * std::max(scale, 1e-300);
+    // every node equals v0 + combination of e1,e2,e3
+    const int comb[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                            {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (int k = 0; k < 8; ++k)
+        for (int d = 0; d < 3; ++d) {
+            double p = v[0][d] + comb[k][0] * (v[1][d] - v[0][d]) + comb[k][1] * (v[3][d] - v[0][d]) +
+                       comb[k][2] * (v[4][d] - v[0][d]);
+            if (std::fabs(p - v[k][d]) > tol) return false;
+        }
+    return true;
+}
+
+// Integration points of a whole cell, exact for polynomials of total degree
+// <= deg (affine hex: tensor GL; otherwise the DECOMPOSE3 sub-tetrahedra).
+void cell_rule(int shape, const double v[8][3], int deg, QPts &q) {
+    q.x.clear();
+    q.w.clear();
+    if (shape == UCNSP_HEX && hex_is_affine(v)) {
+        int n = std::max(1, (deg + 2) / 2);
+        hex_tensor(v, n, q);
+        return;
+    }
+    for (int t = 0; t < nsubtets(shape); ++t) {
+        const int *tt = subtet(shape, t);
+        double tv[4][3];
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) tv[k][d] = v[tt[k]][d];
+        if (deg <= 3)
+            tet_stroud3(tv, q);
+        else
+            tet_collapsed(tv, (deg + 4) / 2, q);
+    }
+}
+
+double tet_volume(const double a[3], const double b[3], const double c[3], const double d[3]) {
+    double e1[3], e2[3], e3[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = b[k] - a[k];
+        e2[k] = c[k] - a[k];
+        e3[k] = d[k] - a[k];
+    }
+    double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                 e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    return std::fabs(det) / 6.0;
+}
+
+// ----------------------------------------------------------------------------
+uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double u01(uint64_t &s) { return (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+template <class T>
+struct Buf {  // uninitialised heap array, filled in parallel (first touch)
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) p = static_cast<T *>(std::malloc(sizeof(T) * count));
+    }
+    void release() {
+        std::free(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~Buf() { release(); }
+};
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+struct ucnsp_case {
+    ucnsp_opts o{};
+    int shape = 0;
+    int64_t ncell = 0, nnode = 0;
+    static const int NF = 6;  // nface_max
+    double per[3] = {0, 0, 0};
+    std::vector<double> xyz;
+    std::vector<int32_t> canon;
+    std::vector<int32_t> cnodes;  // ncell*8
+    std::vector<int32_t> cshape;
+    // connectivity / geometry (per cell-face: ncell*NF)
+    std::vector<int32_t> ifca, ineigh, ineighn, bcode, types_faces, interior, reorient;
+    std::vector<int32_t> fnodes;  // ncell*NF*4 face vertices after CHECKGRADS adoption
+    std::vector<double> angx, angy, surf, normal, vol, minedge, centre;
+    // scheme sizes
+    int iorder = 1, ideg = 3, nnb = 9, imax = 8, typesten = 1, iselem = 0, igq = 1;
+    int qp_quad = 1, qp_tri = 1, nqpmax = 1;
+    std::vector<double> quad_rs, quad_w, tri_bary, tri_w;
+    // stencils and prestore
+    std::vector<int32_t> admis, full, ihexl;
+    Buf<double> invmat;
+    std::vector<double> indicator, integ_basis, volume11, qpoints, frame;
+};
+
+namespace {
+
+// Face matching (NEIGHBOURSS, src/grid_p.f90:1850): faces with the same set
+// of periodic-canonical node ids are neighbours; a pair whose raw node ids
+// differ is a periodic pair (boundary code 5).
+void build_connectivity(ucnsp_case &c) {
+    const int64_t nc = c.ncell;
+    const int NF = ucnsp_case::NF;
+    struct Key {
+        int32_t k[4];
+        int64_t slot;
+    };
+    std::vector<Key> keys;
+    keys.reserve(nc * nfaces_of(c.shape));
+    c.ifca.assign(nc, 0);
+    for (int64_t i = 0; i < nc; ++i) {
+        int sh = c.cshape[i];
+        c.ifca[i] = nfaces_of(sh);
+        for (int f = 0; f < nfaces_of(sh); ++f) {
+            Key K;
+            int nv = face_nv(sh, f);
+            for (int k = 0; k < 4; ++k) K.k[k] = (k < nv) ? c.canon[c.cnodes[i * 8 + face_node(sh, f, k)]] : -1;
+            std::sort(K.k, K.k + 4);
+            K.slot = i * NF + f;
+            keys.push_back(K);
+        }
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+        for (int k = 0; k < 4; ++k)
+            if (a.k[k] != b.k[k]) return a.k[k] < b.k[k];
+        return a.slot < b.slot;
+    });
+    c.ineigh.assign(nc * NF, -1);
+    c.ineighn.assign(nc * NF, -1);
+    c.bcode.assign(nc * NF, 0);
+    c.types_faces.assign(nc * NF, 0);
+    c.interior.assign(nc, 0);
+    for (size_t a = 0; a + 1 < keys.size(); ++a) {
+        const Key &A = keys[a], &B = keys[a + 1];
+        if (std::memcmp(A.k, B.k, sizeof(A.k)) != 0) continue;
+        int64_t i = A.slot / NF, fi = A.slot % NF, j = B.slot / NF, fj = B.slot % NF;
+        c.ineigh[A.slot] = (int32_t)j;
+        c.ineighn[A.slot] = (int32_t)fj;
+        c.ineigh[B.slot] = (int32_t)i;
+        c.ineighn[B.slot] = (int32_t)fi;
+        // periodic if raw node sets differ
+        int sh_i = c.cshape[i], sh_j = c.cshape[j];
+        int32_t ri[4] = {-1, -1, -1, -1}, rj[4] = {-1, -1, -1, -1};
+        for (int k = 0; k < face_nv(sh_i, (int)fi); ++k) ri[k] = c.cnodes[i * 8 + face_node(sh_i, (int)fi, k)];
+        for (int k = 0; k < face_nv(sh_j, (int)fj); ++k) rj[k] = c.cnodes[j * 8 + face_node(sh_j, (int)fj, k)];
+        std::sort(ri, ri + 4);
+        std::sort(rj, rj + 4);
+        if (std::memcmp(ri, rj, sizeof(ri)) != 0) {
+            c.bcode[A.slot] = 5;
+            c.bcode[B.slot] = 5;
+            c.interior[i] = 1;
+            c.interior[j] = 1;
+        }
+        ++a;
+    }
+    for (int64_t i = 0; i < nc; ++i)
+        for (int f = 0; f < c.ifca[i]; ++f) c.types_faces[i * NF + f] = face_nv(c.cshape[i], f) == 4 ? 5 : 6;
+}
+
+inline void node_xyz(const ucnsp_case &c, int32_t node, double *p) {
+    p[0] = c.xyz[node * 3 + 0];
+    p[1] = c.xyz[node * 3 + 1];
+    p[2] = c.xyz[node * 3 + 2];
+}
+
+// Shift a point by whole periods so it lies within half a period of ref
+// (strict test as in LOCALISE_STEN2, src/local_pt.f90:956-981).
+inline void unwrap(const ucnsp_case &c, const double *ref, double *p) {
+    for (int d = 0; d < 3; ++d) {
+        if (c.per[d] <= 0) continue;
+        if (std::fabs(p[d] - ref[d]) > 0.5 * c.per[d]) p[d] += c.per[d] * ((ref[d] - 0.5 * c.per[d]) >= 0 ? 1.0 : -1.0);
+    }
+}
+
+// Angles of the outward normal (ANGLEX/ANGLEY, src/grid_t.f90:4441-4508).
+void normal_angles(double nx, double ny, double nz, double &a1, double &a2) {
+    const double PI = 3.14159265358979323846;
+    a1 = 0.0;
+    if (nx != 0.0 && ny != 0.0) {
+        if (nx > 0 && ny > 0) a1 = std::atan(ny / nx);
+        if (nx < 0 && ny > 0) a1 = PI + std::atan(ny / nx);
+        if (nx < 0 && ny < 0) a1 = PI + std::atan(ny / nx);
+        if (nx > 0 && ny < 0) a1 = 2.0 * PI + std::atan(ny / nx);
+    } else if (nx == 0.0 && ny != 0.0) {
+        a1 = ny > 0 ? PI / 2.0 : 3.0 * (PI / 2.0);
+    } else if (nx != 0.0 && ny == 0.0) {
+        a1 = nx > 0 ? 0.0 : PI;
+    }
+    a2 = std::acos(nz == 0.0 ? 0.0 : nz);
+}
+
+// GEOMETRY_CALC: volumes (DECOMPOSE3 sums), centres (vertex average),
+// face areas (QUADAREA / TRIANGLEAREA), Newell normals and their angles
+// (FIND_ROT_ANGLES), MINEDGE = 3V/sum(S) (EDGE_CALCULATOR3d).
+void build_geometry(ucnsp_case &c) {
+    const int64_t nc = c.ncell;
+    const int NF = ucnsp_case::NF;
+    c.vol.assign(nc, 0.0);
+    c.centre.assign(nc * 3, 0.0);
+    c.minedge.assign(nc, 0.0);
+    c.surf.assign(nc * NF, 0.0);
+    c.angx.assign(nc * NF, 0.0);
+    c.angy.assign(nc * NF, 0.0);
+    c.normal.assign(nc * NF * 3, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nc; ++i) {
+        int sh = c.cshape[i];
+        int nn = nnodes_of(sh);
+        double v[8][3];
+        for (int k = 0; k < nn; ++k) node_xyz(c, c.cnodes[i * 8 + k], v[k]);
+        double cen[3] = {0, 0, 0};
+        for (int k = 0; k < nn; ++k)
+            for (int d = 0; d < 3; ++d) cen[d] += v[k][d];
+        for (int d = 0; d < 3; ++d) c.centre[i * 3 + d] = cen[d] / nn;
+        double vol = 0.0;
+        for (int t = 0; t < nsubtets(sh); ++t) {
+            const int *tt = subtet(sh, t);
+            vol += tet_volume(v[tt[0]], v[tt[1]], v[tt[2]], v[tt[3]]);
+        }
+        c.vol[i] = vol;
+        double ssum = 0.0;
+        for (int f = 0; f < c.ifca[i]; ++f) {
+            int nv = face_nv(sh, f);
+            double fv[4][3];
+            for (int k = 0; k < nv; ++k)
+                for (int d = 0; d < 3; ++d) fv[k][d] = v[face_node(sh, f, k)][d];
+            double area;
+            if (nv == 4) {
+                double e[3], dd[3];
+                for (int d = 0; d < 3; ++d) {
+                    e[d] = fv[3][d] - fv[1][d];
+                    dd[d] = fv[2][d] - fv[0][d];
+                }
+                double cx = e[1] * dd[2] - e[2] * dd[1], cy = e[2] * dd[0] - e[0] * dd[2],
+                       cz = e[0] * dd[1] - e[1] * dd[0];
+                area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+                // two-triangle check of SURFACE_CALCULATOR3 (src/grid_t.f90:1530-1556)
+                auto tri_area = [](const double *a, const double *b, const double *cc) {
+                    double u[3], w[3];
+                    for (int d = 0; d < 3; ++d) {
+                        u[d] = b[d] - a[d];
+                        w[d] = cc[d] - a[d];
+                    }
+                    double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+                    return 0.5 * std::sqrt(x * x + y * y + z * z);
+                };
+                double a2 = tri_area(fv[0], fv[1], fv[2]) + tri_area(fv[0], fv[2], fv[3]);
+                if (std::fabs((a2 - area) / area) * 100.0 > 10.0) area = a2;
+            } else {
+                double u[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    u[d] = fv[1][d] - fv[0][d];
+                    w[d] = fv[2][d] - fv[0][d];
+                }
+                double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+                area = 0.5 * std::sqrt(x * x + y * y + z * z);
+            }
+            c.surf[i * NF + f] = area;
+            ssum += area;
+            // Newell normal over the (outward ordered) polygon
+            double nx = 0, ny = 0, nz = 0;
+            for (int k = 0; k < nv; ++k) {
+                const double *a = fv[k], *b = fv[(k + 1) % nv];
+                nx += (a[1] - b[1]) * (a[2] + b[2]);
+                ny += (a[2] - b[2]) * (a[0] + b[0]);
+                nz += (a[0] - b[0]) * (a[1] + b[1]);
+            }
+            double r = std::sqrt(nx * nx + ny * ny + nz * nz);
+            nx /= r;
+            ny /= r;
+            nz /= r;
+            double a1, a2;
+            normal_angles(nx, ny, nz, a1, a2);
+            c.angx[i * NF + f] = a1;
+            c.angy[i * NF + f] = a2;
+            c.normal[(i * NF + f) * 3 + 0] = nx;
+            c.normal[(i * NF + f) * 3 + 1] = ny;
+            c.normal[(i * NF + f) * 3 + 2] = nz;
+        }
+        c.minedge[i] = 3.0 * vol / ssum;
+    }
+    // CHECKGRADS face-vertex adoption (src/local_pt.f90:2235-2268): on a
+    // shared face the lower-numbered cell takes the higher-numbered cell's
+    // vertex order, so both sides build their Gauss points in the same order.
+    c.fnodes.assign(nc * NF * 4, -1);
+    c.reorient.assign(nc * NF, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nc; ++i) {
+        int sh = c.cshape[i];
+        for (int f = 0; f < c.ifca[i]; ++f) {
+            int64_t j = c.ineigh[i * NF + f];
+            if (j >= 0 && j > i) {
+                int fj = c.ineighn[i * NF + f];
+                int shj = c.cshape[j];
+                for (int k = 0; k < face_nv(shj, fj); ++k)
+                    c.fnodes[(i * NF + f) * 4 + k] = c.cnodes[j * 8 + face_node(shj, fj, k)];
+                c.reorient[i * NF + f] = 1;
+            } else {
+                for (int k = 0; k < face_nv(sh, f); ++k)
+                    c.fnodes[(i * NF + f) * 4 + k] = c.cnodes[i * 8 + face_node(sh, f, k)];
+            }
+        }
+    }
+}
+
+// Trihedral cone test of CHECK_CONDITION (src/grid_p.f90:4582-4811): is
+// point p a non-negative combination of (a-o, b-o, c-o)?
+bool in_cone(const double *o, const double *a, const double *b, const double *cc, const double *p) {
+    double A[3][3];
+    for (int d = 0; d < 3; ++d) {
+        A[d][0] = a[d] - o[d];
+        A[d][1] = b[d] - o[d];
+        A[d][2] = cc[d] - o[d];
+    }
+    double det = A[0][0] * (A[2][2] * A[1][1] - A[2][1] * A[1][2]) - A[1][0] * (A[2][2] * A[0][1] - A[2][1] * A[0][2]) +
+                 A[2][0] * (A[1][2] * A[0][1] - A[1][1] * A[0][2]);
+    double I[3][3];
+    I[0][0] = A[2][2] * A[1][1] - A[2][1] * A[1][2];
+    I[0][1] = -(A[2][2] * A[0][1] - A[2][1] * A[0][2]);
+    I[0][2] = A[1][2] * A[0][1] - A[1][1] * A[0][2];
+    I[1][0] = -(A[2][2] * A[1][0] - A[2][0] * A[1][2]);
+    I[1][1] = A[2][2] * A[0][0] - A[2][0] * A[0][2];
+    I[1][2] = -(A[1][2] * A[0][0] - A[1][0] * A[0][2]);
+    I[2][0] = A[2][1] * A[1][0] - A[2][0] * A[1][1];
+    I[2][1] = -(A[2][1] * A[0][0] - A[2][0] * A[0][1]);
+    I[2][2] = A[1][1] * A[0][0] - A[1][0] * A[0][1];
+    double q[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+    for (int r = 0; r < 3; ++r) {
+        double x = (I[r][0] * q[0] + I[r][1] * q[1] + I[r][2] * q[2]) / det;
+        if (!(x >= 0.0)) return false;
+    }
+    return true;
+}
+
+// Stencils: BFS over face neighbours (ALLSf, src/grid_p.f90:4411-4551) with
+// the candidate pool truncated at ISELEM; central stencil = first nnb BFS
+// cells (ICOMPACT=0: no distance sort); one directional stencil per face
+// with a neighbour, filled with BFS candidates inside the face cone
+// (STENCIILS, src/grid_p.f90:5822-5960); complete stencils are compacted in
+// order (LOCALISE_STENCIL, src/local_pt.f90:653-760) -> ADMIS, FULL
+// (STENCILS, src/grid_p.f90:6452-6520).
+void build_stencils(ucnsp_case &c) {
+    const int64_t nc = c.ncell;
+    const int NF = ucnsp_case::NF;
+    const int TS = c.typesten, NNB = c.nnb;
+    c.admis.assign(nc, 0);
+    c.full.assign(nc, 0);
+    c.ihexl.assign((size_t)nc * TS * NNB, -1);
+    const int nthr = omp_get_max_threads();
+    std::vector<std::vector<int32_t>> stamp(nthr, std::vector<int32_t>(nc, -1));
+#pragma omp parallel
+    {
+        int tid = omp_get_thread_num();
+        std::vector<int32_t> &mark = stamp[tid];
+        std::vector<int32_t> list;
+        list.reserve(c.iselem);
+        std::vector<int32_t> st((size_t)TS * NNB);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < nc; ++i) {
+            list.clear();
+            list.push_back((int32_t)i);
+            mark[i] = (int32_t)i;
+            size_t head = 0;
+            while ((int)list.size() < c.iselem && head < list.size()) {
+                int64_t cur = list[head++];
+                for (int f = 0; f < c.ifca[cur] && (int)list.size() < c.iselem; ++f) {
+                    int32_t nb = c.ineigh[cur * NF + f];
+                    if (nb < 0 || mark[nb] == (int32_t)i) continue;
+                    mark[nb] = (int32_t)i;
+                    list.push_back(nb);
+                }
+            }
+            std::fill(st.begin(), st.end(), -1);
+            // stencil 1: central
+            for (int k = 0; k < NNB && k < (int)list.size(); ++k) st[k] = list[k];
+            // stencils 2..: directional
+            if (TS > 1) {
+                const double *bc = &c.centre[i * 3];
+                int sh = c.cshape[i];
+                for (int f = 0; f < c.ifca[i] && f + 1 < TS; ++f) {
+                    if (c.ineigh[i * NF + f] < 0) continue;
+                    int32_t *row = &st[(size_t)(f + 1) * NNB];
+                    row[0] = (int32_t)i;
+                    int cnt = 1;
+                    int nv = face_nv(sh, f);
+                    double fv[4][3];
+                    for (int k = 0; k < nv; ++k) node_xyz(c, c.cnodes[i * 8 + face_node(sh, f, k)], fv[k]);
+                    for (size_t J = 1; J < list.size() && cnt < NNB; ++J) {
+                        int32_t cand = list[J];
+                        double vg[3] = {c.centre[cand * 3], c.centre[cand * 3 + 1], c.centre[cand * 3 + 2]};
+                        for (int d = 0; d < 3; ++d)
+                            if (c.per[d] > 0 && std::fabs(vg[d] - bc[d]) >= 0.5 * c.per[d])
+                                vg[d] += c.per[d] * ((bc[d] - 0.5 * c.per[d]) >= 0 ? 1.0 : -1.0);
+                        bool ok;
+                        if (nv == 3) {
+                            ok = in_cone(bc, fv[0], fv[2], fv[1], vg);
+                        } else {  // union of the four corner triangles
+                            ok = in_cone(bc, fv[1], fv[0], fv[2], vg) || in_cone(bc, fv[1], fv[3], fv[2], vg) ||
+                                 in_cone(bc, fv[0], fv[3], fv[2], vg) || in_cone(bc, fv[0], fv[3], fv[1], vg);
+                        }
+                        if (ok) row[cnt++] = cand;
+                    }
+                }
+            }
+            // compact complete stencils
+            int ad = 0;
+            for (int s = 0; s < TS; ++s) {
+                bool complete = true;
+                for (int k = 0; k < NNB; ++k)
+                    if (st[(size_t)s * NNB + k] < 0) { complete = false; break; }
+                if (!complete) continue;
+                std::memcpy(&c.ihexl[((size_t)i * TS + ad) * NNB], &st[(size_t)s * NNB], sizeof(int32_t) * NNB);
+                ++ad;
+            }
+            c.admis[i] = ad;
+            c.full[i] = (c.o.iweno == 1 && ad >= c.ifca[i] + 1) ? 1 : 0;
+        }
+    }
+}
+
+// Householder QR of a square matrix (QRDECOMPOSITION, src/matrix.f90:93-146)
+// with the reflector applied in place instead of forming P_l, then the
+// triangular inverse of INVERT (src/prestore.f90:4047-4069);
+// returns invmat = R^-1 Q^T.
+void qr_inverse(int n, const double *M, double *invmat, double *work) {
+    double *R = work, *Q = work + n * n, *v = work + 2 * n * n, *Ri = work + 2 * n * n + n;
+    for (int a = 0; a < n * n; ++a) R[a] = M[a];
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) Q[a * n + b] = (a == b) ? 1.0 : 0.0;
+    for (int l = 0; l < n; ++l) {
+        double nrm = 0.0;
+        for (int r = l; r < n; ++r) nrm += R[r * n + l] * R[r * n + l];
+        nrm = std::sqrt(nrm);
+        for (int r = l; r < n; ++r) v[r] = R[r * n + l];
+        v[l] += (R[l * n + l] >= 0.0 ? 1.0 : -1.0) * nrm;
+        double vn = 0.0;
+        for (int r = l; r < n; ++r) vn += v[r] * v[r];
+        vn = std::sqrt(vn);
+        for (int r = l; r < n; ++r) v[r] /= vn;
+        for (int col = 0; col < n; ++col) {  // R = (I - 2 v v^T) R
+            double s = 0.0;
+            for (int r = l; r < n; ++r) s += v[r] * R[r * n + col];
+            for (int r = l; r < n; ++r) R[r * n + col] -= 2.0 * v[r] * s;
+        }
+        for (int row = 0; row < n; ++row) {  // Q = Q (I - 2 v v^T)
+            double s = 0.0;
+            for (int r = l; r < n; ++r) s += Q[row * n + r] * v[r];
+            for (int r = l; r < n; ++r) Q[row * n + r] -= 2.0 * s * v[r];
+        }
+    }
+    for (int a = 0; a < n * n; ++a) Ri[a] = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+        Ri[i * n + i] = 1.0 / R[i * n + i];
+        for (int j = i + 1; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < j; ++k) s -= R[k * n + j] * Ri[i * n + k];
+            Ri[i * n + j] = s / R[j * n + j];
+        }
+    }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += Ri[a * n + k] * Q[b * n + k];
+            invmat[a * n + b] = s;
+        }
+}
+
+// Local frame of LOCALISE_STEN2 (src/local_pt.f90:1037-1101): the first
+// DECOMPOSE3 sub-tetrahedron; x_local = VVA1 (x - v1), VVA1 = inverse of
+// [v2-v1, v3-v1, v4-v1] as built by COMPUTEJACOBIANS (src/grid_t.f90:581-616).
+void cell_frame(const ucnsp_case &c, int64_t i, double *J9, double *o3, double &detjc) {
+    int sh = c.cshape[i];
+    const int *tt = subtet(sh, 0);
+    double v[4][3];
+    for (int k = 0; k < 4; ++k) node_xyz(c, c.cnodes[i * 8 + tt[k]], v[k]);
+    double A[3][3];
+    for (int d = 0; d < 3; ++d) {
+        A[d][0] = v[1][d] - v[0][d];
+        A[d][1] = v[2][d] - v[0][d];
+        A[d][2] = v[3][d] - v[0][d];
+    }
+    double det = A[0][0] * ((A[2][2] * A[1][1]) - (A[2][1] * A[1][2])) - A[1][0] * ((A[2][2] * A[0][1]) - (A[2][1] * A[0][2])) +
+                 A[2][0] * ((A[1][2] * A[0][1]) - (A[1][1] * A[0][2]));
+    double I[3][3];
+    I[0][0] = A[2][2] * A[1][1] - A[2][1] * A[1][2];
+    I[0][1] = -(A[2][2] * A[0][1] - A[2][1] * A[0][2]);
+    I[0][2] = A[1][2] * A[0][1] - A[1][1] * A[0][2];
+    I[1][0] = -(A[2][2] * A[1][0] - A[2][0] * A[1][2]);
+    I[1][1] = A[2][2] * A[0][0] - A[2][0] * A[0][2];
+    I[1][2] = -(A[1][2] * A[0][0] - A[1][0] * A[0][2]);
+    I[2][0] = A[2][1] * A[1][0] - A[2][0] * A[1][1];
+    I[2][1] = -(A[2][1] * A[0][0] - A[2][0] * A[0][1]);
+    I[2][2] = A[1][1] * A[0][0] - A[1][0] * A[0][1];
+    for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) J9[r * 3 + s] = I[r][s] / det;
+    for (int d = 0; d < 3; ++d) o3[d] = v[0][d];
+    detjc = det;
+}
+
+inline void to_local(const double *J9, const double *o, const double *x, double *y) {
+    double q[3] = {x[0] - o[0], x[1] - o[1], x[2] - o[2]};
+    for (int r = 0; r < 3; ++r) y[r] = J9[r * 3 + 0] * q[0] + J9[r * 3 + 1] * q[1] + J9[r * 3 + 2] * q[2];
+}
+
+// Node coordinates of stencil member j, unwrapped around cell i's centre
+// and mapped into cell i's local frame.
+void member_local_nodes(const ucnsp_case &c, int64_t i, int64_t j, const double *J9, const double *o, double v[8][3]) {
+    const double *ci = &c.centre[i * 3];
+    double cj[3] = {c.centre[j * 3], c.centre[j * 3 + 1], c.centre[j * 3 + 2]};
+    double shift[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+        if (c.per[d] > 0 && std::fabs(cj[d] - ci[d]) > 0.5 * c.per[d])
+            shift[d] = c.per[d] * ((ci[d] - 0.5 * c.per[d]) >= 0 ? 1.0 : -1.0);
+    int nn = nnodes_of(c.cshape[j]);
+    for (int k = 0; k < nn; ++k) {
+        double p[3];
+        node_xyz(c, c.cnodes[j * 8 + k], p);
+        for (int d = 0; d < 3; ++d) p[d] += shift[d];
+        to_local(J9, o, p, v[k]);
+    }
+}
+
+// number of derivative multi-indices 1<=|a|<=p and D^a of monomial d at x
+struct DerivTable {
+    std::vector<int> alpha;  // na*3
+    int na = 0;
+};
+DerivTable make_derivs(int p) {
+    DerivTable t;
+    for (int s = 1; s <= p; ++s)
+        for (int a = s; a >= 0; --a)
+            for (int b = s - a; b >= 0; --b) {
+                int cz = s - a - b;
+                t.alpha.push_back(a);
+                t.alpha.push_back(b);
+                t.alpha.push_back(cz);
+                t.na++;
+            }
+    return t;
+}
+inline double falling(int e, int a) {  // e!/(e-a)!
+    double r = 1.0;
+    for (int k = 0; k < a; ++k) r *= (e - k);
+    return r;
+}
+
+// PRESTORE_RECONSTRUCTION3 per cell: basis integrals, indicator matrix,
+// pseudo-inverses of every admissible stencil, face Gauss points in the
+// local frame (MEMORY_FAST).
+void build_prestore(ucnsp_case &c) {
+    const int64_t nc = c.ncell;
+    const int NF = ucnsp_case::NF;
+    const int TS = c.typesten, NNB = c.nnb, IM = c.imax, ID = c.ideg, NQ = c.nqpmax;
+    c.invmat.alloc((size_t)nc * TS * IM * ID);
+    c.indicator.assign(c.o.iweno == 1 ? (size_t)nc * ID * ID : 0, 0.0);
+    c.integ_basis.assign((size_t)nc * ID, 0.0);
+    c.volume11.assign(nc, 0.0);
+    c.qpoints.assign((size_t)nc * NF * NQ * 3, 0.0);
+    c.frame.assign((size_t)nc * 12, 0.0);
+    const DerivTable dt = make_derivs(c.iorder);
+#pragma omp parallel
+    {
+        std::vector<double> A((size_t)IM * ID), M((size_t)ID * ID), inv((size_t)ID * ID);
+        std::vector<double> work(3 * ID * ID + ID + 8), mono(ID), intg(ID), mbar(ID), volj(IM);
+        std::vector<double> D((size_t)dt.na * ID);
+        QPts q;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nc; ++i) {
+            double J9[9], o[3], detjc;
+            cell_frame(c, i, J9, o, detjc);
+            for (int k = 0; k < 9; ++k) c.frame[i * 12 + k] = J9[k];
+            for (int k = 0; k < 3; ++k) c.frame[i * 12 + 9 + k] = o[k];
+            const double adet = std::fabs(detjc);
+            // own cell in the local frame
+            double vi[8][3];
+            member_local_nodes(c, i, i, J9, o, vi);
+            // INTEG_BASIS: integral of the raw monomials (CALINTBASIS before
+            // INTEG_BASIS is set, src/prestore.f90:222-225)
+            cell_rule(c.cshape[i], vi, c.iorder, q);
+            std::fill(intg.begin(), intg.end(), 0.0);
+            for (size_t p = 0; p < q.w.size(); ++p) {
+                monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, mono.data());
+                for (int d = 0; d < ID; ++d) intg[d] += q.w[p] * mono[d];
+            }
+            const double vol11 = c.vol[i] / adet;
+            c.volume11[i] = vol11;
+            for (int d = 0; d < ID; ++d) {
+                c.integ_basis[i * ID + d] = intg[d];
+                mbar[d] = intg[d] / vol11;
+            }
+            // INDICATOR (INDICATORMATRIX + PH*, src/prestore.f90:1529-2025)
+            if (c.o.iweno == 1) {
+                cell_rule(c.cshape[i], vi, 2 * (c.iorder - 1), q);
+                double *B = &c.indicator[(size_t)i * ID * ID];
+                for (size_t p = 0; p < q.w.size(); ++p) {
+                    double x = q.x[p * 3], y = q.x[p * 3 + 1], z = q.x[p * 3 + 2];
+                    double px[5], py[5], pz[5];
+                    px[0] = py[0] = pz[0] = 1.0;
+                    for (int k = 1; k < 5; ++k) {
+                        px[k] = px[k - 1] * x;
+                        py[k] = py[k - 1] * y;
+                        pz[k] = pz[k - 1] * z;
+                    }
+                    for (int a = 0; a < dt.na; ++a) {
+                        int ax = dt.alpha[a * 3], ay = dt.alpha[a * 3 + 1], az = dt.alpha[a * 3 + 2];
+                        for (int d = 0; d < ID; ++d) {
+                            int ex = MONO[d][0], ey = MONO[d][1], ez = MONO[d][2];
+                            if (ex < ax || ey < ay || ez < az) {
+                                D[a * ID + d] = 0.0;
+                                continue;
+                            }
+                            D[a * ID + d] = falling(ex, ax) * falling(ey, ay) * falling(ez, az) * px[ex - ax] *
+                                            py[ey - ay] * pz[ez - az];
+                        }
+                    }
+                    for (int k = 0; k < ID; ++k)
+                        for (int l = 0; l < ID; ++l) {
+                            double s = 0.0;
+                            for (int a = 0; a < dt.na; ++a) s += D[a * ID + k] * D[a * ID + l];
+                            B[k * ID + l] += q.w[p] * s;
+                        }
+                }
+            }
+            // pseudo-inverses of the admissible stencils
+            double *Pcell = c.invmat.p + (size_t)i * TS * IM * ID;
+            const int ad = c.admis[i];
+            for (int s = 0; s < TS; ++s) {
+                double *P = Pcell + (size_t)s * IM * ID;
+                if (s >= ad) {
+                    std::fill(P, P + (size_t)IM * ID, 0.0);
+                    continue;
+                }
+                const int32_t *mem = &c.ihexl[((size_t)i * TS + s) * NNB];
+                for (int k = 0; k < IM; ++k) {
+                    int64_t j = mem[k + 1];
+                    double vj[8][3];
+                    member_local_nodes(c, i, j, J9, o, vj);
+                    double wl = 1.0;
+                    if (c.o.weight_lsqr == 1) {
+                        double ci[3] = {c.centre[i * 3], c.centre[i * 3 + 1], c.centre[i * 3 + 2]};
+                        double cj[3] = {c.centre[j * 3], c.centre[j * 3 + 1], c.centre[j * 3 + 2]};
+                        unwrap(c, ci, cj);
+                        double dd = 0;
+                        for (int d = 0; d < 3; ++d) dd += (cj[d] - ci[d]) * (cj[d] - ci[d]);
+                        wl = 1.0 / std::sqrt(dd);
+                    }
+                    cell_rule(c.cshape[j], vj, c.iorder, q);
+                    double vq = 0.0;
+                    std::fill(intg.begin(), intg.end(), 0.0);
+                    for (size_t p = 0; p < q.w.size(); ++p) {
+                        monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, mono.data());
+                        for (int d = 0; d < ID; ++d) intg[d] += q.w[p] * mono[d];
+                        vq += q.w[p];
+                    }
+                    // STENCILS(LL,K,:) = w * integral of (m - mbar) over V_j (COMPBASEL)
+                    for (int d = 0; d < ID; ++d) A[(size_t)k * ID + d] = wl * (intg[d] - mbar[d] * vq);
+                    volj[k] = (c.vol[j] / adet) * wl;  // ILOX_VOLUME * WLSQR
+                }
+                // LSCQM = A^T A (src/prestore.f90:437-476)
+                for (int a = 0; a < ID; ++a)
+                    for (int b = 0; b < ID; ++b) {
+                        double s2 = 0.0;
+                        for (int k = 0; k < IM; ++k) s2 += A[(size_t)k * ID + a] * A[(size_t)k * ID + b];
+                        M[a * ID + b] = s2;
+                    }
+                qr_inverse(ID, M.data(), inv.data(), work.data());
+                // invmat_stencilt = invmat A^T, column iq scaled by V_iq w_iq
+                bool bad = false;
+                for (int k = 0; k < IM; ++k)
+                    for (int a = 0; a < ID; ++a) {
+                        double s2 = 0.0;
+                        for (int b = 0; b < ID; ++b) s2 += inv[a * ID + b] * A[(size_t)k * ID + b];
+                        double val = s2 * volj[k];
+                        P[(size_t)k * ID + a] = val;
+                        if (val != val) bad = true;
+                    }
+                if (bad) c.full[i] = 0;  // src/prestore.f90:526-529
+            }
+            // MEMORY_FAST: face Gauss points in the local frame, built from
+            // the adopted vertex order of each face
+            const double *ci = &c.centre[i * 3];
+            for (int f = 0; f < c.ifca[i]; ++f) {
+                int nv = c.types_faces[i * NF + f] == 5 ? 4 : 3;
+                double fv[4][3], lv[4][3];
+                for (int k = 0; k < nv; ++k) {
+                    node_xyz(c, c.fnodes[(i * NF + f) * 4 + k], fv[k]);
+                    unwrap(c, ci, fv[k]);
+                    to_local(J9, o, fv[k], lv[k]);
+                }
+                double *Q = &c.qpoints[((size_t)i * NF + f) * NQ * 3];
+                if (nv == 4) {
+                    for (int g = 0; g < c.qp_quad; ++g) {
+                        double R = c.quad_rs[g * 2], S = c.quad_rs[g * 2 + 1];
+                        double N[4] = {0.25 * (1 - R) * (1 - S), 0.25 * (1 + R) * (1 - S), 0.25 * (1 + R) * (1 + S),
+                                       0.25 * (1 - R) * (1 + S)};
+                        for (int d = 0; d < 3; ++d)
+                            Q[g * 3 + d] = N[0] * lv[0][d] + N[1] * lv[1][d] + N[2] * lv[2][d] + N[3] * lv[3][d];
+                    }
+                } else {
+                    for (int g = 0; g < c.qp_tri; ++g) {
+                        const double *b = &c.tri_bary[g * 3];
+                        for (int d = 0; d < 3; ++d) Q[g * 3 + d] = b[0] * lv[0][d] + b[1] * lv[1][d] + b[2] * lv[2][d];
+                    }
+                }
+            }
+        }
+    }
+}
+
+void gen_box(ucnsp_case &c) {
+    const ucnsp_opts &o = c.o;
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const double hx = o.lx / nx, hy = o.ly / ny, hz = o.lz / nz;
+    c.per[0] = o.lx;
+    c.per[1] = o.ly;
+    c.per[2] = o.lz;
+    c.nnode = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    c.xyz.resize(c.nnode * 3);
+    c.canon.resize(c.nnode);
+    auto nid = [&](int i, int j, int k) { return (int64_t)i + (int64_t)(nx + 1) * (j + (int64_t)(ny + 1) * k); };
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                int64_t n = nid(i, j, k);
+                double p[3] = {i * hx, j * hy, k * hz};
+                if (o.jitter > 0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz) {
+                    uint64_t s = o.seed * 0x100000001B3ULL + (uint64_t)n * 3;
+                    p[0] += o.jitter * hx * (2.0 * u01(s) - 1.0);
+                    p[1] += o.jitter * hy * (2.0 * u01(s) - 1.0);
+                    p[2] += o.jitter * hz * (2.0 * u01(s) - 1.0);
+                }
+                c.xyz[n * 3] = p[0];
+                c.xyz[n * 3 + 1] = p[1];
+                c.xyz[n * 3 + 2] = p[2];
+                c.canon[n] = (int32_t)((i % nx) + nx * ((j % ny) + ny * (k % nz)));
+            }
+    const int64_t nhex = (int64_t)nx * ny * nz;
+    c.shape = o.shape;
+    c.ncell = (o.shape == UCNSP_HEX) ? nhex : 6 * nhex;
+    c.cnodes.assign(c.ncell * 8, -1);
+    c.cshape.assign(c.ncell, o.shape);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k);
+                int64_t v[8] = {nid(i, j, k),     nid(i + 1, j, k),     nid(i + 1, j + 1, k),     nid(i, j + 1, k),
+                                nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
+                if (o.shape == UCNSP_HEX) {
+                    for (int q = 0; q < 8; ++q) c.cnodes[h * 8 + q] = (int32_t)v[q];
+                } else {
+                    // Kuhn split: 6 tets along the 000-111 diagonal, one per axis order
+                    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+                    auto corner = [&](int bx, int by, int bz) {
+                        const int idx[2][2][2] = {{{0, 4}, {3, 7}}, {{1, 5}, {2, 6}}};
+                        return v[idx[bx][by][bz]];
+                    };
+                    for (int t = 0; t < 6; ++t) {
+                        int b[3] = {0, 0, 0};
+                        int64_t tv[4];
+                        tv[0] = corner(0, 0, 0);
+                        b[perm[t][0]] = 1;
+                        tv[1] = corner(b[0], b[1], b[2]);
+                        b[perm[t][1]] = 1;
+                        tv[2] = corner(b[0], b[1], b[2]);
+                        tv[3] = corner(1, 1, 1);
+                        // positive orientation (odd axis orders swap nodes 2,3)
+                        double a0[3], a1[3], a2[3], a3[3];
+                        for (int d = 0; d < 3; ++d) {
+                            a0[d] = c.xyz[tv[0] * 3 + d];
+                            a1[d] = c.xyz[tv[1] * 3 + d];
+                            a2[d] = c.xyz[tv[2] * 3 + d];
+                            a3[d] = c.xyz[tv[3] * 3 + d];
+                        }
+                        double e1[3], e2[3], e3[3];
+                        for (int d = 0; d < 3; ++d) {
+                            e1[d] = a1[d] - a0[d];
+                            e2[d] = a2[d] - a0[d];
+                            e3[d] = a3[d] - a0[d];
+                        }
+                        double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                                     e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+                        if (det < 0) std::swap(tv[1], tv[2]);
+                        int64_t cell = h * 6 + t;
+                        for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)tv[q];
+                    }
+                }
+            }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ucnsp_last_error(void) { return g_err.c_str(); }
+
+int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n) {
+    std::vector<double> a, b;
+    int m = tri ? tri_rule(igq, a, b) : quad_rule(igq, a, b);
+    if (m < 0) return fail(UCNS_EUNSUPPORTED, "face rule not available");
+    for (int g = 0; g < m; ++g) {
+        if (tri) {
+            pts[g * 3] = a[g * 3];
+            pts[g * 3 + 1] = a[g * 3 + 1];
+            pts[g * 3 + 2] = a[g * 3 + 2];
+        } else {
+            pts[g * 3] = a[g * 2];
+            pts[g * 3 + 1] = a[g * 2 + 1];
+            pts[g * 3 + 2] = 0.0;
+        }
+        w[g] = b[g];
+    }
+    *n = m;
+    return UCNS_OK;
+}
+
+int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
+    if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
+    const ucnsp_opts &o = *opts;
+    if (o.shape != UCNSP_HEX && o.shape != UCNSP_TET) return fail(UCNS_EUNSUPPORTED, "shape must be hex(1) or tet(2)");
+    if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "periodic box needs at least 3 cells per direction");
+    if (o.iorder < 1 || o.iorder > 3) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..3");
+    if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
+    if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
+    if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");
+    if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
+    ucnsp_case *c = new ucnsp_case();
+    c->o = o;
+    gen_box(*c);
+    build_connectivity(*c);
+    for (int64_t i = 0; i < c->ncell; ++i)
+        for (int f = 0; f < c->ifca[i]; ++f)
+            if (c->ineigh[i * ucnsp_case::NF + f] < 0) {
+                delete c;
+                return fail(UCNS_EBADARG, "unmatched face in a periodic box");
+            }
+    build_geometry(*c);
+    // scheme sizes (DETERMINE_SIZE, src/grid_p.f90:6097-6288; parameters.f90:1270-1394)
+    c->iorder = o.iorder;
+    c->ideg = ndeg_of(o.iorder);
+    const int ilx = c->ideg + 1;
+    int iextend;
+    if (o.iweno == 1) {
+        c->nnb = ilx * o.extf;
+        c->typesten = 7;
+        iextend = 12;
+        int64_t pool = (int64_t)ilx * o.extf * iextend;
+        c->iselem = (int)std::min<int64_t>(pool, c->ncell - 1);
+    } else {
+        c->nnb = 9;
+        c->typesten = 1;
+        iextend = 2;
+        c->iselem = ilx * o.extf * iextend;
+    }
+    if (c->iselem < c->nnb) {
+        delete c;
+        return fail(UCNS_EBADARG, "mesh too small for the stencil size");
+    }
+    c->imax = c->nnb - 1;
+    c->igq = std::min(o.iorder, 6);
+    c->qp_quad = quad_rule(c->igq, c->quad_rs, c->quad_w);
+    c->qp_tri = tri_rule(c->igq, c->tri_bary, c->tri_w);
+    if (c->qp_quad < 0 || c->qp_tri < 0) {
+        delete c;
+        return fail(UCNS_EUNSUPPORTED, "face quadrature rule not available");
+    }
+    c->nqpmax = std::max(c->qp_quad, c->qp_tri);
+    build_stencils(*c);
+    build_prestore(*c);
+    *out = c;
+    return UCNS_OK;
+}
+
+int ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1, ucns_mesh_v1 *m, ucns_scheme_v1 *s,
+               ucns_prestore_v1 *p) {
+    if (!c || !m || !s || !p) return fail(UCNS_EBADARG, "null argument");
+    std::memset(m, 0, sizeof(*m));
+    std::memset(s, 0, sizeof(*s));
+    std::memset(p, 0, sizeof(*p));
+    m->abi_version = UCNS_ABI_VERSION;
+    m->ncell = (int32_t)c->ncell;
+    m->index_base = 0;
+    m->nface_max = ucnsp_case::NF;
+    m->ifca = c->ifca.data();
+    m->ineigh = c->ineigh.data();
+    m->ineighn = c->ineighn.data();
+    m->bcode = c->bcode.data();
+    m->types_faces = c->types_faces.data();
+    m->faceanglex = c->angx.data();
+    m->faceangley = c->angy.data();
+    m->surf = c->surf.data();
+    m->totvolume = c->vol.data();
+    m->minedge = c->minedge.data();
+    s->abi_version = UCNS_ABI_VERSION;
+    s->precision = precision;
+    s->iweno = c->o.iweno;
+    s->iorder = c->iorder;
+    s->idegfree = c->ideg;
+    s->inumneighbours = c->nnb;
+    s->typesten = c->typesten;
+    s->ees = 0;
+    s->wenoz = 0;
+    s->wenwrt = 1;
+    s->limiter = 1;
+    s->riemann = 1;
+    s->qp_quad = c->qp_quad;
+    s->qp_triangle = c->qp_tri;
+    s->gamma = gamma;
+    s->lwci1 = lwci1;
+    s->weno_eps = 1e-11;
+    s->weights_quad = c->quad_w.data();
+    s->weights_tri = c->tri_w.data();
+    p->admis = c->admis.data();
+    p->full = c->full.data();
+    p->ihexl = c->ihexl.data();
+    p->invmat = c->invmat.p;
+    p->indicator = c->indicator.empty() ? nullptr : c->indicator.data();
+    p->integ_basis = c->integ_basis.data();
+    p->volume11 = c->volume11.data();
+    p->qpoints = c->qpoints.data();
+    return UCNS_OK;
+}
+
+int ucnsp_info(const ucnsp_case *c, int64_t *ints, double *dbls) {
+    if (!c) return fail(UCNS_EBADARG, "null case");
+    if (ints) {
+        int64_t v[16] = {c->ncell, c->nnode, ucnsp_case::NF, c->iorder, c->ideg, c->nnb, c->imax, c->typesten,
+                         c->iselem, c->igq, c->qp_quad, c->qp_tri, c->nqpmax, c->shape, c->o.iweno, 0};
+        std::memcpy(ints, v, sizeof(v));
+    }
+    if (dbls) {
+        double v[8] = {c->per[0], c->per[1], c->per[2], 0, 0, 0, 0, 0};
+        std::memcpy(dbls, v, sizeof(v));
+    }
+    return UCNS_OK;
+}
+
+int ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *count, int32_t *dtype) {
+    if (!c || !name || !ptr || !count || !dtype) return fail(UCNS_EBADARG, "null argument");
+    std::string n(name);
+    auto I = [&](const std::vector<int32_t> &v) {
+        *ptr = (void *)v.data();
+        *count = (int64_t)v.size();
+        *dtype = 0;
+        return UCNS_OK;
+    };
+    auto D = [&](const std::vector<double> &v) {
+        *ptr = (void *)v.data();
+        *count = (int64_t)v.size();
+        *dtype = 1;
+        return UCNS_OK;
+    };
+    if (n == "xyz") return D(c->xyz);
+    if (n == "cell_nodes") return I(c->cnodes);
+    if (n == "cell_shape") return I(c->cshape);
+    if (n == "centre") return D(c->centre);
+    if (n == "volume") return D(c->vol);
+    if (n == "minedge") return D(c->minedge);
+    if (n == "surf") return D(c->surf);
+    if (n == "normal") return D(c->normal);
+    if (n == "ineigh") return I(c->ineigh);
+    if (n == "ineighn") return I(c->ineighn);
+    if (n == "bcode") return I(c->bcode);
+    if (n == "face_nodes") return I(c->fnodes);
+    if (n == "admis") return I(c->admis);
+    if (n == "full") return I(c->full);
+    if (n == "ihexl") return I(c->ihexl);
+    if (n == "indicator") return D(c->indicator);
+    if (n == "integ_basis") return D(c->integ_basis);
+    if (n == "volume11") return D(c->volume11);
+    if (n == "qpoints") return D(c->qpoints);
+    if (n == "frame") return D(c->frame);
+    if (n == "quad_w") return D(c->quad_w);
+    if (n == "tri_w") return D(c->tri_w);
+    if (n == "invmat") {
+        *ptr = (void *)c->invmat.p;
+        *count = (int64_t)c->invmat.n;
+        *dtype = 1;
+        return UCNS_OK;
+    }
+    return fail(UCNS_EBADARG, "unknown array " + n);
+}
+
+int ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint64_t seed, double *u) {
+    if (!c || !u) return fail(UCNS_EBADARG, "null argument");
+    const int64_t nc = c->ncell;
+#pragma omp parallel
+    {
+        QPts q;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < nc; ++i) {
+            double v[8][3];
+            int nn = nnodes_of(c->cshape[i]);
+            for (int k = 0; k < nn; ++k) node_xyz(*c, c->cnodes[i * 8 + k], v[k]);
+            cell_rule(c->cshape[i], v, 5, q);
+            double rho = 1.0;
+            if (noise_amp != 0.0) {
+                uint64_t s = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i;
+                double u1 = std::max(u01(s), 1e-300), u2 = u01(s);
+                rho += noise_amp * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2);
+            }
+            double vol = 0, su = 0, sv = 0, sw = 0, sp = 0, sk = 0;
+            for (size_t p = 0; p < q.w.size(); ++p) {
+                double x = q.x[p * 3], y = q.x[p * 3 + 1], z = q.x[p * 3 + 2], w = q.w[p];
+                double uu = std::sin(x) * std::cos(y) * std::cos(z);
+                double vv = -std::cos(x) * std::sin(y) * std::cos(z);
+                double ww = 0.0;
+                double pp = 100.0 / gamma + (rho / 16.0) * (std::cos(2 * x) + std::cos(2 * y)) * (std::cos(2 * z) + 2.0);
+                vol += w;
+                su += w * uu;
+                sv += w * vv;
+                sw += w * ww;
+                sp += w * pp;
+                sk += w * 0.5 * rho * (uu * uu + vv * vv + ww * ww);
+            }
+            u[i * 5 + 0] = rho;
+            u[i * 5 + 1] = rho * su / vol;
+            u[i * 5 + 2] = rho * sv / vol;
+            u[i * 5 + 3] = rho * sw / vol;
+            u[i * 5 + 4] = sp / vol / (gamma - 1.0) + sk / vol;
+        }
+    }
+    return UCNS_OK;
+}
+
+void ucnsp_free(ucnsp_case *c) { delete c; }
+
+}  // extern "C"

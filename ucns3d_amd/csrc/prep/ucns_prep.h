@@ -1,0 +1,74 @@
+/*
+ * ucns_prep.h -- host-side FP64 preprocessor (libucnsprep.so).
+ *
+ * Builds everything the residual path reads but never writes: the mesh
+ * (synthetic periodic boxes), face connectivity, geometry, stencils and the
+ * PRESTORE arrays (pseudo-inverses, smoothness-indicator matrices, basis
+ * integrals, face Gauss points in each cell's local frame).  It restates the
+ * reference's setup chain:
+ *   NEIGHBOURSS / GEOMETRY_CALC   src/grid_p.f90:1850, src/grid_t.f90:1192-1560
+ *   DETSTEN + ALLSf (BFS)         src/grid_p.f90:3979-4551
+ *   STENCIILS + CHECK_CONDITION   src/grid_p.f90:5822-6090, 4582-4811
+ *   STENCILS (ADMIS, FULL)        src/grid_p.f90:6452-6560
+ *   LOCALISE_STENCIL / _STEN2     src/local_pt.f90:653-1230
+ *   CHECKGRADS (face reorient)    src/local_pt.f90:2220-2286
+ *   FIND_ROT_ANGLES               src/grid_t.f90:274-456
+ *   PRESTORE_RECONSTRUCTION3      src/prestore.f90:167-560, 1529-2025, 3625-4069
+ *   QRDECOMPOSITION               src/matrix.f90:93-146
+ *   MEMORY_FAST (face qps)        src/reconstruct_p.f90:45-302
+ * and fills the ucns_*_v1 structs of include/ucns.h, so the GPU library
+ * receives exactly what a Fortran host would pass.
+ */
+#ifndef UCNS_PREP_H
+#define UCNS_PREP_H
+
+#include <stdint.h>
+#include "../../../include/ucns.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element shapes (IELEM%ISHAPE) */
+#define UCNSP_HEX    1
+#define UCNSP_TET    2
+
+typedef struct ucnsp_opts {
+    int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex */
+    int32_t nx, ny, nz;
+    double  lx, ly, lz;     /* periodic box [0,lx]x[0,ly]x[0,lz] */
+    double  jitter;         /* interior vertices moved by U(-j,j)*spacing per coordinate */
+    uint64_t seed;
+    int32_t iorder;         /* 1 MUSCL (linear), 3 WENO-4 (cubic) ... */
+    int32_t iweno;          /* 1 WENO, -1 MUSCL */
+    int32_t extf;           /* stencil extension factor (profile 8: 2) */
+    int32_t weight_lsqr;    /* 0 unweighted, 1 inverse-distance */
+    int32_t nthreads;       /* OpenMP threads, <=0: library default */
+} ucnsp_opts;
+
+typedef struct ucnsp_case ucnsp_case;
+
+int  ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out);
+/* Fill the boundary structs with pointers into the case's storage (valid
+ * until ucnsp_free).  precision/gamma/lwci1 are the caller's scheme choices. */
+int  ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1,
+                ucns_mesh_v1 *mesh, ucns_scheme_v1 *scheme, ucns_prestore_v1 *pre);
+/* Named internal arrays for tests and drivers: "xyz", "cell_nodes",
+ * "centre", "ihexl", ... ; dtype 0=int32, 1=float64. */
+int  ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *count, int32_t *dtype);
+/* Integer / real metadata: ints[0..15], dbls[0..7] (see ucns_prep.cpp). */
+int  ucnsp_info(const ucnsp_case *c, int64_t *ints, double *dbls);
+/* Cell averages (AoS [ncell][5], conserved) of the periodic vortex
+ * rho=1(+noise), u=sin x cos y cos z, v=-cos x sin y cos z, w=0,
+ * p=100/gamma+(rho/16)(cos2x+cos2y)(cos2z+2).  noise_amp*N(0,1) is added to
+ * each cell's density (seeded). */
+int  ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint64_t seed, double *u_aos);
+/* Gauss-point tables used by the face integration (weights sum to 1). */
+int  ucnsp_face_rule(int32_t igqrules, int32_t tri, double *pts /*[n][3] barycentric or (r,s,0)*/, double *w, int32_t *n);
+void ucnsp_free(ucnsp_case *c);
+const char *ucnsp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

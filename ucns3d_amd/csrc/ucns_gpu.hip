@@ -1,0 +1,1391 @@
+// libucnsgpu.so -- MI355X (gfx950) residual + SSP-RK3 path behind include/ucns.h.
+//
+// Per RK stage three kernels run on the library stream:
+//   k_recon   one wavefront per cell: stencil LSQ contraction (P_s dU_s),
+//             smoothness indicators and WENO weights, the weighted
+//             polynomial at every face Gauss point, CHECKSOL -> MUSCL/Barth
+//             -> CHECKSOLX, ULEFT written once per cell
+//             (ARBITRARY_ORDER, src/reconstruct_p.f90:2954-3015)
+//   k_flux    one thread per unique face: rotate, HLLC, rotate back at each
+//             Gauss point; the area-weighted sum is written once per face
+//             (CALCULATE_FLUXESHI_CONVECTIVE, src/flux_p.f90:478-1030, which
+//             solves every face twice; here each face is solved once and
+//             scattered atomic-free by the gather below)
+//   k_update  one thread per cell: RHS = signed sum of its faces' fluxes, the
+//             Shu-Osher stage update (RUNGE_KUTTA3, src/time_p.f90:885-1027),
+//             and after the last stage the next dt as a min-reduction
+//             (CALCULATE_CFL, src/time_p.f90:27-116)
+// Device layouts are chosen for coalescing and differ from the ABI layouts;
+// ucns_upload_prestore converts.  FP64 everywhere in UCNS_FP64; in UCNS_MIXED
+// the stencil matrices, indicators, Gauss points, reconstruction, weights and
+// HLLC run in FP32 while dU formation, conserved->primitive recovery, flux
+// accumulation, the RK update and dt stay FP64.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ucns.h"
+
+namespace {
+
+constexpr int NV = 5;
+constexpr int WAVE = 64;
+constexpr int NXCD = 8;
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+            return UCNS_EHIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+// monomial exponents of BASIS_REC (src/basis.f90:11-102), cubic max here
+__constant__ int8_t c_mono[34][3] = {
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+    {2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2},
+    {3, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 3, 0}, {2, 0, 1}, {1, 1, 1}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {0, 0, 3},
+    {4, 0, 0}, {3, 1, 0}, {2, 2, 0}, {1, 3, 0}, {0, 4, 0}, {3, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 3, 1},
+    {2, 0, 2}, {1, 1, 2}, {0, 2, 2}, {1, 0, 3}, {0, 1, 3}, {0, 0, 4}};
+
+// face quadrature weights (quad, triangle), sum 1
+__constant__ double c_wq[64];
+__constant__ double c_wt[64];
+
+// per-cell info word
+//   bits 0-2 ifca, 3-5 admis, 6 full, 8-13 face k is a triangle
+__host__ __device__ inline int ci_ifca(int w) { return w & 7; }
+__host__ __device__ inline int ci_admis(int w) { return (w >> 3) & 7; }
+__host__ __device__ inline int ci_full(int w) { return (w >> 6) & 1; }
+__host__ __device__ inline int ci_tri(int w, int f) { return (w >> (8 + f)) & 1; }
+
+template <typename R>
+struct ReconParams {
+    int ncell, nf, nqm, ts, per_xcd;
+    int iweno, qpq, qpt;
+    double gamma, lwci1, eps;
+    const double *U;       // [ncell][5]
+    const int32_t *nbr;    // [ncell][ts][imax]
+    const R *P;            // [ncell][imax][ts][ideg]
+    const R *B;            // [ncell][ideg*(ideg+1)/2] packed upper triangle by rows
+    const R *mbar;         // [ncell][ideg] INTEG_BASIS/VOLUME(1,1)
+    const R *qp;           // [ncell][nf][nqm][3]
+    const int32_t *cinfo;  // [ncell]
+    const int32_t *cnb;    // [ncell][nf] face neighbours
+    R *uleft;              // [ncell][nf][nqm][5]
+    int *counters;         // [0] MUSCL cells, [1] first-order cells
+};
+
+__device__ inline void cons2prim_d(double gamma, const double *u, double *w) {
+    double oo = 1.0 / u[0];
+    w[0] = u[0];
+    w[1] = u[1] * oo;
+    w[2] = u[2] * oo;
+    w[3] = u[3] * oo;
+    w[4] = (gamma - 1.0) * (u[4] - 0.5 * u[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
+}
+
+// CHECKSOL / CHECKSOLX test of one face point against the cell average
+// (src/reconstruct_p.f90:3183-3196): primitive recovery in FP64.
+__device__ inline bool jump_test(double gamma, const double *wi, const double *uq, double jump) {
+    double wq[NV];
+    cons2prim_d(gamma, uq, wq);
+    if (fabs(wq[0] - wi[0]) >= jump * wi[0]) return true;
+    if (fabs(wq[4] - wi[4]) >= jump * wi[4]) return true;
+    if (wq[4] < 0.0 || wq[0] < 0.0) return true;
+    return false;
+}
+
+template <typename R, int IDEG>
+__device__ inline void basis_at(const R *x, const R *mbar, R *phi) {
+    R px[4], py[4], pz[4];
+    px[0] = py[0] = pz[0] = R(1);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        px[k] = px[k - 1] * x[0];
+        py[k] = py[k - 1] * x[1];
+        pz[k] = pz[k - 1] * x[2];
+    }
+#pragma unroll
+    for (int d = 0; d < IDEG; ++d) phi[d] = px[c_mono[d][0]] * py[c_mono[d][1]] * pz[c_mono[d][2]] - mbar[d];
+}
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+template <typename R, int IDEG, int IMAX, int TSMAX>
+struct ReconShared {
+    static constexpr int NQMAX = 6 * 16;
+    union {
+        R du[TSMAX * IMAX * NV];  // phase 1
+        struct {
+            R gh[IDEG * NV];         // combined polynomial
+            R beta[TSMAX * NV];
+            R omega[TSMAX * NV];
+            R ul[NQMAX * NV];        // face-point values
+        } late;
+    } u;
+    R g[TSMAX * IDEG * NV];          // per-stencil coefficients
+    R b[IDEG * IDEG];                // indicator, unpacked
+    R mb[IDEG];
+    double ui[NV];
+};
+
+// Fused reconstruction, one wavefront per cell.
+template <typename R, int IDEG, int IMAX, int TSMAX>
+__global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
+    constexpr int NROW = (TSMAX * IDEG + WAVE - 1) / WAVE;
+    __shared__ ReconShared<R, IDEG, IMAX, TSMAX> sh;
+    const int lane = threadIdx.x;
+    // XCD-aware order: consecutive blocks go to different XCDs, so give
+    // each XCD a contiguous range of cells (its L2 then holds the
+    // neighbourhood of its cells)
+    const int b = blockIdx.x;
+    const int cell = (b % NXCD) * p.per_xcd + b / NXCD;
+    if (cell >= p.ncell) return;
+
+    const int info = p.cinfo[cell];
+    const int ifca = ci_ifca(info), admis = ci_admis(info), full = ci_full(info);
+    const int ts = p.ts;
+    const double *Ui = p.U + (size_t)cell * NV;
+    double ui[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ui[v] = Ui[v];
+    if (lane < NV) sh.ui[lane] = ui[lane];
+
+    // ---- phase 0: dU of every stencil member (FP64, then rounded to R) ----
+    const int32_t *nb = p.nbr + (size_t)cell * ts * IMAX;
+    for (int e = lane; e < admis * IMAX; e += WAVE) {
+        const double *Un = p.U + (size_t)nb[e] * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sh.u.du[e * NV + v] = (R)(Un[v] - ui[v]);
+    }
+    if (p.iweno == 1) {
+        const R *Bc = p.B + (size_t)cell * (IDEG * (IDEG + 1) / 2);
+        for (int e = lane; e < IDEG * (IDEG + 1) / 2; e += WAVE) {
+            // packed upper triangle by rows: row k holds l = k..IDEG-1
+            int k = 0, base = 0;
+            while (e >= base + (IDEG - k)) {
+                base += IDEG - k;
+                ++k;
+            }
+            int l = k + (e - base);
+            R val = Bc[e];
+            sh.b[k * IDEG + l] = val;
+            sh.b[l * IDEG + k] = val;
+        }
+    }
+    for (int e = lane; e < IDEG; e += WAVE) sh.mb[e] = p.mbar[(size_t)cell * IDEG + e];
+    __syncthreads();
+
+    // ---- phase 1: G_s = P_s dU_s, one (stencil,coefficient) row per lane ----
+    {
+        R acc[NROW][NV];
+        int srow[NROW];
+#pragma unroll
+        for (int t = 0; t < NROW; ++t) {
+            srow[t] = (lane + t * WAVE) / IDEG;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[t][v] = R(0);
+        }
+        const int nrows = admis * IDEG;
+        const R *Pc = p.P + (size_t)cell * IMAX * ts * IDEG;
+#pragma unroll 4
+        for (int j = 0; j < IMAX; ++j) {
+            const R *Pj = Pc + (size_t)j * ts * IDEG;
+#pragma unroll
+            for (int t = 0; t < NROW; ++t) {
+                const int r = lane + t * WAVE;
+                if (r < nrows) {
+                    const R pv = Pj[r];
+                    const R *du = &sh.u.du[(srow[t] * IMAX + j) * NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[t][v] += pv * du[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NROW; ++t) {
+            const int r = lane + t * WAVE;
+            if (r < nrows) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sh.g[r * NV + v] = acc[t][v];
+            }
+        }
+    }
+    __syncthreads();
+
+    bool reduce = (full == 0);
+    const int nslot = ifca * p.nqm;
+    if (p.iweno == 1) {
+        // ---- phase 2: beta_{s,v} = g^T (B g) (src/reconstruct_p.f90:995-1004) ----
+        if (lane < admis * NV) {
+            const int s = lane / NV, v = lane % NV;
+            const R *gs = &sh.g[s * IDEG * NV + v];
+            R beta = R(0);
+            for (int k = 0; k < IDEG; ++k) {
+                R bg = R(0);
+#pragma unroll
+                for (int l = 0; l < IDEG; ++l) bg += sh.b[k * IDEG + l] * gs[l * NV];
+                beta += gs[k * NV] * bg;
+            }
+            sh.u.late.beta[lane] = beta;
+        }
+        __syncthreads();
+        // ---- phase 3: weights (src/reconstruct_p.f90:1031-1049) ----
+        if (lane < admis * NV) {
+            const int s = lane / NV, v = lane % NV;
+            const R eps = (R)p.eps;
+            R om;
+            if (sizeof(R) == 8) {
+                // lambda/(eps+beta)^4 as in the reference
+                R sum = R(0), mine = R(0);
+                for (int t = 0; t < admis; ++t) {
+                    R x = eps + sh.u.late.beta[t * NV + v];
+                    R x2 = x * x;
+                    R w = (t == 0 ? (R)p.lwci1 : R(1)) / (x2 * x2);
+                    sum += w;
+                    if (t == s) mine = w;
+                }
+                om = mine / sum;
+            } else {
+                // FP32: (eps+beta)^-4 underflows, so scale every weight by
+                // (eps+beta_min)^4 first; the normalised weights are unchanged
+                R xmin = eps + sh.u.late.beta[v];
+                for (int t = 1; t < admis; ++t) xmin = fminf(xmin, eps + sh.u.late.beta[t * NV + v]);
+                R sum = R(0), mine = R(0);
+                for (int t = 0; t < admis; ++t) {
+                    R r = xmin / (eps + sh.u.late.beta[t * NV + v]);
+                    R r2 = r * r;
+                    R w = (t == 0 ? (R)p.lwci1 : R(1)) * (r2 * r2);
+                    sum += w;
+                    if (t == s) mine = w;
+                }
+                om = mine / sum;
+            }
+            sh.u.late.omega[lane] = om;
+        }
+        __syncthreads();
+        // ---- phase 4: combined coefficients sum_s omega_s G_s ----
+        for (int e = lane; e < IDEG * NV; e += WAVE) {
+            const int v = e % NV;
+            R acc = R(0);
+            for (int s = 0; s < admis; ++s) acc += sh.u.late.omega[s * NV + v] * sh.g[s * IDEG * NV + e];
+            sh.u.late.gh[e] = acc;
+        }
+        __syncthreads();
+        // ---- phase 5: ULEFT at the face points + CHECKSOL (0.85) ----
+        double wi[NV];
+        cons2prim_d(p.gamma, ui, wi);
+        int trouble = 0;
+        const R *qpc = p.qp + (size_t)cell * p.nf * p.nqm * 3;
+        for (int e = lane; e < nslot; e += WAVE) {
+            const int f = e / p.nqm, q = e % p.nqm;
+            const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+            if (q >= nq) continue;
+            R x[3] = {qpc[e * 3], qpc[e * 3 + 1], qpc[e * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, sh.mb, phi);
+            double uq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * sh.u.late.gh[k * NV + v];
+                // ULEFT is kept as the deviation from the FP64 cell average
+                uq[v] = ui[v] + (double)res;
+                sh.u.late.ul[e * NV + v] = res;
+            }
+            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+        }
+        reduce = reduce || (__syncthreads_or(trouble) != 0);
+    } else {
+        reduce = true;  // MUSCL scheme: every cell is non-FULL
+        __syncthreads();
+    }
+
+    // ---- phase 6: MUSCL + Barth-Jespersen (src/reconstruct_p.f90:2044-2280) ----
+    if (reduce) {
+        double umin[NV], umax[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) umin[v] = umax[v] = ui[v];
+        for (int f = 0; f < ifca; ++f) {
+            const int n = p.cnb[(size_t)cell * p.nf + f];
+            if (n < 0) continue;
+            const double *Un = p.U + (size_t)n * NV;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                umin[v] = fmin(umin[v], Un[v]);
+                umax[v] = fmax(umax[v], Un[v]);
+            }
+        }
+        const R *qpc = p.qp + (size_t)cell * p.nf * p.nqm * 3;
+        double slope[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) slope[v] = 1.0e16;
+        for (int e = lane; e < nslot; e += WAVE) {
+            const int f = e / p.nqm, q = e % p.nqm;
+            const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+            if (q >= nq) continue;
+            R x[3] = {qpc[e * 3], qpc[e * 3 + 1], qpc[e * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, sh.mb, phi);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * sh.g[k * NV + v];
+                const double d2 = (double)res;
+                double psi;
+                if (fabs(d2) <= 0.0)
+                    psi = 1.0;
+                else if (d2 > 0.0)
+                    psi = fmin(1.0, (umax[v] - ui[v]) / d2);
+                else
+                    psi = fmin(1.0, (umin[v] - ui[v]) / d2);
+                slope[v] = fmin(slope[v], psi);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) slope[v] = wave_min(slope[v]);
+        for (int e = lane; e < nslot; e += WAVE) {
+            const int f = e / p.nqm, q = e % p.nqm;
+            const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+            if (q >= nq) continue;
+            R x[3] = {qpc[e * 3], qpc[e * 3 + 1], qpc[e * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, sh.mb, phi);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * sh.g[k * NV + v];
+                sh.u.late.ul[e * NV + v] = (R)((double)res * slope[v]);
+            }
+        }
+        if (lane == 0) atomicAdd(&p.counters[0], 1);
+        __syncthreads();
+    }
+
+    // ---- phase 7: CHECKSOLX (0.95) -> first order (src/reconstruct_p.f90:3245-3406) ----
+    {
+        double wi[NV];
+        cons2prim_d(p.gamma, ui, wi);
+        int trouble = 0;
+        for (int e = lane; e < nslot; e += WAVE) {
+            const int f = e / p.nqm, q = e % p.nqm;
+            const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+            if (q >= nq) continue;
+            double uq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) uq[v] = ui[v] + (double)sh.u.late.ul[e * NV + v];
+            if (jump_test(p.gamma, wi, uq, 0.95)) trouble = 1;
+        }
+        const bool first = __syncthreads_or(trouble) != 0;
+        if (first) {
+            for (int e = lane; e < nslot; e += WAVE)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sh.u.late.ul[e * NV + v] = R(0);
+            if (lane == 0) atomicAdd(&p.counters[1], 1);
+            __syncthreads();
+        }
+    }
+
+    // ---- phase 8: ULEFT out, one contiguous block per cell ----
+    R *out = p.uleft + (size_t)cell * p.nf * p.nqm * NV;
+    for (int e = lane; e < nslot * NV; e += WAVE) out[e] = sh.u.late.ul[e];
+}
+
+// ---------------------------------------------------------------------------
+// HLLC (src/riemann.f90:221-426) on rotated FP64 states.  Primitive recovery
+// is FP64; the wave speeds and fluxes run in R.  For R=float the normal
+// momentum flux is carried relative to the face-mean pressure p0 (FP64):
+// rho u^2 + p - p0 has the size of the dynamic pressure, not of p, so the
+// FP32 rounding no longer scales with the ambient pressure; p0 is added back
+// in FP64 by the caller.  For R=double p0 = 0 and the arithmetic is the
+// reference's.
+template <typename R>
+__device__ inline void hllc(double gamma, const double *cl, const double *cr, R *flux, double &p0) {
+    double wl[NV], wr[NV];
+    cons2prim_d(gamma, cl, wl);
+    cons2prim_d(gamma, cr, wr);
+    const bool lowp = sizeof(R) == 4;
+    p0 = lowp ? 0.5 * (wl[4] + wr[4]) : 0.0;
+    const R g = (R)gamma;
+    const R rl = (R)wl[0], uL = (R)wl[1], vl = (R)wl[2], zl = (R)wl[3], pl = (R)wl[4], el = (R)cl[4];
+    const R rr = (R)wr[0], uR = (R)wr[1], vr = (R)wr[2], zr = (R)wr[3], pr = (R)wr[4], er = (R)cr[4];
+    const R dpl = lowp ? (R)(wl[4] - p0) : pl, dpr = lowp ? (R)(wr[4] - p0) : pr;
+    const R dprl = lowp ? (R)(wr[4] - wl[4]) : (pr - pl);
+    const R ccl = sqrt(g * pl / rl), ccr = sqrt(g * pr / rr);
+    const R srl = sqrt(rl), srr = sqrt(rr);
+    const R cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
+                       (R(0.5) * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
+    const R uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
+    R sl = fmin(uL - ccl, uu2 - cc2), sr = fmax(uR + ccr, uu2 + cc2);
+    sl = fmin(sl, R(0));
+    sr = fmax(sr, R(0));
+    const R sm = (dprl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
+    const R cl_[NV] = {(R)cl[0], (R)cl[1], (R)cl[2], (R)cl[3], el};
+    const R cr_[NV] = {(R)cr[0], (R)cr[1], (R)cr[2], (R)cr[3], er};
+    const R fl[NV] = {rl * uL, (rl * (uL * uL)) + dpl, rl * uL * vl, rl * uL * zl, uL * (el + pl)};
+    const R fr[NV] = {rr * uR, (rr * (uR * uR)) + dpr, rr * uR * vr, rr * uR * zr, uR * (er + pr)};
+    const R mul = rl * ((sl - uL) / (sl - sm));
+    const R mur = rr * ((sr - uR) / (sr - sm));
+    const R lastl = (el / rl) + ((sm - uL) * (sm + ((pl) / (rl * (sl - uL)))));
+    const R lastr = (er / rr) + ((sm - uR) * (sm + ((pr) / (rr * (sr - uR)))));
+    const R usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
+    const R usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
+    const R s = copysign(R(1), sm);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        flux[k] = (((R(1) + s) / R(2)) * (fl[k] + sl * (usl[k] - cl_[k]))) +
+                  (((R(1) - s) / R(2)) * (fr[k] + sr * (usr[k] - cr_[k])));
+}
+
+template <typename R>
+struct FluxParams {
+    int nface, nf, nqm, qpq, qpt;
+    double gamma;
+    const int32_t *fcell;   // [nface][2] left (owner), right
+    const int32_t *fslot;   // [nface][2] face slot in left, right; bit 8 of slot 0: triangle
+    const double *frot;     // [nface][4] cos a1, sin a1, cos a2, sin a2 of the owner's outward normal
+    const double *farea;    // [nface]
+    const double *U;        // [ncell][5] cell averages
+    const R *uleft;         // face-point deviations from the cell average
+    double *fflux;          // [nface][5] outward from the owner
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= p.nface) return;
+    const int cL = p.fcell[2 * f], cR = p.fcell[2 * f + 1];
+    const int s0 = p.fslot[2 * f], sR = p.fslot[2 * f + 1];
+    const int sL = s0 & 255;
+    const bool tri = (s0 >> 8) & 1;
+    const int nq = tri ? p.qpt : p.qpq;
+    const double *w = tri ? c_wt : c_wq;
+    const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
+    const double area = p.farea[f];
+    double uL[NV], uR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        uL[v] = p.U[(size_t)cL * NV + v];
+        uR[v] = p.U[(size_t)cR * NV + v];
+    }
+    const R *ulL = p.uleft + ((size_t)cL * p.nf + sL) * p.nqm * NV;
+    const R *ulR = p.uleft + ((size_t)cR * p.nf + sR) * p.nqm * NV;
+    double acc[NV] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < nq; ++q) {
+        double a[NV], bq[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            a[v] = uL[v] + (double)ulL[q * NV + v];
+            bq[v] = uR[v] + (double)ulR[q * NV + v];
+        }
+        // ROTATEF (src/grid_t.f90:4217-4258)
+        const double la[NV] = {a[0], c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3],
+                               c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3], (-s1) * a[1] + c1 * a[2], a[4]};
+        const double ra[NV] = {bq[0], c1 * s2 * bq[1] + s1 * s2 * bq[2] + c2 * bq[3],
+                               c1 * c2 * bq[1] + s1 * c2 * bq[2] + (-s2) * bq[3], (-s1) * bq[1] + c1 * bq[2], bq[4]};
+        R h[NV];
+        double p0;
+        hllc<R>(p.gamma, la, ra, h, p0);
+        const double hd[NV] = {(double)h[0], (double)h[1] + p0, (double)h[2], (double)h[3], (double)h[4]};
+        // ROTATEB (src/grid_t.f90:4261-4305)
+        const double rb[NV] = {hd[0], c1 * s2 * hd[1] + c1 * c2 * hd[2] + (-s1) * hd[3],
+                               s1 * s2 * hd[1] + s1 * c2 * hd[2] + c1 * hd[3], c2 * hd[1] + (-s2) * hd[2], hd[4]};
+        const double wa = w[q] * area;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + (rb[v] * wa);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) p.fflux[(size_t)f * NV + v] = acc[v];
+}
+
+struct UpdateParams {
+    int ncell, nf, stage;
+    double cfl, gamma;
+    const int32_t *cface;    // [ncell][nf] +(face+1) owner, -(face+1) neighbour, 0 none
+    const double *fflux;
+    const double *ovol;      // [ncell] 1/TOTVOLUME
+    const double *minedge;
+    double *U, *U2;
+    double *rhs_out;         // optional [ncell][5]
+    const double *dt;        // this step's dt
+    unsigned long long *dt_next;  // atomic-min target (bits of a positive double), stage 3 only
+    double *t;               // advanced by dt in stage 3
+};
+
+__device__ inline void atomic_min_pos(unsigned long long *addr, double v) {
+    atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// stage 0: gather only (rhs_out); stage 1..3: Shu-Osher update
+__global__ __launch_bounds__(256) void k_update(UpdateParams p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = 1.0e300;
+    if (i < p.ncell) {
+        double r[NV] = {0, 0, 0, 0, 0};
+        for (int f = 0; f < p.nf; ++f) {
+            const int cf = p.cface[(size_t)i * p.nf + f];
+            if (cf == 0) continue;
+            const double *F = p.fflux + (size_t)(abs(cf) - 1) * NV;
+            if (cf > 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) r[v] = r[v] + F[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) r[v] = r[v] - F[v];
+            }
+        }
+        if (p.rhs_out) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) p.rhs_out[(size_t)i * NV + v] = r[v];
+        }
+        if (p.stage > 0) {
+            const double dt = *p.dt, oov = p.ovol[i];
+            double *u = p.U + (size_t)i * NV, *u2 = p.U2 + (size_t)i * NV;
+            const double to4 = 3.0 / 4.0, oo4 = 1.0 / 4.0, to3 = 2.0 / 3.0, oo3 = 1.0 / 3.0;
+            double un[NV];
+            if (p.stage == 1) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    u2[v] = u[v];
+                    un[v] = u[v] - (dt * (r[v] * oov));
+                }
+            } else if (p.stage == 2) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) un[v] = (to4 * u2[v]) + (oo4 * u[v]) - (oo4 * (dt * (r[v] * oov)));
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) un[v] = (oo3 * u2[v]) + (to3 * u[v]) - (to3 * (dt * (r[v] * oov)));
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u[v] = un[v];
+            if (p.stage == 3 && p.dt_next) {
+                double w[NV];
+                cons2prim_d(p.gamma, un, w);
+                const double c = sqrt(w[4] * p.gamma / w[0]);
+                const double veln = fmax(fmax(fabs(w[1]), fabs(w[2])), fabs(w[3])) + c;
+                dtl = (p.cfl / 3.0) * (p.minedge[i] / fabs(veln));
+            }
+        }
+    }
+    if (p.stage == 3) {
+        if (p.dt_next) {
+            dtl = wave_min(dtl);
+            if ((threadIdx.x & (WAVE - 1)) == 0 && dtl < 1.0e300) atomic_min_pos(p.dt_next, dtl);
+        }
+        if (i == 0 && p.t) *p.t += *p.dt;
+    }
+}
+
+struct DtParams {
+    int ncell;
+    double cfl, gamma;
+    const double *U, *minedge;
+    unsigned long long *dt_out;
+};
+
+__global__ __launch_bounds__(256) void k_dt(DtParams p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 1.0e300;
+    if (i < p.ncell) {
+        double w[NV];
+        cons2prim_d(p.gamma, p.U + (size_t)i * NV, w);
+        const double c = sqrt(w[4] * p.gamma / w[0]);
+        const double veln = fmax(fmax(fabs(w[1]), fabs(w[2])), fabs(w[3])) + c;
+        d = (p.cfl / 3.0) * (p.minedge[i] / fabs(veln));
+    }
+    d = wave_min(d);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && d < 1.0e300) atomic_min_pos(p.dt_out, d);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+__global__ void k_nonfinite(const double *u, size_t n, int *flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite(u[i])) bad = 1;
+    if (bad) atomicOr(flag, 1);
+}
+
+// [cells][ts_in][imax][ideg] (FP64, ABI) -> [cells][imax][ts_out][ideg] (R)
+template <typename R>
+__global__ void k_relayout_p(const double *src, R *dst, int ncells, int ts_in, int ts_out, int imax, int ideg) {
+    const size_t per_out = (size_t)imax * ts_out * ideg;
+    const size_t total = per_out * ncells;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = e / per_out;
+        size_t r = e % per_out;
+        const int k = (int)(r % ideg);
+        r /= ideg;
+        const int s = (int)(r % ts_out);
+        const int j = (int)(r / ts_out);
+        dst[e] = (R)src[((c * ts_in + s) * imax + j) * ideg + k];
+    }
+}
+
+template <typename T>
+int dalloc(ucns_ctx *ctx, T **p, size_t n);
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct ucns_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // sizes
+    int ncell = 0, nf = 0, nqm = 0, nface = 0;
+    int precision = UCNS_FP64, iweno = 1, ideg = 0, nnb = 0, imax = 0, ts_abi = 0, ts = 0;
+    int qpq = 0, qpt = 0, nf_abi = 0, nqm_abi = 0, index_base = 0;
+    double gamma = 1.4, lwci1 = 1.0, eps = 1e-11;
+    bool have_prestore = false, have_state = false;
+    // mesh (device)
+    int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
+    double *frot = nullptr, *farea = nullptr, *ovol = nullptr, *minedge = nullptr;
+    std::vector<int32_t> h_cinfo_base;  // ifca + face types (admis/full added at upload)
+    // prestore (device)
+    int32_t *nbr = nullptr;
+    void *P = nullptr, *B = nullptr, *mbar = nullptr, *qp = nullptr, *uleft = nullptr;
+    // state (device)
+    double *U = nullptr, *U2 = nullptr, *fflux = nullptr, *rhs = nullptr, *t = nullptr, *dtfix = nullptr;
+    unsigned long long *dtv = nullptr;  // [2] ping-pong dt slots
+    int *counters = nullptr, *flag = nullptr;
+    // stats
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    ucns_stats_v1 st{};
+};
+
+namespace {
+
+template <typename T>
+int dalloc(ucns_ctx *ctx, T **p, size_t n) {
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " bytes): " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? UCNS_ENOMEM : UCNS_EHIP;
+    }
+    return UCNS_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int per_xcd(int ncell) { return (ncell + NXCD - 1) / NXCD; }
+
+// kernel selection by (precision, ideg, imax)
+template <typename R>
+int launch_recon_t(ucns_ctx *ctx) {
+    ReconParams<R> p{};
+    p.ncell = ctx->ncell;
+    p.nf = ctx->nf;
+    p.nqm = ctx->nqm;
+    p.ts = ctx->ts;
+    p.per_xcd = per_xcd(ctx->ncell);
+    p.iweno = ctx->iweno;
+    p.qpq = ctx->qpq;
+    p.qpt = ctx->qpt;
+    p.gamma = ctx->gamma;
+    p.lwci1 = ctx->lwci1;
+    p.eps = ctx->eps;
+    p.U = ctx->U;
+    p.nbr = ctx->nbr;
+    p.P = (const R *)ctx->P;
+    p.B = (const R *)ctx->B;
+    p.mbar = (const R *)ctx->mbar;
+    p.qp = (const R *)ctx->qp;
+    p.cinfo = ctx->cinfo;
+    p.cnb = ctx->cnb;
+    p.uleft = (R *)ctx->uleft;
+    p.counters = ctx->counters;
+    const dim3 grid(p.per_xcd * NXCD), block(WAVE);
+    if (ctx->ideg == 19 && ctx->imax == 39)
+        hipLaunchKernelGGL((k_recon<R, 19, 39, 7>), grid, block, 0, ctx->stream, p);
+    else if (ctx->ideg == 3 && ctx->imax == 8)
+        hipLaunchKernelGGL((k_recon<R, 3, 8, 1>), grid, block, 0, ctx->stream, p);
+    else {
+        ctx->err = "no reconstruction kernel for idegfree=" + std::to_string(ctx->ideg) +
+                   " imax=" + std::to_string(ctx->imax);
+        return UCNS_EUNSUPPORTED;
+    }
+    return UCNS_OK;
+}
+
+template <typename R>
+void launch_flux_t(ucns_ctx *ctx) {
+    FluxParams<R> p{};
+    p.nface = ctx->nface;
+    p.nf = ctx->nf;
+    p.nqm = ctx->nqm;
+    p.qpq = ctx->qpq;
+    p.qpt = ctx->qpt;
+    p.gamma = ctx->gamma;
+    p.fcell = ctx->fcell;
+    p.fslot = ctx->fslot;
+    p.frot = ctx->frot;
+    p.farea = ctx->farea;
+    p.U = ctx->U;
+    p.uleft = (const R *)ctx->uleft;
+    p.fflux = ctx->fflux;
+    hipLaunchKernelGGL((k_flux<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, p);
+}
+
+void tstart(ucns_ctx *ctx, int k) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[2 * k], ctx->stream);
+}
+void tstop(ucns_ctx *ctx, int k) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[2 * k + 1], ctx->stream);
+}
+void tcollect(ucns_ctx *ctx, int k, double *acc, int64_t *n) {
+    if (!ctx->timing) return;
+    float ms = 0;
+    (void)hipEventSynchronize(ctx->ev[2 * k + 1]);
+    if (hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]) == hipSuccess) {
+        *acc += ms;
+        *n += 1;
+    }
+}
+
+// reconstruction + face fluxes of the current state (ULEFT and face fluxes on device)
+int residual_kernels(ucns_ctx *ctx) {
+    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
+    tstart(ctx, 0);
+    int rc = ctx->precision == UCNS_MIXED ? launch_recon_t<float>(ctx) : launch_recon_t<double>(ctx);
+    if (rc) return rc;
+    tstop(ctx, 0);
+    tcollect(ctx, 0, &ctx->st.ms_recon, &ctx->st.n_recon);
+    tstart(ctx, 1);
+    if (ctx->precision == UCNS_MIXED)
+        launch_flux_t<float>(ctx);
+    else
+        launch_flux_t<double>(ctx);
+    tstop(ctx, 1);
+    tcollect(ctx, 1, &ctx->st.ms_flux, &ctx->st.n_flux);
+    return UCNS_OK;
+}
+
+int update_kernel(ucns_ctx *ctx, int stage, const double *dt, unsigned long long *dt_next, double cfl,
+                  double *rhs_out) {
+    UpdateParams p{};
+    p.ncell = ctx->ncell;
+    p.nf = ctx->nf;
+    p.stage = stage;
+    p.cfl = cfl;
+    p.gamma = ctx->gamma;
+    p.cface = ctx->cface;
+    p.fflux = ctx->fflux;
+    p.ovol = ctx->ovol;
+    p.minedge = ctx->minedge;
+    p.U = ctx->U;
+    p.U2 = ctx->U2;
+    p.rhs_out = rhs_out;
+    p.dt = dt;
+    p.dt_next = dt_next;
+    p.t = ctx->t;
+    tstart(ctx, 2);
+    hipLaunchKernelGGL(k_update, dim3((ctx->ncell + 255) / 256), dim3(256), 0, ctx->stream, p);
+    tstop(ctx, 2);
+    tcollect(ctx, 2, &ctx->st.ms_update, &ctx->st.n_update);
+    return UCNS_OK;
+}
+
+int dt_kernel(ucns_ctx *ctx, double cfl, unsigned long long *out) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, ctx->stream, out,
+                       (unsigned long long)0x7FF0000000000000ULL);
+    DtParams p{};
+    p.ncell = ctx->ncell;
+    p.cfl = cfl;
+    p.gamma = ctx->gamma;
+    p.U = ctx->U;
+    p.minedge = ctx->minedge;
+    p.dt_out = out;
+    tstart(ctx, 3);
+    hipLaunchKernelGGL(k_dt, dim3((ctx->ncell + 255) / 256), dim3(256), 0, ctx->stream, p);
+    tstop(ctx, 3);
+    tcollect(ctx, 3, &ctx->st.ms_dt, &ctx->st.n_dt);
+    return UCNS_OK;
+}
+
+int state_io(ucns_ctx *ctx, const double *src, double *dst, int32_t layout, hipMemcpyKind kind, bool to_dev) {
+    const size_t n = (size_t)ctx->ncell * NV;
+    if (layout == UCNS_LAYOUT_AOS) {
+        if (to_dev)
+            HIP_TRY(hipMemcpyAsync(ctx->U, src, n * sizeof(double), kind, ctx->stream));
+        else
+            HIP_TRY(hipMemcpyAsync(dst, ctx->U, n * sizeof(double), kind, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return UCNS_OK;
+    }
+    if (layout != UCNS_LAYOUT_SOA) {
+        ctx->err = "unknown layout";
+        return UCNS_EBADARG;
+    }
+    // SoA <-> AoS through host memory (layout conversion is not on the hot path)
+    std::vector<double> a(n), b(n);
+    if (to_dev) {
+        if (kind == hipMemcpyDeviceToDevice) {
+            HIP_TRY(hipMemcpy(b.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::memcpy(b.data(), src, n * sizeof(double));
+        }
+        for (int i = 0; i < ctx->ncell; ++i)
+            for (int v = 0; v < NV; ++v) a[(size_t)i * NV + v] = b[(size_t)v * ctx->ncell + i];
+        HIP_TRY(hipMemcpy(ctx->U, a.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy(a.data(), ctx->U, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < ctx->ncell; ++i)
+            for (int v = 0; v < NV; ++v) b[(size_t)v * ctx->ncell + i] = a[(size_t)i * NV + v];
+        if (kind == hipMemcpyDeviceToDevice) {
+            HIP_TRY(hipMemcpy(dst, b.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::memcpy(dst, b.data(), n * sizeof(double));
+        }
+    }
+    return UCNS_OK;
+}
+
+template <typename R>
+int upload_real(ucns_ctx *ctx, void **dst, const std::vector<double> &src) {
+    std::vector<R> tmp(src.size());
+    for (size_t e = 0; e < src.size(); ++e) tmp[e] = (R)src[e];
+    int rc = dalloc(ctx, (R **)dst, tmp.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice));
+    return UCNS_OK;
+}
+
+int upload_real_any(ucns_ctx *ctx, void **dst, const std::vector<double> &src) {
+    return ctx->precision == UCNS_MIXED ? upload_real<float>(ctx, dst, src) : upload_real<double>(ctx, dst, src);
+}
+
+void release_prestore(ucns_ctx *ctx) {
+    dfree(ctx->nbr);
+    dfree(ctx->P);
+    dfree(ctx->B);
+    dfree(ctx->mbar);
+    dfree(ctx->qp);
+    ctx->have_prestore = false;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ucns_abi_version(void) { return UCNS_ABI_VERSION; }
+
+const char *ucns_last_error(const ucns_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) {
+    if (!out) return UCNS_EBADARG;
+    *out = nullptr;
+    if (!m || !s) return UCNS_EBADARG;
+    ucns_ctx *ctx = new ucns_ctx();
+    *out = ctx;
+    auto bad = [&](int code, const std::string &msg) {
+        ctx->err = msg;
+        return code;
+    };
+    if (m->abi_version != UCNS_ABI_VERSION || s->abi_version != UCNS_ABI_VERSION)
+        return bad(UCNS_EBADARG, "abi_version mismatch");
+    if (m->ncell <= 0 || m->nface_max <= 0 || m->nface_max > 6) return bad(UCNS_EBADARG, "bad ncell/nface_max");
+    if (!m->ifca || !m->ineigh || !m->ineighn || !m->bcode || !m->types_faces || !m->faceanglex ||
+        !m->faceangley || !m->surf || !m->totvolume || !m->minedge)
+        return bad(UCNS_EBADARG, "null mesh array");
+    if (s->precision != UCNS_FP64 && s->precision != UCNS_MIXED) return bad(UCNS_EBADARG, "bad precision");
+    if (s->iweno != 1 && s->iweno != -1) return bad(UCNS_EUNSUPPORTED, "iweno must be 1 (WENO) or -1 (MUSCL)");
+    if (s->ees != 0) return bad(UCNS_EUNSUPPORTED, "only EES=0 stencils are supported");
+    if (s->wenoz != 0) return bad(UCNS_EUNSUPPORTED, "wenoz=1 is not supported");
+    if (s->wenwrt != 1) return bad(UCNS_EUNSUPPORTED, "only conserved-variable reconstruction (wenwrt=1)");
+    if (s->limiter != 1) return bad(UCNS_EUNSUPPORTED, "only the Barth-Jespersen limiter (1)");
+    if (s->riemann != 1) return bad(UCNS_EUNSUPPORTED, "only the HLLC solver (1)");
+    if (s->typesten < 1 || s->typesten > 7) return bad(UCNS_EUNSUPPORTED, "typesten must be 1..7");
+    if (s->qp_quad < 1 || s->qp_quad > 64 || s->qp_triangle < 1 || s->qp_triangle > 64 || !s->weights_quad ||
+        !s->weights_tri)
+        return bad(UCNS_EBADARG, "bad face quadrature");
+    ctx->ncell = m->ncell;
+    ctx->nf_abi = m->nface_max;
+    ctx->index_base = m->index_base;
+    ctx->precision = s->precision;
+    ctx->iweno = s->iweno;
+    ctx->ideg = s->idegfree;
+    ctx->nnb = s->inumneighbours;
+    ctx->imax = s->inumneighbours - 1;
+    ctx->ts_abi = s->typesten;
+    ctx->qpq = s->qp_quad;
+    ctx->qpt = s->qp_triangle;
+    ctx->nqm_abi = std::max(s->qp_quad, s->qp_triangle);
+    ctx->gamma = s->gamma;
+    ctx->lwci1 = s->lwci1;
+    ctx->eps = s->weno_eps;
+    if (!((ctx->ideg == 19 && ctx->imax == 39) || (ctx->ideg == 3 && ctx->imax == 8)))
+        return bad(UCNS_EUNSUPPORTED, "supported (idegfree, inumneighbours): (19, 40) WENO-4 and (3, 9) MUSCL");
+    if (ctx->iweno == -1 && ctx->ideg != 3) return bad(UCNS_EUNSUPPORTED, "MUSCL path is linear (idegfree 3)");
+    HIP_TRY(hipGetDevice(&ctx->device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+
+    const int nc = m->ncell, nfa = m->nface_max, base = m->index_base;
+    // faces in use, Gauss points per face type present
+    int nf = 0, nqm = 0;
+    bool any_quad = false, any_tri = false;
+    for (int i = 0; i < nc; ++i) {
+        const int ifca = m->ifca[i];
+        if (ifca < 4 || ifca > nfa) return bad(UCNS_EBADARG, "ifca out of range at cell " + std::to_string(i));
+        nf = std::max(nf, ifca);
+        for (int f = 0; f < ifca; ++f) {
+            const int t = m->types_faces[(size_t)i * nfa + f];
+            if (t == 5) any_quad = true;
+            else if (t == 6) any_tri = true;
+            else return bad(UCNS_EBADARG, "types_faces must be 5 or 6");
+        }
+    }
+    if (any_quad) nqm = std::max(nqm, s->qp_quad);
+    if (any_tri) nqm = std::max(nqm, s->qp_triangle);
+    ctx->nf = nf;
+    ctx->nqm = nqm;
+    if (nf * nqm > 6 * 16) return bad(UCNS_EUNSUPPORTED, "too many face Gauss points per cell");
+    // unique faces: owner = lower cell id (equal ids: lower slot)
+    std::vector<int32_t> cinfo(nc, 0), cnb((size_t)nc * nf, -1), cface((size_t)nc * nf, 0);
+    std::vector<int32_t> fcell, fslot;
+    std::vector<double> frot, farea;
+    for (int i = 0; i < nc; ++i) {
+        const int ifca = m->ifca[i];
+        int w = ifca;
+        for (int f = 0; f < ifca; ++f)
+            if (m->types_faces[(size_t)i * nfa + f] == 6) w |= 1 << (8 + f);
+        cinfo[i] = w;
+        for (int f = 0; f < ifca; ++f) {
+            const size_t sl = (size_t)i * nfa + f;
+            const int bc = m->bcode[sl];
+            if (bc != 0 && bc != 5)
+                return bad(UCNS_EUNSUPPORTED, "only interior and periodic (code 5) faces are supported");
+            const int j = m->ineigh[sl] - base, fj = m->ineighn[sl] - base;
+            if (j < 0 || j >= nc || fj < 0 || fj >= m->ifca[j])
+                return bad(UCNS_EBADARG, "face neighbour out of range at cell " + std::to_string(i));
+            const size_t slj = (size_t)j * nfa + fj;
+            if (m->ineigh[slj] - base != i || m->ineighn[slj] - base != f)
+                return bad(UCNS_EBADARG, "face neighbours are not symmetric at cell " + std::to_string(i));
+            cnb[(size_t)i * nf + f] = j;
+            const bool owner = (i < j) || (i == j && f < fj);
+            if (!owner) continue;
+            const int id = (int)farea.size();
+            fcell.push_back(i);
+            fcell.push_back(j);
+            fslot.push_back(f | (m->types_faces[sl] == 6 ? 256 : 0));
+            fslot.push_back(fj);
+            const double a1 = m->faceanglex[sl], a2 = m->faceangley[sl];
+            frot.push_back(std::cos(a1));
+            frot.push_back(std::sin(a1));
+            frot.push_back(std::cos(a2));
+            frot.push_back(std::sin(a2));
+            farea.push_back(m->surf[sl]);
+            cface[(size_t)i * nf + f] = id + 1;
+            cface[(size_t)j * nf + fj] = -(id + 1);
+        }
+    }
+    ctx->nface = (int)farea.size();
+    ctx->h_cinfo_base = cinfo;
+    std::vector<double> ovol(nc), minedge(nc);
+    for (int i = 0; i < nc; ++i) {
+        if (!(m->totvolume[i] > 0)) return bad(UCNS_EBADARG, "non-positive volume");
+        ovol[i] = 1.0 / m->totvolume[i];
+        minedge[i] = m->minedge[i];
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->cinfo, nc)) || (rc = dalloc(ctx, &ctx->cnb, cnb.size())) ||
+        (rc = dalloc(ctx, &ctx->cface, cface.size())) || (rc = dalloc(ctx, &ctx->fcell, fcell.size())) ||
+        (rc = dalloc(ctx, &ctx->fslot, fslot.size())) || (rc = dalloc(ctx, &ctx->frot, frot.size())) ||
+        (rc = dalloc(ctx, &ctx->farea, farea.size())) || (rc = dalloc(ctx, &ctx->ovol, nc)) ||
+        (rc = dalloc(ctx, &ctx->minedge, nc)) || (rc = dalloc(ctx, &ctx->U, (size_t)nc * NV)) ||
+        (rc = dalloc(ctx, &ctx->U2, (size_t)nc * NV)) || (rc = dalloc(ctx, &ctx->rhs, (size_t)nc * NV)) ||
+        (rc = dalloc(ctx, &ctx->fflux, (size_t)ctx->nface * NV)) || (rc = dalloc(ctx, &ctx->t, 1)) ||
+        (rc = dalloc(ctx, &ctx->dtfix, 1)) || (rc = dalloc(ctx, &ctx->dtv, 2)) ||
+        (rc = dalloc(ctx, &ctx->counters, 2)) || (rc = dalloc(ctx, &ctx->flag, 1)))
+        return rc;
+    const size_t ul_elems = (size_t)nc * nf * nqm * NV;
+    rc = ctx->precision == UCNS_MIXED ? dalloc(ctx, (float **)&ctx->uleft, ul_elems)
+                                      : dalloc(ctx, (double **)&ctx->uleft, ul_elems);
+    if (rc) return rc;
+    HIP_TRY(hipMemset(ctx->uleft, 0, ul_elems * (ctx->precision == UCNS_MIXED ? 4 : 8)));
+    HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->cnb, cnb.data(), cnb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->cface, cface.data(), cface.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->fcell, fcell.data(), fcell.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->fslot, fslot.data(), fslot.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->frot, frot.data(), frot.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->farea, farea.data(), farea.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->ovol, ovol.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->minedge, minedge.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    double wq[64] = {0}, wt[64] = {0};
+    for (int q = 0; q < s->qp_quad; ++q) wq[q] = s->weights_quad[q];
+    for (int q = 0; q < s->qp_triangle; ++q) wt[q] = s->weights_tri[q];
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wq), wq, sizeof(wq)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wt), wt, sizeof(wt)));
+    HIP_TRY(hipMemset(ctx->t, 0, sizeof(double)));
+    return UCNS_OK;
+}
+
+int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
+    if (!ctx) return UCNS_EBADARG;
+    if (!pre || !pre->admis || !pre->full || !pre->ihexl || !pre->invmat || !pre->integ_basis || !pre->volume11 ||
+        !pre->qpoints) {
+        ctx->err = "null prestore array";
+        return UCNS_EBADARG;
+    }
+    if (ctx->iweno == 1 && !pre->indicator) {
+        ctx->err = "WENO needs the indicator matrices";
+        return UCNS_EBADARG;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    release_prestore(ctx);
+    const int nc = ctx->ncell, ID = ctx->ideg, IM = ctx->imax, NNB = ctx->nnb, TSA = ctx->ts_abi;
+    const int base = ctx->index_base;
+    // admissible stencils, flags
+    int ts = 1;
+    std::vector<int32_t> cinfo = ctx->h_cinfo_base;
+    for (int i = 0; i < nc; ++i) {
+        const int ad = pre->admis[i];
+        if (ad < 1 || ad > TSA) {
+            ctx->err = "admis out of range at cell " + std::to_string(i);
+            return UCNS_EBADARG;
+        }
+        ts = std::max(ts, ad);
+        cinfo[i] |= (ad << 3) | ((pre->full[i] ? 1 : 0) << 6);
+    }
+    if (ctx->iweno == -1 && ts != 1) {
+        ctx->err = "MUSCL uses the central stencil only (admis must be 1)";
+        return UCNS_EUNSUPPORTED;
+    }
+    ctx->ts = ts;
+    // stencil members without the cell itself: [cell][ts][imax]
+    std::vector<int32_t> nbr((size_t)nc * ts * IM, 0);
+    for (int i = 0; i < nc; ++i)
+        for (int s = 0; s < pre->admis[i]; ++s) {
+            const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
+            if (row[0] - base != i) {
+                ctx->err = "IHEXL(s,1) must be the cell itself (cell " + std::to_string(i) + ")";
+                return UCNS_EBADARG;
+            }
+            for (int k = 0; k < IM; ++k) {
+                const int j = row[k + 1] - base;
+                if (j < 0 || j >= nc) {
+                    ctx->err = "stencil member out of range at cell " + std::to_string(i);
+                    return UCNS_EBADARG;
+                }
+                nbr[((size_t)i * ts + s) * IM + k] = j;
+            }
+        }
+    int rc = dalloc(ctx, &ctx->nbr, nbr.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    // pseudo-inverses: staged in chunks, re-laid out on the device
+    const bool mixed = ctx->precision == UCNS_MIXED;
+    const size_t per_cell_abi = (size_t)TSA * IM * ID, per_cell_dev = (size_t)IM * ts * ID;
+    rc = mixed ? dalloc(ctx, (float **)&ctx->P, per_cell_dev * nc) : dalloc(ctx, (double **)&ctx->P, per_cell_dev * nc);
+    if (rc) return rc;
+    {
+        const size_t chunk_cells = std::max<size_t>(1, (size_t)(256u << 20) / (per_cell_abi * sizeof(double)));
+        double *stage = nullptr, *host_pin = nullptr;
+        if ((rc = dalloc(ctx, &stage, chunk_cells * per_cell_abi))) return rc;
+        if (hipHostMalloc((void **)&host_pin, chunk_cells * per_cell_abi * sizeof(double), hipHostMallocDefault) !=
+            hipSuccess)
+            host_pin = nullptr;
+        for (size_t c0 = 0; c0 < (size_t)nc; c0 += chunk_cells) {
+            const size_t cn = std::min(chunk_cells, (size_t)nc - c0);
+            const double *src = pre->invmat + c0 * per_cell_abi;
+            const size_t bytes = cn * per_cell_abi * sizeof(double);
+            if (host_pin) {
+                std::memcpy(host_pin, src, bytes);
+                HIP_TRY(hipMemcpyAsync(stage, host_pin, bytes, hipMemcpyHostToDevice, ctx->stream));
+            } else {
+                HIP_TRY(hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+            }
+            const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
+            if (mixed)
+                hipLaunchKernelGGL(k_relayout_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                   (float *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+            else
+                hipLaunchKernelGGL(k_relayout_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                   (double *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        dfree(stage);
+        if (host_pin) (void)hipHostFree(host_pin);
+    }
+    // indicators, packed upper triangle
+    if (ctx->iweno == 1) {
+        const size_t np = (size_t)ID * (ID + 1) / 2;
+        std::vector<double> bp((size_t)nc * np);
+        for (int i = 0; i < nc; ++i) {
+            const double *Bi = pre->indicator + (size_t)i * ID * ID;
+            size_t e = (size_t)i * np;
+            for (int k = 0; k < ID; ++k)
+                for (int l = k; l < ID; ++l) bp[e++] = Bi[(size_t)k * ID + l];
+        }
+        if ((rc = upload_real_any(ctx, &ctx->B, bp))) return rc;
+    } else {
+        std::vector<double> bp(1, 0.0);
+        if ((rc = upload_real_any(ctx, &ctx->B, bp))) return rc;
+    }
+    // basis means
+    {
+        std::vector<double> mb((size_t)nc * ID);
+        for (int i = 0; i < nc; ++i) {
+            const double oov = 1.0 / pre->volume11[i];
+            for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[(size_t)i * ID + d] * oov;
+        }
+        if ((rc = upload_real_any(ctx, &ctx->mbar, mb))) return rc;
+    }
+    // Gauss points in the local frames: [cell][nf][nqm][3]
+    {
+        const int nf = ctx->nf, nqm = ctx->nqm, nfa = ctx->nf_abi, nqa = ctx->nqm_abi;
+        std::vector<double> q((size_t)nc * nf * nqm * 3, 0.0);
+        for (int i = 0; i < nc; ++i)
+            for (int f = 0; f < nf; ++f)
+                for (int g = 0; g < nqm && g < nqa; ++g)
+                    for (int d = 0; d < 3; ++d)
+                        q[(((size_t)i * nf + f) * nqm + g) * 3 + d] =
+                            pre->qpoints[(((size_t)i * nfa + f) * nqa + g) * 3 + d];
+        if ((rc = upload_real_any(ctx, &ctx->qp, q))) return rc;
+    }
+    ctx->have_prestore = true;
+    ctx->st.bytes_recon_per_cell =
+        (double)((size_t)IM * ts * ID * (mixed ? 4 : 8)) + (ctx->iweno == 1 ? (double)ID * (ID + 1) / 2 * (mixed ? 4 : 8) : 0) +
+        (double)ts * IM * 4 + (double)ctx->nf * ctx->nqm * 3 * (mixed ? 4 : 8) + (double)ID * (mixed ? 4 : 8);
+    return UCNS_OK;
+}
+
+int ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout) {
+    if (!ctx || !u) return UCNS_EBADARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = state_io(ctx, u, nullptr, layout, hipMemcpyHostToDevice, true);
+    if (rc == 0) ctx->have_state = true;
+    return rc;
+}
+
+int ucns_get_state(ucns_ctx *ctx, double *u, int32_t layout) {
+    if (!ctx || !u) return UCNS_EBADARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return state_io(ctx, nullptr, u, layout, hipMemcpyDeviceToHost, false);
+}
+
+int ucns_set_state_device(ucns_ctx *ctx, const double *u, int32_t layout) {
+    if (!ctx || !u) return UCNS_EBADARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = state_io(ctx, u, nullptr, layout, hipMemcpyDeviceToDevice, true);
+    if (rc == 0) ctx->have_state = true;
+    return rc;
+}
+
+int ucns_get_state_device(ucns_ctx *ctx, double *u, int32_t layout) {
+    if (!ctx || !u) return UCNS_EBADARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return state_io(ctx, nullptr, u, layout, hipMemcpyDeviceToDevice, false);
+}
+
+static int ready(ucns_ctx *ctx) {
+    if (!ctx->have_prestore) {
+        ctx->err = "ucns_upload_prestore has not been called";
+        return UCNS_ESTATE;
+    }
+    if (!ctx->have_state) {
+        ctx->err = "ucns_set_state has not been called";
+        return UCNS_ESTATE;
+    }
+    return UCNS_OK;
+}
+
+int ucns_compute_dt(ucns_ctx *ctx, double cfl, double *dt_out) {
+    if (!ctx || !dt_out || !(cfl > 0)) return UCNS_EBADARG;
+    if (!ctx->have_state) {
+        ctx->err = "ucns_set_state has not been called";
+        return UCNS_ESTATE;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    dt_kernel(ctx, cfl, &ctx->dtv[0]);
+    HIP_TRY(hipMemcpyAsync(dt_out, &ctx->dtv[0], sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UCNS_OK;
+}
+
+int ucns_reconstruct(ucns_ctx *ctx, double *uleft_out) {
+    if (!ctx || !uleft_out) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream));
+    rc = ctx->precision == UCNS_MIXED ? launch_recon_t<float>(ctx) : launch_recon_t<double>(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    const size_t n = (size_t)ctx->ncell * ctx->nf * ctx->nqm * NV;
+    std::vector<double> d(n);
+    if (ctx->precision == UCNS_MIXED) {
+        std::vector<float> f(n);
+        HIP_TRY(hipMemcpyAsync(f.data(), ctx->uleft, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t e = 0; e < n; ++e) d[e] = f[e];
+    } else {
+        HIP_TRY(hipMemcpyAsync(d.data(), ctx->uleft, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    // the device keeps deviations from the cell average; hand out states
+    std::vector<double> u((size_t)ctx->ncell * NV);
+    HIP_TRY(hipMemcpy(u.data(), ctx->U, u.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int nfa = ctx->nf_abi, nqa = ctx->nqm_abi;
+    std::memset(uleft_out, 0, sizeof(double) * (size_t)ctx->ncell * nfa * nqa * NV);
+    for (int i = 0; i < ctx->ncell; ++i)
+        for (int f = 0; f < ctx->nf; ++f)
+            for (int q = 0; q < ctx->nqm; ++q)
+                for (int v = 0; v < NV; ++v)
+                    uleft_out[(((size_t)i * nfa + f) * nqa + q) * NV + v] =
+                        u[(size_t)i * NV + v] + d[(((size_t)i * ctx->nf + f) * ctx->nqm + q) * NV + v];
+    int cnt[2];
+    HIP_TRY(hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->st.n_reduced_last = cnt[0];
+    ctx->st.n_firstorder_last = cnt[1];
+    return UCNS_OK;
+}
+
+int ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout) {
+    if (!ctx) return UCNS_EBADARG;
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = residual_kernels(ctx))) return rc;
+    update_kernel(ctx, 0, nullptr, nullptr, 0.0, ctx->rhs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int cnt[2];
+    HIP_TRY(hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->st.n_reduced_last = cnt[0];
+    ctx->st.n_firstorder_last = cnt[1];
+    if (rhs_out) {
+        const size_t n = (size_t)ctx->ncell * NV;
+        std::vector<double> a(n);
+        HIP_TRY(hipMemcpy(a.data(), ctx->rhs, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (layout == UCNS_LAYOUT_AOS)
+            std::memcpy(rhs_out, a.data(), n * sizeof(double));
+        else
+            for (int i = 0; i < ctx->ncell; ++i)
+                for (int v = 0; v < NV; ++v) rhs_out[(size_t)v * ctx->ncell + i] = a[(size_t)i * NV + v];
+    }
+    return UCNS_OK;
+}
+
+int ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, double *t_io) {
+    if (!ctx || nsteps < 0) return UCNS_EBADARG;
+    if (!(dt_fixed > 0) && !(cfl > 0)) {
+        ctx->err = "need cfl > 0 or dt_fixed > 0";
+        return UCNS_EBADARG;
+    }
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const double t0 = t_io ? *t_io : 0.0;
+    HIP_TRY(hipMemcpyAsync(ctx->t, &t0, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const bool fixed = dt_fixed > 0;
+    if (fixed) HIP_TRY(hipMemcpyAsync(ctx->dtfix, &dt_fixed, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int cur = 0;
+    if (!fixed && nsteps > 0) dt_kernel(ctx, cfl, &ctx->dtv[cur]);
+    for (int it = 0; it < nsteps; ++it) {
+        const double *dt = fixed ? ctx->dtfix : (const double *)&ctx->dtv[cur];
+        unsigned long long *next = fixed ? nullptr : &ctx->dtv[1 - cur];
+        for (int stage = 1; stage <= 3; ++stage) {
+            if ((rc = residual_kernels(ctx))) return rc;
+            if (stage == 3 && next)
+                hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, ctx->stream, next,
+                                   (unsigned long long)0x7FF0000000000000ULL);
+            update_kernel(ctx, stage, dt, stage == 3 ? next : nullptr, cfl, nullptr);
+        }
+        HIP_TRY(hipGetLastError());
+        if (!fixed) cur = 1 - cur;
+    }
+    HIP_TRY(hipMemsetAsync(ctx->flag, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_nonfinite, dim3(1024), dim3(256), 0, ctx->stream, ctx->U, (size_t)ctx->ncell * NV,
+                       ctx->flag);
+    int flag = 0;
+    double t1 = t0;
+    HIP_TRY(hipMemcpyAsync(&flag, ctx->flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&t1, ctx->t, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t_io) *t_io = t1;
+    int cnt[2];
+    HIP_TRY(hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->st.n_reduced_last = cnt[0];
+    ctx->st.n_firstorder_last = cnt[1];
+    if (flag) {
+        ctx->err = "non-finite state after the RK3 steps";
+        return UCNS_ENONFINITE;
+    }
+    return UCNS_OK;
+}
+
+int ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out) {
+    if (!ctx || !out) return UCNS_EBADARG;
+    *out = ctx->st;
+    return UCNS_OK;
+}
+
+int ucns_reset_stats(ucns_ctx *ctx) {
+    if (!ctx) return UCNS_EBADARG;
+    const double b = ctx->st.bytes_recon_per_cell;
+    ctx->st = ucns_stats_v1{};
+    ctx->st.bytes_recon_per_cell = b;
+    ctx->timing = true;
+    return UCNS_OK;
+}
+
+void ucns_destroy(ucns_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    release_prestore(ctx);
+    dfree(ctx->cinfo);
+    dfree(ctx->cnb);
+    dfree(ctx->cface);
+    dfree(ctx->fcell);
+    dfree(ctx->fslot);
+    dfree(ctx->frot);
+    dfree(ctx->farea);
+    dfree(ctx->ovol);
+    dfree(ctx->minedge);
+    dfree(ctx->U);
+    dfree(ctx->U2);
+    dfree(ctx->rhs);
+    dfree(ctx->fflux);
+    dfree(ctx->t);
+    dfree(ctx->dtfix);
+    dfree(ctx->dtv);
+    dfree(ctx->counters);
+    dfree(ctx->flag);
+    dfree(ctx->uleft);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"

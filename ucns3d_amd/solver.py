@@ -1,0 +1,101 @@
+"""Host-side mirror of the reference's time-marching interface over the C-ABI.
+
+    RUNGE_KUTTA3 / TIME_MARCHING  (src/time_p.f90:885-1027, 4026-4580) -> Solver.rk3_steps
+    CALL_FLUX_SUBROUTINES_3D       (src/time_p.f90:2253-2497)            -> Solver.residual
+    ARBITRARY_ORDER                (src/reconstruct_p.f90:2954-3015)     -> Solver.reconstruct
+    CALCULATE_CFL                  (src/time_p.f90:27-191)               -> Solver.compute_dt
+
+Every call goes through libucnsgpu.so (include/ucns.h); there is no CPU
+fallback: constructing a Solver fails if the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .native import UCNS_FP64, UCNS_LAYOUT_AOS, UCNS_MIXED  # noqa: F401
+
+
+class UcnsError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with code {code}: {msg}")
+        self.code = code
+
+
+class Solver:
+    def __init__(self, case, precision: int = UCNS_FP64):
+        self.case = case
+        self.precision = precision
+        self.L = native.gpu_lib()
+        self._m, self._s, self._p = case.structs(precision)
+        h = C.c_void_p()
+        rc = self.L.ucns_create(C.byref(self._m), C.byref(self._s), C.byref(h))
+        self._h = h
+        self._check("ucns_create", rc)
+        rc = self.L.ucns_upload_prestore(self._h, C.byref(self._p))
+        self._check("ucns_upload_prestore", rc)
+        self.ncell = case.ncell
+        self.uleft_shape = (case.ncell, case.nface_max, max(case.qp_quad, case.qp_tri), 5)
+
+    def _check(self, fn: str, rc: int) -> None:
+        if rc != 0:
+            msg = self.L.ucns_last_error(self._h).decode() if self._h else ""
+            raise UcnsError(fn, rc, msg)
+
+    # -- state -----------------------------------------------------------------
+    def set_state(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.ncell, 5)
+        self._check("ucns_set_state", self.L.ucns_set_state(self._h, native.f64p(u), UCNS_LAYOUT_AOS))
+
+    def get_state(self) -> np.ndarray:
+        u = np.empty((self.ncell, 5))
+        self._check("ucns_get_state", self.L.ucns_get_state(self._h, native.f64p(u), UCNS_LAYOUT_AOS))
+        return u
+
+    def set_state_device(self, ptr: int) -> None:
+        self._check("ucns_set_state_device", self.L.ucns_set_state_device(self._h, C.c_void_p(ptr), UCNS_LAYOUT_AOS))
+
+    def get_state_device(self, ptr: int) -> None:
+        self._check("ucns_get_state_device", self.L.ucns_get_state_device(self._h, C.c_void_p(ptr), UCNS_LAYOUT_AOS))
+
+    # -- path ------------------------------------------------------------------
+    def residual(self) -> np.ndarray:
+        rhs = np.empty((self.ncell, 5))
+        self._check("ucns_residual", self.L.ucns_residual(self._h, native.f64p(rhs), UCNS_LAYOUT_AOS))
+        return rhs
+
+    def reconstruct(self) -> np.ndarray:
+        ul = np.empty(self.uleft_shape)
+        self._check("ucns_reconstruct", self.L.ucns_reconstruct(self._h, native.f64p(ul)))
+        return ul
+
+    def compute_dt(self, cfl: float) -> float:
+        dt = C.c_double()
+        self._check("ucns_compute_dt", self.L.ucns_compute_dt(self._h, cfl, C.byref(dt)))
+        return dt.value
+
+    def rk3_steps(self, nsteps: int, cfl: float = 0.5, dt_fixed: float = 0.0, t0: float = 0.0) -> float:
+        t = C.c_double(t0)
+        self._check("ucns_rk3_steps", self.L.ucns_rk3_steps(self._h, nsteps, cfl, dt_fixed, C.byref(t)))
+        return t.value
+
+    def reset_stats(self) -> None:
+        self._check("ucns_reset_stats", self.L.ucns_reset_stats(self._h))
+
+    def stats(self) -> dict:
+        st = native.StatsV1()
+        self._check("ucns_stats", self.L.ucns_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in native.StatsV1._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.L.ucns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
