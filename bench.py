@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s per SSP-RK3 stage of the WENO-4 HLLC residual
+path on one MI355X, FP64 and mixed FP32/FP64.
+
+A "step" is one SSP-RK3 time step (three residual evaluations + stage
+updates) over the whole mesh; the state, the PRESTORE arrays and the face
+data are resident in HBM before the timed region starts.  `value` is the
+FP64 path on the default workload (64^3 hexes split into 6 jittered
+tetrahedra each, 1,572,864 cells); the mixed path runs on the same mesh and
+is reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "cell-updates/s per RK stage (WENO-4 HLLC) FP64 vs mixed; adjoint/primal cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def recon_bytes_per_cell(case, precision: int) -> float:
+    """Algorithmic bytes one cell costs the reconstruction kernel: stencil
+    pseudo-inverses, packed indicator, stencil indices, face Gauss points,
+    basis means and the cell's own state (neighbour re-reads and the ULEFT
+    round trip are implementation overheads and not counted)."""
+    b = 4 if precision == 1 else 8
+    adm = case.array("admis").astype(np.float64).mean()
+    faces = 6 if case.shape == 1 else 4
+    nqp = faces * (case.qp_quad if case.shape == 1 else case.qp_tri)
+    by = adm * case.imax * case.ideg * b
+    if case.iweno == 1:
+        by += case.ideg * (case.ideg + 1) / 2 * b
+    by += adm * case.imax * 4 + nqp * 3 * b + case.ideg * b + 5 * 8
+    return by
+
+
+def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.ndarray):
+    import torch
+    from ucns3d_amd.solver import Solver
+
+    t0 = time.time()
+    s = Solver(case, precision)
+    s.set_state(u0)
+    log(f"  upload {'mixed' if precision else 'fp64'}: {time.time() - t0:.1f}s")
+    if warmup > 0:
+        s.rk3_steps(warmup, cfl=cfl)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    t_sim = s.rk3_steps(steps, cfl=cfl)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    u = s.get_state()
+    # per-kernel durations with HIP events on the library stream (short extra run)
+    s.reset_stats()
+    s.rk3_steps(2, cfl=cfl)
+    st = s.stats()
+    rhs = s.residual()
+    s.close()
+    cons = float(np.abs(rhs.sum(axis=0)).max() / max(np.abs(rhs).sum(axis=0).max(), 1e-300))
+    return {
+        "elapsed": elapsed, "t_sim": t_sim, "u": u, "stats": st, "conservation": cons,
+    }
+
+
+def cpu_baseline(spec_key: str, seconds: float):
+    """The CPU oracle (C restatement of the reference algorithm, OpenMP) on a
+    reduced mesh of the same construction, timed on this host."""
+    from oracle.oracle import Oracle
+    from ucns3d_amd.case import Case, reduced
+
+    n = 24 if spec_key == "c2" else 40
+    case = Case(reduced(spec_key, n))
+    orc = Oracle(case)
+    u = case.init_state()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and steps < 50:
+        u, _ = orc.rk3(u, 1, cfl=case.spec.cfl)
+        steps += 1
+    el = time.perf_counter() - t0
+    val = case.ncell * 3 * steps / el
+    return {"value": val, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{case.spec.name}: {case.ncell} cells, {steps} SSP-RK3 steps ({3 * steps} stages) in "
+                      f"{el:.1f}s, oracle/ucns_oracle.c with OpenMP"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs the whole mesh as one partition on one GPU; "
+                                   "--gpus must be 1 (the multi-GPU halo design is documented, not built)"}))
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    import torch
+    from ucns3d_amd.case import PRESETS, Case
+    from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    torch.cuda.set_device(0)
+    spec = PRESETS[args.config]
+    t0 = time.time()
+    case = Case(spec)
+    u0 = case.init_state()
+    log(f"{spec.name}: {case.ncell} cells, preprocessing {time.time() - t0:.1f}s")
+
+    fp = run_path(case, UCNS_FP64, args.steps, args.warmup, spec.cfl, u0)
+    ms_step = fp["elapsed"] / args.steps * 1e3
+    value = case.ncell * 3 * args.steps / fp["elapsed"]
+    st = fp["stats"]
+    rk_ms = st["ms_recon"] / max(st["n_recon"], 1)
+    bytes_cell = recon_bytes_per_cell(case, UCNS_FP64)
+    achieved = bytes_cell * case.ncell / (rk_ms * 1e-3) / 1e9
+    log(f"fp64: {ms_step:.2f} ms/step, {value / 1e6:.1f} M cell-updates/s/stage, recon kernel {rk_ms:.3f} ms "
+        f"({achieved:.0f} GB/s), flux {st['ms_flux'] / max(st['n_flux'], 1):.3f} ms, "
+        f"update {st['ms_update'] / max(st['n_update'], 1):.3f} ms")
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": spec.name, "ncell": case.ncell, "cfl": spec.cfl, "stages_per_step": 3,
+                   "wenocentralweight": spec.lwci1, "seed": spec.seed, "jitter": spec.jitter},
+        "roofline": {"bound": "hbm", "kernel": "k_recon (fused LSQ/WENO/extrapolation/CHECKSOL)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_cell": bytes_cell, "kernel_ms": rk_ms},
+        "kernels_ms": {"recon": rk_ms, "flux": st["ms_flux"] / max(st["n_flux"], 1),
+                       "update": st["ms_update"] / max(st["n_update"], 1)},
+        "conservation_fp64": fp["conservation"],
+    }
+    if not args.no_mixed:
+        mx = run_path(case, UCNS_MIXED, args.steps, args.warmup, spec.cfl, u0)
+        v_mx = case.ncell * 3 * args.steps / mx["elapsed"]
+        stm = mx["stats"]
+        rk_mx = stm["ms_recon"] / max(stm["n_recon"], 1)
+        bytes_mx = recon_bytes_per_cell(case, UCNS_MIXED)
+        ach_mx = bytes_mx * case.ncell / (rk_mx * 1e-3) / 1e9
+        a, b = mx["u"], fp["u"]
+        err = (np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))).tolist()
+        log(f"mixed: {mx['elapsed'] / args.steps * 1e3:.2f} ms/step, {v_mx / 1e6:.1f} M/s, recon {rk_mx:.3f} ms "
+            f"({ach_mx:.0f} GB/s), rel L2 vs fp64 after {args.warmup + args.steps} steps {err}")
+        out["mixed"] = {"value": v_mx, "ms_per_step": mx["elapsed"] / args.steps * 1e3, "speedup_vs_fp64": v_mx / value,
+                        "roofline_frac": ach_mx / HBM_PEAK_GBS, "recon_ms": rk_mx, "bytes_per_cell": bytes_mx,
+                        "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

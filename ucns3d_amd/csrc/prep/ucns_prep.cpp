@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 #include <omp.h>
+#include <sys/mman.h>
 
 namespace {
 
@@ -108,6 +109,18 @@ void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
     for (int i = 0; i < n; ++i) { xs[i] = x[idx[i]]; ws[i] = w[idx[i]]; }
     x = xs;
     w = ws;
+}
+
+// Gauss-Legendre tables for n = 1..8, solved once.
+struct GLTable {
+    std::vector<double> x[9], w[9];
+    GLTable() {
+        for (int n = 1; n <= 8; ++n) gauss_legendre(n, x[n], w[n]);
+    }
+};
+const GLTable &gl_table() {
+    static const GLTable t;
+    return t;
 }
 
 // Quadrilateral face rule in the reference's point order
@@ -245,8 +258,7 @@ void tet_stroud3(const double v[4][3], QPts &q) {
 }
 
 void tet_collapsed(const double v[4][3], int n, QPts &q) {
-    std::vector<double> g, gw;
-    gauss_legendre(n, g, gw);
+    const std::vector<double> &g = gl_table().x[n], &gw = gl_table().w[n];
     double e1[3], e2[3], e3[3];
     for (int d = 0; d < 3; ++d) {
         e1[d] = v[1][d] - v[0][d];
@@ -271,8 +283,7 @@ void tet_collapsed(const double v[4][3], int n, QPts &q) {
 
 void hex_tensor(const double v[8][3], int n, QPts &q) {
     // parallelepiped spanned at node 1 by (2-1), (4-1), (5-1)
-    std::vector<double> g, gw;
-    gauss_legendre(n, g, gw);
+    const std::vector<double> &g = gl_table().x[n], &gw = gl_table().w[n];
     double e1[3], e2[3], e3[3];
     for (int d = 0; d < 3; ++d) {
         e1[d] = v[1][d] - v[0][d];
@@ -351,13 +362,17 @@ uint64_t splitmix64(uint64_t &s) {
 double u01(uint64_t &s) { return (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 template <class T>
-struct Buf {  // uninitialised heap array, filled in parallel (first touch)
+struct Buf {  // uninitialised heap array on 2 MiB pages, filled in parallel (first touch)
     T *p = nullptr;
     size_t n = 0;
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) p = static_cast<T *>(std::malloc(sizeof(T) * count));
+        if (!count) return;
+        const size_t align = (size_t)2 << 20;
+        size_t bytes = ((sizeof(T) * count + align - 1) / align) * align;
+        p = static_cast<T *>(std::aligned_alloc(align, bytes));
+        if (p) madvise(p, bytes, MADV_HUGEPAGE);
     }
     void release() {
         std::free(p);
@@ -392,6 +407,8 @@ struct ucnsp_case {
     std::vector<int32_t> admis, full, ihexl;
     Buf<double> invmat;
     std::vector<double> indicator, integ_basis, volume11, qpoints, frame;
+    std::vector<double> framedet;
+    int64_t n_solved = 0;  // stencils whose pseudo-inverse was solved (others copied from a congruent one)
 };
 
 namespace {
@@ -604,18 +621,28 @@ void build_geometry(ucnsp_case &c) {
     }
 }
 
-// Trihedral cone test of CHECK_CONDITION (src/grid_p.f90:4582-4811): is
-// point p a non-negative combination of (a-o, b-o, c-o)?
-bool in_cone(const double *o, const double *a, const double *b, const double *cc, const double *p) {
+// Trihedral cone of CHECK_CONDITION (src/grid_p.f90:4582-4811): a point p
+// is inside when VVA1 (p - o) >= 0 componentwise, VVA1 the inverse of
+// [a-o, b-o, c-o] as COMPUTEJACOBIANS builds it (adjugate divided by the
+// determinant, src/grid_t.f90:581-616).  The inverse is formed once per
+// cone; the test keeps the reference's operation order (ties on cone faces
+// are decided by this rounding).
+struct Cone {
+    double I[3][3];
+    double o[3];
+};
+
+void make_cone(const double *o, const double *a, const double *b, const double *cc, Cone &k) {
     double A[3][3];
     for (int d = 0; d < 3; ++d) {
         A[d][0] = a[d] - o[d];
         A[d][1] = b[d] - o[d];
         A[d][2] = cc[d] - o[d];
+        k.o[d] = o[d];
     }
     double det = A[0][0] * (A[2][2] * A[1][1] - A[2][1] * A[1][2]) - A[1][0] * (A[2][2] * A[0][1] - A[2][1] * A[0][2]) +
                  A[2][0] * (A[1][2] * A[0][1] - A[1][1] * A[0][2]);
-    double I[3][3];
+    double (&I)[3][3] = k.I;
     I[0][0] = A[2][2] * A[1][1] - A[2][1] * A[1][2];
     I[0][1] = -(A[2][2] * A[0][1] - A[2][1] * A[0][2]);
     I[0][2] = A[1][2] * A[0][1] - A[1][1] * A[0][2];
@@ -625,9 +652,16 @@ bool in_cone(const double *o, const double *a, const double *b, const double *cc
     I[2][0] = A[2][1] * A[1][0] - A[2][0] * A[1][1];
     I[2][1] = -(A[2][1] * A[0][0] - A[2][0] * A[0][1]);
     I[2][2] = A[1][1] * A[0][0] - A[1][0] * A[0][1];
-    double q[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+    for (int r = 0; r < 3; ++r)
+        for (int c2 = 0; c2 < 3; ++c2) I[r][c2] = I[r][c2] / det;
+}
+
+inline bool in_cone(const Cone &k, const double *p) {
+    const double q[3] = {p[0] - k.o[0], p[1] - k.o[1], p[2] - k.o[2]};
     for (int r = 0; r < 3; ++r) {
-        double x = (I[r][0] * q[0] + I[r][1] * q[1] + I[r][2] * q[2]) / det;
+        double x = k.I[r][0] * q[0];
+        x = x + k.I[r][1] * q[1];
+        x = x + k.I[r][2] * q[2];
         if (!(x >= 0.0)) return false;
     }
     return true;
@@ -686,19 +720,28 @@ void build_stencils(ucnsp_case &c) {
                     int nv = face_nv(sh, f);
                     double fv[4][3];
                     for (int k = 0; k < nv; ++k) node_xyz(c, c.cnodes[i * 8 + face_node(sh, f, k)], fv[k]);
+                    // sub-cones: triangle (n1,n3,n2); quadrilateral the four
+                    // corner triangles (n2,n1,n3) (n2,n4,n3) (n1,n4,n3) (n1,n4,n2)
+                    Cone cones[4];
+                    int ncone;
+                    if (nv == 3) {
+                        make_cone(bc, fv[0], fv[2], fv[1], cones[0]);
+                        ncone = 1;
+                    } else {
+                        make_cone(bc, fv[1], fv[0], fv[2], cones[0]);
+                        make_cone(bc, fv[1], fv[3], fv[2], cones[1]);
+                        make_cone(bc, fv[0], fv[3], fv[2], cones[2]);
+                        make_cone(bc, fv[0], fv[3], fv[1], cones[3]);
+                        ncone = 4;
+                    }
                     for (size_t J = 1; J < list.size() && cnt < NNB; ++J) {
                         int32_t cand = list[J];
                         double vg[3] = {c.centre[cand * 3], c.centre[cand * 3 + 1], c.centre[cand * 3 + 2]};
                         for (int d = 0; d < 3; ++d)
                             if (c.per[d] > 0 && std::fabs(vg[d] - bc[d]) >= 0.5 * c.per[d])
                                 vg[d] += c.per[d] * ((bc[d] - 0.5 * c.per[d]) >= 0 ? 1.0 : -1.0);
-                        bool ok;
-                        if (nv == 3) {
-                            ok = in_cone(bc, fv[0], fv[2], fv[1], vg);
-                        } else {  // union of the four corner triangles
-                            ok = in_cone(bc, fv[1], fv[0], fv[2], vg) || in_cone(bc, fv[1], fv[3], fv[2], vg) ||
-                                 in_cone(bc, fv[0], fv[3], fv[2], vg) || in_cone(bc, fv[0], fv[3], fv[1], vg);
-                        }
+                        bool ok = false;
+                        for (int kc = 0; kc < ncone && !ok; ++kc) ok = in_cone(cones[kc], vg);
                         if (ok) row[cnt++] = cand;
                     }
                 }
@@ -845,167 +888,388 @@ inline double falling(int e, int a) {  // e!/(e-a)!
     return r;
 }
 
-// PRESTORE_RECONSTRUCTION3 per cell: basis integrals, indicator matrix,
-// pseudo-inverses of every admissible stencil, face Gauss points in the
-// local frame (MEMORY_FAST).
+// Geometry signatures.  On unjittered meshes many cells are translates of
+// each other; their PRESTORE data then agree to rounding, so the
+// preprocessor solves one representative and copies it.  Two levels:
+//   base signature    the cell's own node offsets from its centre and the
+//                     adopted face-vertex order -> frame-relative data
+//                     (INTEG_BASIS, VOLUME(1,1), INDICATOR, QPOINTS)
+//   stencil signature base signature + the stencil's members' node offsets
+//                     -> that stencil's pseudo-inverse
+// Offsets are quantised at 1e-9 of the box size; equal hashes are verified
+// element by element.  Never used on jittered meshes.
+struct SigWriter {
+    double q;
+    uint64_t h = 1469598103934665603ULL;
+    std::vector<int64_t> *out;
+    void mix(int64_t v) {
+        if (out) out->push_back(v);
+        h ^= (uint64_t)v;
+        h *= 1099511628211ULL;
+    }
+};
+
+void sig_cell_nodes(const ucnsp_case &c, int64_t i, int64_t j, SigWriter &w) {
+    const double *ci = &c.centre[i * 3];
+    double shift[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+        if (c.per[d] > 0 && std::fabs(c.centre[j * 3 + d] - ci[d]) > 0.5 * c.per[d])
+            shift[d] = c.per[d] * ((ci[d] - 0.5 * c.per[d]) >= 0 ? 1.0 : -1.0);
+    w.mix(c.cshape[j]);
+    for (int k = 0; k < nnodes_of(c.cshape[j]); ++k)
+        for (int d = 0; d < 3; ++d) w.mix(std::llround((c.xyz[c.cnodes[j * 8 + k] * 3 + d] + shift[d] - ci[d]) / w.q));
+}
+
+uint64_t base_signature(const ucnsp_case &c, int64_t i, std::vector<int64_t> *out) {
+    SigWriter w{1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2])), 1469598103934665603ULL, out};
+    sig_cell_nodes(c, i, i, w);
+    const double *ci = &c.centre[i * 3];
+    for (int f = 0; f < c.ifca[i]; ++f)
+        for (int k = 0; k < 4; ++k) {
+            int32_t n = c.fnodes[(i * ucnsp_case::NF + f) * 4 + k];
+            if (n < 0) continue;
+            double p[3] = {c.xyz[n * 3], c.xyz[n * 3 + 1], c.xyz[n * 3 + 2]};
+            unwrap(c, ci, p);
+            for (int d = 0; d < 3; ++d) w.mix(std::llround((p[d] - ci[d]) / w.q));
+        }
+    return w.h;
+}
+
+uint64_t stencil_signature(const ucnsp_case &c, int64_t i, int s, const std::vector<int64_t> &brep,
+                           std::vector<int64_t> *out) {
+    // cells are described by their verified geometry class (node offsets
+    // from the centre) and members also by their centre offset
+    SigWriter w{1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2])), 1469598103934665603ULL, out};
+    w.mix(brep[i]);
+    const double *ci = &c.centre[i * 3];
+    const int32_t *mem = &c.ihexl[((size_t)i * c.typesten + s) * c.nnb];
+    for (int k = 1; k < c.nnb; ++k) {
+        const int64_t j = mem[k];
+        double cj[3] = {c.centre[j * 3], c.centre[j * 3 + 1], c.centre[j * 3 + 2]};
+        unwrap(c, ci, cj);
+        w.mix(brep[j]);
+        for (int d = 0; d < 3; ++d) w.mix(std::llround((cj[d] - ci[d]) / w.q));
+    }
+    return w.h;
+}
+
+struct PrestoreScratch {
+    std::vector<double> A, M, inv, work, mono, intg, mbar, volj, nzv, memint;
+    std::vector<int> nzk;
+    QPts q;
+    PrestoreScratch(int ID, int IM) {
+        A.resize((size_t)IM * ID);
+        M.resize((size_t)ID * ID);
+        inv.resize((size_t)ID * ID);
+        work.resize(3 * ID * ID + ID + 8);
+        mono.resize(ID);
+        intg.resize(ID);
+        mbar.resize(ID);
+        volj.resize(IM);
+        nzv.resize(ID);
+        nzk.resize(ID);
+        memint.resize((size_t)IM * (ID + 1));
+    }
+};
+
+// Frame-relative data of one cell: INTEG_BASIS, VOLUME(1,1), INDICATOR
+// (INDICATORMATRIX + PH*, src/prestore.f90:1529-2025) and the face Gauss
+// points in the local frame (MEMORY_FAST, src/reconstruct_p.f90:178-244).
+void base_cell(ucnsp_case &c, int64_t i, const DerivTable &dt, PrestoreScratch &S) {
+    const int NF = ucnsp_case::NF;
+    const int ID = c.ideg, NQ = c.nqpmax;
+    QPts &q = S.q;
+    const double *J9 = &c.frame[i * 12], *o = &c.frame[i * 12 + 9];
+    const double adet = std::fabs(c.framedet[i]);
+    double vi[8][3];
+    member_local_nodes(c, i, i, J9, o, vi);
+    // INTEG_BASIS: integral of the raw monomials (CALINTBASIS runs before
+    // INTEG_BASIS is set, src/prestore.f90:222-225)
+    cell_rule(c.cshape[i], vi, c.iorder, q);
+    std::fill(S.intg.begin(), S.intg.end(), 0.0);
+    for (size_t p = 0; p < q.w.size(); ++p) {
+        monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, S.mono.data());
+        for (int d = 0; d < ID; ++d) S.intg[d] += q.w[p] * S.mono[d];
+    }
+    c.volume11[i] = c.vol[i] / adet;
+    for (int d = 0; d < ID; ++d) c.integ_basis[i * ID + d] = S.intg[d];
+    if (c.o.iweno == 1) {
+        cell_rule(c.cshape[i], vi, 2 * (c.iorder - 1), q);
+        double *B = &c.indicator[(size_t)i * ID * ID];
+        std::fill(B, B + (size_t)ID * ID, 0.0);
+        for (size_t p = 0; p < q.w.size(); ++p) {
+            double x = q.x[p * 3], y = q.x[p * 3 + 1], z = q.x[p * 3 + 2];
+            double px[5], py[5], pz[5];
+            px[0] = py[0] = pz[0] = 1.0;
+            for (int k = 1; k < 5; ++k) {
+                px[k] = px[k - 1] * x;
+                py[k] = py[k - 1] * y;
+                pz[k] = pz[k - 1] * z;
+            }
+            for (int a = 0; a < dt.na; ++a) {
+                int ax = dt.alpha[a * 3], ay = dt.alpha[a * 3 + 1], az = dt.alpha[a * 3 + 2];
+                // D^a m_d is non-zero only for monomials that contain x^ax y^ay z^az
+                int nz = 0;
+                for (int d = 0; d < ID; ++d) {
+                    int ex = MONO[d][0], ey = MONO[d][1], ez = MONO[d][2];
+                    if (ex < ax || ey < ay || ez < az) continue;
+                    S.nzk[nz] = d;
+                    S.nzv[nz] = falling(ex, ax) * falling(ey, ay) * falling(ez, az) * px[ex - ax] * py[ey - ay] *
+                                pz[ez - az];
+                    ++nz;
+                }
+                const double wq = q.w[p];
+                for (int u = 0; u < nz; ++u) {
+                    const double wu = wq * S.nzv[u];
+                    double *Brow = B + (size_t)S.nzk[u] * ID;
+                    for (int t = 0; t < nz; ++t) Brow[S.nzk[t]] += wu * S.nzv[t];
+                }
+            }
+        }
+    }
+    const double *ci = &c.centre[i * 3];
+    for (int f = 0; f < c.ifca[i]; ++f) {
+        int nv = c.types_faces[i * NF + f] == 5 ? 4 : 3;
+        double fv[4][3], lv[4][3];
+        for (int k = 0; k < nv; ++k) {
+            node_xyz(c, c.fnodes[(i * NF + f) * 4 + k], fv[k]);
+            unwrap(c, ci, fv[k]);
+            to_local(J9, o, fv[k], lv[k]);
+        }
+        double *Q = &c.qpoints[((size_t)i * NF + f) * NQ * 3];
+        if (nv == 4) {
+            for (int g = 0; g < c.qp_quad; ++g) {
+                double R = c.quad_rs[g * 2], Ss = c.quad_rs[g * 2 + 1];
+                double N[4] = {0.25 * (1 - R) * (1 - Ss), 0.25 * (1 + R) * (1 - Ss), 0.25 * (1 + R) * (1 + Ss),
+                               0.25 * (1 - R) * (1 + Ss)};
+                for (int d = 0; d < 3; ++d)
+                    Q[g * 3 + d] = N[0] * lv[0][d] + N[1] * lv[1][d] + N[2] * lv[2][d] + N[3] * lv[3][d];
+            }
+        } else {
+            for (int g = 0; g < c.qp_tri; ++g) {
+                const double *b = &c.tri_bary[g * 3];
+                for (int d = 0; d < 3; ++d) Q[g * 3 + d] = b[0] * lv[0][d] + b[1] * lv[1][d] + b[2] * lv[2][d];
+            }
+        }
+    }
+}
+
+// Pseudo-inverse of stencil s of cell i (PRESTORE_RECONSTRUCTION3,
+// src/prestore.f90:285-529): A_jk = w_j * integral over V_j of (m_k - mbar_k),
+// M = A^T A, Householder QR, invmat = R^-1 Q^T, P = invmat A^T with column j
+// scaled by |V_j| w_j.  Returns false if the result has a NaN.
+bool stencil_p(ucnsp_case &c, int64_t i, int s, PrestoreScratch &S) {
+    const int IM = c.imax, ID = c.ideg, NNB = c.nnb, TS = c.typesten;
+    QPts &q = S.q;
+    const double *J9 = &c.frame[i * 12], *o = &c.frame[i * 12 + 9];
+    const double adet = std::fabs(c.framedet[i]);
+    const double vol11 = c.volume11[i];
+    for (int d = 0; d < ID; ++d) S.mbar[d] = c.integ_basis[i * ID + d] / vol11;
+    const int32_t *mem = &c.ihexl[((size_t)i * TS + s) * NNB];
+    double *A = S.A.data();
+    for (int k = 0; k < IM; ++k) {
+        int64_t j = mem[k + 1];
+        double wl = 1.0;
+        if (c.o.weight_lsqr == 1) {
+            double ci[3] = {c.centre[i * 3], c.centre[i * 3 + 1], c.centre[i * 3 + 2]};
+            double cj[3] = {c.centre[j * 3], c.centre[j * 3 + 1], c.centre[j * 3 + 2]};
+            unwrap(c, ci, cj);
+            double dd = 0;
+            for (int d = 0; d < 3; ++d) dd += (cj[d] - ci[d]) * (cj[d] - ci[d]);
+            wl = 1.0 / std::sqrt(dd);
+        }
+        double vj[8][3];
+        member_local_nodes(c, i, j, J9, o, vj);
+        cell_rule(c.cshape[j], vj, c.iorder, q);
+        double vq = 0.0;
+        std::fill(S.intg.begin(), S.intg.end(), 0.0);
+        for (size_t p = 0; p < q.w.size(); ++p) {
+            monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, S.mono.data());
+            for (int d = 0; d < ID; ++d) S.intg[d] += q.w[p] * S.mono[d];
+            vq += q.w[p];
+        }
+        // STENCILS(LL,K,:) = w * integral of (m - mbar) over V_j (COMPBASEL)
+        for (int d = 0; d < ID; ++d) A[(size_t)k * ID + d] = wl * (S.intg[d] - S.mbar[d] * vq);
+        S.volj[k] = (c.vol[j] / adet) * wl;  // ILOX_VOLUME * WLSQR
+    }
+    // LSCQM = A^T A (src/prestore.f90:437-476)
+    double *M = S.M.data();
+    for (int a2 = 0; a2 < ID; ++a2)
+        for (int b2 = a2; b2 < ID; ++b2) {
+            double s2 = 0.0;
+            for (int k = 0; k < IM; ++k) s2 += A[(size_t)k * ID + a2] * A[(size_t)k * ID + b2];
+            M[a2 * ID + b2] = s2;
+            M[b2 * ID + a2] = s2;
+        }
+    qr_inverse(ID, M, S.inv.data(), S.work.data());
+    bool bad = false;
+    const double *inv = S.inv.data();
+    double *P = c.invmat.p + ((size_t)i * TS + s) * IM * ID;
+    for (int k = 0; k < IM; ++k)
+        for (int a2 = 0; a2 < ID; ++a2) {
+            double s2 = 0.0;
+            for (int b2 = 0; b2 < ID; ++b2) s2 += inv[a2 * ID + b2] * A[(size_t)k * ID + b2];
+            double val = s2 * S.volj[k];
+            P[(size_t)k * ID + a2] = val;
+            if (val != val) bad = true;
+        }
+    return !bad;
+}
+
+// representative of every item: rep[k] = smallest index with an equal
+// (verified) signature
+template <class Hash, class Full>
+void classes(int64_t n, std::vector<int64_t> &rep, Hash hash, Full full) {
+    std::vector<std::pair<uint64_t, int64_t>> sig(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) sig[k] = {hash(k), k};
+    std::sort(sig.begin(), sig.end());
+    rep.resize(n);
+    for (int64_t k = 0; k < n; ++k) rep[k] = k;
+    for (size_t a = 1; a < sig.size(); ++a)
+        if (sig[a].first == sig[a - 1].first) rep[sig[a].second] = rep[sig[a - 1].second];
+#pragma omp parallel
+    {
+        std::vector<int64_t> va, vb;
+#pragma omp for schedule(static)
+        for (int64_t k = 0; k < n; ++k) {
+            if (rep[k] == k) continue;
+            va.clear();
+            vb.clear();
+            full(k, &va);
+            full(rep[k], &vb);
+            if (va != vb) rep[k] = k;
+        }
+    }
+}
+
 void build_prestore(ucnsp_case &c) {
     const int64_t nc = c.ncell;
     const int NF = ucnsp_case::NF;
-    const int TS = c.typesten, NNB = c.nnb, IM = c.imax, ID = c.ideg, NQ = c.nqpmax;
+    const int TS = c.typesten, IM = c.imax, ID = c.ideg, NQ = c.nqpmax;
     c.invmat.alloc((size_t)nc * TS * IM * ID);
     c.indicator.assign(c.o.iweno == 1 ? (size_t)nc * ID * ID : 0, 0.0);
     c.integ_basis.assign((size_t)nc * ID, 0.0);
     c.volume11.assign(nc, 0.0);
     c.qpoints.assign((size_t)nc * NF * NQ * 3, 0.0);
     c.frame.assign((size_t)nc * 12, 0.0);
+    c.framedet.assign(nc, 0.0);
     const DerivTable dt = make_derivs(c.iorder);
+    const bool tim = std::getenv("UCNSP_TIMING") != nullptr;
+    double t0 = omp_get_wtime();
+    auto lap = [&](const char *what) {
+        if (!tim) return;
+        double t1 = omp_get_wtime();
+        std::fprintf(stderr, "[ucnsp]   %-18s %8.3f s\n", what, t1 - t0);
+        t0 = t1;
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nc; ++i) {
+        double J9[9], o[3], det;
+        cell_frame(c, i, J9, o, det);
+        for (int k = 0; k < 9; ++k) c.frame[i * 12 + k] = J9[k];
+        for (int k = 0; k < 3; ++k) c.frame[i * 12 + 9 + k] = o[k];
+        c.framedet[i] = det;
+    }
+    const bool cache = (c.o.jitter == 0.0);
+    lap("frames");
+    // 1. frame-relative per-cell data
+    std::vector<int64_t> brep;
+    std::vector<uint64_t> bhash(nc, 0);
+    if (cache) {
+        classes(nc, brep, [&](int64_t i) { return bhash[i] = base_signature(c, i, nullptr); },
+                [&](int64_t i, std::vector<int64_t> *v) { base_signature(c, i, v); });
+    } else {
+        brep.resize(nc);
+        for (int64_t i = 0; i < nc; ++i) brep[i] = i;
+    }
+    lap("base classes");
 #pragma omp parallel
     {
-        std::vector<double> A((size_t)IM * ID), M((size_t)ID * ID), inv((size_t)ID * ID);
-        std::vector<double> work(3 * ID * ID + ID + 8), mono(ID), intg(ID), mbar(ID), volj(IM);
-        std::vector<double> D((size_t)dt.na * ID);
-        QPts q;
+        PrestoreScratch S(ID, IM);
 #pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nc; ++i)
+            if (brep[i] == i) base_cell(c, i, dt, S);
+#pragma omp for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
-            double J9[9], o[3], detjc;
-            cell_frame(c, i, J9, o, detjc);
-            for (int k = 0; k < 9; ++k) c.frame[i * 12 + k] = J9[k];
-            for (int k = 0; k < 3; ++k) c.frame[i * 12 + 9 + k] = o[k];
-            const double adet = std::fabs(detjc);
-            // own cell in the local frame
-            double vi[8][3];
-            member_local_nodes(c, i, i, J9, o, vi);
-            // INTEG_BASIS: integral of the raw monomials (CALINTBASIS before
-            // INTEG_BASIS is set, src/prestore.f90:222-225)
-            cell_rule(c.cshape[i], vi, c.iorder, q);
-            std::fill(intg.begin(), intg.end(), 0.0);
-            for (size_t p = 0; p < q.w.size(); ++p) {
-                monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, mono.data());
-                for (int d = 0; d < ID; ++d) intg[d] += q.w[p] * mono[d];
-            }
-            const double vol11 = c.vol[i] / adet;
-            c.volume11[i] = vol11;
-            for (int d = 0; d < ID; ++d) {
-                c.integ_basis[i * ID + d] = intg[d];
-                mbar[d] = intg[d] / vol11;
-            }
-            // INDICATOR (INDICATORMATRIX + PH*, src/prestore.f90:1529-2025)
-            if (c.o.iweno == 1) {
-                cell_rule(c.cshape[i], vi, 2 * (c.iorder - 1), q);
-                double *B = &c.indicator[(size_t)i * ID * ID];
-                for (size_t p = 0; p < q.w.size(); ++p) {
-                    double x = q.x[p * 3], y = q.x[p * 3 + 1], z = q.x[p * 3 + 2];
-                    double px[5], py[5], pz[5];
-                    px[0] = py[0] = pz[0] = 1.0;
-                    for (int k = 1; k < 5; ++k) {
-                        px[k] = px[k - 1] * x;
-                        py[k] = py[k - 1] * y;
-                        pz[k] = pz[k - 1] * z;
-                    }
-                    for (int a = 0; a < dt.na; ++a) {
-                        int ax = dt.alpha[a * 3], ay = dt.alpha[a * 3 + 1], az = dt.alpha[a * 3 + 2];
-                        for (int d = 0; d < ID; ++d) {
-                            int ex = MONO[d][0], ey = MONO[d][1], ez = MONO[d][2];
-                            if (ex < ax || ey < ay || ez < az) {
-                                D[a * ID + d] = 0.0;
-                                continue;
-                            }
-                            D[a * ID + d] = falling(ex, ax) * falling(ey, ay) * falling(ez, az) * px[ex - ax] *
-                                            py[ey - ay] * pz[ez - az];
-                        }
-                    }
-                    for (int k = 0; k < ID; ++k)
-                        for (int l = 0; l < ID; ++l) {
-                            double s = 0.0;
-                            for (int a = 0; a < dt.na; ++a) s += D[a * ID + k] * D[a * ID + l];
-                            B[k * ID + l] += q.w[p] * s;
-                        }
-                }
-            }
-            // pseudo-inverses of the admissible stencils
-            double *Pcell = c.invmat.p + (size_t)i * TS * IM * ID;
-            const int ad = c.admis[i];
-            for (int s = 0; s < TS; ++s) {
-                double *P = Pcell + (size_t)s * IM * ID;
-                if (s >= ad) {
-                    std::fill(P, P + (size_t)IM * ID, 0.0);
-                    continue;
-                }
-                const int32_t *mem = &c.ihexl[((size_t)i * TS + s) * NNB];
-                for (int k = 0; k < IM; ++k) {
-                    int64_t j = mem[k + 1];
-                    double vj[8][3];
-                    member_local_nodes(c, i, j, J9, o, vj);
-                    double wl = 1.0;
-                    if (c.o.weight_lsqr == 1) {
-                        double ci[3] = {c.centre[i * 3], c.centre[i * 3 + 1], c.centre[i * 3 + 2]};
-                        double cj[3] = {c.centre[j * 3], c.centre[j * 3 + 1], c.centre[j * 3 + 2]};
-                        unwrap(c, ci, cj);
-                        double dd = 0;
-                        for (int d = 0; d < 3; ++d) dd += (cj[d] - ci[d]) * (cj[d] - ci[d]);
-                        wl = 1.0 / std::sqrt(dd);
-                    }
-                    cell_rule(c.cshape[j], vj, c.iorder, q);
-                    double vq = 0.0;
-                    std::fill(intg.begin(), intg.end(), 0.0);
-                    for (size_t p = 0; p < q.w.size(); ++p) {
-                        monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID, mono.data());
-                        for (int d = 0; d < ID; ++d) intg[d] += q.w[p] * mono[d];
-                        vq += q.w[p];
-                    }
-                    // STENCILS(LL,K,:) = w * integral of (m - mbar) over V_j (COMPBASEL)
-                    for (int d = 0; d < ID; ++d) A[(size_t)k * ID + d] = wl * (intg[d] - mbar[d] * vq);
-                    volj[k] = (c.vol[j] / adet) * wl;  // ILOX_VOLUME * WLSQR
-                }
-                // LSCQM = A^T A (src/prestore.f90:437-476)
-                for (int a = 0; a < ID; ++a)
-                    for (int b = 0; b < ID; ++b) {
-                        double s2 = 0.0;
-                        for (int k = 0; k < IM; ++k) s2 += A[(size_t)k * ID + a] * A[(size_t)k * ID + b];
-                        M[a * ID + b] = s2;
-                    }
-                qr_inverse(ID, M.data(), inv.data(), work.data());
-                // invmat_stencilt = invmat A^T, column iq scaled by V_iq w_iq
-                bool bad = false;
-                for (int k = 0; k < IM; ++k)
-                    for (int a = 0; a < ID; ++a) {
-                        double s2 = 0.0;
-                        for (int b = 0; b < ID; ++b) s2 += inv[a * ID + b] * A[(size_t)k * ID + b];
-                        double val = s2 * volj[k];
-                        P[(size_t)k * ID + a] = val;
-                        if (val != val) bad = true;
-                    }
-                if (bad) c.full[i] = 0;  // src/prestore.f90:526-529
-            }
-            // MEMORY_FAST: face Gauss points in the local frame, built from
-            // the adopted vertex order of each face
-            const double *ci = &c.centre[i * 3];
-            for (int f = 0; f < c.ifca[i]; ++f) {
-                int nv = c.types_faces[i * NF + f] == 5 ? 4 : 3;
-                double fv[4][3], lv[4][3];
-                for (int k = 0; k < nv; ++k) {
-                    node_xyz(c, c.fnodes[(i * NF + f) * 4 + k], fv[k]);
-                    unwrap(c, ci, fv[k]);
-                    to_local(J9, o, fv[k], lv[k]);
-                }
-                double *Q = &c.qpoints[((size_t)i * NF + f) * NQ * 3];
-                if (nv == 4) {
-                    for (int g = 0; g < c.qp_quad; ++g) {
-                        double R = c.quad_rs[g * 2], S = c.quad_rs[g * 2 + 1];
-                        double N[4] = {0.25 * (1 - R) * (1 - S), 0.25 * (1 + R) * (1 - S), 0.25 * (1 + R) * (1 + S),
-                                       0.25 * (1 - R) * (1 + S)};
-                        for (int d = 0; d < 3; ++d)
-                            Q[g * 3 + d] = N[0] * lv[0][d] + N[1] * lv[1][d] + N[2] * lv[2][d] + N[3] * lv[3][d];
-                    }
-                } else {
-                    for (int g = 0; g < c.qp_tri; ++g) {
-                        const double *b = &c.tri_bary[g * 3];
-                        for (int d = 0; d < 3; ++d) Q[g * 3 + d] = b[0] * lv[0][d] + b[1] * lv[1][d] + b[2] * lv[2][d];
-                    }
-                }
-            }
+            const int64_t r = brep[i];
+            if (r == i) continue;
+            if (c.o.iweno == 1)
+                std::memcpy(&c.indicator[(size_t)i * ID * ID], &c.indicator[(size_t)r * ID * ID], sizeof(double) * ID * ID);
+            std::memcpy(&c.integ_basis[(size_t)i * ID], &c.integ_basis[(size_t)r * ID], sizeof(double) * ID);
+            c.volume11[i] = c.volume11[r];
+            const size_t nq = (size_t)NF * NQ * 3;
+            std::memcpy(&c.qpoints[(size_t)i * nq], &c.qpoints[(size_t)r * nq], sizeof(double) * nq);
         }
     }
+    lap("base data");
+    // 2. pseudo-inverses, one per stencil class
+    const int64_t ns = nc * TS;
+    std::vector<int64_t> srep, grep;
+    if (cache) {
+        const double qg = 1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2]));
+        classes(nc, grep,
+                [&](int64_t i) {
+                    SigWriter w{qg, 1469598103934665603ULL, nullptr};
+                    sig_cell_nodes(c, i, i, w);
+                    return w.h;
+                },
+                [&](int64_t i, std::vector<int64_t> *v) {
+                    SigWriter w{qg, 1469598103934665603ULL, v};
+                    sig_cell_nodes(c, i, i, w);
+                });
+        classes(ns, srep,
+                [&](int64_t k) {
+                    const int64_t i = k / TS;
+                    const int s = (int)(k % TS);
+                    if (s >= c.admis[i]) return (uint64_t)0;
+                    return stencil_signature(c, i, s, grep, nullptr);
+                },
+                [&](int64_t k, std::vector<int64_t> *v) {
+                    const int64_t i = k / TS;
+                    const int s = (int)(k % TS);
+                    if (s >= c.admis[i]) {
+                        v->push_back(-1);
+                        return;
+                    }
+                    stencil_signature(c, i, s, grep, v);
+                });
+    } else {
+        srep.resize(ns);
+        for (int64_t k = 0; k < ns; ++k) srep[k] = k;
+    }
+    lap("stencil classes");
+    std::vector<char> ok(ns, 1);
+    int64_t solved = 0;
+#pragma omp parallel reduction(+ : solved)
+    {
+        PrestoreScratch S(ID, IM);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t k = 0; k < ns; ++k) {
+            const int64_t i = k / TS;
+            const int s = (int)(k % TS);
+            if (s >= c.admis[i]) {
+                double *P = c.invmat.p + (size_t)k * IM * ID;
+                std::fill(P, P + (size_t)IM * ID, 0.0);
+                continue;
+            }
+            if (srep[k] != k) continue;
+            ok[k] = stencil_p(c, i, s, S) ? 1 : 0;
+            ++solved;
+        }
+#pragma omp for schedule(static)
+        for (int64_t k = 0; k < ns; ++k) {
+            const int64_t r = srep[k];
+            const int64_t i = k / TS;
+            if (r == k || (int)(k % TS) >= c.admis[i]) continue;
+            std::memcpy(c.invmat.p + (size_t)k * IM * ID, c.invmat.p + (size_t)r * IM * ID, sizeof(double) * IM * ID);
+            ok[k] = ok[r];
+        }
+    }
+    lap("pseudo-inverses");
+    c.n_solved = solved;
+    for (int64_t k = 0; k < ns; ++k)
+        if (!ok[k]) c.full[k / TS] = 0;  // NaN stencils disable FULL (src/prestore.f90:526-529)
 }
 
 void gen_box(ucnsp_case &c) {
@@ -1126,8 +1390,18 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
     ucnsp_case *c = new ucnsp_case();
     c->o = o;
+    const bool tim = std::getenv("UCNSP_TIMING") != nullptr;
+    double t0 = omp_get_wtime();
+    auto lap = [&](const char *what) {
+        if (!tim) return;
+        double t1 = omp_get_wtime();
+        std::fprintf(stderr, "[ucnsp] %-14s %8.3f s\n", what, t1 - t0);
+        t0 = t1;
+    };
     gen_box(*c);
+    lap("mesh");
     build_connectivity(*c);
+    lap("connectivity");
     for (int64_t i = 0; i < c->ncell; ++i)
         for (int f = 0; f < c->ifca[i]; ++f)
             if (c->ineigh[i * ucnsp_case::NF + f] < 0) {
@@ -1135,6 +1409,7 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
                 return fail(UCNS_EBADARG, "unmatched face in a periodic box");
             }
     build_geometry(*c);
+    lap("geometry");
     // scheme sizes (DETERMINE_SIZE, src/grid_p.f90:6097-6288; parameters.f90:1270-1394)
     c->iorder = o.iorder;
     c->ideg = ndeg_of(o.iorder);
@@ -1166,7 +1441,23 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     }
     c->nqpmax = std::max(c->qp_quad, c->qp_tri);
     build_stencils(*c);
+    // leading stencil dimension = the largest ADMIS present (tets use 5 of
+    // the 7 slots of TYPESTEN); complete stencils are already compacted
+    {
+        int ts = 1;
+        for (int64_t i = 0; i < c->ncell; ++i) ts = std::max(ts, (int)c->admis[i]);
+        if (ts < c->typesten) {
+            std::vector<int32_t> h((size_t)c->ncell * ts * c->nnb);
+            for (int64_t i = 0; i < c->ncell; ++i)
+                std::memcpy(&h[(size_t)i * ts * c->nnb], &c->ihexl[(size_t)i * c->typesten * c->nnb],
+                            sizeof(int32_t) * ts * c->nnb);
+            c->ihexl.swap(h);
+            c->typesten = ts;
+        }
+    }
+    lap("stencils");
     build_prestore(*c);
+    lap("prestore");
     *out = c;
     return UCNS_OK;
 }
@@ -1225,7 +1516,7 @@ int ucnsp_info(const ucnsp_case *c, int64_t *ints, double *dbls) {
     if (!c) return fail(UCNS_EBADARG, "null case");
     if (ints) {
         int64_t v[16] = {c->ncell, c->nnode, ucnsp_case::NF, c->iorder, c->ideg, c->nnb, c->imax, c->typesten,
-                         c->iselem, c->igq, c->qp_quad, c->qp_tri, c->nqpmax, c->shape, c->o.iweno, 0};
+                         c->iselem, c->igq, c->qp_quad, c->qp_tri, c->nqpmax, c->shape, c->o.iweno, c->n_solved};
         std::memcpy(ints, v, sizeof(v));
     }
     if (dbls) {
