@@ -409,6 +409,311 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
 }
 
 // ---------------------------------------------------------------------------
+// WENO reconstruction, lane per (cell, stencil).  A wavefront owns a tile of
+// 8 consecutive cells; lane = 8*local_cell + stencil.  Each lane streams its
+// stencil's pseudo-inverse from the tiled layout [tile][j][k][8*ts] (one
+// contiguous block per load instruction), forms dU_j on the fly from the
+// FP64 states and keeps its ideg x 5 coefficients in registers.  The
+// indicators, weights and the weighted sum over stencils use shuffles
+// inside the 8-lane group of a cell; face points are split over the group.
+template <typename R>
+struct ReconWParams {
+    int ncell, ntile, tiles_per_xcd, ts, nf, nqm, qpq, qpt;
+    double gamma, lwci1, eps;
+    const double *U;
+    const int32_t *nbr;    // [ntile][imax][8*ts]
+    const R *P;            // [ntile][imax][ideg][8*ts]
+    const R *B;            // [ncell][ideg*(ideg+1)/2]
+    const R *mbar;         // [ncell][ideg]
+    const R *qp;           // [ncell][nf][nqm][3]
+    const int32_t *cinfo;
+    const int32_t *cnb;    // [ncell][nf]
+    R *uleft;              // [ncell][nf][nqm][5] deviations from the cell average
+    int *counters;
+};
+
+template <typename T>
+__device__ inline T grp_sum8(T v) {
+    v += __shfl_xor(v, 1, WAVE);
+    v += __shfl_xor(v, 2, WAVE);
+    v += __shfl_xor(v, 4, WAVE);
+    return v;
+}
+template <typename T>
+__device__ inline T grp_min8(T v) {
+    v = fmin(v, __shfl_xor(v, 1, WAVE));
+    v = fmin(v, __shfl_xor(v, 2, WAVE));
+    v = fmin(v, __shfl_xor(v, 4, WAVE));
+    return v;
+}
+__device__ inline int grp_or8(int v) {
+    v |= __shfl_xor(v, 1, WAVE);
+    v |= __shfl_xor(v, 2, WAVE);
+    v |= __shfl_xor(v, 4, WAVE);
+    return v;
+}
+
+// index of B(k,l) in the packed upper triangle stored by rows
+__host__ __device__ constexpr int pidx(int ideg, int k, int l) {
+    return k <= l ? k * ideg - k * (k - 1) / 2 + (l - k) : l * ideg - l * (l - 1) / 2 + (k - l);
+}
+
+template <typename R, int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
+    constexpr int NP = IDEG * (IDEG + 1) / 2;
+    __shared__ R sB[8][NP];
+    __shared__ R sG0[8][IDEG * NV];
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    const int info = cvalid ? p.cinfo[cell] : 0;
+    const int ifca = ci_ifca(info), admis = ci_admis(info), full = ci_full(info);
+    const bool act = cvalid && s < admis;
+    const int TP = 8 * p.ts, pos = lc * p.ts + s;
+    double ui[NV] = {1.0, 0.0, 0.0, 0.0, 1.0};
+    if (cvalid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
+    }
+    // indicators of the tile's cells -> LDS (one contiguous block)
+    {
+        const R *Bt = p.B + (size_t)tile * 8 * NP;
+        const int nvc = min(8, p.ncell - tile * 8);
+        for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+    }
+    // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
+    R g[IDEG][NV];
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[k][v] = R(0);
+    if (act) {
+        const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
+        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + pos;
+#pragma unroll 2
+        for (int j = 0; j < IMAX; ++j) {
+            const int n = nt[j * TP];
+            R du[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) du[v] = (R)(p.U[(size_t)n * NV + v] - ui[v]);
+            const R *Pj = Pt + (size_t)j * IDEG * TP;
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                const R pk = Pj[k * TP];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) g[k][v] += pk * du[v];
+            }
+        }
+    }
+    // central-stencil coefficients kept for the MUSCL fallback
+    if (s == 0) {
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sG0[lc][k * NV + v] = g[k][v];
+    }
+    __syncthreads();
+
+    // ---- smoothness indicators beta = g^T (B g) (src/reconstruct_p.f90:995-1004) ----
+    R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k) {
+            R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+            for (int l = 0; l < IDEG; ++l) {
+                const R bkl = sB[lc][pidx(IDEG, k, l)];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
+        }
+    }
+    // ---- weights (src/reconstruct_p.f90:1031-1049) and the weighted sum ----
+    const R lam = (s == 0) ? (R)p.lwci1 : R(1);
+    const R eps = (R)p.eps;
+    R om[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        R w;
+        if (sizeof(R) == 8) {
+            const R x = eps + beta[v];
+            const R x2 = x * x;
+            w = act ? lam / (x2 * x2) : R(0);
+        } else {
+            // FP32: scale by (eps+beta_min)^4 so (eps+beta)^-4 cannot underflow
+            const R x = eps + beta[v];
+            const R xmin = grp_min8(act ? x : (R)INFINITY);
+            const R r = xmin / x;
+            const R r2 = r * r;
+            w = act ? lam * (r2 * r2) : R(0);
+        }
+        const R sum = grp_sum8(w);
+        om[v] = act ? w / sum : R(0);
+    }
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[k][v] = grp_sum8(om[v] * g[k][v]);
+
+    // ---- face points: deviation from the average + CHECKSOL (0.85) ----
+    const int nslot = ifca * p.nqm;
+    const R *qpc = p.qp + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * 3;
+    R mb[IDEG];
+#pragma unroll
+    for (int d = 0; d < IDEG; ++d) mb[d] = cvalid ? p.mbar[(size_t)cell * IDEG + d] : R(0);
+    double wi[NV];
+    cons2prim_d(p.gamma, ui, wi);
+    R ul[NQL][NV];
+    int trouble = 0;
+#pragma unroll
+    for (int r = 0; r < NQL; ++r) {
+        const int q = s + 8 * r;
+        const int f = q / p.nqm, qq = q - f * p.nqm;
+        const bool ok = cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ul[r][v] = R(0);
+        if (ok) {
+            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, mb, phi);
+            double uq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * g[k][v];
+                ul[r][v] = res;
+                uq[v] = ui[v] + (double)res;
+            }
+            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+        }
+    }
+    const int any_trouble = grp_or8(trouble);  // every lane takes part in the shuffle
+    const bool reduce = cvalid && (!full || any_trouble);
+
+    // ---- MUSCL + Barth-Jespersen on the central polynomial (src/reconstruct_p.f90:2044-2280) ----
+    if (reduce) {
+        double umin[NV], umax[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) umin[v] = umax[v] = ui[v];
+        for (int f = 0; f < ifca; ++f) {
+            const int n = p.cnb[(size_t)cell * p.nf + f];
+            if (n < 0) continue;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double un = p.U[(size_t)n * NV + v];
+                umin[v] = fmin(umin[v], un);
+                umax[v] = fmax(umax[v], un);
+            }
+        }
+        double slope[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) slope[v] = 1.0e16;
+        R res0[NQL][NV];
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            const int f = q / p.nqm, qq = q - f * p.nqm;
+            const bool ok = q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) res0[r][v] = R(0);
+            if (ok) {
+                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                R phi[IDEG];
+                basis_at<R, IDEG>(x, mb, phi);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    R res = R(0);
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) res += phi[k] * sG0[lc][k * NV + v];
+                    res0[r][v] = res;
+                    const double d2 = (double)res;
+                    double psi;
+                    if (fabs(d2) <= 0.0)
+                        psi = 1.0;
+                    else if (d2 > 0.0)
+                        psi = fmin(1.0, (umax[v] - ui[v]) / d2);
+                    else
+                        psi = fmin(1.0, (umin[v] - ui[v]) / d2);
+                    slope[v] = fmin(slope[v], psi);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) slope[v] = grp_min8(slope[v]);
+#pragma unroll
+        for (int r = 0; r < NQL; ++r)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ul[r][v] = (R)((double)res0[r][v] * slope[v]);
+        if (s == 0) atomicAdd(&p.counters[0], 1);
+    }
+
+    // ---- CHECKSOLX (0.95) -> first order (src/reconstruct_p.f90:3245-3406) ----
+    trouble = 0;
+#pragma unroll
+    for (int r = 0; r < NQL; ++r) {
+        const int q = s + 8 * r;
+        const int f = q / p.nqm, qq = q - f * p.nqm;
+        const bool ok = cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+        if (ok) {
+            double uq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) uq[v] = ui[v] + (double)ul[r][v];
+            if (jump_test(p.gamma, wi, uq, 0.95)) trouble = 1;
+        }
+    }
+    if (grp_or8(trouble)) {
+#pragma unroll
+        for (int r = 0; r < NQL; ++r)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ul[r][v] = R(0);
+        if (s == 0 && cvalid) atomicAdd(&p.counters[1], 1);
+    }
+    // ---- ULEFT out ----
+    if (cvalid) {
+        R *out = p.uleft + (size_t)cell * p.nf * p.nqm * NV;
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            if (q < nslot) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) out[q * NV + v] = ul[r][v];
+            }
+        }
+    }
+}
+
+// [cells][ts_in][imax][ideg] (FP64, ABI) -> tiled [tile][imax][ideg][8*ts] (R)
+template <typename R>
+__global__ void k_tile_p(const double *src, R *dst, int cell0, int ncells_chunk, int ncell, int ts_in, int ts,
+                         int imax, int ideg) {
+    // dst covers the tiles of cells [cell0, cell0+ncells_chunk), cell0 % 8 == 0
+    const int TP = 8 * ts;
+    const size_t per_tile = (size_t)imax * ideg * TP;
+    const size_t ntile = ((size_t)ncells_chunk + 7) / 8;
+    const size_t total = per_tile * ntile;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t t = e / per_tile;
+        size_t r = e % per_tile;
+        const int pos = (int)(r % TP);
+        r /= TP;
+        const int k = (int)(r % ideg);
+        const int j = (int)(r / ideg);
+        const int lc = pos / ts, s = pos % ts;
+        const size_t cl = t * 8 + lc;  // cell within the chunk
+        R val = R(0);
+        if (cl < (size_t)ncells_chunk && cell0 + cl < (size_t)ncell && s < ts_in)
+            val = (R)src[((cl * ts_in + s) * imax + j) * ideg + k];
+        dst[e] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // HLLC (src/riemann.f90:221-426) on rotated FP64 states.  Primitive recovery
 // is FP64; the wave speeds and fluxes run in R.  For R=float the normal
 // momentum flux is carried relative to the face-mean pressure p0 (FP64):
@@ -656,6 +961,8 @@ struct ucns_ctx {
     int qpq = 0, qpt = 0, nf_abi = 0, nqm_abi = 0, index_base = 0;
     double gamma = 1.4, lwci1 = 1.0, eps = 1e-11;
     bool have_prestore = false, have_state = false;
+    bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
+    int ntile = 0;
     // mesh (device)
     int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
     double *frot = nullptr, *farea = nullptr, *ovol = nullptr, *minedge = nullptr;
@@ -718,6 +1025,36 @@ int launch_recon_t(ucns_ctx *ctx) {
     p.cnb = ctx->cnb;
     p.uleft = (R *)ctx->uleft;
     p.counters = ctx->counters;
+    if (ctx->tiled) {
+        ReconWParams<R> w{};
+        w.ncell = ctx->ncell;
+        w.ntile = ctx->ntile;
+        w.tiles_per_xcd = (ctx->ntile + NXCD - 1) / NXCD;
+        w.ts = ctx->ts;
+        w.nf = ctx->nf;
+        w.nqm = ctx->nqm;
+        w.qpq = ctx->qpq;
+        w.qpt = ctx->qpt;
+        w.gamma = ctx->gamma;
+        w.lwci1 = ctx->lwci1;
+        w.eps = ctx->eps;
+        w.U = ctx->U;
+        w.nbr = ctx->nbr;
+        w.P = (const R *)ctx->P;
+        w.B = (const R *)ctx->B;
+        w.mbar = (const R *)ctx->mbar;
+        w.qp = (const R *)ctx->qp;
+        w.cinfo = ctx->cinfo;
+        w.cnb = ctx->cnb;
+        w.uleft = (R *)ctx->uleft;
+        w.counters = ctx->counters;
+        const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
+        if (ctx->nf * ctx->nqm <= 24)
+            hipLaunchKernelGGL((k_recon_w<R, 19, 39, 3>), grid, block, 0, ctx->stream, w);
+        else
+            hipLaunchKernelGGL((k_recon_w<R, 19, 39, 7>), grid, block, 0, ctx->stream, w);
+        return UCNS_OK;
+    }
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);
     if (ctx->ideg == 19 && ctx->imax == 39)
         hipLaunchKernelGGL((k_recon<R, 19, 39, 7>), grid, block, 0, ctx->stream, p);
@@ -1078,8 +1415,13 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         return UCNS_EUNSUPPORTED;
     }
     ctx->ts = ts;
-    // stencil members without the cell itself: [cell][ts][imax]
-    std::vector<int32_t> nbr((size_t)nc * ts * IM, 0);
+    const bool mixed = ctx->precision == UCNS_MIXED;
+    ctx->tiled = (ctx->iweno == 1 && ID == 19 && IM == 39 && ts <= 8 && ctx->nf * ctx->nqm <= 56);
+    ctx->ntile = (nc + 7) / 8;
+    // stencil members without the cell itself
+    //   tiled: [tile][imax][8*ts], lane position 8-cell-tile local
+    //   plain: [cell][ts][imax]
+    std::vector<int32_t> nbr(ctx->tiled ? (size_t)ctx->ntile * IM * 8 * ts : (size_t)nc * ts * IM, 0);
     for (int i = 0; i < nc; ++i)
         for (int s = 0; s < pre->admis[i]; ++s) {
             const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
@@ -1093,7 +1435,12 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                     ctx->err = "stencil member out of range at cell " + std::to_string(i);
                     return UCNS_EBADARG;
                 }
-                nbr[((size_t)i * ts + s) * IM + k] = j;
+                if (ctx->tiled) {
+                    const size_t TP = 8 * (size_t)ts;
+                    nbr[((size_t)(i / 8) * IM + k) * TP + (size_t)(i % 8) * ts + s] = j;
+                } else {
+                    nbr[((size_t)i * ts + s) * IM + k] = j;
+                }
             }
         }
     int rc = dalloc(ctx, &ctx->nbr, nbr.size());
@@ -1101,12 +1448,13 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     HIP_TRY(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
     // pseudo-inverses: staged in chunks, re-laid out on the device
-    const bool mixed = ctx->precision == UCNS_MIXED;
     const size_t per_cell_abi = (size_t)TSA * IM * ID, per_cell_dev = (size_t)IM * ts * ID;
-    rc = mixed ? dalloc(ctx, (float **)&ctx->P, per_cell_dev * nc) : dalloc(ctx, (double **)&ctx->P, per_cell_dev * nc);
+    const size_t dev_elems = ctx->tiled ? (size_t)ctx->ntile * 8 * per_cell_dev : per_cell_dev * nc;
+    rc = mixed ? dalloc(ctx, (float **)&ctx->P, dev_elems) : dalloc(ctx, (double **)&ctx->P, dev_elems);
     if (rc) return rc;
     {
-        const size_t chunk_cells = std::max<size_t>(1, (size_t)(256u << 20) / (per_cell_abi * sizeof(double)));
+        size_t chunk_cells = std::max<size_t>(8, (size_t)(256u << 20) / (per_cell_abi * sizeof(double)));
+        chunk_cells -= chunk_cells % 8;
         double *stage = nullptr, *host_pin = nullptr;
         if ((rc = dalloc(ctx, &stage, chunk_cells * per_cell_abi))) return rc;
         if (hipHostMalloc((void **)&host_pin, chunk_cells * per_cell_abi * sizeof(double), hipHostMallocDefault) !=
@@ -1122,13 +1470,25 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             } else {
                 HIP_TRY(hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, ctx->stream));
             }
-            const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
-            if (mixed)
-                hipLaunchKernelGGL(k_relayout_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
-                                   (float *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
-            else
-                hipLaunchKernelGGL(k_relayout_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
-                                   (double *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+            if (ctx->tiled) {
+                const size_t out_elems = ((cn + 7) / 8) * 8 * per_cell_dev;
+                const int blocks = (int)std::min<size_t>(4096, (out_elems + 255) / 256);
+                if (mixed)
+                    hipLaunchKernelGGL(k_tile_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       (float *)ctx->P + c0 * per_cell_dev, (int)c0, (int)cn, nc, TSA, ts, IM, ID);
+                else
+                    hipLaunchKernelGGL(k_tile_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       (double *)ctx->P + c0 * per_cell_dev, (int)c0, (int)cn, nc, TSA, ts, IM, ID);
+            } else {
+                const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
+                if (mixed)
+                    hipLaunchKernelGGL(k_relayout_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       (float *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+                else
+                    hipLaunchKernelGGL(k_relayout_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       (double *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+            }
+            HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(ctx->stream));
         }
         dfree(stage);
