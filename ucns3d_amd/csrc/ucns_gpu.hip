@@ -715,48 +715,81 @@ __global__ void k_tile_p(const double *src, R *dst, int cell0, int ncells_chunk,
 
 // ---------------------------------------------------------------------------
 // HLLC (src/riemann.f90:221-426) on rotated FP64 states.  Primitive recovery
-// is FP64; the wave speeds and fluxes run in R.  For R=float the normal
-// momentum flux is carried relative to the face-mean pressure p0 (FP64):
-// rho u^2 + p - p0 has the size of the dynamic pressure, not of p, so the
-// FP32 rounding no longer scales with the ambient pressure; p0 is added back
-// in FP64 by the caller.  For R=double p0 = 0 and the arithmetic is the
-// reference's.
-template <typename R>
-__device__ inline void hllc(double gamma, const double *cl, const double *cr, R *flux, double &p0) {
+// is FP64.  R=double keeps the reference's arithmetic.  R=float evaluates the
+// wave speeds and the star-state correction in FP32 and each side's
+// physical flux F_K in FP64:
+//   F = F_K + S_K (U*_K - U_K),  K = L if S_M >= 0 else R,
+// with U*_K - U_K in its factored form, every component carrying
+// (S_M - u_K) / (S_K - S_M):
+//   mass rho d, normal momentum rho S_K d, tangential rho v_t d,
+//   energy E d + rho chi (S_M - u_K)(S_M + p/(rho (S_K - u_K))),
+//   d = (S_M - u_K)/(S_K - S_M), chi = (S_K - u_K)/(S_K - S_M),
+// so the FP32 rounding scales with the jump across the face and not with
+// the size of the state (the pressure, the energy) as it would if U* and
+// U_K were rounded separately and subtracted.
+__device__ inline void hllc64(double gamma, const double *cl, const double *cr, double *flux) {
     double wl[NV], wr[NV];
     cons2prim_d(gamma, cl, wl);
     cons2prim_d(gamma, cr, wr);
-    const bool lowp = sizeof(R) == 4;
-    p0 = lowp ? 0.5 * (wl[4] + wr[4]) : 0.0;
-    const R g = (R)gamma;
-    const R rl = (R)wl[0], uL = (R)wl[1], vl = (R)wl[2], zl = (R)wl[3], pl = (R)wl[4], el = (R)cl[4];
-    const R rr = (R)wr[0], uR = (R)wr[1], vr = (R)wr[2], zr = (R)wr[3], pr = (R)wr[4], er = (R)cr[4];
-    const R dpl = lowp ? (R)(wl[4] - p0) : pl, dpr = lowp ? (R)(wr[4] - p0) : pr;
-    const R dprl = lowp ? (R)(wr[4] - wl[4]) : (pr - pl);
-    const R ccl = sqrt(g * pl / rl), ccr = sqrt(g * pr / rr);
-    const R srl = sqrt(rl), srr = sqrt(rr);
-    const R cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
-                       (R(0.5) * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
-    const R uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
-    R sl = fmin(uL - ccl, uu2 - cc2), sr = fmax(uR + ccr, uu2 + cc2);
-    sl = fmin(sl, R(0));
-    sr = fmax(sr, R(0));
-    const R sm = (dprl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
-    const R cl_[NV] = {(R)cl[0], (R)cl[1], (R)cl[2], (R)cl[3], el};
-    const R cr_[NV] = {(R)cr[0], (R)cr[1], (R)cr[2], (R)cr[3], er};
-    const R fl[NV] = {rl * uL, (rl * (uL * uL)) + dpl, rl * uL * vl, rl * uL * zl, uL * (el + pl)};
-    const R fr[NV] = {rr * uR, (rr * (uR * uR)) + dpr, rr * uR * vr, rr * uR * zr, uR * (er + pr)};
-    const R mul = rl * ((sl - uL) / (sl - sm));
-    const R mur = rr * ((sr - uR) / (sr - sm));
-    const R lastl = (el / rl) + ((sm - uL) * (sm + ((pl) / (rl * (sl - uL)))));
-    const R lastr = (er / rr) + ((sm - uR) * (sm + ((pr) / (rr * (sr - uR)))));
-    const R usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
-    const R usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
-    const R s = copysign(R(1), sm);
+    const double rl = wl[0], uL = wl[1], vl = wl[2], zl = wl[3], pl = wl[4], el = cl[4];
+    const double rr = wr[0], uR = wr[1], vr = wr[2], zr = wr[3], pr = wr[4], er = cr[4];
+    const double ccl = sqrt(gamma * pl / rl), ccr = sqrt(gamma * pr / rr);
+    const double srl = sqrt(rl), srr = sqrt(rr);
+    const double cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
+                            (0.5 * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
+    const double uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
+    double sl = fmin(uL - ccl, uu2 - cc2), sr = fmax(uR + ccr, uu2 + cc2);
+    sl = fmin(sl, 0.0);
+    sr = fmax(sr, 0.0);
+    const double sm = (pr - pl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
+    const double fl[NV] = {rl * uL, (rl * (uL * uL)) + pl, rl * uL * vl, rl * uL * zl, uL * (el + pl)};
+    const double fr[NV] = {rr * uR, (rr * (uR * uR)) + pr, rr * uR * vr, rr * uR * zr, uR * (er + pr)};
+    const double mul = rl * ((sl - uL) / (sl - sm));
+    const double mur = rr * ((sr - uR) / (sr - sm));
+    const double lastl = (el / rl) + ((sm - uL) * (sm + ((pl) / (rl * (sl - uL)))));
+    const double lastr = (er / rr) + ((sm - uR) * (sm + ((pr) / (rr * (sr - uR)))));
+    const double usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
+    const double usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
+    const double s = copysign(1.0, sm);
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-        flux[k] = (((R(1) + s) / R(2)) * (fl[k] + sl * (usl[k] - cl_[k]))) +
-                  (((R(1) - s) / R(2)) * (fr[k] + sr * (usr[k] - cr_[k])));
+        flux[k] = (((1.0 + s) / 2.0) * (fl[k] + sl * (usl[k] - cl[k]))) +
+                  (((1.0 - s) / 2.0) * (fr[k] + sr * (usr[k] - cr[k])));
+}
+
+__device__ inline void hllc32(double gamma, const double *cl, const double *cr, double *flux) {
+    double wl[NV], wr[NV];
+    cons2prim_d(gamma, cl, wl);
+    cons2prim_d(gamma, cr, wr);
+    const float g = (float)gamma;
+    const float rl = (float)wl[0], uL = (float)wl[1], pl = (float)wl[4];
+    const float rr = (float)wr[0], uR = (float)wr[1], pr = (float)wr[4];
+    const float ccl = sqrtf(g * pl / rl), ccr = sqrtf(g * pr / rr);
+    const float srl = sqrtf(rl), srr = sqrtf(rr);
+    const float cc2 = sqrtf((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
+                            (0.5f * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
+    const float uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
+    float sl = fminf(uL - ccl, uu2 - cc2), sr = fmaxf(uR + ccr, uu2 + cc2);
+    sl = fminf(sl, 0.0f);
+    sr = fmaxf(sr, 0.0f);
+    const float dp = (float)(wr[4] - wl[4]);
+    const float sm = (dp + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
+    const bool left = !(copysignf(1.0f, sm) < 0.0f);
+    const double *w = left ? wl : wr;
+    const double *c = left ? cl : cr;
+    const float sk = left ? sl : sr;
+    const float rk = (float)w[0], uk = (float)w[1], pk = (float)w[4];
+    // physical flux of the upwind side, FP64
+    const double fk[NV] = {w[0] * w[1], (w[0] * (w[1] * w[1])) + w[4], w[0] * w[1] * w[2], w[0] * w[1] * w[3],
+                           w[1] * (c[4] + w[4])};
+    // star-state correction, FP32, factored
+    const float den = sk - sm;
+    const float d = (sm - uk) / den;
+    const float chi = (sk - uk) / den;
+    const float du[NV] = {rk * d, rk * sk * d, rk * (float)w[2] * d, rk * (float)w[3] * d,
+                          (float)c[4] * d + rk * chi * (sm - uk) * (sm + pk / (rk * (sk - uk)))};
+#pragma unroll
+    for (int k = 0; k < NV; ++k) flux[k] = fk[k] + (double)(sk * du[k]);
 }
 
 template <typename R>
@@ -805,10 +838,11 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
                                c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3], (-s1) * a[1] + c1 * a[2], a[4]};
         const double ra[NV] = {bq[0], c1 * s2 * bq[1] + s1 * s2 * bq[2] + c2 * bq[3],
                                c1 * c2 * bq[1] + s1 * c2 * bq[2] + (-s2) * bq[3], (-s1) * bq[1] + c1 * bq[2], bq[4]};
-        R h[NV];
-        double p0;
-        hllc<R>(p.gamma, la, ra, h, p0);
-        const double hd[NV] = {(double)h[0], (double)h[1] + p0, (double)h[2], (double)h[3], (double)h[4]};
+        double hd[NV];
+        if (sizeof(R) == 8)
+            hllc64(p.gamma, la, ra, hd);
+        else
+            hllc32(p.gamma, la, ra, hd);
         // ROTATEB (src/grid_t.f90:4261-4305)
         const double rb[NV] = {hd[0], c1 * s2 * hd[1] + c1 * c2 * hd[2] + (-s1) * hd[3],
                                s1 * s2 * hd[1] + s1 * c2 * hd[2] + c1 * hd[3], c2 * hd[1] + (-s2) * hd[2], hd[4]};
