@@ -458,11 +458,12 @@ __host__ __device__ constexpr int pidx(int ideg, int k, int l) {
     return k <= l ? k * ideg - k * (k - 1) / 2 + (l - k) : l * ideg - l * (l - 1) / 2 + (k - l);
 }
 
-template <typename R, int IDEG, int IMAX, int NQL>
-__global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
+template <typename R, int IDEG, int IMAX, int NQL, int D>
+__device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
     __shared__ R sB[8][NP];
     __shared__ R sG0[8][IDEG * NV];
+    __shared__ int sN[IMAX * 64];  // stencil members of the tile, [j][pos]
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
@@ -478,13 +479,18 @@ __global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
     }
-    // indicators of the tile's cells -> LDS (one contiguous block)
+    // indicators and stencil members of the tile -> LDS (contiguous blocks)
     {
         const R *Bt = p.B + (size_t)tile * 8 * NP;
         const int nvc = min(8, p.ncell - tile * 8);
         for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP;
+        for (int e = lane; e < IMAX * TP; e += WAVE) sN[e] = nt[e];
     }
+    __syncthreads();
     // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
+    // D members are in flight (their states and pseudo-inverse columns in
+    // registers) while one is accumulated.
     R g[IDEG][NV];
 #pragma unroll
     for (int k = 0; k < IDEG; ++k)
@@ -492,19 +498,42 @@ __global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
         for (int v = 0; v < NV; ++v) g[k][v] = R(0);
     if (act) {
         const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
-        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + pos;
-#pragma unroll 2
-        for (int j = 0; j < IMAX; ++j) {
-            const int n = nt[j * TP];
-            R du[NV];
+        R pb[D][IDEG];
+        double ub[D][NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) du[v] = (R)(p.U[(size_t)n * NV + v] - ui[v]);
-            const R *Pj = Pt + (size_t)j * IDEG * TP;
+        for (int d = 0; d < D; ++d) {
+            const int n = sN[d * TP + pos];
 #pragma unroll
-            for (int k = 0; k < IDEG; ++k) {
-                const R pk = Pj[k * TP];
+            for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+            const R *Pj = Pt + (size_t)d * IDEG * TP;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) g[k][v] += pk * du[v];
+            for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+        }
+#pragma unroll 1
+        for (int j0 = 0; j0 < IMAX; j0 += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int j = j0 + d;
+                if (j < IMAX) {
+                    R du[NV], pc[IDEG];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) du[v] = (R)(ub[d][v] - ui[v]);
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) pc[k] = pb[d][k];
+                    const int jn = j + D;
+                    if (jn < IMAX) {
+                        const int n = sN[jn * TP + pos];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+                        const R *Pj = Pt + (size_t)jn * IDEG * TP;
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+                    }
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+                }
             }
         }
     }
@@ -561,42 +590,46 @@ __global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
         for (int v = 0; v < NV; ++v) g[k][v] = grp_sum8(om[v] * g[k][v]);
 
     // ---- face points: deviation from the average + CHECKSOL (0.85) ----
+    // Values are stored as they are formed.  A cell that passes CHECKSOL
+    // cannot fail CHECKSOLX (its 0.95 jumps imply the 0.85 ones), so only
+    // the MUSCL branch below re-tests and overwrites its face points.
     const int nslot = ifca * p.nqm;
     const R *qpc = p.qp + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * 3;
-    R mb[IDEG];
-#pragma unroll
-    for (int d = 0; d < IDEG; ++d) mb[d] = cvalid ? p.mbar[(size_t)cell * IDEG + d] : R(0);
+    const R *mb = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
+    R *out = p.uleft + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * NV;
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
-    R ul[NQL][NV];
     int trouble = 0;
-#pragma unroll
-    for (int r = 0; r < NQL; ++r) {
-        const int q = s + 8 * r;
+#pragma unroll 1
+    for (int q = s; q < NQL * 8; q += 8) {
         const int f = q / p.nqm, qq = q - f * p.nqm;
         const bool ok = cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+        if (!ok) {
+            if (cvalid && q < nslot) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ul[r][v] = R(0);
-        if (ok) {
-            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
-            R phi[IDEG];
-            basis_at<R, IDEG>(x, mb, phi);
-            double uq[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                R res = R(0);
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) res += phi[k] * g[k][v];
-                ul[r][v] = res;
-                uq[v] = ui[v] + (double)res;
+                for (int v = 0; v < NV; ++v) out[q * NV + v] = R(0);
             }
-            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+            continue;
         }
+        R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+        R phi[IDEG];
+        basis_at<R, IDEG>(x, mb, phi);
+        double uq[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R res = R(0);
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) res += phi[k] * g[k][v];
+            out[q * NV + v] = res;
+            uq[v] = ui[v] + (double)res;
+        }
+        if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
     }
     const int any_trouble = grp_or8(trouble);  // every lane takes part in the shuffle
     const bool reduce = cvalid && (!full || any_trouble);
 
     // ---- MUSCL + Barth-Jespersen on the central polynomial (src/reconstruct_p.f90:2044-2280) ----
+    // then CHECKSOLX (0.95) -> first order (src/reconstruct_p.f90:3245-3406)
     if (reduce) {
         double umin[NV], umax[NV];
 #pragma unroll
@@ -646,46 +679,55 @@ __global__ __launch_bounds__(WAVE) void k_recon_w(ReconWParams<R> p) {
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) slope[v] = grp_min8(slope[v]);
+        int tr2 = 0;
 #pragma unroll
-        for (int r = 0; r < NQL; ++r)
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            const int f = q / p.nqm, qq = q - f * p.nqm;
+            const bool ok = q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ul[r][v] = (R)((double)res0[r][v] * slope[v]);
-        if (s == 0) atomicAdd(&p.counters[0], 1);
-    }
-
-    // ---- CHECKSOLX (0.95) -> first order (src/reconstruct_p.f90:3245-3406) ----
-    trouble = 0;
+            for (int v = 0; v < NV; ++v) res0[r][v] = (R)((double)res0[r][v] * slope[v]);
+            if (ok) {
+                double uq[NV];
 #pragma unroll
-    for (int r = 0; r < NQL; ++r) {
-        const int q = s + 8 * r;
-        const int f = q / p.nqm, qq = q - f * p.nqm;
-        const bool ok = cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
-        if (ok) {
-            double uq[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) uq[v] = ui[v] + (double)ul[r][v];
-            if (jump_test(p.gamma, wi, uq, 0.95)) trouble = 1;
+                for (int v = 0; v < NV; ++v) uq[v] = ui[v] + (double)res0[r][v];
+                if (jump_test(p.gamma, wi, uq, 0.95)) tr2 = 1;
+            }
         }
-    }
-    if (grp_or8(trouble)) {
-#pragma unroll
-        for (int r = 0; r < NQL; ++r)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ul[r][v] = R(0);
-        if (s == 0 && cvalid) atomicAdd(&p.counters[1], 1);
-    }
-    // ---- ULEFT out ----
-    if (cvalid) {
-        R *out = p.uleft + (size_t)cell * p.nf * p.nqm * NV;
+        const bool first = grp_or8(tr2) != 0;
 #pragma unroll
         for (int r = 0; r < NQL; ++r) {
             const int q = s + 8 * r;
             if (q < nslot) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) out[q * NV + v] = ul[r][v];
+                for (int v = 0; v < NV; ++v) out[q * NV + v] = first ? R(0) : res0[r][v];
             }
         }
+        if (s == 0) {
+            atomicAdd(&p.counters[0], 1);
+            if (first) atomicAdd(&p.counters[1], 1);
+        }
     }
+}
+
+// FP64: registers for the 19x5 coefficients dominate, one wave per SIMD;
+// FP32: the compiler is asked for WAVES32 waves per SIMD.
+#ifndef UCNS_WAVES32
+#define UCNS_WAVES32 1
+#endif
+#ifndef UCNS_DEPTH64
+#define UCNS_DEPTH64 1
+#endif
+#ifndef UCNS_DEPTH32
+#define UCNS_DEPTH32 2
+#endif
+template <int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE) void k_recon_w64(ReconWParams<double> p) {
+    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64>(p);
+}
+template <int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE, UCNS_WAVES32) void k_recon_w32(ReconWParams<float> p) {
+    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32>(p);
 }
 
 // [cells][ts_in][imax][ideg] (FP64, ABI) -> tiled [tile][imax][ideg][8*ts] (R)
@@ -1083,10 +1125,17 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.uleft = (R *)ctx->uleft;
         w.counters = ctx->counters;
         const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
-        if (ctx->nf * ctx->nqm <= 24)
-            hipLaunchKernelGGL((k_recon_w<R, 19, 39, 3>), grid, block, 0, ctx->stream, w);
-        else
-            hipLaunchKernelGGL((k_recon_w<R, 19, 39, 7>), grid, block, 0, ctx->stream, w);
+        if constexpr (sizeof(R) == 8) {
+            if (ctx->nf * ctx->nqm <= 24)
+                hipLaunchKernelGGL((k_recon_w64<19, 39, 3>), grid, block, 0, ctx->stream, w);
+            else
+                hipLaunchKernelGGL((k_recon_w64<19, 39, 7>), grid, block, 0, ctx->stream, w);
+        } else {
+            if (ctx->nf * ctx->nqm <= 24)
+                hipLaunchKernelGGL((k_recon_w32<19, 39, 3>), grid, block, 0, ctx->stream, w);
+            else
+                hipLaunchKernelGGL((k_recon_w32<19, 39, 7>), grid, block, 0, ctx->stream, w);
+        }
         return UCNS_OK;
     }
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);

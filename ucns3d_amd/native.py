@@ -104,7 +104,7 @@ _libs: dict[str, C.CDLL] = {}
 def _load(name: str, api: dict) -> C.CDLL:
     if name in _libs:
         return _libs[name]
-    path = os.path.join(LIBDIR, name)
+    path = name if os.path.isabs(name) else os.path.join(LIBDIR, name)
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built; run `python -m ucns3d_amd.build` (or __graft_entry__.build())")
     lib = C.CDLL(path)
@@ -120,8 +120,9 @@ def prep_lib() -> C.CDLL:
     return _load("libucnsprep.so", PREP_API)
 
 
-def gpu_lib() -> C.CDLL:
-    return _load("libucnsgpu.so", GPU_API)
+def gpu_lib(path: str | None = None) -> C.CDLL:
+    """The HIP library; `path` selects another build of it (an absolute path)."""
+    return _load(path or "libucnsgpu.so", GPU_API)
 
 
 def gpu_lib_path() -> str:

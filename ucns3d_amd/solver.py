@@ -25,10 +25,10 @@ class UcnsError(RuntimeError):
 
 
 class Solver:
-    def __init__(self, case, precision: int = UCNS_FP64):
+    def __init__(self, case, precision: int = UCNS_FP64, lib_path: str | None = None):
         self.case = case
         self.precision = precision
-        self.L = native.gpu_lib()
+        self.L = native.gpu_lib(lib_path)
         self._m, self._s, self._p = case.structs(precision)
         h = C.c_void_p()
         rc = self.L.ucns_create(C.byref(self._m), C.byref(self._s), C.byref(h))
