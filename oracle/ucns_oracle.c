@@ -125,6 +125,20 @@ static double *uleft_at(const ctx_t *c, double *ul, int64_t i, int f, int q) {
     return ul + (((size_t)i * c->nf + f) * c->nqm + q) * NV;
 }
 
+/* omega_s = lambda_s / (eps + beta_s)^4, normalised; lambda = lwci1 for the
+ * central stencil, 1 for the directional ones (reconstruct_p.f90:1031-1049) */
+static void weno_weights(int admis, const double *beta, double lwci1, double eps, double *omega) {
+    double wt[16], sum = 0.0;
+    for (int ll = 0; ll < admis; ++ll) {
+        double lam = (ll == 0) ? lwci1 : 1.0;
+        double x = eps + beta[ll];
+        double x2 = x * x;
+        wt[ll] = lam / (x2 * x2);
+    }
+    for (int ll = 0; ll < admis; ++ll) sum += wt[ll];
+    for (int ll = 0; ll < admis; ++ll) omega[ll] = wt[ll] / sum;
+}
+
 /* one cell of ARBITRARY_ORDER: LEAST_SQUARES + WENOWEIGHTS (CP_RECONSTRUCTION) */
 static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, double *grad /*ts*ideg*5*/,
                       double *work) {
@@ -155,7 +169,7 @@ static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, do
     const double *B = c->p->indicator + (size_t)i * ideg * ideg;
     const double eps = c->s->weno_eps;
     for (int v = 0; v < NV; ++v) {
-        double beta[8], wt[8], sum = 0.0;
+        double beta[8], wt[8];
         for (int ll = 0; ll < admis; ++ll) {
             for (int a = 0; a < ideg; ++a) {
                 double acc = 0.0;
@@ -166,14 +180,8 @@ static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, do
             for (int a = 0; a < ideg; ++a) dot += grad[((size_t)ll * ideg + a) * NV + v] * bg[a];
             beta[ll] = dot;
         }
-        for (int ll = 0; ll < admis; ++ll) {
-            double lam = (ll == 0) ? c->s->lwci1 : 1.0;
-            double x = eps + beta[ll];
-            double x2 = x * x;
-            wt[ll] = lam / (x2 * x2);
-        }
-        for (int ll = 0; ll < admis; ++ll) sum += wt[ll];
-        for (int ll = 0; ll < admis; ++ll) omega[ll][v] = wt[ll] / sum;
+        weno_weights(admis, beta, c->s->lwci1, eps, wt);
+        for (int ll = 0; ll < admis; ++ll) omega[ll][v] = wt[ll];
     }
     /* extrapolation to the face Gauss points, stencil by stencil */
     const double *integ = c->p->integ_basis + (size_t)i * ideg;
@@ -211,6 +219,14 @@ static int trouble_test(const ctx_t *c, const double *u, const double *ul, int64
     return 0;
 }
 
+/* Barth-Jespersen psi of one face point value usol (reconstruct_p.f90:3439-3453) */
+static double barth_psi(double ui, double umin, double umax, double usol) {
+    double d2 = usol - ui;
+    if (fabs(d2) <= 0.0) return 1.0;
+    if (d2 > 0.0) return fmin(1.0, (umax - ui) / d2);
+    return fmin(1.0, (umin - ui) / d2);
+}
+
 /* MUSCL on one cell: FIND_BOUNDS (strict) + unlimited central polynomial +
  * Barth-Jespersen (reconstruct_p.f90:1821-2039, 2044-2280, 3421-3453) */
 static void muscl_cell(const ctx_t *c, const double *u, int64_t i, double *ul, const double *grad, double *work) {
@@ -240,15 +256,7 @@ static void muscl_cell(const ctx_t *c, const double *u, int64_t i, double *ul, c
                 double res = 0.0;
                 for (int k = 0; k < ideg; ++k) res += phi[k] * grad[(size_t)k * NV + v];
                 double usol = ui[v] + res;
-                double d2 = usol - ui[v];
-                double psi;
-                if (fabs(d2) <= 0.0)
-                    psi = 1.0;
-                else if (d2 > 0.0)
-                    psi = fmin(1.0, (umax[v] - ui[v]) / d2);
-                else
-                    psi = fmin(1.0, (umin[v] - ui[v]) / d2);
-                slope[v] = fmin(slope[v], psi);
+                slope[v] = fmin(slope[v], barth_psi(ui[v], umin[v], umax[v], usol));
             }
         }
     /* pass 2: ULEFT = U_i + (USOL - U_i) * SLOPE */
@@ -485,24 +493,13 @@ int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_
     return rc;
 }
 
-/* WENO weights of one cell and variable from given indicators (for the
- * closed-form tests of reconstruct_p.f90:1031-1049) */
+/* WENO weights of one cell and variable from given indicators (the closed-form
+ * tests of reconstruct_p.f90:1031-1049 call this through the same helper the
+ * reconstruction uses) */
 void oracle_weno_weights(int admis, const double *beta, double lwci1, double eps, double *omega) {
-    double wt[16], sum = 0.0;
-    for (int ll = 0; ll < admis; ++ll) {
-        double lam = (ll == 0) ? lwci1 : 1.0;
-        double x = eps + beta[ll];
-        double x2 = x * x;
-        wt[ll] = lam / (x2 * x2);
-        sum += wt[ll];
-    }
-    for (int ll = 0; ll < admis; ++ll) omega[ll] = wt[ll] / sum;
+    weno_weights(admis, beta, lwci1, eps, omega);
 }
 
-/* Barth-Jespersen psi of one face point (reconstruct_p.f90:3439-3453) */
-double oracle_barth(double ui, double umin, double umax, double usol) {
-    double d2 = usol - ui;
-    if (fabs(d2) <= 0.0) return 1.0;
-    if (d2 > 0.0) return fmin(1.0, (umax - ui) / d2);
-    return fmin(1.0, (umin - ui) / d2);
-}
+/* Barth-Jespersen psi of one face point (reconstruct_p.f90:3439-3453), the
+ * helper the MUSCL reconstruction uses */
+double oracle_barth(double ui, double umin, double umax, double usol) { return barth_psi(ui, umin, umax, usol); }

@@ -1,0 +1,196 @@
+"""Test-side geometry written independently of the preprocessor: cell
+vertices, quadrature over tetrahedra (collapsed Gauss-Legendre) and
+trilinear hexahedra (tensor Gauss-Legendre), local-frame maps and exact
+cell averages of polynomials.  Used to pin the preprocessor's PRESTORE
+arrays and the oracle against closed forms."""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+
+from ucns3d_amd.case import HEX, TET
+
+def monomial_exponents(ideg: int):
+    """exponents of the reference's monomial basis (BASIS_REC, src/basis.f90:27-102):
+    degree by degree; within a degree, z-power ascending, then y-power ascending"""
+    out = []
+    p = 1
+    while len(out) < ideg:
+        for cz in range(0, p + 1):
+            for b in range(0, p - cz + 1):
+                out.append((p - cz - b, b, cz))
+        p += 1
+    return out[:ideg]
+
+
+def gauss_legendre01(n: int):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def tet_rule(n: int):
+    """points (barycentric weights for vertices 1..3) and weights summing to 1/6
+    on the unit tetrahedron, by the collapsed (Duffy) map of [0,1]^3."""
+    g, w = gauss_legendre01(n)
+    pts, wts = [], []
+    for (a, wa), (b, wb), (c, wc) in itertools.product(zip(g, w), repeat=3):
+        xi = a
+        eta = b * (1 - a)
+        zeta = c * (1 - a) * (1 - b)
+        pts.append((xi, eta, zeta))
+        wts.append(wa * wb * wc * (1 - a) ** 2 * (1 - b))
+    return np.array(pts), np.array(wts)
+
+
+HEX_CORNERS = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)],
+                       dtype=float)
+
+
+def hex_map(v, xi):
+    """trilinear map of the unit cube through the 8 vertices v (VTK order);
+    returns points and |det J|"""
+    xi = np.atleast_2d(xi)
+    N = np.ones((len(xi), 8))
+    dN = np.zeros((len(xi), 8, 3))
+    for a, c in enumerate(HEX_CORNERS):
+        f = [xi[:, d] if c[d] else 1 - xi[:, d] for d in range(3)]
+        df = [np.ones(len(xi)) if c[d] else -np.ones(len(xi)) for d in range(3)]
+        N[:, a] = f[0] * f[1] * f[2]
+        dN[:, a, 0] = df[0] * f[1] * f[2]
+        dN[:, a, 1] = f[0] * df[1] * f[2]
+        dN[:, a, 2] = f[0] * f[1] * df[2]
+    x = N @ v
+    J = np.einsum("pad,ae->ped", dN, v)
+    return x, np.abs(np.linalg.det(J))
+
+
+class Mesh:
+    """numpy views of one Case's arrays"""
+
+    def __init__(self, case):
+        self.case = case
+        nc = case.ncell
+        self.nc = nc
+        self.L = np.array(case.period)
+        self.xyz = case.array("xyz").reshape(-1, 3)
+        self.cnodes = case.array("cell_nodes").reshape(nc, 8)
+        self.shape = case.array("cell_shape")
+        self.centre = case.array("centre").reshape(nc, 3)
+        self.vol = case.array("volume")
+        self.minedge = case.array("minedge")
+        self.surf = case.array("surf").reshape(nc, 6)
+        self.normal = case.array("normal").reshape(nc, 6, 3)
+        self.ineigh = case.array("ineigh").reshape(nc, 6)
+        self.ineighn = case.array("ineighn").reshape(nc, 6)
+        self.fnodes = case.array("face_nodes").reshape(nc, 6, 4)
+        self.frame = case.array("frame").reshape(nc, 12)
+
+    def nfaces(self, i):
+        return 6 if self.shape[i] == HEX else 4
+
+    def verts(self, i, shift=None):
+        n = 8 if self.shape[i] == HEX else 4
+        v = self.xyz[self.cnodes[i, :n]].copy()
+        if shift is not None:
+            v += shift
+        return v
+
+    def unwrap_shift(self, ref, p):
+        """whole-period shift moving p within half a period of ref"""
+        d = p - ref
+        return -np.round(d / self.L) * self.L
+
+    def cell_rule(self, i, n=4, shift=None):
+        """global quadrature points and weights (sum = volume) of cell i"""
+        v = self.verts(i, shift)
+        if self.shape[i] == TET:
+            p, w = tet_rule(n)
+            A = np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]], axis=1)
+            return v[0] + p @ A.T, w * abs(np.linalg.det(A))
+        g, gw = gauss_legendre01(n)
+        xi = np.array(list(itertools.product(g, g, g)))
+        ww = np.array([a * b * c for a, b, c in itertools.product(gw, gw, gw)])
+        x, dj = hex_map(v, xi)
+        return x, ww * dj
+
+    def rules(self, n=4):
+        """quadrature points [nc, nq, 3] and weights [nc, nq] of every cell
+        (meshes of one cell shape)"""
+        key = ("rules", n)
+        if key not in self.__dict__:
+            pts, wts = zip(*(self.cell_rule(i, n) for i in range(self.nc)))
+            self.__dict__[key] = (np.stack(pts), np.stack(wts))
+        return self.__dict__[key]
+
+    def member_means(self, i, members, exps, n=4):
+        """local-frame (cell i) means of the monomials over each member,
+        members unwrapped to lie next to cell i"""
+        pts, wts = self.rules(n)
+        members = np.asarray(members)
+        sh = self.centre[members] - self.centre[i]
+        shift = -np.round(sh / self.L) * self.L
+        x = pts[members] + shift[:, None, :]
+        y = self.to_local(i, x.reshape(-1, 3)).reshape(len(members), -1, 3)
+        w = wts[members]
+        out = np.zeros((len(members), len(exps)))
+        for k, (a, b, c) in enumerate(exps):
+            out[:, k] = (w * y[..., 0] ** a * y[..., 1] ** b * y[..., 2] ** c).sum(1) / w.sum(1)
+        return out
+
+    def design_matrix(self, i, members, exps):
+        """A_jk = mean over member j of (m_k - mbar_k(cell i)), and the member volumes"""
+        mb = self.member_means(i, [i], exps)[0]
+        return self.member_means(i, members, exps) - mb, self.vol[np.asarray(members)]
+
+    def to_local(self, i, x):
+        J = self.frame[i, :9].reshape(3, 3)
+        o = self.frame[i, 9:]
+        return (np.atleast_2d(x) - o) @ J.T
+
+    def to_global(self, i, y):
+        J = self.frame[i, :9].reshape(3, 3)
+        o = self.frame[i, 9:]
+        return np.atleast_2d(y) @ np.linalg.inv(J).T + o
+
+    def face_verts(self, i, f):
+        """face vertices in the stored (adopted) order, unwrapped to lie next
+        to cell i (a periodic face adopted from the neighbour carries that
+        neighbour's node ids on the far side of the box)"""
+        ids = [k for k in self.fnodes[i, f] if k >= 0]
+        v = self.xyz[ids]
+        return v + self.unwrap_shift(self.centre[i], v.mean(0))
+
+
+def monomials(y, exps):
+    y = np.atleast_2d(y)
+    return np.stack([y[:, 0] ** a * y[:, 1] ** b * y[:, 2] ** c for a, b, c in exps], axis=1)
+
+
+class Cubic:
+    """a random polynomial of total degree <= deg in global coordinates"""
+
+    def __init__(self, rng, deg, centre, scale):
+        self.terms = [(a, b, c) for a in range(deg + 1) for b in range(deg + 1) for c in range(deg + 1)
+                      if a + b + c <= deg]
+        self.coef = rng.standard_normal(len(self.terms))
+        self.centre = np.asarray(centre, dtype=float)
+        self.scale = float(scale)
+
+    def __call__(self, x):
+        y = (np.atleast_2d(x) - self.centre) / self.scale
+        out = np.zeros(len(y))
+        for (a, b, c), k in zip(self.terms, self.coef):
+            out += k * y[:, 0] ** a * y[:, 1] ** b * y[:, 2] ** c
+        return out
+
+
+def cell_average(mesh: Mesh, i: int, fn, n=4):
+    x, w = mesh.cell_rule(i, n)
+    return float((fn(x) * w).sum() / w.sum())
+
+
+def factorial_ratio_tri(a, b, c):
+    """mean over a triangle of l1^a l2^b l3^c (barycentric), = 2 a! b! c! / (a+b+c+2)!"""
+    return 2.0 * math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 2)

@@ -1803,8 +1803,12 @@ int ucns_reset_stats(ucns_ctx *ctx) {
 
 void ucns_destroy(ucns_ctx *ctx) {
     if (!ctx) return;
+    if (!ctx->stream) {  // rejected before any device resource was created
+        delete ctx;
+        return;
+    }
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     release_prestore(ctx);
     dfree(ctx->cinfo);
     dfree(ctx->cnb);
