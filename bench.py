@@ -48,6 +48,29 @@ def recon_bytes_per_cell(case, precision: int) -> float:
     return by
 
 
+def recon_kernel_name(case, precision: int) -> str:
+    """the reconstruction kernel the library launches for this case"""
+    nql = 3 if case.nface_max * max(case.qp_quad, case.qp_tri) <= 24 or case.shape == 2 else 7
+    return f"k_recon_{'w32' if precision else 'w64'}<19,39,{nql}>"
+
+
+def committed_traffic(kernel: str, config: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy calibration) and committed under
+    profiles/ as 'traffic <kernel> <config> <bytes>' lines; None if absent"""
+    if os.environ.get("UCNS_RING") == "1":
+        return None
+    d = os.path.join(REPO, "profiles")
+    for fn in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        if not fn.endswith(".txt"):
+            continue
+        for line in open(os.path.join(d, fn)):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic" and f[1] == kernel and f[2] == config:
+                return float(f[3])
+    return None
+
+
 def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.ndarray):
     import torch
     from ucns3d_amd.solver import Solver
@@ -99,6 +122,48 @@ def cpu_baseline(spec_key: str, seconds: float):
                       f"{el:.1f}s, oracle/ucns_oracle.c with OpenMP"}
 
 
+def rel_errors(a: np.ndarray, b: np.ndarray) -> dict:
+    """per conserved variable: relative L2 and relative Linf of a against b"""
+    d = a - b
+    l2 = np.sqrt((d ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))
+    linf = np.abs(d).max(0) / np.abs(b).max(0)
+    return {"rel_l2": l2.tolist(), "rel_linf": linf.tolist()}
+
+
+def accuracy(case, u0: np.ndarray, steps: list) -> dict:
+    """FP64 and mixed contexts side by side from the same state with the same
+    fixed dt (the FP64 CFL step of the initial state), compared at each
+    requested step count; the CPU oracle joins at step 1."""
+    from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+    from ucns3d_amd.solver import Solver
+
+    fp = Solver(case, UCNS_FP64)
+    mx = Solver(case, UCNS_MIXED)
+    fp.set_state(u0)
+    mx.set_state(u0)
+    dt = fp.compute_dt(case.spec.cfl)
+    out = {"dt": dt, "steps": {}}
+    done = 0
+    for n in sorted(steps):
+        fp.rk3_steps(n - done, dt_fixed=dt)
+        mx.rk3_steps(n - done, dt_fixed=dt)
+        done = n
+        a, b = mx.get_state(), fp.get_state()
+        rec = {"mixed_vs_fp64": rel_errors(a, b)}
+        if n == 1:
+            from oracle.oracle import Oracle
+            t0 = time.time()
+            ref, _ = Oracle(case).rk3(u0, 1, dt_fixed=dt)
+            rec["fp64_vs_oracle"] = rel_errors(b, ref)
+            rec["mixed_vs_oracle"] = rel_errors(a, ref)
+            log(f"  oracle step: {time.time() - t0:.1f}s")
+        out["steps"][str(n)] = rec
+        log(f"step {n}: {rec}")
+    fp.close()
+    mx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +173,11 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--accuracy", default="",
+                    help="comma-separated step counts: report the mixed path's per-variable relative L2 and Linf "
+                         "against the FP64 path (and, at step 1, the CPU oracle) at each, then exit")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the whole mesh as one partition on one GPU; "
@@ -122,11 +192,25 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = 1 << 28
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+        return 0
     spec = PRESETS[args.config]
     t0 = time.time()
     case = Case(spec)
     u0 = case.init_state()
     log(f"{spec.name}: {case.ncell} cells, preprocessing {time.time() - t0:.1f}s")
+
+    if args.accuracy:
+        rep = accuracy(case, u0, [int(x) for x in args.accuracy.split(",")])
+        print(json.dumps({"workload": spec.name, "ncell": case.ncell, "accuracy": rep}), flush=True)
+        return 0
 
     fp = run_path(case, UCNS_FP64, args.steps, args.warmup, spec.cfl, u0)
     ms_step = fp["elapsed"] / args.steps * 1e3
@@ -144,9 +228,11 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": spec.name, "ncell": case.ncell, "cfl": spec.cfl, "stages_per_step": 3,
                    "wenocentralweight": spec.lwci1, "seed": spec.seed, "jitter": spec.jitter},
-        "roofline": {"bound": "hbm", "kernel": "k_recon (fused LSQ/WENO/extrapolation/CHECKSOL)",
+        "roofline": {"bound": "hbm", "kernel": recon_kernel_name(case, UCNS_FP64),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_cell": bytes_cell, "kernel_ms": rk_ms},
+                     "traffic": committed_traffic(recon_kernel_name(case, UCNS_FP64), args.config),
+                     "algorithmic_bytes": bytes_cell * case.ncell, "bytes_per_cell": bytes_cell,
+                     "kernel_ms": rk_ms},
         "kernels_ms": {"recon": rk_ms, "flux": st["ms_flux"] / max(st["n_flux"], 1),
                        "update": st["ms_update"] / max(st["n_update"], 1)},
         "conservation_fp64": fp["conservation"],
@@ -164,6 +250,8 @@ def main() -> int:
             f"({ach_mx:.0f} GB/s), rel L2 vs fp64 after {args.warmup + args.steps} steps {err}")
         out["mixed"] = {"value": v_mx, "ms_per_step": mx["elapsed"] / args.steps * 1e3, "speedup_vs_fp64": v_mx / value,
                         "roofline_frac": ach_mx / HBM_PEAK_GBS, "recon_ms": rk_mx, "bytes_per_cell": bytes_mx,
+                        "kernel": recon_kernel_name(case, UCNS_MIXED),
+                        "traffic": committed_traffic(recon_kernel_name(case, UCNS_MIXED), args.config),
                         "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)

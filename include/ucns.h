@@ -65,7 +65,7 @@ typedef struct ucns_mesh_v1 {
     const int32_t *ifca;        /* [ncell] faces per cell (4 tet, 5 prism/pyramid, 6 hex) */
     const int32_t *ineigh;      /* [ncell*nface_max] cell across face L (interior or periodic) */
     const int32_t *ineighn;     /* [ncell*nface_max] face slot of L inside that neighbour */
-    const int32_t *bcode;       /* [ncell*nface_max] 0 interior, else boundary code (5 periodic) */
+    const int32_t *bcode;       /* [ncell*nface_max] 0 interior, else boundary code (5 or 50 periodic) */
     const int32_t *types_faces; /* [ncell*nface_max] 5 quadrilateral, 6 triangle */
     const double *faceanglex;   /* [ncell*nface_max] outward normal angle a1 */
     const double *faceangley;   /* [ncell*nface_max] outward normal angle a2 */

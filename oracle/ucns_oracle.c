@@ -236,7 +236,7 @@ static void muscl_cell(const ctx_t *c, const double *u, int64_t i, double *ul, c
     for (int v = 0; v < NV; ++v) umin[v] = umax[v] = ui[v];
     for (int f = 0; f < c->m->ifca[i]; ++f) {
         int bc = c->m->bcode[i * nf + f];
-        if (bc != 0 && bc != 5) continue;  /* non-periodic boundary faces add no neighbour */
+        if (bc != 0 && bc != 5 && bc != 50) continue;  /* non-periodic boundary faces add no neighbour */
         const double *un = u + (size_t)(c->m->ineigh[i * nf + f] - c->base) * NV;
         for (int v = 0; v < NV; ++v) {
             umin[v] = fmin(umin[v], un[v]);

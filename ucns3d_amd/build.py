@@ -23,7 +23,7 @@ CXX = os.environ.get("CXX", "g++")
 
 # packed-FP32 instructions are disabled for every HIP translation unit
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
 ]

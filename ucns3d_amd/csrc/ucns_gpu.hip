@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ucns.h"
@@ -458,12 +459,48 @@ __host__ __device__ constexpr int pidx(int ideg, int k, int l) {
     return k <= l ? k * ideg - k * (k - 1) / 2 + (l - k) : l * ideg - l * (l - 1) / 2 + (k - l);
 }
 
-template <typename R, int IDEG, int IMAX, int NQL, int D>
+// Shape of the LDS ring that streams a tile's pseudo-inverse rows: NCH
+// chunks of 1 KiB per row (one global_load_lds_dwordx4 each), DEPTH rows in
+// flight.  After the LSQ loop the ring is reused for the tile's indicator
+// matrices and the central-stencil coefficients.  For tetrahedra the
+// indicators are fetched into the slots that fall free during the last
+// DEPTH rows, so their latency hides behind the end of the stream.
+template <typename R, int IDEG, int IMAX, int NCH, int DEPTH>
+struct RingShape {
+    static constexpr int NP = IDEG * (IDEG + 1) / 2;
+    static constexpr int SLOTB = (NCH > 0 ? NCH : 1) * 1024;
+    static constexpr int BBYTES = 8 * NP * (int)sizeof(R);
+    static constexpr int NBC = (BBYTES + 1023) / 1024;          // indicator chunks
+    static constexpr int NB = (BBYTES + SLOTB - 1) / SLOTB;     // ring slots they take
+    static constexpr int S0 = (IMAX - DEPTH) % DEPTH;           // first slot freed in the tail
+    static constexpr bool TAILB = NCH > 0 && S0 + NB <= DEPTH && NB < DEPTH;
+    static constexpr int BSLOT = TAILB ? S0 : 0;
+    static constexpr int GSLOT = TAILB ? (S0 + NB) % DEPTH : NB;
+    static constexpr int BYTES = NCH > 0 ? DEPTH * SLOTB : 1;
+    static_assert(NCH == 0 || GSLOT < DEPTH, "ring too shallow for the late data");
+    static_assert(NCH == 0 || 8 * IDEG * NV * (int)sizeof(R) <= SLOTB, "central coefficients exceed a slot");
+    static_assert(NCH == 0 || (DEPTH - 1) * NCH + NBC <= 63, "vmcnt immediate out of range");
+};
+
+// one 1 KiB piece per lane group: lane l moves bytes [16 l, 16 l + 16) of the
+// chunk, when they lie below `valid`; the LDS side is M0-based and lane-ordered
+__device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int lane, int valid) {
+    if (lane * 16 < valid)
+        __builtin_amdgcn_global_load_lds((const void *)(src + lane * 16),
+                                         (__attribute__((address_space(3))) void *)lds_dst, 16, 0, 0);
+}
+
+template <typename R, int IDEG, int IMAX, int NQL, int D, int NCH, int DEPTH>
 __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
-    __shared__ R sB[8][NP];
-    __shared__ R sG0[8][IDEG * NV];
-    __shared__ int sN[IMAX * 64];  // stencil members of the tile, [j][pos]
+    using RS = RingShape<R, IDEG, IMAX, NCH, DEPTH>;
+    constexpr bool RING = NCH > 0;
+    __shared__ __attribute__((aligned(1024))) char ring[RS::BYTES];
+    __shared__ R sBx[RING ? 1 : 8][RING ? 1 : NP];
+    __shared__ R sG0x[RING ? 1 : 8][RING ? 1 : IDEG * NV];
+    __shared__ int sN[RING ? 1 : IMAX * 64];  // stencil members of the tile, [j][pos]
+    R (*sB)[NP] = RING ? (R (*)[NP])(ring + RS::BSLOT * RS::SLOTB) : (R (*)[NP])&sBx[0][0];
+    R (*sG0)[IDEG * NV] = RING ? (R (*)[IDEG * NV])(ring + RS::GSLOT * RS::SLOTB) : (R (*)[IDEG * NV])&sG0x[0][0];
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
@@ -479,64 +516,140 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
     }
-    // indicators and stencil members of the tile -> LDS (contiguous blocks)
-    {
-        const R *Bt = p.B + (size_t)tile * 8 * NP;
-        const int nvc = min(8, p.ncell - tile * 8);
-        for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
-        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP;
-        for (int e = lane; e < IMAX * TP; e += WAVE) sN[e] = nt[e];
-    }
-    __syncthreads();
-    // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
-    // D members are in flight (their states and pseudo-inverse columns in
-    // registers) while one is accumulated.
+    const int nvc = min(8, p.ncell - tile * 8);
     R g[IDEG][NV];
 #pragma unroll
     for (int k = 0; k < IDEG; ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) g[k][v] = R(0);
-    if (act) {
-        const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
-        R pb[D][IDEG];
-        double ub[D][NV];
+    if constexpr (RING) {
+        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497), rows streamed
+        // through the LDS ring by direct global->LDS loads ----
+        const int rowb = IDEG * TP * (int)sizeof(R);
+        const char *Pt = (const char *)(p.P + (size_t)tile * IMAX * IDEG * TP);
+        auto issue_row = [&](int j) {
+            const char *src = Pt + (size_t)j * rowb;
+            char *dst = ring + (j % DEPTH) * RS::SLOTB;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int n = sN[d * TP + pos];
+            for (int c = 0; c < NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, rowb - c * 1024);
+        };
+        auto issue_b = [&]() {
+            const char *src = (const char *)(p.B + (size_t)tile * 8 * NP);
+            char *dst = (char *)&sB[0][0];
+            const int bb = nvc * NP * (int)sizeof(R);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
-            const R *Pj = Pt + (size_t)d * IDEG * TP;
+            for (int c = 0; c < RS::NBC; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, bb - c * 1024);
+        };
 #pragma unroll
-            for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+        for (int j = 0; j < DEPTH; ++j) issue_row(j);
+        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + (act ? pos : 0);
+        double ub[NV] = {0, 0, 0, 0, 0};
+        int nnext = 0;
+        if (act) {
+            const int n0 = nt[0];
+            nnext = nt[TP];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)n0 * NV + v];
         }
+        const int rl = act ? pos : 0;
+        // one member: dU from the prefetched state, prefetch the next state,
+        // wait for the row, accumulate
+        auto member = [&](int j, auto wait) {
+            R du[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) du[v] = (R)(ub[v] - ui[v]);
+            if (act && j + 1 < IMAX) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)nnext * NV + v];
+                if (j + 2 < IMAX) nnext = nt[(size_t)(j + 2) * TP];
+            }
+            wait();
+            if (act) {
+                const R *row = (const R *)(ring + (j % DEPTH) * RS::SLOTB) + rl;
+                R pc[IDEG];
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) pc[k] = row[k * TP];
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        };
+        // steady state: DEPTH-1 rows behind row j are in flight
 #pragma unroll 1
-        for (int j0 = 0; j0 < IMAX; j0 += D) {
+        for (int j = 0; j <= IMAX - DEPTH; ++j) {
+            member(j, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * NCH) : "memory"); });
+            if (j + DEPTH < IMAX) issue_row(j + DEPTH);
+        }
+        if (RS::TAILB && RS::NB == 1) issue_b();
+        // tail: rows IMAX-DEPTH+1 .. IMAX-1, nothing new but the indicators
+        [&]<int... T>(std::integer_sequence<int, T...>) {
+            (
+                [&] {
+                    constexpr int t = T + 1;
+                    constexpr int pend = (DEPTH - 1 - t) * NCH + ((RS::TAILB && RS::NB <= t) ? RS::NBC : 0);
+                    member(IMAX - DEPTH + t, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(pend) : "memory"); });
+                    if constexpr (RS::TAILB && RS::NB == t + 1) issue_b();
+                }(),
+                ...);
+        }(std::make_integer_sequence<int, DEPTH - 1>{});
+        if constexpr (!RS::TAILB) issue_b();
+    } else {
+        // indicators and stencil members of the tile -> LDS (contiguous blocks)
+        {
+            const R *Bt = p.B + (size_t)tile * 8 * NP;
+            for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+            const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP;
+            for (int e = lane; e < IMAX * TP; e += WAVE) sN[e] = nt[e];
+        }
+        __syncthreads();
+        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
+        // D members are in flight (their states and pseudo-inverse columns in
+        // registers) while one is accumulated.
+        if (act) {
+            const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
+            R pb[D][IDEG];
+            double ub[D][NV];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const int j = j0 + d;
-                if (j < IMAX) {
-                    R du[NV], pc[IDEG];
+                const int n = sN[d * TP + pos];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) du[v] = (R)(ub[d][v] - ui[v]);
+                for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+                const R *Pj = Pt + (size_t)d * IDEG * TP;
 #pragma unroll
-                    for (int k = 0; k < IDEG; ++k) pc[k] = pb[d][k];
-                    const int jn = j + D;
-                    if (jn < IMAX) {
-                        const int n = sN[jn * TP + pos];
+                for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+            }
+#pragma unroll 1
+            for (int j0 = 0; j0 < IMAX; j0 += D) {
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
-                        const R *Pj = Pt + (size_t)jn * IDEG * TP;
+                for (int d = 0; d < D; ++d) {
+                    const int j = j0 + d;
+                    if (j < IMAX) {
+                        R du[NV], pc[IDEG];
 #pragma unroll
-                        for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+                        for (int v = 0; v < NV; ++v) du[v] = (R)(ub[d][v] - ui[v]);
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k) pc[k] = pb[d][k];
+                        const int jn = j + D;
+                        if (jn < IMAX) {
+                            const int n = sN[jn * TP + pos];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+                            const R *Pj = Pt + (size_t)jn * IDEG * TP;
+#pragma unroll
+                            for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+                        }
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
                     }
-#pragma unroll
-                    for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
                 }
             }
         }
     }
+    if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
     // central-stencil coefficients kept for the MUSCL fallback
     if (s == 0) {
 #pragma unroll
@@ -723,11 +836,21 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
 #endif
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE) void k_recon_w64(ReconWParams<double> p) {
-    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64>(p);
+    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64, 0, 1>(p);
 }
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE, UCNS_WAVES32) void k_recon_w32(ReconWParams<float> p) {
-    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32>(p);
+    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32, 0, 1>(p);
+}
+// ring variants: NCH = ceil(19 * 8 ts * sizeof(R) / 1024) for the stencil
+// count ts they are instantiated for
+template <int IDEG, int IMAX, int NQL, int NCH, int DEPTH>
+__global__ __launch_bounds__(WAVE) void k_recon_r64(ReconWParams<double> p) {
+    recon_w_body<double, IDEG, IMAX, NQL, 1, NCH, DEPTH>(p);
+}
+template <int IDEG, int IMAX, int NQL, int NCH, int DEPTH>
+__global__ __launch_bounds__(WAVE, 2) void k_recon_r32(ReconWParams<float> p) {
+    recon_w_body<float, IDEG, IMAX, NQL, 1, NCH, DEPTH>(p);
 }
 
 // [cells][ts_in][imax][ideg] (FP64, ABI) -> tiled [tile][imax][ideg][8*ts] (R)
@@ -1038,6 +1161,7 @@ struct ucns_ctx {
     double gamma = 1.4, lwci1 = 1.0, eps = 1e-11;
     bool have_prestore = false, have_state = false;
     bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
+    bool ring = false;  // UCNS_RING=1 selects the LDS-ring kernels (experimental)
     int ntile = 0;
     // mesh (device)
     int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
@@ -1125,6 +1249,24 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.uleft = (R *)ctx->uleft;
         w.counters = ctx->counters;
         const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
+        // pseudo-inverse rows streamed through an LDS ring (tetrahedra: 5
+        // stencils, hexahedra: 7)
+        const bool ring_ok = ctx->ring && ((ctx->ts == 5 && ctx->nf * ctx->nqm <= 24) ||
+                                               (ctx->ts == 7 && ctx->nf * ctx->nqm > 24));
+        if (ring_ok) {
+            if constexpr (sizeof(R) == 8) {
+                if (ctx->ts == 5)
+                    hipLaunchKernelGGL((k_recon_r64<19, 39, 3, 6, 6>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_r64<19, 39, 7, 9, 4>), grid, block, 0, ctx->stream, w);
+            } else {
+                if (ctx->ts == 5)
+                    hipLaunchKernelGGL((k_recon_r32<19, 39, 3, 3, 6>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_r32<19, 39, 7, 5, 4>), grid, block, 0, ctx->stream, w);
+            }
+            return UCNS_OK;
+        }
         if constexpr (sizeof(R) == 8) {
             if (ctx->nf * ctx->nqm <= 24)
                 hipLaunchKernelGGL((k_recon_w64<19, 39, 3>), grid, block, 0, ctx->stream, w);
@@ -1398,8 +1540,8 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
         for (int f = 0; f < ifca; ++f) {
             const size_t sl = (size_t)i * nfa + f;
             const int bc = m->bcode[sl];
-            if (bc != 0 && bc != 5)
-                return bad(UCNS_EUNSUPPORTED, "only interior and periodic (code 5) faces are supported");
+            if (bc != 0 && bc != 5 && bc != 50)  // periodic: 5 or 50 (src/dg_functions.f90:1786)
+                return bad(UCNS_EUNSUPPORTED, "only interior and periodic (code 5/50) faces are supported");
             const int j = m->ineigh[sl] - base, fj = m->ineighn[sl] - base;
             if (j < 0 || j >= nc || fj < 0 || fj >= m->ifca[j])
                 return bad(UCNS_EBADARG, "face neighbour out of range at cell " + std::to_string(i));
@@ -1500,6 +1642,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     ctx->ts = ts;
     const bool mixed = ctx->precision == UCNS_MIXED;
     ctx->tiled = (ctx->iweno == 1 && ID == 19 && IM == 39 && ts <= 8 && ctx->nf * ctx->nqm <= 56);
+    {
+        const char *e = std::getenv("UCNS_RING");
+        ctx->ring = e && e[0] == '1';
+    }
     ctx->ntile = (nc + 7) / 8;
     // stencil members without the cell itself
     //   tiled: [tile][imax][8*ts], lane position 8-cell-tile local
@@ -1587,6 +1733,9 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             for (int k = 0; k < ID; ++k)
                 for (int l = k; l < ID; ++l) bp[e++] = Bi[(size_t)k * ID + l];
         }
+        // zero pad: the tile loads move whole 16-byte pieces, the last of
+        // which may reach past the final cell's block
+        bp.resize(bp.size() + 512, 0.0);
         if ((rc = upload_real_any(ctx, &ctx->B, bp))) return rc;
     } else {
         std::vector<double> bp(1, 0.0);
