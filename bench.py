@@ -49,9 +49,14 @@ def recon_bytes_per_cell(case, precision: int) -> float:
 
 
 def recon_kernel_name(case, precision: int) -> str:
-    """the reconstruction kernel the library launches for this case"""
-    nql = 3 if case.nface_max * max(case.qp_quad, case.qp_tri) <= 24 or case.shape == 2 else 7
-    return f"k_recon_{'w32' if precision else 'w64'}<19,39,{nql}>"
+    """the reconstruction kernel the library launches for this case: the
+    row-stream kernels for tetrahedra (5 stencils) and hexahedra (7), the
+    register-prefetch ones when UCNS_STREAM=0"""
+    tet = case.shape == 2
+    ts, nql = (5, 3) if tet else (7, 7)
+    if os.environ.get("UCNS_STREAM") == "0" or os.environ.get("UCNS_RING") == "1":
+        return f"k_recon_{'w32' if precision else 'w64'}<19,39,{nql}>"
+    return f"k_recon_{'s32' if precision else 's64'}<19,39,{nql},{ts}>"
 
 
 def committed_traffic(kernel: str, config: str):

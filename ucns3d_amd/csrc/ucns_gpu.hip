@@ -431,6 +431,8 @@ struct ReconWParams {
     const int32_t *cnb;    // [ncell][nf]
     R *uleft;              // [ncell][nf][nqm][5] deviations from the cell average
     int *counters;
+    const unsigned char *stream;  // per-tile row stream of the DMA kernels (StreamShape)
+    long long tile_bytes;
 };
 
 template <typename T>
@@ -482,174 +484,32 @@ struct RingShape {
     static_assert(NCH == 0 || (DEPTH - 1) * NCH + NBC <= 63, "vmcnt immediate out of range");
 };
 
-// one 1 KiB piece per lane group: lane l moves bytes [16 l, 16 l + 16) of the
-// chunk, when they lie below `valid`; the LDS side is M0-based and lane-ordered
-__device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int lane, int valid) {
-    if (lane * 16 < valid)
-        __builtin_amdgcn_global_load_lds((const void *)(src + lane * 16),
-                                         (__attribute__((address_space(3))) void *)lds_dst, 16, 0, 0);
+// Direct global->LDS loads (global_load_lds_dword / _dwordx4): lane l's 4 or
+// 16 bytes from its own global address land at LDS byte (M0 + 4 l) or
+// (M0 + 16 l).  Issued as inline asm so that the compiler neither counts nor
+// waits for them; every reader waits with an explicit s_waitcnt.  The nop
+// covers the M0-write hazard ahead of the load.
+__device__ __forceinline__ unsigned lds_addr(const void *p) {
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) char *)p;
+}
+__device__ __forceinline__ void lds_load16(const void *g, unsigned lds_base) {
+    asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "{m0}"(lds_base) : "memory");
+}
+__device__ __forceinline__ void lds_load4(const void *g, unsigned lds_base) {
+    asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "{m0}"(lds_base) : "memory");
 }
 
-template <typename R, int IDEG, int IMAX, int NQL, int D, int NCH, int DEPTH>
-__device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
-    constexpr int NP = IDEG * (IDEG + 1) / 2;
-    using RS = RingShape<R, IDEG, IMAX, NCH, DEPTH>;
-    constexpr bool RING = NCH > 0;
-    __shared__ __attribute__((aligned(1024))) char ring[RS::BYTES];
-    __shared__ R sBx[RING ? 1 : 8][RING ? 1 : NP];
-    __shared__ R sG0x[RING ? 1 : 8][RING ? 1 : IDEG * NV];
-    __shared__ int sN[RING ? 1 : IMAX * 64];  // stencil members of the tile, [j][pos]
-    R (*sB)[NP] = RING ? (R (*)[NP])(ring + RS::BSLOT * RS::SLOTB) : (R (*)[NP])&sBx[0][0];
-    R (*sG0)[IDEG * NV] = RING ? (R (*)[IDEG * NV])(ring + RS::GSLOT * RS::SLOTB) : (R (*)[IDEG * NV])&sG0x[0][0];
-    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
-    const int b = blockIdx.x;
-    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
-    if (tile >= p.ntile) return;
-    const int cell = tile * 8 + lc;
-    const bool cvalid = cell < p.ncell;
-    const int info = cvalid ? p.cinfo[cell] : 0;
-    const int ifca = ci_ifca(info), admis = ci_admis(info), full = ci_full(info);
-    const bool act = cvalid && s < admis;
-    const int TP = 8 * p.ts, pos = lc * p.ts + s;
-    double ui[NV] = {1.0, 0.0, 0.0, 0.0, 1.0};
-    if (cvalid) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
-    }
-    const int nvc = min(8, p.ncell - tile * 8);
-    R g[IDEG][NV];
-#pragma unroll
-    for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) g[k][v] = R(0);
-    if constexpr (RING) {
-        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497), rows streamed
-        // through the LDS ring by direct global->LDS loads ----
-        const int rowb = IDEG * TP * (int)sizeof(R);
-        const char *Pt = (const char *)(p.P + (size_t)tile * IMAX * IDEG * TP);
-        auto issue_row = [&](int j) {
-            const char *src = Pt + (size_t)j * rowb;
-            char *dst = ring + (j % DEPTH) * RS::SLOTB;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, rowb - c * 1024);
-        };
-        auto issue_b = [&]() {
-            const char *src = (const char *)(p.B + (size_t)tile * 8 * NP);
-            char *dst = (char *)&sB[0][0];
-            const int bb = nvc * NP * (int)sizeof(R);
-#pragma unroll
-            for (int c = 0; c < RS::NBC; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, bb - c * 1024);
-        };
-#pragma unroll
-        for (int j = 0; j < DEPTH; ++j) issue_row(j);
-        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + (act ? pos : 0);
-        double ub[NV] = {0, 0, 0, 0, 0};
-        int nnext = 0;
-        if (act) {
-            const int n0 = nt[0];
-            nnext = nt[TP];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)n0 * NV + v];
-        }
-        const int rl = act ? pos : 0;
-        // one member: dU from the prefetched state, prefetch the next state,
-        // wait for the row, accumulate
-        auto member = [&](int j, auto wait) {
-            R du[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) du[v] = (R)(ub[v] - ui[v]);
-            if (act && j + 1 < IMAX) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)nnext * NV + v];
-                if (j + 2 < IMAX) nnext = nt[(size_t)(j + 2) * TP];
-            }
-            wait();
-            if (act) {
-                const R *row = (const R *)(ring + (j % DEPTH) * RS::SLOTB) + rl;
-                R pc[IDEG];
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) pc[k] = row[k * TP];
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
-            }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        };
-        // steady state: DEPTH-1 rows behind row j are in flight
-#pragma unroll 1
-        for (int j = 0; j <= IMAX - DEPTH; ++j) {
-            member(j, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * NCH) : "memory"); });
-            if (j + DEPTH < IMAX) issue_row(j + DEPTH);
-        }
-        if (RS::TAILB && RS::NB == 1) issue_b();
-        // tail: rows IMAX-DEPTH+1 .. IMAX-1, nothing new but the indicators
-        [&]<int... T>(std::integer_sequence<int, T...>) {
-            (
-                [&] {
-                    constexpr int t = T + 1;
-                    constexpr int pend = (DEPTH - 1 - t) * NCH + ((RS::TAILB && RS::NB <= t) ? RS::NBC : 0);
-                    member(IMAX - DEPTH + t, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(pend) : "memory"); });
-                    if constexpr (RS::TAILB && RS::NB == t + 1) issue_b();
-                }(),
-                ...);
-        }(std::make_integer_sequence<int, DEPTH - 1>{});
-        if constexpr (!RS::TAILB) issue_b();
-    } else {
-        // indicators and stencil members of the tile -> LDS (contiguous blocks)
-        {
-            const R *Bt = p.B + (size_t)tile * 8 * NP;
-            for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
-            const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP;
-            for (int e = lane; e < IMAX * TP; e += WAVE) sN[e] = nt[e];
-        }
-        __syncthreads();
-        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
-        // D members are in flight (their states and pseudo-inverse columns in
-        // registers) while one is accumulated.
-        if (act) {
-            const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
-            R pb[D][IDEG];
-            double ub[D][NV];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int n = sN[d * TP + pos];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
-                const R *Pj = Pt + (size_t)d * IDEG * TP;
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
-            }
-#pragma unroll 1
-            for (int j0 = 0; j0 < IMAX; j0 += D) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const int j = j0 + d;
-                    if (j < IMAX) {
-                        R du[NV], pc[IDEG];
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) du[v] = (R)(ub[d][v] - ui[v]);
-#pragma unroll
-                        for (int k = 0; k < IDEG; ++k) pc[k] = pb[d][k];
-                        const int jn = j + D;
-                        if (jn < IMAX) {
-                            const int n = sN[jn * TP + pos];
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
-                            const R *Pj = Pt + (size_t)jn * IDEG * TP;
-#pragma unroll
-                            for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
-                        }
-#pragma unroll
-                        for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
-                    }
-                }
-            }
-        }
-    }
-    if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
+// one 1 KiB piece: lane l moves bytes [16 l, 16 l + 16) of the chunk when
+// they lie below `valid`
+__device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int lane, int valid) {
+    if (lane * 16 < valid) lds_load16(src + lane * 16, lds_addr(lds_dst));
+}
+
+template <typename R, int IDEG, int NQL>
+__device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG][NV], const double (&ui)[NV],
+                                           const R (*sB)[IDEG * (IDEG + 1) / 2], R (*sG0)[IDEG * NV], int lc, int s,
+                                           int cell, bool cvalid, int info, bool act) {
+    const int ifca = ci_ifca(info), full = ci_full(info);
     // central-stencil coefficients kept for the MUSCL fallback
     if (s == 0) {
 #pragma unroll
@@ -820,6 +680,394 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
             atomicAdd(&p.counters[0], 1);
             if (first) atomicAdd(&p.counters[1], 1);
         }
+    }
+}
+
+template <typename R, int IDEG, int IMAX, int NQL, int D, int NCH, int DEPTH>
+__device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
+    constexpr int NP = IDEG * (IDEG + 1) / 2;
+    using RS = RingShape<R, IDEG, IMAX, NCH, DEPTH>;
+    constexpr bool RING = NCH > 0;
+    __shared__ __attribute__((aligned(1024))) char ring[RS::BYTES];
+    __shared__ R sBx[RING ? 1 : 8][RING ? 1 : NP];
+    __shared__ R sG0x[RING ? 1 : 8][RING ? 1 : IDEG * NV];
+    __shared__ int sN[RING ? 1 : IMAX * 64];  // stencil members of the tile, [j][pos]
+    R (*sB)[NP] = RING ? (R (*)[NP])(ring + RS::BSLOT * RS::SLOTB) : (R (*)[NP])&sBx[0][0];
+    R (*sG0)[IDEG * NV] = RING ? (R (*)[IDEG * NV])(ring + RS::GSLOT * RS::SLOTB) : (R (*)[IDEG * NV])&sG0x[0][0];
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    const int info = cvalid ? p.cinfo[cell] : 0;
+    const int ifca = ci_ifca(info), admis = ci_admis(info), full = ci_full(info);
+    const bool act = cvalid && s < admis;
+    const int TP = 8 * p.ts, pos = lc * p.ts + s;
+    double ui[NV] = {1.0, 0.0, 0.0, 0.0, 1.0};
+    if (cvalid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
+    }
+    const int nvc = min(8, p.ncell - tile * 8);
+    R g[IDEG][NV];
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[k][v] = R(0);
+    if constexpr (RING) {
+        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497), rows streamed
+        // through the LDS ring by direct global->LDS loads ----
+        const int rowb = IDEG * TP * (int)sizeof(R);
+        const char *Pt = (const char *)(p.P + (size_t)tile * IMAX * IDEG * TP);
+        auto issue_row = [&](int j) {
+            const char *src = Pt + (size_t)j * rowb;
+            char *dst = ring + (j % DEPTH) * RS::SLOTB;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, rowb - c * 1024);
+        };
+        auto issue_b = [&]() {
+            const char *src = (const char *)(p.B + (size_t)tile * 8 * NP);
+            char *dst = (char *)&sB[0][0];
+            const int bb = nvc * NP * (int)sizeof(R);
+#pragma unroll
+            for (int c = 0; c < RS::NBC; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, bb - c * 1024);
+        };
+#pragma unroll
+        for (int j = 0; j < DEPTH; ++j) issue_row(j);
+        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + (act ? pos : 0);
+        double ub[NV] = {0, 0, 0, 0, 0};
+        int nnext = 0;
+        if (act) {
+            const int n0 = nt[0];
+            nnext = nt[TP];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)n0 * NV + v];
+        }
+        const int rl = act ? pos : 0;
+        // one member: dU from the prefetched state, prefetch the next state,
+        // wait for the row, accumulate
+        auto member = [&](int j, auto wait) {
+            R du[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) du[v] = (R)(ub[v] - ui[v]);
+            if (act && j + 1 < IMAX) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)nnext * NV + v];
+                if (j + 2 < IMAX) nnext = nt[(size_t)(j + 2) * TP];
+            }
+            wait();
+            if (act) {
+                const R *row = (const R *)(ring + (j % DEPTH) * RS::SLOTB) + rl;
+                R pc[IDEG];
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) pc[k] = row[k * TP];
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        };
+        // steady state: DEPTH-1 rows behind row j are in flight
+#pragma unroll 1
+        for (int j = 0; j <= IMAX - DEPTH; ++j) {
+            member(j, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * NCH) : "memory"); });
+            if (j + DEPTH < IMAX) issue_row(j + DEPTH);
+        }
+        if (RS::TAILB && RS::NB == 1) issue_b();
+        // tail: rows IMAX-DEPTH+1 .. IMAX-1, nothing new but the indicators
+        [&]<int... T>(std::integer_sequence<int, T...>) {
+            (
+                [&] {
+                    constexpr int t = T + 1;
+                    constexpr int pend = (DEPTH - 1 - t) * NCH + ((RS::TAILB && RS::NB <= t) ? RS::NBC : 0);
+                    member(IMAX - DEPTH + t, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(pend) : "memory"); });
+                    if constexpr (RS::TAILB && RS::NB == t + 1) issue_b();
+                }(),
+                ...);
+        }(std::make_integer_sequence<int, DEPTH - 1>{});
+        if constexpr (!RS::TAILB) issue_b();
+    } else {
+        // indicators and stencil members of the tile -> LDS (contiguous blocks)
+        {
+            const R *Bt = p.B + (size_t)tile * 8 * NP;
+            for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+            const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP;
+            for (int e = lane; e < IMAX * TP; e += WAVE) sN[e] = nt[e];
+        }
+        __syncthreads();
+        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
+        // D members are in flight (their states and pseudo-inverse columns in
+        // registers) while one is accumulated.
+        if (act) {
+            const R *Pt = p.P + (size_t)tile * IMAX * IDEG * TP + pos;
+            R pb[D][IDEG];
+            double ub[D][NV];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int n = sN[d * TP + pos];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+                const R *Pj = Pt + (size_t)d * IDEG * TP;
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+            }
+#pragma unroll 1
+            for (int j0 = 0; j0 < IMAX; j0 += D) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const int j = j0 + d;
+                    if (j < IMAX) {
+                        R du[NV], pc[IDEG];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) du[v] = (R)(ub[d][v] - ui[v]);
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k) pc[k] = pb[d][k];
+                        const int jn = j + D;
+                        if (jn < IMAX) {
+                            const int n = sN[jn * TP + pos];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) ub[d][v] = p.U[(size_t)n * NV + v];
+                            const R *Pj = Pt + (size_t)jn * IDEG * TP;
+#pragma unroll
+                            for (int k = 0; k < IDEG; ++k) pb[d][k] = Pj[k * TP];
+                        }
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
+    recon_late<R, IDEG, NQL>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+}
+
+// ---------------------------------------------------------------------------
+// Row-stream reconstruction: every byte the LSQ loop needs arrives by direct
+// global->LDS loads (global_load_lds), so no register load is outstanding
+// while rows are in flight.  Vector-memory counters complete in order, and
+// the compiler does not count these loads, so the loop waits with explicit
+// s_waitcnt vmcnt(N) where N is a lower bound on the loads issued after the
+// row it needs.
+//
+// HBM stream of one tile: header = member indices of rows 0..DEPTH-1, then
+// per member row j: the pseudo-inverse block [k][8 ts] (R) followed by the
+// member indices of row j + DEPTH ([8 ts] int32), so the indices of a row
+// are in LDS one ring turn before its states are fetched.
+//
+// LDS slot of a row: the row block as streamed, then the 40-byte states of
+// the lanes' members as four lane-ordered pieces (16, 16, 4, 4 bytes); the
+// two 4-byte pieces sit in the slack of the row's last 1 KiB chunk when it
+// has room.  After the loop the slots hold the tile's indicators and the
+// central coefficients.
+constexpr int stream_depth(bool mixed, int ts) { return ts == 5 ? (mixed ? 3 : 4) : (mixed ? 2 : 3); }
+
+template <typename R, int IDEG, int IMAX, int TS, int DEPTH>
+struct StreamShape {
+    static constexpr int NP = IDEG * (IDEG + 1) / 2;
+    static constexpr int TP = 8 * TS;
+    static constexpr int PB = IDEG * TP * (int)sizeof(R);  // pseudo-inverse bytes of a row
+    static constexpr int RD = PB + TP * 4;                 // row bytes in HBM
+    static constexpr int HD = DEPTH * TP * 4;              // tile header bytes
+    static constexpr int NCH = (RD + 1023) / 1024;         // 1 KiB loads per row
+    static constexpr bool CD_IN_P = NCH * 1024 - RD >= 512;
+    static constexpr int UA = NCH * 1024, UB = UA + 1024;
+    static constexpr int UC = CD_IN_P ? RD : UB + 1024;
+    static constexpr int UD = UC + 256;
+    static constexpr int SB = CD_IN_P ? UB + 1024 : UD + 256;  // slot bytes
+    static constexpr int PER = NCH + 4;                    // loads per row
+    static constexpr int BBYTES = 8 * NP * (int)sizeof(R);
+    static constexpr int NBC = (BBYTES + 1023) / 1024;
+    static constexpr int NB = (BBYTES + SB - 1) / SB;
+    static constexpr int S0 = (IMAX - DEPTH) % DEPTH;      // slot of the first tail row
+    static constexpr bool TAILB = S0 + NB <= DEPTH && NB < DEPTH;
+    static constexpr int TB = NB - 1;                      // tail step after which they are free
+    static constexpr int BSLOT = TAILB ? S0 : 0;
+    static constexpr int GSLOT = TAILB ? (S0 + NB) % DEPTH : NB;
+    static constexpr long long TILE = HD + (long long)IMAX * RD;
+    static_assert(RD % 16 == 0 && HD % 16 == 0, "16-byte pieces");
+    static_assert(HD <= 1024, "header is one load");
+    static_assert(GSLOT < DEPTH && NBC * 1024 <= NB * SB, "late data must fit the ring");
+    static_assert(8 * IDEG * NV * (int)sizeof(R) <= SB, "central coefficients exceed a slot");
+    static_assert((DEPTH - 1) * PER + NBC <= 63, "vmcnt immediate out of range");
+};
+
+template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH>
+__device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
+    using SS = StreamShape<R, IDEG, IMAX, TS, DEPTH>;
+    constexpr int NP = SS::NP, TP = SS::TP;
+    // ring slots, then the header's index rows
+    __shared__ __attribute__((aligned(1024))) char ring[DEPTH * SS::SB + SS::HD];
+    const int *hdr = (const int *)(ring + DEPTH * SS::SB);
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    int info = cvalid ? p.cinfo[cell] : 0;
+    double ui[NV] = {1.0, 0.0, 0.0, 0.0, 1.0};
+    if (cvalid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
+    }
+    // the cell's own data is in registers before the first row load
+    asm volatile("s_waitcnt vmcnt(0)"
+                 : "+v"(info), "+v"(ui[0]), "+v"(ui[1]), "+v"(ui[2]), "+v"(ui[3]), "+v"(ui[4])
+                 :
+                 : "memory");
+    const int admis = ci_admis(info);
+    const bool act = cvalid && s < admis;
+    const int rl = act ? lc * TS + s : 0;
+    const int nvc = min(8, p.ncell - tile * 8);
+    const char *T = (const char *)p.stream + (size_t)tile * SS::TILE;
+
+    auto issue_row = [&](int j) {
+        const char *src = T + SS::HD + (size_t)j * SS::RD;
+        char *dst = ring + (j % DEPTH) * SS::SB;
+#pragma unroll
+        for (int c = 0; c < SS::NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, SS::RD - c * 1024);
+    };
+    // lane 0 of cell 0 of every tile is active, so these four loads are issued in every wave
+    auto issue_states = [&](int j, int n) {
+        char *dst = ring + (j % DEPTH) * SS::SB;
+        if (act) {
+            const char *src = (const char *)(p.U + (size_t)n * NV);
+            lds_load16(src, lds_addr(dst + SS::UA));
+            lds_load16(src + 16, lds_addr(dst + SS::UB));
+            lds_load4(src + 32, lds_addr(dst + SS::UC));
+            lds_load4(src + 36, lds_addr(dst + SS::UD));
+        }
+    };
+    // every chunk is issued, also past the last cell's block (lane 0 then
+    // re-reads the block's first bytes), so the loads counted below exist
+    auto issue_b = [&]() {
+        const char *src = (const char *)(p.B + (size_t)tile * 8 * NP);
+        char *dst = ring + SS::BSLOT * SS::SB;
+        const int bb = nvc * NP * (int)sizeof(R);
+#pragma unroll
+        for (int c = 0; c < SS::NBC; ++c) {
+            const int valid = bb - c * 1024;
+            dma_chunk(valid > 0 ? src + c * 1024 : src, dst + c * 1024, lane, valid > 0 ? valid : 16);
+        }
+    };
+
+    // prologue: header, then (row, states) pairs for rows 0..DEPTH-1
+    dma_chunk(T, ring + DEPTH * SS::SB, lane, SS::HD);
+    issue_row(0);
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(SS::NCH) : "memory");  // header landed
+    issue_states(0, hdr[rl]);
+#pragma unroll
+    for (int r = 1; r < DEPTH; ++r) {
+        issue_row(r);
+        issue_states(r, hdr[r * TP + rl]);
+    }
+
+    // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
+    R g[IDEG][NV];
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[k][v] = R(0);
+    auto consume = [&](int i) {
+        if (act) {
+            const char *slot = ring + (i % DEPTH) * SS::SB;
+            const double2 a = *(const double2 *)(slot + SS::UA + lane * 16);
+            const double2 c = *(const double2 *)(slot + SS::UB + lane * 16);
+            const unsigned lo = *(const unsigned *)(slot + SS::UC + lane * 4);
+            const unsigned hi = *(const unsigned *)(slot + SS::UD + lane * 4);
+            const double u4 = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+            const R du[NV] = {(R)(a.x - ui[0]), (R)(a.y - ui[1]), (R)(c.x - ui[2]), (R)(c.y - ui[3]),
+                              (R)(u4 - ui[4])};
+            const R *row = (const R *)slot + rl;
+            R pc[IDEG];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) pc[k] = row[k * TP];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+        }
+    };
+#pragma unroll 1
+    for (int i = 0; i < IMAX - DEPTH; ++i) {
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * SS::PER) : "memory");
+        consume(i);
+        const int nn = *(const int *)(ring + (i % DEPTH) * SS::SB + SS::PB + rl * 4);  // members of row i+DEPTH
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        issue_row(i + DEPTH);
+        issue_states(i + DEPTH, nn);
+    }
+    // tail: the last DEPTH rows; the indicators follow into the first free slots
+    [&]<int... TT>(std::integer_sequence<int, TT...>) {
+        (
+            [&] {
+                constexpr int t = TT;
+                constexpr int pend = (DEPTH - 1 - t) * SS::PER + ((SS::TAILB && t > SS::TB) ? SS::NBC : 0);
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"(pend) : "memory");
+                consume(IMAX - DEPTH + t);
+                if constexpr (SS::TAILB && t == SS::TB) {
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                    issue_b();
+                }
+            }(),
+            ...);
+    }(std::make_integer_sequence<int, DEPTH>{});
+    if constexpr (!SS::TAILB) {
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        issue_b();
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
+    const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
+    R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
+    recon_late<R, IDEG, NQL>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+}
+
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE) void k_recon_s64(ReconWParams<double> p) {
+    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS)>(p);
+}
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE, 2) void k_recon_s32(ReconWParams<float> p) {
+    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS)>(p);
+}
+
+// upload: pseudo-inverses of cells cell0 .. cell0+cn-1 ([cell][tsa][imax][ideg]
+// FP64, the ABI order) into the row stream
+template <typename R>
+__global__ void k_stream_p(const double *src, unsigned char *stream, int cell0, int cn, int tsa, int ts, int imax,
+                           int ideg, int hd, int rd, long long tile_bytes) {
+    const size_t n = (size_t)cn * ts * imax * ideg;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % ideg);
+        const int j = (int)((e / ideg) % imax);
+        const int st = (int)((e / ((size_t)ideg * imax)) % ts);
+        const int cl = (int)(e / ((size_t)ideg * imax * ts));
+        const int i = cell0 + cl;
+        const double v = src[(((size_t)cl * tsa + st) * imax + j) * ideg + k];
+        R *row = (R *)(stream + (size_t)(i / 8) * tile_bytes + hd + (size_t)j * rd);
+        row[(size_t)k * 8 * ts + (i % 8) * ts + st] = (R)v;
+    }
+}
+
+// upload: member-index rows [ntile][depth+imax][8 ts] (row r = members j=r,
+// zero past imax) into the header (r < depth) and behind the pseudo-inverse
+// block of row r - depth
+__global__ void k_stream_idx(const int32_t *idx, unsigned char *stream, int ntile, int depth, int imax, int tp, int hd,
+                             int pb, int rd, long long tile_bytes) {
+    const size_t n = (size_t)ntile * (depth + imax) * tp;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int q = (int)(e % tp);
+        const int r = (int)((e / tp) % (depth + imax));
+        const size_t t = e / ((size_t)tp * (depth + imax));
+        unsigned char *base = stream + t * tile_bytes;
+        int32_t *dst = r < depth ? (int32_t *)(base + (size_t)r * tp * 4) + q
+                                 : (int32_t *)(base + hd + (size_t)(r - depth) * rd + pb) + q;
+        *dst = idx[e];
     }
 }
 
@@ -1162,6 +1410,7 @@ struct ucns_ctx {
     bool have_prestore = false, have_state = false;
     bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
     bool ring = false;  // UCNS_RING=1 selects the LDS-ring kernels (experimental)
+    int smode = 0;      // stencils per cell of the row-stream kernel in use (0: none)
     int ntile = 0;
     // mesh (device)
     int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
@@ -1170,6 +1419,8 @@ struct ucns_ctx {
     // prestore (device)
     int32_t *nbr = nullptr;
     void *P = nullptr, *B = nullptr, *mbar = nullptr, *qp = nullptr, *uleft = nullptr;
+    unsigned char *rows = nullptr;  // row stream of the DMA reconstruction kernels
+    long long tile_bytes = 0;
     // state (device)
     double *U = nullptr, *U2 = nullptr, *fflux = nullptr, *rhs = nullptr, *t = nullptr, *dtfix = nullptr;
     unsigned long long *dtv = nullptr;  // [2] ping-pong dt slots
@@ -1249,6 +1500,22 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.uleft = (R *)ctx->uleft;
         w.counters = ctx->counters;
         const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
+        w.stream = ctx->rows;
+        w.tile_bytes = ctx->tile_bytes;
+        if (ctx->smode) {
+            if constexpr (sizeof(R) == 8) {
+                if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_s64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_s64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+            } else {
+                if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_s32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_s32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+            }
+            return UCNS_OK;
+        }
         // pseudo-inverse rows streamed through an LDS ring (tetrahedra: 5
         // stencils, hexahedra: 7)
         const bool ring_ok = ctx->ring && ((ctx->ts == 5 && ctx->nf * ctx->nqm <= 24) ||
@@ -1443,6 +1710,7 @@ int upload_real_any(ucns_ctx *ctx, void **dst, const std::vector<double> &src) {
 void release_prestore(ucns_ctx *ctx) {
     dfree(ctx->nbr);
     dfree(ctx->P);
+    dfree(ctx->rows);
     dfree(ctx->B);
     dfree(ctx->mbar);
     dfree(ctx->qp);
@@ -1645,12 +1913,18 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     {
         const char *e = std::getenv("UCNS_RING");
         ctx->ring = e && e[0] == '1';
+        // row-stream kernels: tetrahedra (5 stencils, 24 face points) and
+        // hexahedra (7 stencils, 54 face points); UCNS_STREAM=0 opts out
+        const char *st = std::getenv("UCNS_STREAM");
+        const bool stream_ok = !(st && st[0] == '0') && !ctx->ring && ctx->tiled &&
+                               ((ts == 5 && ctx->nf * ctx->nqm <= 24) || (ts == 7 && ctx->nf * ctx->nqm > 24));
+        ctx->smode = stream_ok ? ts : 0;
     }
     ctx->ntile = (nc + 7) / 8;
     // stencil members without the cell itself
     //   tiled: [tile][imax][8*ts], lane position 8-cell-tile local
     //   plain: [cell][ts][imax]
-    std::vector<int32_t> nbr(ctx->tiled ? (size_t)ctx->ntile * IM * 8 * ts : (size_t)nc * ts * IM, 0);
+    std::vector<int32_t> nbr(ctx->smode ? (size_t)0 : ctx->tiled ? (size_t)ctx->ntile * IM * 8 * ts : (size_t)nc * ts * IM, 0);
     for (int i = 0; i < nc; ++i)
         for (int s = 0; s < pre->admis[i]; ++s) {
             const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
@@ -1664,7 +1938,9 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                     ctx->err = "stencil member out of range at cell " + std::to_string(i);
                     return UCNS_EBADARG;
                 }
-                if (ctx->tiled) {
+                if (ctx->smode) {
+                    continue;  // carried by the row stream
+                } else if (ctx->tiled) {
                     const size_t TP = 8 * (size_t)ts;
                     nbr[((size_t)(i / 8) * IM + k) * TP + (size_t)(i % 8) * ts + s] = j;
                 } else {
@@ -1672,15 +1948,39 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                 }
             }
         }
-    int rc = dalloc(ctx, &ctx->nbr, nbr.size());
+    int rc = dalloc(ctx, &ctx->nbr, std::max<size_t>(nbr.size(), 1));
     if (rc) return rc;
-    HIP_TRY(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!nbr.empty())
+        HIP_TRY(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
     // pseudo-inverses: staged in chunks, re-laid out on the device
     const size_t per_cell_abi = (size_t)TSA * IM * ID, per_cell_dev = (size_t)IM * ts * ID;
-    const size_t dev_elems = ctx->tiled ? (size_t)ctx->ntile * 8 * per_cell_dev : per_cell_dev * nc;
+    const size_t dev_elems = ctx->smode ? 1 : ctx->tiled ? (size_t)ctx->ntile * 8 * per_cell_dev : per_cell_dev * nc;
     rc = mixed ? dalloc(ctx, (float **)&ctx->P, dev_elems) : dalloc(ctx, (double **)&ctx->P, dev_elems);
     if (rc) return rc;
+    const int sTP = 8 * ts, sD = stream_depth(mixed, ts), sPB = ID * sTP * (mixed ? 4 : 8);
+    const int sRD = sPB + sTP * 4, sHD = sD * sTP * 4;
+    if (ctx->smode) {
+        ctx->tile_bytes = sHD + (long long)IM * sRD;
+        if ((rc = dalloc(ctx, &ctx->rows, (size_t)ctx->ntile * ctx->tile_bytes))) return rc;
+        // member-index rows: row r holds members j = r (zero past imax)
+        const size_t rows = (size_t)sD + IM;
+        std::vector<int32_t> idx((size_t)ctx->ntile * rows * sTP, 0);
+        for (int i = 0; i < nc; ++i)
+            for (int s = 0; s < pre->admis[i]; ++s) {
+                const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
+                for (int r = 0; r < IM; ++r)
+                    idx[((size_t)(i / 8) * rows + r) * sTP + (size_t)(i % 8) * ts + s] = row[r + 1] - base;
+            }
+        int32_t *didx = nullptr;
+        if ((rc = dalloc(ctx, &didx, idx.size()))) return rc;
+        HIP_TRY(hipMemcpy(didx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_stream_idx, dim3(4096), dim3(256), 0, ctx->stream, didx, ctx->rows, ctx->ntile, sD, IM,
+                           sTP, sHD, sPB, sRD, ctx->tile_bytes);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dfree(didx);
+    }
     {
         size_t chunk_cells = std::max<size_t>(8, (size_t)(256u << 20) / (per_cell_abi * sizeof(double)));
         chunk_cells -= chunk_cells % 8;
@@ -1699,7 +1999,15 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             } else {
                 HIP_TRY(hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, ctx->stream));
             }
-            if (ctx->tiled) {
+            if (ctx->smode) {
+                const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
+                if (mixed)
+                    hipLaunchKernelGGL(k_stream_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       ctx->rows, (int)c0, (int)cn, TSA, ts, IM, ID, sHD, sRD, ctx->tile_bytes);
+                else
+                    hipLaunchKernelGGL(k_stream_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
+                                       ctx->rows, (int)c0, (int)cn, TSA, ts, IM, ID, sHD, sRD, ctx->tile_bytes);
+            } else if (ctx->tiled) {
                 const size_t out_elems = ((cn + 7) / 8) * 8 * per_cell_dev;
                 const int blocks = (int)std::min<size_t>(4096, (out_elems + 255) / 256);
                 if (mixed)
