@@ -63,9 +63,9 @@ typedef struct ucns_mesh_v1 {
     int32_t index_base;         /* 0 or 1 */
     int32_t nface_max;          /* leading dimension of the per-face arrays */
     const int32_t *ifca;        /* [ncell] faces per cell (4 tet, 5 prism/pyramid, 6 hex) */
-    const int32_t *ineigh;      /* [ncell*nface_max] cell across face L (interior or periodic) */
+    const int32_t *ineigh;      /* [ncell*nface_max] cell across face L (interior or periodic; ignored on boundary faces) */
     const int32_t *ineighn;     /* [ncell*nface_max] face slot of L inside that neighbour */
-    const int32_t *bcode;       /* [ncell*nface_max] 0 interior, else boundary code (5 or 50 periodic) */
+    const int32_t *bcode;       /* [ncell*nface_max] 0 interior, 5 or 50 periodic, 1/2/3/4/6 boundary (ucns_boundary_v1) */
     const int32_t *types_faces; /* [ncell*nface_max] 5 quadrilateral, 6 triangle */
     const double *faceanglex;   /* [ncell*nface_max] outward normal angle a1 */
     const double *faceangley;   /* [ncell*nface_max] outward normal angle a2 */
@@ -120,12 +120,28 @@ typedef struct ucns_stats_v1 {
     int64_t n_firstorder_last;                    /* cells sent to first order by CHECKSOLX */
 } ucns_stats_v1;
 
+/* Ghost states of boundary faces (BOUNDARYS, src/flow_operations.f90:2860-3375),
+ * Euler 3D, no rotating frame.  Face codes (bcode): 1 inflow, 2 outflow,
+ * 3 symmetry, 4 wall (slip, the inviscid branch), 6 farfield.  inflow/outflow
+ * are the conserved states INFLOW() and OUTFLOW() return; this version takes
+ * them uniform over the boundary (a free stream). */
+typedef struct ucns_boundary_v1 {
+    int32_t abi_version;
+    int32_t boundtype;          /* BOUNDTYPE: 0 supersonic (inflow = free stream, outflow = copy),
+                                   1 characteristic (subsonic/supersonic from the interior Mach number) */
+    double inflow[5];           /* conserved */
+    double outflow[5];          /* conserved */
+} ucns_boundary_v1;
+
 typedef struct ucns_ctx ucns_ctx;
 
 /* Set up one partition on the current HIP device (topology, faces, scheme). */
 int  ucns_create(const ucns_mesh_v1 *mesh, const ucns_scheme_v1 *scheme, ucns_ctx **out);
 /* Upload PRESTORE data (pseudo-inverses, indicators, qp coordinates ...). */
 int  ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre);
+/* Free-stream data of the boundary codes 1, 2 and 6 (needed before the first
+ * residual when the mesh has such faces; codes 3 and 4 need none). */
+int  ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc);
 /* Cell averages U_C(i)%VAL(1,1:5) in / out. */
 int  ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout);
 int  ucns_get_state(ucns_ctx *ctx, double *u, int32_t layout);

@@ -45,6 +45,10 @@ def lib() -> C.CDLL:
         L.oracle_weno_weights.restype = None
         L.oracle_barth.argtypes = [C.c_double] * 4
         L.oracle_barth.restype = C.c_double
+        L.oracle_set_boundary.argtypes = [vp]
+        L.oracle_set_boundary.restype = None
+        L.oracle_ghost.argtypes = [C.c_int, C.c_double, f64, C.c_double, C.c_double, f64]
+        L.oracle_ghost.restype = None
         _LIB = L
     return _LIB
 
@@ -131,3 +135,18 @@ def weno_weights(beta, lwci1, eps=1e-11):
 
 def barth(ui, umin, umax, usol):
     return lib().oracle_barth(ui, umin, umax, usol)
+
+
+def set_boundary(boundtype, inflow, outflow):
+    """free-stream data the oracle's boundary faces use (process-wide)"""
+    from ucns3d_amd import native
+    b = native.BoundaryV1(abi_version=native.UCNS_ABI_VERSION, boundtype=boundtype,
+                          inflow=(C.c_double * 5)(*inflow), outflow=(C.c_double * 5)(*outflow))
+    lib().oracle_set_boundary(C.byref(b))
+
+
+def ghost(code, gamma, cl, a1, a2):
+    cl = np.ascontiguousarray(cl, dtype=np.float64)
+    out = np.zeros(5)
+    lib().oracle_ghost(code, gamma, _p(cl), a1, a2, _p(out))
+    return out

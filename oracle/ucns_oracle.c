@@ -59,6 +59,126 @@ void oracle_rotateb(double *vec, const double *rot, double a1, double a2) {
     vec[4] = rot[4];
 }
 
+/* Free-stream data of the boundary codes (set by oracle_set_boundary). */
+static ucns_boundary_v1 g_bc = {UCNS_ABI_VERSION, 0, {1, 0, 0, 0, 2.5}, {1, 0, 0, 0, 2.5}};
+
+void oracle_set_boundary(const ucns_boundary_v1 *bc) { g_bc = *bc; }
+
+#define TOLSMALL 1.0e-13 /* grid_p.f90:18 */
+
+/* PRIM2CONS of (rho, u, v, w, p) with the reference's energy form
+ * rho*(p/((gamma-1) rho)) + rho*(|u|^2/2) (flow_operations.f90:2929-2935) */
+static void bc_prim2cons(double gamma, const double *s3, double *cr) {
+    cr[0] = s3[0];
+    cr[1] = s3[1] * s3[0];
+    cr[2] = s3[2] * s3[0];
+    cr[3] = s3[3] * s3[0];
+    double skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
+    double ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
+    cr[4] = (s3[0] * (ikins)) + (s3[0] * skins);
+}
+
+/* subsonic inflow: pressure from the outgoing characteristic, the rest from
+ * the free stream (flow_operations.f90:2907-2915); s1 free stream, s2 interior */
+static void bc_sub_in(double gamma, const double *s1, const double *s2, double sps, double nx, double ny, double nz,
+                      double *cr) {
+    double s3[NV];
+    s3[4] = 0.5 * ((s1[4]) + (s2[4]) - (s2[0] * sps * ((nx * (s1[1] - s2[1])) + (ny * (s1[2] - s2[2])) + (nz * (s1[3] - s2[3])))));
+    s3[0] = s1[0] + (s3[4] - s1[4]) / (sps * sps);
+    s3[1] = s1[1] - (nx * (s1[4] - s3[4])) / (sps * s2[0]);
+    s3[2] = s1[2] - (ny * (s1[4] - s3[4])) / (sps * s2[0]);
+    s3[3] = s1[3] - (nz * (s1[4] - s3[4])) / (sps * s2[0]);
+    bc_prim2cons(gamma, s3, cr);
+}
+
+/* subsonic outflow: free-stream pressure, the rest from the interior
+ * (flow_operations.f90:2995-3010); s1 free stream, s2 interior */
+static void bc_sub_out(double gamma, const double *s1, const double *s2, double sps, double nx, double ny, double nz,
+                       double *cr) {
+    double s3[NV];
+    s3[4] = s1[4];
+    s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
+    s3[1] = s2[1] + (nx * (s2[4] - s3[4])) / (sps * s2[0]);
+    s3[2] = s2[2] + (ny * (s2[4] - s3[4])) / (sps * s2[0]);
+    s3[3] = s2[3] + (nz * (s2[4] - s3[4])) / (sps * s2[0]);
+    bc_prim2cons(gamma, s3, cr);
+}
+
+/* BOUNDARYS (flow_operations.f90:2860-3375), Euler 3D without a rotating
+ * frame: ghost state cr of a boundary face from the face-point state cl */
+void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2, double *cr) {
+    const double nx = (cos(a1) * sin(a2)), ny = (sin(a1) * sin(a2)), nz = (cos(a2));
+    double s1[NV], s2[NV];
+    switch (code) {
+    case 1: /* inflow (2880-2925) */
+        if (g_bc.boundtype == 0) {
+            for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+        } else {
+            cons2prim(gamma, g_bc.inflow, s1);
+            cons2prim(gamma, cl, s2);
+            double sps = sqrt((gamma * s2[4]) / (s2[0]));
+            double vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
+            if (vel / (sps + TOLSMALL) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+            } else {
+                bc_sub_in(gamma, s1, s2, sps, nx, ny, nz, cr);
+            }
+        }
+        break;
+    case 2: /* outflow (2965-3013) */
+        if (g_bc.boundtype == 0) {
+            for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+        } else {
+            cons2prim(gamma, g_bc.outflow, s1);
+            cons2prim(gamma, cl, s2);
+            double sps = sqrt((gamma * s2[4]) / (s2[0]));
+            double vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
+            if (vel / (sps + TOLSMALL) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+            } else {
+                bc_sub_out(gamma, s1, s2, sps, nx, ny, nz, cr);
+            }
+        }
+        break;
+    case 3: /* symmetry (3086-3131) */
+    case 4: /* wall, inviscid branch (3132-3160) */
+    {
+        double rot[NV], rr[NV];
+        oracle_rotatef(rot, cl, a1, a2);
+        for (int v = 0; v < NV; ++v) rr[v] = rot[v];
+        rr[1] = -rot[1];
+        oracle_rotateb(cr, rr, a1, a2);
+        break;
+    }
+    case 6: /* farfield (3203-3296): direction from the rotated normal momentum */
+    {
+        double rot[NV];
+        oracle_rotatef(rot, cl, a1, a2);
+        const double vnb = rot[1];
+        cons2prim(gamma, cl, s2);
+        double sps = sqrt((gamma * s2[4]) / (s2[0]));
+        if (vnb <= 0.0) {
+            if (fabs(vnb) >= sps) {
+                for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+            } else {
+                cons2prim(gamma, g_bc.inflow, s1);
+                bc_sub_in(gamma, s1, s2, sps, nx, ny, nz, cr);
+            }
+        } else {
+            if (fabs(vnb) >= sps) {
+                for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+            } else {
+                cons2prim(gamma, g_bc.outflow, s1);
+                bc_sub_out(gamma, s1, s2, sps, nx, ny, nz, cr);
+            }
+        }
+        break;
+    }
+    default:
+        for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+    }
+}
+
 /* HLLC_RIEMANN_SOLVER, Euler, no MRF (riemann.f90:221-426) */
 void oracle_hllc(double gamma, const double *cl, const double *cr, double *flux) {
     double wl[NV], wr[NV];
@@ -378,12 +498,21 @@ int oracle_fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_pre
             const double a1 = m->faceanglex[slot], a2 = m->faceangley[slot];
             const int iqp = nqp_of(&c, i, f);
             const double *wts = m->types_faces[slot] == 5 ? s->weights_quad : s->weights_tri;
-            const int64_t j = m->ineigh[slot] - c.base;
-            const int fj = m->ineighn[slot] - c.base;
+            const int bcode = m->bcode[slot];
+            const int bnd = bcode != 0 && bcode != 5 && bcode != 50;
+            const int64_t j = bnd ? 0 : m->ineigh[slot] - c.base;
+            const int fj = bnd ? 0 : m->ineighn[slot] - c.base;
             double god[NV] = {0, 0, 0, 0, 0};
             for (int q = 0; q < iqp; ++q) {
                 const double *cl = uleft_at(&c, (double *)ul, i, f, q);
-                const double *cr = uleft_at(&c, (double *)ul, j, fj, q);
+                double ghost[NV];
+                const double *cr;
+                if (bnd) { /* GET_STATES_BOUNDS -> BOUNDARYS (dg_functions.f90:1808-1828) */
+                    oracle_ghost(bcode, s->gamma, cl, a1, a2, ghost);
+                    cr = ghost;
+                } else {
+                    cr = uleft_at(&c, (double *)ul, j, fj, q);
+                }
                 double clr[NV], crr[NV], h[NV], rh[NV];
                 oracle_rotatef(crr, cr, a1, a2);
                 oracle_rotatef(clr, cl, a1, a2);

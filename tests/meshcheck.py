@@ -98,9 +98,11 @@ class Mesh:
         return v
 
     def unwrap_shift(self, ref, p):
-        """whole-period shift moving p within half a period of ref"""
+        """whole-period shift moving p within half a period of ref (bounded
+        axes have period 0 and never shift)"""
         d = p - ref
-        return -np.round(d / self.L) * self.L
+        L = np.where(self.L > 0, self.L, 1.0)
+        return np.where(self.L > 0, -np.round(d / L) * L, 0.0)
 
     def cell_rule(self, i, n=4, shift=None):
         """global quadrature points and weights (sum = volume) of cell i"""
@@ -129,8 +131,7 @@ class Mesh:
         members unwrapped to lie next to cell i"""
         pts, wts = self.rules(n)
         members = np.asarray(members)
-        sh = self.centre[members] - self.centre[i]
-        shift = -np.round(sh / self.L) * self.L
+        shift = self.unwrap_shift(self.centre[i], self.centre[members])
         x = pts[members] + shift[:, None, :]
         y = self.to_local(i, x.reshape(-1, 3)).reshape(len(members), -1, 3)
         w = wts[members]

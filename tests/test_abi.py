@@ -45,7 +45,7 @@ def test_header_lists_the_path():
     for fn in ["ucns_create", "ucns_upload_prestore", "ucns_set_state", "ucns_get_state", "ucns_residual",
                "ucns_reconstruct", "ucns_rk3_steps", "ucns_compute_dt", "ucns_destroy", "ucns_last_error"]:
         assert fn in fns
-    assert len(fns) == 15
+    assert len(fns) == 16
 
 
 def test_every_declared_symbol_is_exported(glib):
@@ -69,7 +69,8 @@ def test_struct_layouts_match_header():
     """ctypes mirrors: field count and order of each struct in the header."""
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in [("ucns_mesh_v1", native.MeshV1), ("ucns_scheme_v1", native.SchemeV1),
-                      ("ucns_prestore_v1", native.PrestoreV1), ("ucns_stats_v1", native.StatsV1)]:
+                      ("ucns_prestore_v1", native.PrestoreV1), ("ucns_stats_v1", native.StatsV1),
+                      ("ucns_boundary_v1", native.BoundaryV1)]:
         body = re.search(r"typedef struct %s \{(.*?)\}" % cname, text, flags=re.S).group(1)
         fields = []
         for decl in body.split(";"):
@@ -77,7 +78,7 @@ def test_struct_layouts_match_header():
             if not decl:
                 continue
             names = decl.split(None, 1)[1] if not decl.startswith("const") else decl.split(None, 2)[2]
-            fields += [n.strip().lstrip("*").strip() for n in names.split(",")]
+            fields += [n.strip().lstrip("*").strip().split("[")[0] for n in names.split(",")]
         assert fields == [f for f, _ in py._fields_], (cname, fields)
 
 
@@ -130,6 +131,7 @@ def test_null_context_is_rejected(glib):
     d = C.c_double()
     st = native.StatsV1()
     assert glib.ucns_upload_prestore(None, None) == -1
+    assert glib.ucns_set_boundary(None, None) == -1
     assert glib.ucns_set_state(None, None, 0) == -1
     assert glib.ucns_get_state(None, None, 0) == -1
     assert glib.ucns_set_state_device(None, None, 0) == -1

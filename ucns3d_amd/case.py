@@ -34,6 +34,7 @@ class CaseSpec:
     length: float = 2.0 * math.pi
     lwci1: float = 10000.0   # wenocentralweight of tests/taylor_green_vortex/UCNS3D.DAT
     gamma: float = 1.4
+    bc: tuple = (0, 0, 0, 0, 0, 0)  # codes of the x-, x+, y-, y+, z-, z+ sides; 0 = periodic axis
 
     @property
     def iorder(self) -> int:
@@ -67,7 +68,7 @@ PRESETS = {
 def reduced(key: str, n: int) -> CaseSpec:
     s = PRESETS[key]
     return CaseSpec(f"{key}-reduced-{n}", s.shape, n, s.scheme, s.jitter, s.seed, s.noise, s.cfl, s.steps,
-                    s.length, s.lwci1, s.gamma)
+                    s.length, s.lwci1, s.gamma, s.bc)
 
 
 class Case:
@@ -76,7 +77,8 @@ class Case:
         lib = native.prep_lib()
         o = native.PrepOpts(shape=spec.shape, nx=spec.n, ny=spec.n, nz=spec.n,
                             lx=spec.length, ly=spec.length, lz=spec.length, jitter=spec.jitter, seed=spec.seed,
-                            iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads)
+                            iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads,
+                            bc=(C.c_int32 * 6)(*spec.bc))
         h = C.c_void_p()
         rc = lib.ucnsp_build(C.byref(o), C.byref(h))
         if rc != 0:

@@ -473,6 +473,23 @@ void build_connectivity(ucnsp_case &c) {
     }
     for (int64_t i = 0; i < nc; ++i)
         for (int f = 0; f < c.ifca[i]; ++f) c.types_faces[i * NF + f] = face_nv(c.cshape[i], f) == 4 ? 5 : 6;
+    // unmatched faces lie on a bounded side of the box: code of that side
+    const double L[3] = {c.o.lx, c.o.ly, c.o.lz};
+    for (int64_t i = 0; i < nc; ++i)
+        for (int f = 0; f < c.ifca[i]; ++f) {
+            if (c.ineigh[i * NF + f] >= 0) continue;
+            int sh = c.cshape[i], nv = face_nv(sh, f), side = -1;
+            for (int d = 0; d < 3 && side < 0; ++d)
+                for (int e = 0; e < 2 && side < 0; ++e) {
+                    bool on = true;
+                    for (int k = 0; k < nv; ++k) {
+                        const double x = c.xyz[(size_t)c.cnodes[i * 8 + face_node(sh, f, k)] * 3 + d];
+                        if (std::fabs(x - e * L[d]) > 1e-9 * L[d]) on = false;
+                    }
+                    if (on) side = 2 * d + e;
+                }
+            c.bcode[i * NF + f] = side >= 0 ? c.o.bc[side] : -1;
+        }
 }
 
 inline void node_xyz(const ucnsp_case &c, int32_t node, double *p) {
@@ -921,7 +938,7 @@ void sig_cell_nodes(const ucnsp_case &c, int64_t i, int64_t j, SigWriter &w) {
 }
 
 uint64_t base_signature(const ucnsp_case &c, int64_t i, std::vector<int64_t> *out) {
-    SigWriter w{1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2])), 1469598103934665603ULL, out};
+    SigWriter w{1e-9 * std::max(c.o.lx, std::max(c.o.ly, c.o.lz)), 1469598103934665603ULL, out};
     sig_cell_nodes(c, i, i, w);
     const double *ci = &c.centre[i * 3];
     for (int f = 0; f < c.ifca[i]; ++f)
@@ -939,7 +956,7 @@ uint64_t stencil_signature(const ucnsp_case &c, int64_t i, int s, const std::vec
                            std::vector<int64_t> *out) {
     // cells are described by their verified geometry class (node offsets
     // from the centre) and members also by their centre offset
-    SigWriter w{1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2])), 1469598103934665603ULL, out};
+    SigWriter w{1e-9 * std::max(c.o.lx, std::max(c.o.ly, c.o.lz)), 1469598103934665603ULL, out};
     w.mix(brep[i]);
     const double *ci = &c.centre[i * 3];
     const int32_t *mem = &c.ihexl[((size_t)i * c.typesten + s) * c.nnb];
@@ -1207,7 +1224,7 @@ void build_prestore(ucnsp_case &c) {
     const int64_t ns = nc * TS;
     std::vector<int64_t> srep, grep;
     if (cache) {
-        const double qg = 1e-9 * std::max(c.per[0], std::max(c.per[1], c.per[2]));
+        const double qg = 1e-9 * std::max(c.o.lx, std::max(c.o.ly, c.o.lz));
         classes(nc, grep,
                 [&](int64_t i) {
                     SigWriter w{qg, 1469598103934665603ULL, nullptr};
@@ -1276,9 +1293,10 @@ void gen_box(ucnsp_case &c) {
     const ucnsp_opts &o = c.o;
     const int nx = o.nx, ny = o.ny, nz = o.nz;
     const double hx = o.lx / nx, hy = o.ly / ny, hz = o.lz / nz;
-    c.per[0] = o.lx;
-    c.per[1] = o.ly;
-    c.per[2] = o.lz;
+    const bool pax[3] = {o.bc[0] == 0, o.bc[2] == 0, o.bc[4] == 0};  // periodic axes
+    c.per[0] = pax[0] ? o.lx : 0.0;
+    c.per[1] = pax[1] ? o.ly : 0.0;
+    c.per[2] = pax[2] ? o.lz : 0.0;
     c.nnode = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
     c.xyz.resize(c.nnode * 3);
     c.canon.resize(c.nnode);
@@ -1297,7 +1315,9 @@ void gen_box(ucnsp_case &c) {
                 c.xyz[n * 3] = p[0];
                 c.xyz[n * 3 + 1] = p[1];
                 c.xyz[n * 3 + 2] = p[2];
-                c.canon[n] = (int32_t)((i % nx) + nx * ((j % ny) + ny * (k % nz)));
+                // nodes on opposite sides of a periodic axis share a canonical id
+                const int ci = pax[0] ? i % nx : i, cj = pax[1] ? j % ny : j, ck = pax[2] ? k % nz : k;
+                c.canon[n] = (int32_t)(ci + (nx + 1) * (cj + (int64_t)(ny + 1) * ck));
             }
     const int64_t nhex = (int64_t)nx * ny * nz;
     c.shape = o.shape;
@@ -1382,7 +1402,14 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
     if (o.shape != UCNSP_HEX && o.shape != UCNSP_TET) return fail(UCNS_EUNSUPPORTED, "shape must be hex(1) or tet(2)");
-    if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "periodic box needs at least 3 cells per direction");
+    if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
+    for (int d = 0; d < 3; ++d) {
+        const int a = o.bc[2 * d], b = o.bc[2 * d + 1];
+        if ((a == 0) != (b == 0)) return fail(UCNS_EBADARG, "a periodic axis needs code 0 on both sides");
+        for (int v : {a, b})
+            if (v != 0 && v != 1 && v != 2 && v != 3 && v != 4 && v != 6)
+                return fail(UCNS_EUNSUPPORTED, "side codes: 0 periodic, 1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield");
+    }
     if (o.iorder < 1 || o.iorder > 3) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..3");
     if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
     if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
@@ -1404,9 +1431,9 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     lap("connectivity");
     for (int64_t i = 0; i < c->ncell; ++i)
         for (int f = 0; f < c->ifca[i]; ++f)
-            if (c->ineigh[i * ucnsp_case::NF + f] < 0) {
+            if (c->ineigh[i * ucnsp_case::NF + f] < 0 && c->bcode[i * ucnsp_case::NF + f] <= 0) {
                 delete c;
-                return fail(UCNS_EBADARG, "unmatched face in a periodic box");
+                return fail(UCNS_EBADARG, "unmatched face off the box sides");
             }
     build_geometry(*c);
     lap("geometry");

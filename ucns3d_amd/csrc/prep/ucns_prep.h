@@ -2,7 +2,7 @@
  * ucns_prep.h -- host-side FP64 preprocessor (libucnsprep.so).
  *
  * Builds everything the residual path reads but never writes: the mesh
- * (synthetic periodic boxes), face connectivity, geometry, stencils and the
+ * (synthetic boxes, periodic or bounded per axis), face connectivity, geometry, stencils and the
  * PRESTORE arrays (pseudo-inverses, smoothness-indicator matrices, basis
  * integrals, face Gauss points in each cell's local frame).  It restates the
  * reference's setup chain:
@@ -44,6 +44,9 @@ typedef struct ucnsp_opts {
     int32_t extf;           /* stencil extension factor (profile 8: 2) */
     int32_t weight_lsqr;    /* 0 unweighted, 1 inverse-distance */
     int32_t nthreads;       /* OpenMP threads, <=0: library default */
+    int32_t bc[6];          /* boundary code of the x-, x+, y-, y+, z-, z+ sides; 0 on both sides of an
+                               axis makes it periodic (code 5 on the paired faces).  Codes as IBOUND%ICODE:
+                               1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield */
 } ucnsp_opts;
 
 typedef struct ucnsp_case ucnsp_case;

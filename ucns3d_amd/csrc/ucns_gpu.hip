@@ -568,7 +568,14 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     // the MUSCL branch below re-tests and overwrites its face points.
     const int nslot = ifca * p.nqm;
     const R *qpc = p.qp + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * 3;
-    const R *mb = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
+    // the cell's basis means, once (p.mbar may alias p.uleft for the
+    // compiler, so loads inside the face-point loop would not be hoisted)
+    R mb[IDEG];
+    {
+        const R *mbg = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k) mb[k] = mbg[k];
+    }
     R *out = p.uleft + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * NV;
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
@@ -1216,7 +1223,133 @@ struct FluxParams {
     const double *U;        // [ncell][5] cell averages
     const R *uleft;         // face-point deviations from the cell average
     double *fflux;          // [nface][5] outward from the owner
+    ucns_boundary_v1 bc;    // free-stream data of the boundary codes
 };
+
+// ---------------------------------------------------------------------------
+// Ghost states of boundary faces: BOUNDARYS (src/flow_operations.f90:2860-3375),
+// Euler 3D without a rotating frame, in FP64 in both precisions.
+#define UCNS_TOLSMALL 1.0e-13  // src/grid_p.f90:18
+
+__device__ inline void bc_prim2cons(double gamma, const double *s3, double *cr) {
+    cr[0] = s3[0];
+    cr[1] = s3[1] * s3[0];
+    cr[2] = s3[2] * s3[0];
+    cr[3] = s3[3] * s3[0];
+    const double skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
+    const double ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
+    cr[4] = (s3[0] * (ikins)) + (s3[0] * skins);
+}
+
+// subsonic inflow (src/flow_operations.f90:2907-2915): s1 free stream, s2 interior
+__device__ inline void bc_sub_in(double gamma, const double *s1, const double *s2, double sps, double nx, double ny,
+                                 double nz, double *cr) {
+    double s3[NV];
+    s3[4] = 0.5 * ((s1[4]) + (s2[4]) - (s2[0] * sps * ((nx * (s1[1] - s2[1])) + (ny * (s1[2] - s2[2])) + (nz * (s1[3] - s2[3])))));
+    s3[0] = s1[0] + (s3[4] - s1[4]) / (sps * sps);
+    s3[1] = s1[1] - (nx * (s1[4] - s3[4])) / (sps * s2[0]);
+    s3[2] = s1[2] - (ny * (s1[4] - s3[4])) / (sps * s2[0]);
+    s3[3] = s1[3] - (nz * (s1[4] - s3[4])) / (sps * s2[0]);
+    bc_prim2cons(gamma, s3, cr);
+}
+
+// subsonic outflow (src/flow_operations.f90:2995-3010): s1 free stream, s2 interior
+__device__ inline void bc_sub_out(double gamma, const double *s1, const double *s2, double sps, double nx, double ny,
+                                  double nz, double *cr) {
+    double s3[NV];
+    s3[4] = s1[4];
+    s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
+    s3[1] = s2[1] + (nx * (s2[4] - s3[4])) / (sps * s2[0]);
+    s3[2] = s2[2] + (ny * (s2[4] - s3[4])) / (sps * s2[0]);
+    s3[3] = s2[3] + (nz * (s2[4] - s3[4])) / (sps * s2[0]);
+    bc_prim2cons(gamma, s3, cr);
+}
+
+// ROTATEF / ROTATEB with precomputed cos/sin of the face angles
+__device__ inline void rotf(const double *a, double c1, double s1, double c2, double s2, double *r) {
+    r[0] = a[0];
+    r[1] = c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3];
+    r[2] = c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3];
+    r[3] = (-s1) * a[1] + c1 * a[2];
+    r[4] = a[4];
+}
+__device__ inline void rotb(const double *h, double c1, double s1, double c2, double s2, double *r) {
+    r[0] = h[0];
+    r[1] = c1 * s2 * h[1] + c1 * c2 * h[2] + (-s1) * h[3];
+    r[2] = s1 * s2 * h[1] + s1 * c2 * h[2] + c1 * h[3];
+    r[3] = c2 * h[1] + (-s2) * h[2];
+    r[4] = h[4];
+}
+
+__device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double gamma, const double *cl, double c1,
+                                   double s1, double c2, double s2, double *cr) {
+    const double nx = c1 * s2, ny = s1 * s2, nz = c2;
+    double f[NV], w[NV];
+    switch (code) {
+    case 1:  // inflow (src/flow_operations.f90:2880-2925)
+        if (bc.boundtype == 0) {
+            for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+        } else {
+            cons2prim_d(gamma, bc.inflow, f);
+            cons2prim_d(gamma, cl, w);
+            const double sps = sqrt((gamma * w[4]) / (w[0]));
+            const double vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            if (vel / (sps + UCNS_TOLSMALL) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+            } else {
+                bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
+            }
+        }
+        break;
+    case 2:  // outflow (src/flow_operations.f90:2965-3013)
+        if (bc.boundtype == 0) {
+            for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+        } else {
+            cons2prim_d(gamma, bc.outflow, f);
+            cons2prim_d(gamma, cl, w);
+            const double sps = sqrt((gamma * w[4]) / (w[0]));
+            const double vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            if (vel / (sps + UCNS_TOLSMALL) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+            } else {
+                bc_sub_out(gamma, f, w, sps, nx, ny, nz, cr);
+            }
+        }
+        break;
+    case 3:  // symmetry (src/flow_operations.f90:3086-3131)
+    case 4:  // wall, inviscid branch (src/flow_operations.f90:3132-3160)
+    {
+        double rot[NV];
+        rotf(cl, c1, s1, c2, s2, rot);
+        rot[1] = -rot[1];
+        rotb(rot, c1, s1, c2, s2, cr);
+        break;
+    }
+    default:  // 6, farfield (src/flow_operations.f90:3203-3296)
+    {
+        double rot[NV];
+        rotf(cl, c1, s1, c2, s2, rot);
+        const double vnb = rot[1];
+        cons2prim_d(gamma, cl, w);
+        const double sps = sqrt((gamma * w[4]) / (w[0]));
+        if (vnb <= 0.0) {
+            if (fabs(vnb) >= sps) {
+                for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+            } else {
+                cons2prim_d(gamma, bc.inflow, f);
+                bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
+            }
+        } else {
+            if (fabs(vnb) >= sps) {
+                for (int v = 0; v < NV; ++v) cr[v] = cl[v];
+            } else {
+                cons2prim_d(gamma, bc.outflow, f);
+                bc_sub_out(gamma, f, w, sps, nx, ny, nz, cr);
+            }
+        }
+    }
+    }
+}
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
@@ -1230,14 +1363,15 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
     const double *w = tri ? c_wt : c_wq;
     const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
     const double area = p.farea[f];
+    const bool bnd = cR < 0;  // boundary face: -code
     double uL[NV], uR[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         uL[v] = p.U[(size_t)cL * NV + v];
-        uR[v] = p.U[(size_t)cR * NV + v];
+        uR[v] = bnd ? 0.0 : p.U[(size_t)cR * NV + v];
     }
     const R *ulL = p.uleft + ((size_t)cL * p.nf + sL) * p.nqm * NV;
-    const R *ulR = p.uleft + ((size_t)cR * p.nf + sR) * p.nqm * NV;
+    const R *ulR = p.uleft + ((size_t)(bnd ? cL : cR) * p.nf + (bnd ? sL : sR)) * p.nqm * NV;
     double acc[NV] = {0, 0, 0, 0, 0};
     for (int q = 0; q < nq; ++q) {
         double a[NV], bq[NV];
@@ -1246,6 +1380,7 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
             a[v] = uL[v] + (double)ulL[q * NV + v];
             bq[v] = uR[v] + (double)ulR[q * NV + v];
         }
+        if (bnd) ghost_state(-cR, p.bc, p.gamma, a, c1, s1, c2, s2, bq);  // GET_STATES_BOUNDS
         // ROTATEF (src/grid_t.f90:4217-4258)
         const double la[NV] = {a[0], c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3],
                                c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3], (-s1) * a[1] + c1 * a[2], a[4]};
@@ -1411,6 +1546,8 @@ struct ucns_ctx {
     bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
     bool ring = false;  // UCNS_RING=1 selects the LDS-ring kernels (experimental)
     int smode = 0;      // stencils per cell of the row-stream kernel in use (0: none)
+    bool needs_bc = false, have_bc = false;  // boundary codes 1/2/6 present; ucns_set_boundary called
+    ucns_boundary_v1 bc{};
     int ntile = 0;
     // mesh (device)
     int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
@@ -1576,6 +1713,7 @@ void launch_flux_t(ucns_ctx *ctx) {
     p.U = ctx->U;
     p.uleft = (const R *)ctx->uleft;
     p.fflux = ctx->fflux;
+    p.bc = ctx->bc;
     hipLaunchKernelGGL((k_flux<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, p);
 }
 
@@ -1808,8 +1946,26 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
         for (int f = 0; f < ifca; ++f) {
             const size_t sl = (size_t)i * nfa + f;
             const int bc = m->bcode[sl];
+            if (bc == 1 || bc == 2 || bc == 3 || bc == 4 || bc == 6) {
+                // boundary face: owner only, the right state is a ghost (BOUNDARYS)
+                if (bc != 3 && bc != 4) ctx->needs_bc = true;
+                cnb[(size_t)i * nf + f] = -1;
+                const int id = (int)farea.size();
+                fcell.push_back(i);
+                fcell.push_back(-bc);
+                fslot.push_back(f | (m->types_faces[sl] == 6 ? 256 : 0));
+                fslot.push_back(0);
+                const double a1 = m->faceanglex[sl], a2 = m->faceangley[sl];
+                frot.push_back(std::cos(a1));
+                frot.push_back(std::sin(a1));
+                frot.push_back(std::cos(a2));
+                frot.push_back(std::sin(a2));
+                farea.push_back(m->surf[sl]);
+                cface[(size_t)i * nf + f] = id + 1;
+                continue;
+            }
             if (bc != 0 && bc != 5 && bc != 50)  // periodic: 5 or 50 (src/dg_functions.f90:1786)
-                return bad(UCNS_EUNSUPPORTED, "only interior and periodic (code 5/50) faces are supported");
+                return bad(UCNS_EUNSUPPORTED, "face codes: 0 interior, 5/50 periodic, 1/2/3/4/6 boundary");
             const int j = m->ineigh[sl] - base, fj = m->ineighn[sl] - base;
             if (j < 0 || j >= nc || fj < 0 || fj >= m->ifca[j])
                 return bad(UCNS_EBADARG, "face neighbour out of range at cell " + std::to_string(i));
@@ -2077,6 +2233,30 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     return UCNS_OK;
 }
 
+int ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc) {
+    if (!ctx || !bc) return UCNS_EBADARG;
+    if (bc->abi_version != UCNS_ABI_VERSION) {
+        ctx->err = "abi_version mismatch";
+        return UCNS_EBADARG;
+    }
+    if (bc->boundtype != 0 && bc->boundtype != 1) {
+        ctx->err = "boundtype must be 0 or 1";
+        return UCNS_EBADARG;
+    }
+    for (int v = 0; v < NV; ++v)
+        if (!std::isfinite(bc->inflow[v]) || !std::isfinite(bc->outflow[v])) {
+            ctx->err = "non-finite free-stream state";
+            return UCNS_EBADARG;
+        }
+    if (!(bc->inflow[0] > 0) || !(bc->outflow[0] > 0)) {
+        ctx->err = "free-stream density must be positive";
+        return UCNS_EBADARG;
+    }
+    ctx->bc = *bc;
+    ctx->have_bc = true;
+    return UCNS_OK;
+}
+
 int ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -2106,6 +2286,10 @@ int ucns_get_state_device(ucns_ctx *ctx, double *u, int32_t layout) {
 }
 
 static int ready(ucns_ctx *ctx) {
+    if (ctx->needs_bc && !ctx->have_bc) {
+        ctx->err = "the mesh has inflow/outflow/farfield faces: call ucns_set_boundary first";
+        return UCNS_ESTATE;
+    }
     if (!ctx->have_prestore) {
         ctx->err = "ucns_upload_prestore has not been called";
         return UCNS_ESTATE;

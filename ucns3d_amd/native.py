@@ -48,6 +48,13 @@ class PrestoreV1(C.Structure):
     ]
 
 
+class BoundaryV1(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("boundtype", C.c_int32),
+        ("inflow", C.c_double * 5), ("outflow", C.c_double * 5),
+    ]
+
+
 class StatsV1(C.Structure):
     _fields_ = [
         ("n_recon", C.c_int64), ("n_flux", C.c_int64), ("n_update", C.c_int64), ("n_dt", C.c_int64),
@@ -62,7 +69,7 @@ class PrepOpts(C.Structure):
         ("shape", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double), ("jitter", C.c_double),
         ("seed", C.c_uint64), ("iorder", C.c_int32), ("iweno", C.c_int32), ("extf", C.c_int32),
-        ("weight_lsqr", C.c_int32), ("nthreads", C.c_int32),
+        ("weight_lsqr", C.c_int32), ("nthreads", C.c_int32), ("bc", C.c_int32 * 6),
     ]
 
 
@@ -70,6 +77,7 @@ class PrepOpts(C.Structure):
 GPU_API = {
     "ucns_create": (C.c_int, [C.POINTER(MeshV1), C.POINTER(SchemeV1), C.POINTER(C.c_void_p)]),
     "ucns_upload_prestore": (C.c_int, [C.c_void_p, C.POINTER(PrestoreV1)]),
+    "ucns_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(BoundaryV1)]),
     "ucns_set_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_get_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_set_state_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),

@@ -44,6 +44,14 @@ class Solver:
             msg = self.L.ucns_last_error(self._h).decode() if self._h else ""
             raise UcnsError(fn, rc, msg)
 
+    # -- boundaries --------------------------------------------------------------
+    def set_boundary(self, boundtype: int, inflow, outflow) -> None:
+        """free-stream data of the inflow/outflow/farfield codes (BOUNDARYS,
+        src/flow_operations.f90:2860-3375); conserved 5-vectors"""
+        b = native.BoundaryV1(abi_version=native.UCNS_ABI_VERSION, boundtype=boundtype,
+                              inflow=(C.c_double * 5)(*inflow), outflow=(C.c_double * 5)(*outflow))
+        self._check("ucns_set_boundary", self.L.ucns_set_boundary(self._h, C.byref(b)))
+
     # -- state -----------------------------------------------------------------
     def set_state(self, u: np.ndarray) -> None:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.ncell, 5)
