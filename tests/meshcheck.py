@@ -84,6 +84,7 @@ class Mesh:
         self.normal = case.array("normal").reshape(nc, 6, 3)
         self.ineigh = case.array("ineigh").reshape(nc, 6)
         self.ineighn = case.array("ineighn").reshape(nc, 6)
+        self.bcode = case.array("bcode").reshape(nc, 6)
         self.fnodes = case.array("face_nodes").reshape(nc, 6, 4)
         self.frame = case.array("frame").reshape(nc, 12)
 

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from ucns3d_amd.case import PRESETS, Case, reduced
+from ucns3d_amd.case import HEX, PRESETS, TET, Case, CaseSpec, reduced
 from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
@@ -20,6 +20,8 @@ CASES = {
     "hex-muscl-8": lambda: reduced("c1", 8),
     "hex-weno4-12": lambda: reduced("c3", 12),
     "tet-weno4-6": lambda: reduced("c2", 6),
+    "hex-weno3-10": lambda: CaseSpec("hex-weno3-10", HEX, 10, "weno3", seed=11),
+    "tet-weno3-5": lambda: CaseSpec("tet-weno3-5", TET, 5, "weno3", jitter=0.1, seed=12),
 }
 
 _cache = {}
@@ -142,7 +144,7 @@ def test_config1_full_size_100_steps():
     s.close()
 
 
-@pytest.mark.parametrize("name", ["hex-weno4-12", "tet-weno4-6"])
+@pytest.mark.parametrize("name", ["hex-weno4-12", "tet-weno4-6", "hex-weno3-10", "tet-weno3-5"])
 def test_mixed_vs_fp64(name):
     case, orc, u = get(name)
     print(f"[mixed] {name}", flush=True)

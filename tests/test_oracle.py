@@ -293,6 +293,8 @@ def interior_cells(case, m, exps):
     ("tet-weno4-cubic", lambda: CaseSpec("tet-weno4-6", TET, 6, "weno4", jitter=0.1, seed=5), 3),
     ("hex-muscl-linear", lambda: reduced("c1", 6), 1),
     ("tet-muscl-linear", lambda: CaseSpec("tet-muscl-6", TET, 6, "muscl", jitter=0.15, seed=7), 1),
+    ("hex-weno3-quadratic", lambda: CaseSpec("hex-weno3", HEX, 10, "weno3", seed=5), 2),
+    ("tet-weno3-quadratic", lambda: CaseSpec("tet-weno3", TET, 5, "weno3", jitter=0.1, seed=5), 2),
 ])
 def test_reconstruction_reproduces_polynomials(name, spec, deg):
     """Cell averages of a polynomial of the scheme's degree are reconstructed
@@ -329,7 +331,7 @@ def test_reconstruction_reproduces_polynomials(name, spec, deg):
         for v in range(5):
             exact = base[v] + amp[v] * polys[v](xg)
             got = ul[i, :nf, :nq, v].ravel()
-            if deg == 3:
+            if deg >= 2:
                 worst = max(worst, np.abs(got - exact).max() / amp[v])
             else:
                 # MUSCL: the Barth-Jespersen limiter scales the exact linear
@@ -442,6 +444,7 @@ def wave_averages(m, k, u0):
     (HEX, "weno4", (12, 24), 3.0),
     (TET, "weno4", (4, 8), 2.7),
     (HEX, "muscl", (8, 16), 1.4),  # Barth clips the slope at every extremum
+    (HEX, "weno3", (12, 24), 2.4),
 ])
 def test_residual_converges_at_the_scheme_order(shape, scheme, ns, order):
     """-RHS/V approximates d/dt of the exact cell averages of a smooth

@@ -25,7 +25,7 @@ class CaseSpec:
     name: str
     shape: int            # HEX or TET (6 Kuhn tets per hex)
     n: int                # hexes per direction
-    scheme: str           # "muscl" (2nd order, Barth) or "weno4" (IORDER 3)
+    scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2) or "weno4" (IORDER 3)
     jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
     seed: int = 1
     noise: float = 0.0    # N(0,1) density perturbation amplitude per cell
@@ -38,7 +38,7 @@ class CaseSpec:
 
     @property
     def iorder(self) -> int:
-        return 1 if self.scheme == "muscl" else 3
+        return {"muscl": 1, "weno3": 2, "weno4": 3}[self.scheme]
 
     @property
     def iweno(self) -> int:

@@ -1640,22 +1640,33 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
         if (ctx->smode) {
+            // (ideg, imax, face-point slots, stencils): WENO-4 tets / hexes, WENO-3 tets / hexes
             if constexpr (sizeof(R) == 8) {
-                if (ctx->smode == 5)
-                    hipLaunchKernelGGL((k_recon_s64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_s64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_s64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_s64<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
                 else
-                    hipLaunchKernelGGL((k_recon_s64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                    hipLaunchKernelGGL((k_recon_s64<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
             } else {
-                if (ctx->smode == 5)
-                    hipLaunchKernelGGL((k_recon_s32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_s32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_s32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
                 else
-                    hipLaunchKernelGGL((k_recon_s32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                    hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
             }
             return UCNS_OK;
         }
         // pseudo-inverse rows streamed through an LDS ring (tetrahedra: 5
         // stencils, hexahedra: 7)
-        const bool ring_ok = ctx->ring && ((ctx->ts == 5 && ctx->nf * ctx->nqm <= 24) ||
+        const bool ring_ok = ctx->ring && ctx->ideg == 19 && ((ctx->ts == 5 && ctx->nf * ctx->nqm <= 24) ||
                                                (ctx->ts == 7 && ctx->nf * ctx->nqm > 24));
         if (ring_ok) {
             if constexpr (sizeof(R) == 8) {
@@ -1672,12 +1683,16 @@ int launch_recon_t(ucns_ctx *ctx) {
             return UCNS_OK;
         }
         if constexpr (sizeof(R) == 8) {
-            if (ctx->nf * ctx->nqm <= 24)
+            if (ctx->ideg == 9)
+                hipLaunchKernelGGL((k_recon_w64<9, 19, 3>), grid, block, 0, ctx->stream, w);
+            else if (ctx->nf * ctx->nqm <= 24)
                 hipLaunchKernelGGL((k_recon_w64<19, 39, 3>), grid, block, 0, ctx->stream, w);
             else
                 hipLaunchKernelGGL((k_recon_w64<19, 39, 7>), grid, block, 0, ctx->stream, w);
         } else {
-            if (ctx->nf * ctx->nqm <= 24)
+            if (ctx->ideg == 9)
+                hipLaunchKernelGGL((k_recon_w32<9, 19, 3>), grid, block, 0, ctx->stream, w);
+            else if (ctx->nf * ctx->nqm <= 24)
                 hipLaunchKernelGGL((k_recon_w32<19, 39, 3>), grid, block, 0, ctx->stream, w);
             else
                 hipLaunchKernelGGL((k_recon_w32<19, 39, 7>), grid, block, 0, ctx->stream, w);
@@ -1687,6 +1702,8 @@ int launch_recon_t(ucns_ctx *ctx) {
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);
     if (ctx->ideg == 19 && ctx->imax == 39)
         hipLaunchKernelGGL((k_recon<R, 19, 39, 7>), grid, block, 0, ctx->stream, p);
+    else if (ctx->ideg == 9 && ctx->imax == 19)
+        hipLaunchKernelGGL((k_recon<R, 9, 19, 7>), grid, block, 0, ctx->stream, p);
     else if (ctx->ideg == 3 && ctx->imax == 8)
         hipLaunchKernelGGL((k_recon<R, 3, 8, 1>), grid, block, 0, ctx->stream, p);
     else {
@@ -1906,8 +1923,9 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
     ctx->gamma = s->gamma;
     ctx->lwci1 = s->lwci1;
     ctx->eps = s->weno_eps;
-    if (!((ctx->ideg == 19 && ctx->imax == 39) || (ctx->ideg == 3 && ctx->imax == 8)))
-        return bad(UCNS_EUNSUPPORTED, "supported (idegfree, inumneighbours): (19, 40) WENO-4 and (3, 9) MUSCL");
+    if (!((ctx->ideg == 19 && ctx->imax == 39) || (ctx->ideg == 9 && ctx->imax == 19) || (ctx->ideg == 3 && ctx->imax == 8)))
+        return bad(UCNS_EUNSUPPORTED,
+                   "supported (idegfree, inumneighbours): (19, 40) WENO-4, (9, 20) WENO-3 and (3, 9) MUSCL");
     if (ctx->iweno == -1 && ctx->ideg != 3) return bad(UCNS_EUNSUPPORTED, "MUSCL path is linear (idegfree 3)");
     HIP_TRY(hipGetDevice(&ctx->device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -2065,15 +2083,18 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     }
     ctx->ts = ts;
     const bool mixed = ctx->precision == UCNS_MIXED;
-    ctx->tiled = (ctx->iweno == 1 && ID == 19 && IM == 39 && ts <= 8 && ctx->nf * ctx->nqm <= 56);
+    ctx->tiled = (ctx->iweno == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) && ts <= 8 &&
+                  ctx->nf * ctx->nqm <= 56);
     {
         const char *e = std::getenv("UCNS_RING");
         ctx->ring = e && e[0] == '1';
         // row-stream kernels: tetrahedra (5 stencils, 24 face points) and
         // hexahedra (7 stencils, 54 face points); UCNS_STREAM=0 opts out
         const char *st = std::getenv("UCNS_STREAM");
-        const bool stream_ok = !(st && st[0] == '0') && !ctx->ring && ctx->tiled &&
-                               ((ts == 5 && ctx->nf * ctx->nqm <= 24) || (ts == 7 && ctx->nf * ctx->nqm > 24));
+        const int nfq = ctx->nf * ctx->nqm;
+        const bool shape_ok = ID == 19 ? ((ts == 5 && nfq <= 24) || (ts == 7 && nfq > 24))
+                                       : ((ts == 5 || ts == 7) && nfq <= 24);
+        const bool stream_ok = !(st && st[0] == '0') && !ctx->ring && ctx->tiled && shape_ok;
         ctx->smode = stream_ok ? ts : 0;
     }
     ctx->ntile = (nc + 7) / 8;
