@@ -568,14 +568,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     // the MUSCL branch below re-tests and overwrites its face points.
     const int nslot = ifca * p.nqm;
     const R *qpc = p.qp + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * 3;
-    // the cell's basis means, once (p.mbar may alias p.uleft for the
-    // compiler, so loads inside the face-point loop would not be hoisted)
-    R mb[IDEG];
-    {
-        const R *mbg = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
-#pragma unroll
-        for (int k = 0; k < IDEG; ++k) mb[k] = mbg[k];
-    }
+    const R *mb = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
     R *out = p.uleft + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * NV;
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
