@@ -124,7 +124,7 @@ def cpu_baseline(spec_key: str, seconds: float):
     val = case.ncell * 3 * steps / el
     return {"value": val, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"{case.spec.name}: {case.ncell} cells, {steps} SSP-RK3 steps ({3 * steps} stages) in "
-                      f"{el:.1f}s, oracle/ucns_oracle.c with OpenMP"}
+                      f"{el:.1f}s, oracle/ucns_oracle.cpp with OpenMP"}
 
 
 def rel_errors(a: np.ndarray, b: np.ndarray) -> dict:

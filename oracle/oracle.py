@@ -49,6 +49,12 @@ def lib() -> C.CDLL:
         L.oracle_set_boundary.restype = None
         L.oracle_ghost.argtypes = [C.c_int, C.c_double, f64, C.c_double, C.c_double, f64]
         L.oracle_ghost.restype = None
+        L.oracle_residual_tl.argtypes = [vp, vp, vp, f64, f64, f64, f64, f64]
+        L.oracle_rk3_tl.argtypes = [vp, vp, vp, f64, f64, f64, C.c_int32, C.c_double]
+        L.oracle_hllc_tl.argtypes = [C.c_double, f64, f64, f64, f64, f64, f64]
+        L.oracle_hllc_tl.restype = None
+        L.oracle_ghost_tl.argtypes = [C.c_int, C.c_double, f64, f64, f64, C.c_double, C.c_double, f64, f64]
+        L.oracle_ghost_tl.restype = None
         _LIB = L
     return _LIB
 
@@ -103,6 +109,28 @@ class Oracle:
         assert rc == 0
         return u, t.value
 
+    def residual_tl(self, u, du, dbc=None):
+        """tangent-linear residual: (R(u), J(u) du + dR/dbc dbc); dbc is the
+        tangent of the boundary data (inflow[5], outflow[5])"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        du = np.ascontiguousarray(du, dtype=np.float64)
+        rhs, drhs = np.zeros_like(u), np.zeros_like(u)
+        m, s, p = self._refs()
+        db = None if dbc is None else np.ascontiguousarray(dbc, dtype=np.float64)
+        rc = self.L.oracle_residual_tl(m, s, p, _p(u), _p(du), None if db is None else _p(db), _p(rhs), _p(drhs))
+        assert rc == 0
+        return rhs, drhs
+
+    def rk3_tl(self, u, du, nsteps, dt, dbc=None):
+        """nsteps SSP-RK3 steps of fixed dt with the tangent: (u_N, du_N)"""
+        u = np.array(u, dtype=np.float64, copy=True)
+        du = np.array(du, dtype=np.float64, copy=True)
+        m, s, p = self._refs()
+        db = None if dbc is None else np.ascontiguousarray(dbc, dtype=np.float64)
+        rc = self.L.oracle_rk3_tl(m, s, p, _p(u), _p(du), None if db is None else _p(db), nsteps, dt)
+        assert rc == 0
+        return u, du
+
 
 def hllc(gamma, cl, cr):
     cl = np.ascontiguousarray(cl, dtype=np.float64)
@@ -150,3 +178,19 @@ def ghost(code, gamma, cl, a1, a2):
     out = np.zeros(5)
     lib().oracle_ghost(code, gamma, _p(cl), a1, a2, _p(out))
     return out
+
+
+def hllc_tl(gamma, cl, dcl, cr, dcr):
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (cl, dcl, cr, dcr)]
+    f, df = np.zeros(5), np.zeros(5)
+    lib().oracle_hllc_tl(gamma, *[_p(x) for x in a], _p(f), _p(df))
+    return f, df
+
+
+def ghost_tl(code, gamma, cl, dcl, a1, a2, dbc=None):
+    cl = np.ascontiguousarray(cl, dtype=np.float64)
+    dcl = np.ascontiguousarray(dcl, dtype=np.float64)
+    db = None if dbc is None else np.ascontiguousarray(dbc, dtype=np.float64)
+    g, dg = np.zeros(5), np.zeros(5)
+    lib().oracle_ghost_tl(code, gamma, _p(cl), _p(dcl), None if db is None else _p(db), a1, a2, _p(g), _p(dg))
+    return g, dg

@@ -1,6 +1,7 @@
 /*
- * ucns_oracle.c -- CPU restatement of UCNS3D's explicit residual and SSP-RK3
- * update, in FP64, following the reference's own loop structure.
+ * ucns_oracle.cpp -- CPU restatement of UCNS3D's explicit residual and SSP-RK3
+ * update, in FP64, following the reference's own loop structure, plus its
+ * tangent-linear (forward-mode) derivative.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
@@ -18,20 +19,71 @@
  *   - Shu-Osher SSP-RK3 with the U_C slots 1..3                 (time_p.f90:885-1027)
  *   - dt = (CFL/3) min MINEDGE/(max|u_k| + c)                    (time_p.f90:27-116)
  *
- * Parity status of each function is listed in DESIGN.md (oracle section).
+ * Every function on the residual path is a template on the scalar type T.
+ * T = double is the restatement above (the same expressions, evaluated in
+ * the same order, as the plain-C form it replaces).  T = Dual carries one
+ * directional derivative next to each value: the same code then evaluates
+ * the tangent-linear model J(u) du exactly (to rounding), with every branch
+ * (HLLC wave-speed clamps and sign(S_M), WENO, CHECKSOL/CHECKSOLX, Barth
+ * min/max, boundary regimes) decided by the values, as the primal decides it.
  */
-#include <math.h>
-#include <stdint.h>
-#include <stdlib.h>
-#include <string.h>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
 
 #include "../include/ucns.h"
 
 #define NV 5
 
+/* ---- forward-mode dual numbers ---- */
+struct Dual {
+    double v, d;
+    Dual() = default;
+    Dual(double x, double y) : v(x), d(y) {}
+    explicit Dual(double x) : v(x), d(0.0) {}
+};
+static inline double val(double x) { return x; }
+static inline double val(const Dual &x) { return x.v; }
+static inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+static inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+static inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+static inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline Dual operator/(Dual a, Dual b) {
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+static inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+static inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+static inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+static inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+static inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+static inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+static inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+static inline Dual operator/(double a, Dual b) {
+    const double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+static inline Dual sqrt(Dual a) {
+    const double r = std::sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+static inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+static inline Dual fmin(Dual a, Dual b) { return (b.v < a.v) ? b : a; }
+static inline Dual fmax(Dual a, Dual b) { return (b.v > a.v) ? b : a; }
+static inline Dual fmin(double a, Dual b) { return (b.v < a) ? b : Dual{a, 0.0}; }
+static inline Dual fmin(Dual a, double b) { return (b < a.v) ? Dual{b, 0.0} : a; }
+static inline Dual fmax(Dual a, double b) { return (b > a.v) ? Dual{b, 0.0} : a; }
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+
 /* CONS2PRIM2 for one state, Euler 3D (flow_operations.f90:447-467) */
-static void cons2prim(double gamma, const double *u, double *w) {
-    double oo = 1.0 / u[0];
+template <class T>
+static void cons2prim(double gamma, const T *u, T *w) {
+    T oo = 1.0 / u[0];
     w[0] = u[0];
     w[1] = u[1] * oo;
     w[2] = u[2] * oo;
@@ -40,7 +92,8 @@ static void cons2prim(double gamma, const double *u, double *w) {
 }
 
 /* ROTATEF (grid_t.f90:4217-4258): rot = TRI * vec */
-void oracle_rotatef(double *rot, const double *vec, double a1, double a2) {
+template <class T>
+static void rotatef(T *rot, const T *vec, double a1, double a2) {
     double coa1 = cos(a1), sia1 = sin(a1), coa2 = cos(a2), sia2 = sin(a2);
     rot[0] = vec[0];
     rot[1] = coa1 * sia2 * vec[1] + sia1 * sia2 * vec[2] + coa2 * vec[3];
@@ -50,7 +103,8 @@ void oracle_rotatef(double *rot, const double *vec, double a1, double a2) {
 }
 
 /* ROTATEB (grid_t.f90:4261-4305): vec = INVTRI * rot */
-void oracle_rotateb(double *vec, const double *rot, double a1, double a2) {
+template <class T>
+static void rotateb(T *vec, const T *rot, double a1, double a2) {
     double coa1 = cos(a1), sia1 = sin(a1), coa2 = cos(a2), sia2 = sin(a2);
     vec[0] = rot[0];
     vec[1] = coa1 * sia2 * rot[1] + coa1 * coa2 * rot[2] + (-sia1) * rot[3];
@@ -59,30 +113,42 @@ void oracle_rotateb(double *vec, const double *rot, double a1, double a2) {
     vec[4] = rot[4];
 }
 
-/* Free-stream data of the boundary codes (set by oracle_set_boundary). */
+/* Free-stream data of the boundary codes (set by oracle_set_boundary), and
+ * the tangent of the inflow[5], outflow[5] data for the Dual evaluation. */
 static ucns_boundary_v1 g_bc = {UCNS_ABI_VERSION, 0, {1, 0, 0, 0, 2.5}, {1, 0, 0, 0, 2.5}};
+static double g_dbc[2 * NV];
 
-void oracle_set_boundary(const ucns_boundary_v1 *bc) { g_bc = *bc; }
+template <class T>
+static T bc_at(const double *x, int k);
+template <>
+double bc_at<double>(const double *x, int k) {
+    return x[k];
+}
+template <>
+Dual bc_at<Dual>(const double *x, int k) {
+    return {x[k], x == g_bc.inflow ? g_dbc[k] : g_dbc[NV + k]};
+}
 
 #define TOLSMALL 1.0e-13 /* grid_p.f90:18 */
 
 /* PRIM2CONS of (rho, u, v, w, p) with the reference's energy form
  * rho*(p/((gamma-1) rho)) + rho*(|u|^2/2) (flow_operations.f90:2929-2935) */
-static void bc_prim2cons(double gamma, const double *s3, double *cr) {
+template <class T>
+static void bc_prim2cons(double gamma, const T *s3, T *cr) {
     cr[0] = s3[0];
     cr[1] = s3[1] * s3[0];
     cr[2] = s3[2] * s3[0];
     cr[3] = s3[3] * s3[0];
-    double skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
-    double ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
+    T skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
+    T ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
     cr[4] = (s3[0] * (ikins)) + (s3[0] * skins);
 }
 
 /* subsonic inflow: pressure from the outgoing characteristic, the rest from
  * the free stream (flow_operations.f90:2907-2915); s1 free stream, s2 interior */
-static void bc_sub_in(double gamma, const double *s1, const double *s2, double sps, double nx, double ny, double nz,
-                      double *cr) {
-    double s3[NV];
+template <class T>
+static void bc_sub_in(double gamma, const T *s1, const T *s2, T sps, double nx, double ny, double nz, T *cr) {
+    T s3[NV];
     s3[4] = 0.5 * ((s1[4]) + (s2[4]) - (s2[0] * sps * ((nx * (s1[1] - s2[1])) + (ny * (s1[2] - s2[2])) + (nz * (s1[3] - s2[3])))));
     s3[0] = s1[0] + (s3[4] - s1[4]) / (sps * sps);
     s3[1] = s1[1] - (nx * (s1[4] - s3[4])) / (sps * s2[0]);
@@ -93,9 +159,9 @@ static void bc_sub_in(double gamma, const double *s1, const double *s2, double s
 
 /* subsonic outflow: free-stream pressure, the rest from the interior
  * (flow_operations.f90:2995-3010); s1 free stream, s2 interior */
-static void bc_sub_out(double gamma, const double *s1, const double *s2, double sps, double nx, double ny, double nz,
-                       double *cr) {
-    double s3[NV];
+template <class T>
+static void bc_sub_out(double gamma, const T *s1, const T *s2, T sps, double nx, double ny, double nz, T *cr) {
+    T s3[NV];
     s3[4] = s1[4];
     s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
     s3[1] = s2[1] + (nx * (s2[4] - s3[4])) / (sps * s2[0]);
@@ -106,20 +172,25 @@ static void bc_sub_out(double gamma, const double *s1, const double *s2, double 
 
 /* BOUNDARYS (flow_operations.f90:2860-3375), Euler 3D without a rotating
  * frame: ghost state cr of a boundary face from the face-point state cl */
-void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2, double *cr) {
+template <class T>
+static void ghost(int code, double gamma, const T *cl, double a1, double a2, T *cr) {
     const double nx = (cos(a1) * sin(a2)), ny = (sin(a1) * sin(a2)), nz = (cos(a2));
-    double s1[NV], s2[NV];
+    T s1[NV], s2[NV], fin[NV], fout[NV];
+    for (int v = 0; v < NV; ++v) {
+        fin[v] = bc_at<T>(g_bc.inflow, v);
+        fout[v] = bc_at<T>(g_bc.outflow, v);
+    }
     switch (code) {
     case 1: /* inflow (2880-2925) */
         if (g_bc.boundtype == 0) {
-            for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+            for (int v = 0; v < NV; ++v) cr[v] = fin[v];
         } else {
-            cons2prim(gamma, g_bc.inflow, s1);
+            cons2prim(gamma, fin, s1);
             cons2prim(gamma, cl, s2);
-            double sps = sqrt((gamma * s2[4]) / (s2[0]));
-            double vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
-            if (vel / (sps + TOLSMALL) > 1.0) {
-                for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+            T sps = sqrt((gamma * s2[4]) / (s2[0]));
+            T vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
+            if (val(vel / (sps + TOLSMALL)) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = fin[v];
             } else {
                 bc_sub_in(gamma, s1, s2, sps, nx, ny, nz, cr);
             }
@@ -129,11 +200,11 @@ void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2
         if (g_bc.boundtype == 0) {
             for (int v = 0; v < NV; ++v) cr[v] = cl[v];
         } else {
-            cons2prim(gamma, g_bc.outflow, s1);
+            cons2prim(gamma, fout, s1);
             cons2prim(gamma, cl, s2);
-            double sps = sqrt((gamma * s2[4]) / (s2[0]));
-            double vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
-            if (vel / (sps + TOLSMALL) > 1.0) {
+            T sps = sqrt((gamma * s2[4]) / (s2[0]));
+            T vel = sqrt(s2[1] * s2[1] + s2[2] * s2[2] + s2[3] * s2[3]);
+            if (val(vel / (sps + TOLSMALL)) > 1.0) {
                 for (int v = 0; v < NV; ++v) cr[v] = cl[v];
             } else {
                 bc_sub_out(gamma, s1, s2, sps, nx, ny, nz, cr);
@@ -143,32 +214,32 @@ void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2
     case 3: /* symmetry (3086-3131) */
     case 4: /* wall, inviscid branch (3132-3160) */
     {
-        double rot[NV], rr[NV];
-        oracle_rotatef(rot, cl, a1, a2);
+        T rot[NV], rr[NV];
+        rotatef(rot, cl, a1, a2);
         for (int v = 0; v < NV; ++v) rr[v] = rot[v];
         rr[1] = -rot[1];
-        oracle_rotateb(cr, rr, a1, a2);
+        rotateb(cr, rr, a1, a2);
         break;
     }
     case 6: /* farfield (3203-3296): direction from the rotated normal momentum */
     {
-        double rot[NV];
-        oracle_rotatef(rot, cl, a1, a2);
-        const double vnb = rot[1];
+        T rot[NV];
+        rotatef(rot, cl, a1, a2);
+        const T vnb = rot[1];
         cons2prim(gamma, cl, s2);
-        double sps = sqrt((gamma * s2[4]) / (s2[0]));
-        if (vnb <= 0.0) {
-            if (fabs(vnb) >= sps) {
-                for (int v = 0; v < NV; ++v) cr[v] = g_bc.inflow[v];
+        T sps = sqrt((gamma * s2[4]) / (s2[0]));
+        if (val(vnb) <= 0.0) {
+            if (val(fabs(vnb)) >= val(sps)) {
+                for (int v = 0; v < NV; ++v) cr[v] = fin[v];
             } else {
-                cons2prim(gamma, g_bc.inflow, s1);
+                cons2prim(gamma, fin, s1);
                 bc_sub_in(gamma, s1, s2, sps, nx, ny, nz, cr);
             }
         } else {
-            if (fabs(vnb) >= sps) {
+            if (val(fabs(vnb)) >= val(sps)) {
                 for (int v = 0; v < NV; ++v) cr[v] = cl[v];
             } else {
-                cons2prim(gamma, g_bc.outflow, s1);
+                cons2prim(gamma, fout, s1);
                 bc_sub_out(gamma, s1, s2, sps, nx, ny, nz, cr);
             }
         }
@@ -180,30 +251,31 @@ void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2
 }
 
 /* HLLC_RIEMANN_SOLVER, Euler, no MRF (riemann.f90:221-426) */
-void oracle_hllc(double gamma, const double *cl, const double *cr, double *flux) {
-    double wl[NV], wr[NV];
+template <class T>
+static void hllc(double gamma, const T *cl, const T *cr, T *flux) {
+    T wl[NV], wr[NV];
     cons2prim(gamma, cl, wl);
     cons2prim(gamma, cr, wr);
-    double rl = wl[0], ul = wl[1], vl = wl[2], zl = wl[3], pl = wl[4], el = cl[4];
-    double rr = wr[0], ur = wr[1], vr = wr[2], zr = wr[3], pr = wr[4], er = cr[4];
-    double ccl = sqrt(gamma * pl / rl), ccr = sqrt(gamma * pr / rr);
-    double srl = sqrt(rl), srr = sqrt(rr);
-    double cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
-                      (0.5 * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((ur - ul) * (ur - ul))));
-    double uu2 = ((ul * srl) + (ur * srr)) / (srl + srr);
-    double sl = fmin(ul - ccl, uu2 - cc2), sr = fmax(ur + ccr, uu2 + cc2);
+    T rl = wl[0], ul = wl[1], vl = wl[2], zl = wl[3], pl = wl[4], el = cl[4];
+    T rr = wr[0], ur = wr[1], vr = wr[2], zr = wr[3], pr = wr[4], er = cr[4];
+    T ccl = sqrt(gamma * pl / rl), ccr = sqrt(gamma * pr / rr);
+    T srl = sqrt(rl), srr = sqrt(rr);
+    T cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
+                 (0.5 * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((ur - ul) * (ur - ul))));
+    T uu2 = ((ul * srl) + (ur * srr)) / (srl + srr);
+    T sl = fmin(ul - ccl, uu2 - cc2), sr = fmax(ur + ccr, uu2 + cc2);
     sl = fmin(sl, 0.0);
     sr = fmax(sr, 0.0);
-    double sm = (pr - pl + (rl * ul * (sl - ul)) - (rr * ur * (sr - ur))) / ((rl * (sl - ul)) - (rr * (sr - ur)));
-    double fl[NV] = {rl * ul, (rl * (ul * ul)) + pl, rl * ul * vl, rl * ul * zl, ul * (el + pl)};
-    double fr[NV] = {rr * ur, (rr * (ur * ur)) + pr, rr * ur * vr, rr * ur * zr, ur * (er + pr)};
-    double mul = rl * ((sl - ul) / (sl - sm));
-    double mur = rr * ((sr - ur) / (sr - sm));
-    double lastl = (el / rl) + ((sm - ul) * (sm + ((pl) / (rl * (sl - ul)))));
-    double lastr = (er / rr) + ((sm - ur) * (sm + ((pr) / (rr * (sr - ur)))));
-    double usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
-    double usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
-    double s = copysign(1.0, sm);
+    T sm = (pr - pl + (rl * ul * (sl - ul)) - (rr * ur * (sr - ur))) / ((rl * (sl - ul)) - (rr * (sr - ur)));
+    T fl[NV] = {rl * ul, (rl * (ul * ul)) + pl, rl * ul * vl, rl * ul * zl, ul * (el + pl)};
+    T fr[NV] = {rr * ur, (rr * (ur * ur)) + pr, rr * ur * vr, rr * ur * zr, ur * (er + pr)};
+    T mul = rl * ((sl - ul) / (sl - sm));
+    T mur = rr * ((sr - ur) / (sr - sm));
+    T lastl = (el / rl) + ((sm - ul) * (sm + ((pl) / (rl * (sl - ul)))));
+    T lastr = (er / rr) + ((sm - ur) * (sm + ((pr) / (rr * (sr - ur)))));
+    T usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
+    T usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
+    double s = copysign(1.0, val(sm));
     for (int k = 0; k < NV; ++k)
         flux[k] = (((1.0 + s) / 2.0) * (fl[k] + sl * (usl[k] - cl[k]))) +
                   (((1.0 - s) / 2.0) * (fr[k] + sr * (usr[k] - cr[k])));
@@ -230,58 +302,68 @@ static void basis_rec(const double *x, int ideg, const double *integ, double vol
         phi[d] = px[MONO[d][0]] * py[MONO[d][1]] * pz[MONO[d][2]] - integ[d] * oov;
 }
 
-typedef struct {
+struct ctx_t {
     const ucns_mesh_v1 *m;
     const ucns_scheme_v1 *s;
     const ucns_prestore_v1 *p;
     int nf, nqm, ideg, nnb, imax, ts, base;
-} ctx_t;
+};
 
 static int nqp_of(const ctx_t *c, int64_t i, int f) {
     return c->m->types_faces[i * c->nf + f] == 5 ? c->s->qp_quad : c->s->qp_triangle;
 }
 
-static double *uleft_at(const ctx_t *c, double *ul, int64_t i, int f, int q) {
+template <class T>
+static T *uleft_at(const ctx_t *c, T *ul, int64_t i, int f, int q) {
     return ul + (((size_t)i * c->nf + f) * c->nqm + q) * NV;
 }
 
 /* omega_s = lambda_s / (eps + beta_s)^4, normalised; lambda = lwci1 for the
  * central stencil, 1 for the directional ones (reconstruct_p.f90:1031-1049) */
-static void weno_weights(int admis, const double *beta, double lwci1, double eps, double *omega) {
-    double wt[16], sum = 0.0;
+template <class T>
+static void weno_weights(int admis, const T *beta, double lwci1, double eps, T *omega) {
+    T wt[16], sum = T(0.0);
     for (int ll = 0; ll < admis; ++ll) {
         double lam = (ll == 0) ? lwci1 : 1.0;
-        double x = eps + beta[ll];
-        double x2 = x * x;
+        T x = eps + beta[ll];
+        T x2 = x * x;
         wt[ll] = lam / (x2 * x2);
     }
-    for (int ll = 0; ll < admis; ++ll) sum += wt[ll];
+    for (int ll = 0; ll < admis; ++ll) sum = sum + wt[ll];
     for (int ll = 0; ll < admis; ++ll) omega[ll] = wt[ll] / sum;
 }
 
+/* per-thread scratch of the cell routines */
+template <class T>
+struct Work {
+    std::vector<T> grad, mat1, bg;
+    std::vector<double> phi;
+    Work(const ctx_t &c)
+        : grad((size_t)c.ts * c.ideg * NV), mat1((size_t)c.imax * NV), bg(c.ideg), phi(c.ideg) {}
+};
+
 /* one cell of ARBITRARY_ORDER: LEAST_SQUARES + WENOWEIGHTS (CP_RECONSTRUCTION) */
-static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, double *grad /*ts*ideg*5*/,
-                      double *work) {
+template <class T>
+static void weno_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) {
     const int ideg = c->ideg, imax = c->imax, nnb = c->nnb, ts = c->ts;
     const int admis = c->p->admis[i];
-    const double *ui = u + i * NV;
-    double *mat1 = work;                  /* imax*5 */
-    double *phi = work + imax * NV;       /* ideg */
-    double *bg = phi + ideg;              /* ideg */
-    double omega[8][NV];
+    const T *ui = u + i * NV;
+    T *grad = w.grad.data(), *mat1 = w.mat1.data(), *bg = w.bg.data();
+    double *phi = w.phi.data();
+    T omega[8][NV];
     /* LSQ (COMPUTE_GRADIENTS_MEAN_LSQ, LOCAL=1 branch) */
     for (int ll = 0; ll < admis; ++ll) {
         const int32_t *ih = c->p->ihexl + ((size_t)i * ts + ll) * nnb;
-        const double *s1 = u + (size_t)(ih[0] - c->base) * NV;
+        const T *s1 = u + (size_t)(ih[0] - c->base) * NV;
         for (int iq = 0; iq < imax; ++iq) {
-            const double *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
+            const T *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
             for (int v = 0; v < NV; ++v) mat1[iq * NV + v] = s2[v] - s1[v];
         }
         const double *P = c->p->invmat + ((size_t)i * ts + ll) * imax * ideg;
         for (int k = 0; k < ideg; ++k)
             for (int v = 0; v < NV; ++v) {
-                double acc = 0.0;
-                for (int iq = 0; iq < imax; ++iq) acc += P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
+                T acc = T(0.0);
+                for (int iq = 0; iq < imax; ++iq) acc = acc + P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
                 grad[((size_t)ll * ideg + k) * NV + v] = acc;
             }
     }
@@ -289,15 +371,15 @@ static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, do
     const double *B = c->p->indicator + (size_t)i * ideg * ideg;
     const double eps = c->s->weno_eps;
     for (int v = 0; v < NV; ++v) {
-        double beta[8], wt[8];
+        T beta[8], wt[8];
         for (int ll = 0; ll < admis; ++ll) {
             for (int a = 0; a < ideg; ++a) {
-                double acc = 0.0;
-                for (int b = 0; b < ideg; ++b) acc += B[(size_t)b * ideg + a] * grad[((size_t)ll * ideg + b) * NV + v];
+                T acc = T(0.0);
+                for (int b = 0; b < ideg; ++b) acc = acc + B[(size_t)b * ideg + a] * grad[((size_t)ll * ideg + b) * NV + v];
                 bg[a] = acc;
             }
-            double dot = 0.0;
-            for (int a = 0; a < ideg; ++a) dot += grad[((size_t)ll * ideg + a) * NV + v] * bg[a];
+            T dot = T(0.0);
+            for (int a = 0; a < ideg; ++a) dot = dot + grad[((size_t)ll * ideg + a) * NV + v] * bg[a];
             beta[ll] = dot;
         }
         weno_weights(admis, beta, c->s->lwci1, eps, wt);
@@ -308,74 +390,78 @@ static void weno_cell(const ctx_t *c, const double *u, int64_t i, double *ul, do
     const double vol11 = c->p->volume11[i];
     for (int f = 0; f < c->m->ifca[i]; ++f)
         for (int q = 0; q < nqp_of(c, i, f); ++q) {
-            double *o = uleft_at(c, ul, i, f, q);
-            for (int v = 0; v < NV; ++v) o[v] = 0.0;
+            T *o = uleft_at(c, ul, i, f, q);
+            for (int v = 0; v < NV; ++v) o[v] = T(0.0);
         }
     for (int ll = 0; ll < admis; ++ll)
         for (int f = 0; f < c->m->ifca[i]; ++f)
             for (int q = 0; q < nqp_of(c, i, f); ++q) {
                 const double *x = c->p->qpoints + (((size_t)i * c->nf + f) * c->nqm + q) * 3;
                 basis_rec(x, ideg, integ, vol11, phi);
-                double *o = uleft_at(c, ul, i, f, q);
+                T *o = uleft_at(c, ul, i, f, q);
                 for (int v = 0; v < NV; ++v) {
-                    double res = 0.0;
-                    for (int k = 0; k < ideg; ++k) res += phi[k] * grad[((size_t)ll * ideg + k) * NV + v];
+                    T res = T(0.0);
+                    for (int k = 0; k < ideg; ++k) res = res + phi[k] * grad[((size_t)ll * ideg + k) * NV + v];
                     o[v] = o[v] + (ui[v] + res) * omega[ll][v];
                 }
             }
 }
 
 /* CHECKSOL / CHECKSOLX face-point test (reconstruct_p.f90:3183-3196, 3294-3310) */
-static int trouble_test(const ctx_t *c, const double *u, const double *ul, int64_t i, double jump) {
-    double wi[NV], wq[NV];
+template <class T>
+static int trouble_test(const ctx_t *c, const T *u, const T *ul, int64_t i, double jump) {
+    T wi[NV], wq[NV];
     cons2prim(c->s->gamma, u + i * NV, wi);
     for (int f = 0; f < c->m->ifca[i]; ++f)
         for (int q = 0; q < nqp_of(c, i, f); ++q) {
-            cons2prim(c->s->gamma, uleft_at(c, (double *)ul, i, f, q), wq);
-            if (fabs(wq[0] - wi[0]) >= jump * wi[0]) return 1;
-            if (fabs(wq[4] - wi[4]) >= jump * wi[4]) return 1;
-            if (wq[4] < 0.0 || wq[0] < 0.0) return 1;
+            cons2prim(c->s->gamma, uleft_at(c, (T *)ul, i, f, q), wq);
+            if (fabs(val(wq[0]) - val(wi[0])) >= jump * val(wi[0])) return 1;
+            if (fabs(val(wq[4]) - val(wi[4])) >= jump * val(wi[4])) return 1;
+            if (val(wq[4]) < 0.0 || val(wq[0]) < 0.0) return 1;
         }
     return 0;
 }
 
 /* Barth-Jespersen psi of one face point value usol (reconstruct_p.f90:3439-3453) */
-static double barth_psi(double ui, double umin, double umax, double usol) {
-    double d2 = usol - ui;
-    if (fabs(d2) <= 0.0) return 1.0;
-    if (d2 > 0.0) return fmin(1.0, (umax - ui) / d2);
+template <class T>
+static T barth_psi(T ui, T umin, T umax, T usol) {
+    T d2 = usol - ui;
+    if (fabs(val(d2)) <= 0.0) return T(1.0);
+    if (val(d2) > 0.0) return fmin(1.0, (umax - ui) / d2);
     return fmin(1.0, (umin - ui) / d2);
 }
 
 /* MUSCL on one cell: FIND_BOUNDS (strict) + unlimited central polynomial +
  * Barth-Jespersen (reconstruct_p.f90:1821-2039, 2044-2280, 3421-3453) */
-static void muscl_cell(const ctx_t *c, const double *u, int64_t i, double *ul, const double *grad, double *work) {
+template <class T>
+static void muscl_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) {
     const int ideg = c->ideg, nf = c->nf;
-    const double *ui = u + i * NV;
-    double umin[NV], umax[NV], slope[NV];
+    const T *grad = w.grad.data();
+    const T *ui = u + i * NV;
+    T umin[NV], umax[NV], slope[NV];
     for (int v = 0; v < NV; ++v) umin[v] = umax[v] = ui[v];
     for (int f = 0; f < c->m->ifca[i]; ++f) {
         int bc = c->m->bcode[i * nf + f];
         if (bc != 0 && bc != 5 && bc != 50) continue;  /* non-periodic boundary faces add no neighbour */
-        const double *un = u + (size_t)(c->m->ineigh[i * nf + f] - c->base) * NV;
+        const T *un = u + (size_t)(c->m->ineigh[i * nf + f] - c->base) * NV;
         for (int v = 0; v < NV; ++v) {
             umin[v] = fmin(umin[v], un[v]);
             umax[v] = fmax(umax[v], un[v]);
         }
     }
-    double *phi = work;
+    double *phi = w.phi.data();
     const double *integ = c->p->integ_basis + (size_t)i * ideg;
     const double vol11 = c->p->volume11[i];
-    for (int v = 0; v < NV; ++v) slope[v] = 1.0e16; /* TOLBIG */
+    for (int v = 0; v < NV; ++v) slope[v] = T(1.0e16); /* TOLBIG */
     /* pass 1: USOL and the limiter */
     for (int f = 0; f < c->m->ifca[i]; ++f)
         for (int q = 0; q < nqp_of(c, i, f); ++q) {
             const double *x = c->p->qpoints + (((size_t)i * nf + f) * c->nqm + q) * 3;
             basis_rec(x, ideg, integ, vol11, phi);
             for (int v = 0; v < NV; ++v) {
-                double res = 0.0;
-                for (int k = 0; k < ideg; ++k) res += phi[k] * grad[(size_t)k * NV + v];
-                double usol = ui[v] + res;
+                T res = T(0.0);
+                for (int k = 0; k < ideg; ++k) res = res + phi[k] * grad[(size_t)k * NV + v];
+                T usol = ui[v] + res;
                 slope[v] = fmin(slope[v], barth_psi(ui[v], umin[v], umax[v], usol));
             }
         }
@@ -384,30 +470,31 @@ static void muscl_cell(const ctx_t *c, const double *u, int64_t i, double *ul, c
         for (int q = 0; q < nqp_of(c, i, f); ++q) {
             const double *x = c->p->qpoints + (((size_t)i * nf + f) * c->nqm + q) * 3;
             basis_rec(x, ideg, integ, vol11, phi);
-            double *o = uleft_at(c, ul, i, f, q);
+            T *o = uleft_at(c, ul, i, f, q);
             for (int v = 0; v < NV; ++v) {
-                double res = 0.0;
-                for (int k = 0; k < ideg; ++k) res += phi[k] * grad[(size_t)k * NV + v];
-                double usol = ui[v] + res;
+                T res = T(0.0);
+                for (int k = 0; k < ideg; ++k) res = res + phi[k] * grad[(size_t)k * NV + v];
+                T usol = ui[v] + res;
                 o[v] = 0.0 + (ui[v] + ((usol - ui[v]) * slope[v]));
             }
         }
 }
 
-static void lsq_central(const ctx_t *c, const double *u, int64_t i, double *grad, double *work) {
+template <class T>
+static void lsq_central(const ctx_t *c, const T *u, int64_t i, Work<T> &w) {
     const int ideg = c->ideg, imax = c->imax, nnb = c->nnb;
     const int32_t *ih = c->p->ihexl + (size_t)i * c->ts * nnb;
-    const double *s1 = u + (size_t)(ih[0] - c->base) * NV;
-    double *mat1 = work;
+    const T *s1 = u + (size_t)(ih[0] - c->base) * NV;
+    T *mat1 = w.mat1.data(), *grad = w.grad.data();
     for (int iq = 0; iq < imax; ++iq) {
-        const double *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
+        const T *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
         for (int v = 0; v < NV; ++v) mat1[iq * NV + v] = s2[v] - s1[v];
     }
     const double *P = c->p->invmat + (size_t)i * c->ts * imax * ideg;
     for (int k = 0; k < ideg; ++k)
         for (int v = 0; v < NV; ++v) {
-            double acc = 0.0;
-            for (int iq = 0; iq < imax; ++iq) acc += P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
+            T acc = T(0.0);
+            for (int iq = 0; iq < imax; ++iq) acc = acc + P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
             grad[(size_t)k * NV + v] = acc;
         }
 }
@@ -427,23 +514,22 @@ static void init_ctx(ctx_t *c, const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, c
 
 /* ARBITRARY_ORDER (reconstruct_p.f90:2954-3015) for all cells -> ULEFT.
  * counters (optional): [0] cells sent to MUSCL, [1] cells set to first order. */
-int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
-                       double *ul, int64_t *counters) {
+template <class T>
+static int reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *u, T *ul,
+                       int64_t *counters) {
     ctx_t c;
     init_ctx(&c, m, s, p);
     const int64_t nc = m->ncell;
     int64_t n_red = 0, n_first = 0;
-    int *reduce = (int *)calloc((size_t)nc, sizeof(int));
-    if (!reduce) return UCNS_ENOMEM;
+    std::vector<int> reduce((size_t)nc, 0);
 #pragma omp parallel
     {
-        double *grad = (double *)malloc(sizeof(double) * (size_t)c.ts * c.ideg * NV);
-        double *work = (double *)malloc(sizeof(double) * ((size_t)c.imax * NV + 4 * c.ideg + 16));
+        Work<T> w(c);
         /* WENOWEIGHTS + CHECKSOL, or nothing for MUSCL (IWENO=-1) */
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             if (s->iweno == 1) {
-                weno_cell(&c, u, i, ul, grad, work);
+                weno_cell(&c, u, i, ul, w);
                 int red = 0;
                 if (p->full[i] == 0) red = 1;
                 else red = trouble_test(&c, u, ul, i, 0.85);
@@ -456,8 +542,8 @@ int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucn
 #pragma omp for schedule(static) reduction(+ : n_red)
         for (int64_t i = 0; i < nc; ++i) {
             if (reduce[i] == 1 || p->full[i] == 0) {
-                lsq_central(&c, u, i, grad, work);
-                muscl_cell(&c, u, i, ul, grad, work);
+                lsq_central(&c, u, i, w);
+                muscl_cell(&c, u, i, ul, w);
                 n_red += 1;
             }
         }
@@ -467,16 +553,13 @@ int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucn
             if (trouble_test(&c, u, ul, i, 0.95)) {
                 for (int f = 0; f < m->ifca[i]; ++f)
                     for (int q = 0; q < c.nqm; ++q) {
-                        double *o = uleft_at(&c, ul, i, f, q);
+                        T *o = uleft_at(&c, ul, i, f, q);
                         for (int v = 0; v < NV; ++v) o[v] = u[i * NV + v];
                     }
                 n_first += 1;
             }
         }
-        free(grad);
-        free(work);
     }
-    free(reduce);
     if (counters) {
         counters[0] = n_red;
         counters[1] = n_first;
@@ -485,14 +568,14 @@ int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucn
 }
 
 /* CALCULATE_FLUXESHI_CONVECTIVE, cell based, every face solved from both sides */
-int oracle_fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *ul,
-                  double *rhs) {
+template <class T>
+static int fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *ul, T *rhs) {
     ctx_t c;
     init_ctx(&c, m, s, p);
     const int64_t nc = m->ncell;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nc; ++i) {
-        double r[NV] = {0, 0, 0, 0, 0};
+        T r[NV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
         for (int f = 0; f < m->ifca[i]; ++f) {
             const int64_t slot = i * c.nf + f;
             const double a1 = m->faceanglex[slot], a2 = m->faceangley[slot];
@@ -502,22 +585,22 @@ int oracle_fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_pre
             const int bnd = bcode != 0 && bcode != 5 && bcode != 50;
             const int64_t j = bnd ? 0 : m->ineigh[slot] - c.base;
             const int fj = bnd ? 0 : m->ineighn[slot] - c.base;
-            double god[NV] = {0, 0, 0, 0, 0};
+            T god[NV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
             for (int q = 0; q < iqp; ++q) {
-                const double *cl = uleft_at(&c, (double *)ul, i, f, q);
-                double ghost[NV];
-                const double *cr;
+                const T *cl = uleft_at(&c, (T *)ul, i, f, q);
+                T gst[NV];
+                const T *cr;
                 if (bnd) { /* GET_STATES_BOUNDS -> BOUNDARYS (dg_functions.f90:1808-1828) */
-                    oracle_ghost(bcode, s->gamma, cl, a1, a2, ghost);
-                    cr = ghost;
+                    ghost(bcode, s->gamma, cl, a1, a2, gst);
+                    cr = gst;
                 } else {
-                    cr = uleft_at(&c, (double *)ul, j, fj, q);
+                    cr = uleft_at(&c, (T *)ul, j, fj, q);
                 }
-                double clr[NV], crr[NV], h[NV], rh[NV];
-                oracle_rotatef(crr, cr, a1, a2);
-                oracle_rotatef(clr, cl, a1, a2);
-                oracle_hllc(s->gamma, clr, crr, h);
-                oracle_rotateb(rh, h, a1, a2);
+                T clr[NV], crr[NV], h[NV], rh[NV];
+                rotatef(crr, cr, a1, a2);
+                rotatef(clr, cl, a1, a2);
+                hllc(s->gamma, clr, crr, h);
+                rotateb(rh, h, a1, a2);
                 for (int v = 0; v < NV; ++v) god[v] = god[v] + (rh[v] * (wts[q] * m->surf[slot]));
             }
             for (int v = 0; v < NV; ++v) r[v] = r[v] + god[v];
@@ -532,18 +615,17 @@ static size_t uleft_size(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s) {
     return (size_t)m->ncell * m->nface_max * nqm * NV;
 }
 
-int oracle_residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
-                    double *rhs, int64_t *counters) {
-    double *ul = (double *)calloc(uleft_size(m, s), sizeof(double));
-    if (!ul) return UCNS_ENOMEM;
-    int rc = oracle_reconstruct(m, s, p, u, ul, counters);
-    if (rc == 0) rc = oracle_fluxes(m, s, p, ul, rhs);
-    free(ul);
+template <class T>
+static int residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *u, T *rhs,
+                    int64_t *counters) {
+    std::vector<T> ul(uleft_size(m, s), T(0.0));
+    int rc = reconstruct(m, s, p, u, ul.data(), counters);
+    if (rc == 0) rc = fluxes(m, s, p, ul.data(), rhs);
     return rc;
 }
 
 /* CALCULATE_CFL, Euler branch (time_p.f90:46, 67-116) */
-int oracle_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, double cfl, double *dt_out) {
+static void cfl_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, double cfl, double *dt_out) {
     const double ccfl = cfl / 3.0;
     double dt = 1.0e16;
     const int64_t nc = m->ncell;
@@ -557,32 +639,36 @@ int oracle_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, d
         dt = fmin(dt, d);
     }
     *dt_out = dt;
-    return UCNS_OK;
 }
 
-/* RUNGE_KUTTA3 (time_p.f90:885-1027) for nsteps steps; u is slot 1 (in/out). */
-int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, double *u,
-                     int32_t nsteps, double cfl, double dt_fixed, double *t_io) {
+static void values(const double *u, double *out, size_t n) { std::memcpy(out, u, n * sizeof(double)); }
+static void values(const Dual *u, double *out, size_t n) {
+    for (size_t e = 0; e < n; ++e) out[e] = u[e].v;
+}
+
+/* RUNGE_KUTTA3 (time_p.f90:885-1027) for nsteps steps; u is slot 1 (in/out).
+ * dt_fixed <= 0 takes dt from the CFL condition of each step's state (its
+ * derivative is not carried: the tangent runs use dt_fixed). */
+template <class T>
+static int rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, T *u, int32_t nsteps,
+                     double cfl, double dt_fixed, double *t_io) {
     const int64_t nc = m->ncell;
     const size_t n = (size_t)nc * NV;
-    double *u2 = (double *)malloc(sizeof(double) * n);
-    double *u3 = (double *)malloc(sizeof(double) * n);
-    double *rhs = (double *)malloc(sizeof(double) * n);
-    double *ul = (double *)calloc(uleft_size(m, s), sizeof(double));
-    if (!u2 || !u3 || !rhs || !ul) {
-        free(u2); free(u3); free(rhs); free(ul);
-        return UCNS_ENOMEM;
-    }
+    std::vector<T> u2(n), u3(n), rhs(n), ul(uleft_size(m, s), T(0.0));
+    std::vector<double> uv(dt_fixed > 0.0 ? 0 : n);
     const double to4 = 3.0 / 4.0, oo4 = 1.0 / 4.0, to3 = 2.0 / 3.0, oo3 = 1.0 / 3.0;
     double t = t_io ? *t_io : 0.0;
     int rc = UCNS_OK;
     for (int it = 0; it < nsteps && rc == UCNS_OK; ++it) {
         double dt = dt_fixed;
-        if (!(dt_fixed > 0.0)) oracle_dt(m, s, u, cfl, &dt);
+        if (!(dt_fixed > 0.0)) {
+            values(u, uv.data(), n);
+            cfl_dt(m, s, uv.data(), cfl, &dt);
+        }
         /* stage 1 */
-        rc = oracle_reconstruct(m, s, p, u, ul, NULL);
+        rc = reconstruct(m, s, p, u, ul.data(), (int64_t *)nullptr);
         if (rc) break;
-        oracle_fluxes(m, s, p, ul, rhs);
+        fluxes(m, s, p, ul.data(), rhs.data());
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             double oov = 1.0 / m->totvolume[i];
@@ -592,9 +678,9 @@ int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_
             }
         }
         /* stage 2 */
-        rc = oracle_reconstruct(m, s, p, u, ul, NULL);
+        rc = reconstruct(m, s, p, u, ul.data(), (int64_t *)nullptr);
         if (rc) break;
-        oracle_fluxes(m, s, p, ul, rhs);
+        fluxes(m, s, p, ul.data(), rhs.data());
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             double oov = 1.0 / m->totvolume[i];
@@ -605,9 +691,9 @@ int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_
             }
         }
         /* stage 3 */
-        rc = oracle_reconstruct(m, s, p, u, ul, NULL);
+        rc = reconstruct(m, s, p, u, ul.data(), (int64_t *)nullptr);
         if (rc) break;
-        oracle_fluxes(m, s, p, ul, rhs);
+        fluxes(m, s, p, ul.data(), rhs.data());
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             double oov = 1.0 / m->totvolume[i];
@@ -618,8 +704,52 @@ int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_
         t += dt;
     }
     if (t_io) *t_io = t;
-    free(u2); free(u3); free(rhs); free(ul);
     return rc;
+}
+
+static std::vector<Dual> make_dual(const double *u, const double *du, size_t n) {
+    std::vector<Dual> x(n);
+    for (size_t e = 0; e < n; ++e) x[e] = {u[e], du ? du[e] : 0.0};
+    return x;
+}
+
+static void set_dbc(const double *dbc) {
+    for (int k = 0; k < 2 * NV; ++k) g_dbc[k] = dbc ? dbc[k] : 0.0;
+}
+
+extern "C" {
+
+void oracle_rotatef(double *rot, const double *vec, double a1, double a2) { rotatef(rot, vec, a1, a2); }
+void oracle_rotateb(double *vec, const double *rot, double a1, double a2) { rotateb(vec, rot, a1, a2); }
+void oracle_set_boundary(const ucns_boundary_v1 *bc) { g_bc = *bc; }
+void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2, double *cr) {
+    ghost(code, gamma, cl, a1, a2, cr);
+}
+void oracle_hllc(double gamma, const double *cl, const double *cr, double *flux) { hllc(gamma, cl, cr, flux); }
+
+int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+                       double *ul, int64_t *counters) {
+    return reconstruct(m, s, p, u, ul, counters);
+}
+
+int oracle_fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *ul,
+                  double *rhs) {
+    return fluxes(m, s, p, ul, rhs);
+}
+
+int oracle_residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+                    double *rhs, int64_t *counters) {
+    return residual(m, s, p, u, rhs, counters);
+}
+
+int oracle_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, double cfl, double *dt_out) {
+    cfl_dt(m, s, u, cfl, dt_out);
+    return UCNS_OK;
+}
+
+int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, double *u,
+                     int32_t nsteps, double cfl, double dt_fixed, double *t_io) {
+    return rk3_steps(m, s, p, u, nsteps, cfl, dt_fixed, t_io);
 }
 
 /* WENO weights of one cell and variable from given indicators (the closed-form
@@ -632,3 +762,68 @@ void oracle_weno_weights(int admis, const double *beta, double lwci1, double eps
 /* Barth-Jespersen psi of one face point (reconstruct_p.f90:3439-3453), the
  * helper the MUSCL reconstruction uses */
 double oracle_barth(double ui, double umin, double umax, double usol) { return barth_psi(ui, umin, umax, usol); }
+
+/* ---- tangent-linear model: the Dual instantiation of the same code ----
+ * du: [ncell][5] state tangent; dbc (or null): tangent of the boundary
+ * data inflow[0..4], outflow[0..4] set by oracle_set_boundary. */
+int oracle_residual_tl(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+                       const double *du, const double *dbc, double *rhs, double *drhs) {
+    const size_t n = (size_t)m->ncell * NV;
+    set_dbc(dbc);
+    std::vector<Dual> x = make_dual(u, du, n), r(n);
+    int rc = residual(m, s, p, x.data(), r.data(), (int64_t *)nullptr);
+    for (size_t e = 0; e < n; ++e) {
+        if (rhs) rhs[e] = r[e].v;
+        drhs[e] = r[e].d;
+    }
+    set_dbc(nullptr);
+    return rc;
+}
+
+/* nsteps SSP-RK3 steps of fixed dt with the tangent carried along:
+ * u, du in/out */
+int oracle_rk3_tl(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, double *u, double *du,
+                  const double *dbc, int32_t nsteps, double dt_fixed) {
+    if (!(dt_fixed > 0.0)) return UCNS_EBADARG;
+    const size_t n = (size_t)m->ncell * NV;
+    set_dbc(dbc);
+    std::vector<Dual> x = make_dual(u, du, n);
+    int rc = rk3_steps(m, s, p, x.data(), nsteps, 0.0, dt_fixed, (double *)nullptr);
+    for (size_t e = 0; e < n; ++e) {
+        u[e] = x[e].v;
+        du[e] = x[e].d;
+    }
+    set_dbc(nullptr);
+    return rc;
+}
+
+/* Jacobian-vector products of the point routines, for their own tests:
+ * HLLC of (cl + t dcl, cr + t dcr) and the ghost state of cl + t dcl */
+void oracle_hllc_tl(double gamma, const double *cl, const double *dcl, const double *cr, const double *dcr,
+                    double *flux, double *dflux) {
+    Dual a[NV], b[NV], f[NV];
+    for (int v = 0; v < NV; ++v) {
+        a[v] = {cl[v], dcl[v]};
+        b[v] = {cr[v], dcr[v]};
+    }
+    hllc(gamma, a, b, f);
+    for (int v = 0; v < NV; ++v) {
+        flux[v] = f[v].v;
+        dflux[v] = f[v].d;
+    }
+}
+
+void oracle_ghost_tl(int code, double gamma, const double *cl, const double *dcl, const double *dbc, double a1,
+                     double a2, double *cr, double *dcr) {
+    Dual a[NV], g[NV];
+    for (int v = 0; v < NV; ++v) a[v] = {cl[v], dcl[v]};
+    set_dbc(dbc);
+    ghost(code, gamma, a, a1, a2, g);
+    set_dbc(nullptr);
+    for (int v = 0; v < NV; ++v) {
+        cr[v] = g[v].v;
+        dcr[v] = g[v].d;
+    }
+}
+
+}  // extern "C"

@@ -1,4 +1,4 @@
-"""The CPU oracle (oracle/ucns_oracle.c) pinned against what the mathematics
+"""The CPU oracle (oracle/ucns_oracle.cpp) pinned against what the mathematics
 fixes: the textbook HLLC solver, rotation algebra, closed forms of the WENO
 weights, Barth-Jespersen limiter and CFL step, free-stream preservation,
 conservation, the Shu-Osher SSP-RK3 composition, exact reproduction of

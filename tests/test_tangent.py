@@ -1,0 +1,174 @@
+"""The oracle's tangent-linear model (the Dual instantiation of the same
+templated code as the FP64 restatement) pinned against central finite
+differences of the FP64 oracle itself: the HLLC solver in each wave-speed
+regime, every boundary ghost state including its dependence on the
+free-stream data, the full residual (WENO-4, WENO-3, MUSCL, bounded boxes)
+and several SSP-RK3 steps.  A wrong derivative rule (a dropped term, a sign,
+the wrong branch) shows as a mismatch far above the O(h^2) truncation and
+O(eps/h) rounding of the differences."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from oracle.oracle import Oracle
+from ucns3d_amd.case import HEX, TET, Case, CaseSpec, reduced
+
+from test_oracle import G, prim2cons, random_state
+
+
+def fd_dir(f, x, dx, h):
+    return (f(x + h * dx) - f(x - h * dx)) / (2 * h)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(autouse=True)
+def _reset_boundary():
+    oracle.set_boundary(0, prim2cons(1, 0, 0, 0, 1), prim2cons(1, 0, 0, 0, 1))
+    yield
+
+
+def test_hllc_tangent_matches_differences():
+    rng = np.random.default_rng(31)
+    seen = set()
+    for _ in range(400):
+        UL = random_state(rng, rng.uniform(0, 2.5))
+        UR = random_state(rng, rng.uniform(0, 2.5))
+        dl, dr = rng.normal(size=5) * np.abs(UL), rng.normal(size=5) * np.abs(UR)
+        f, df = oracle.hllc_tl(G, UL, dl, UR, dr)
+        np.testing.assert_array_equal(f, oracle.hllc(G, UL, UR))
+        h = 1e-6
+        fd = (oracle.hllc(G, UL + h * dl, UR + h * dr) - oracle.hllc(G, UL - h * dl, UR - h * dr)) / (2 * h)
+        err = rel(df, fd)
+        assert err < 1e-6, (err, UL, UR)
+        rl, ul = UL[0], UL[1] / UL[0]
+        pl = (G - 1) * (UL[4] - 0.5 * (UL[1:4] @ UL[1:4]) / rl)
+        seen.add(("supersonic" if abs(ul) > math.sqrt(G * pl / rl) else "subsonic", ul > 0))
+    assert len(seen) == 4, seen
+
+
+@pytest.mark.parametrize("code,boundtype", [(1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (4, 1), (6, 1)])
+def test_ghost_tangent_matches_differences(code, boundtype):
+    rng = np.random.default_rng(32 + code)
+    for _ in range(100):
+        far, out = random_state(rng, 0.6), random_state(rng, 0.6)
+        oracle.set_boundary(boundtype, far, out)
+        a1, a2 = rng.uniform(-math.pi, math.pi), rng.uniform(0.1, math.pi - 0.1)
+        U = random_state(rng, rng.uniform(0.1, 1.6))
+        dU = rng.normal(size=5) * np.abs(U)
+        dbc = rng.normal(size=10) * np.abs(np.r_[far, out])
+        g, dg = oracle.ghost_tl(code, G, U, dU, a1, a2, dbc)
+        np.testing.assert_array_equal(g, oracle.ghost(code, G, U, a1, a2))
+        h = 1e-6
+
+        def f(t):
+            oracle.set_boundary(boundtype, far + t * dbc[:5], out + t * dbc[5:])
+            return oracle.ghost(code, G, U + t * dU, a1, a2)
+        fd = (f(h) - f(-h)) / (2 * h)
+        oracle.set_boundary(boundtype, far, out)
+        if np.linalg.norm(dg) == 0 and np.linalg.norm(fd) == 0:
+            continue
+        assert rel(dg, fd) < 1e-6, (code, boundtype, dg, fd)
+
+
+CASES = {
+    "tet-weno4": lambda: (reduced("c2", 4), None),
+    "hex-weno4": lambda: (reduced("c3", 12), None),
+    "hex-muscl": lambda: (reduced("c1", 6), None),
+    "tet-weno3-farfield": lambda: (CaseSpec("farfield", TET, 4, "weno3", jitter=0.1, seed=4, bc=(6,) * 6),
+                                   (1, prim2cons(1.0, 0.3, 0.1, 0.0, 100 / 1.4), prim2cons(1.0, 0.3, 0.1, 0.0, 90 / 1.4))),
+    "hex-weno4-channel": lambda: (CaseSpec("channel", HEX, 12, "weno4", seed=3, bc=(1, 2, 4, 4, 3, 3)),
+                                  (1, prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4), prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4))),
+}
+_cache = {}
+
+
+def setup(name):
+    if name not in _cache:
+        spec, bc = CASES[name]()
+        case = Case(spec)
+        rng = np.random.default_rng(7)
+        u = case.init_state()
+        if bc is None:
+            # carry the vortex with a uniform velocity larger than its own, so
+            # that no face sits at S_M = 0, where the HLLC flux switches from
+            # the left to the right star state (continuous, with a kink)
+            r = u[:, :1]
+            vel = u[:, 1:4] / r + np.array([1.5, 1.2, 1.3])
+            p = (G - 1) * (u[:, 4] - 0.5 * (u[:, 1:4] ** 2).sum(1) / u[:, 0])
+            u = np.c_[r, r * vel, p / (G - 1) + 0.5 * r[:, 0] * (vel ** 2).sum(1)]
+        # break the symmetries of the vortex (cells mirrored about its axes
+        # hold equal values) with a smooth modulation, then add cell noise,
+        # so that no limiter bound or weight decision is a tie between equal
+        # values
+        x = case.array("centre").reshape(-1, 3)
+        u = u * (1 + 0.02 * np.sin(0.37 * x[:, :1] + 0.71 * x[:, 1:2] + 0.53 * x[:, 2:3] + 0.2))
+        u = u * (1 + 1e-3 * rng.uniform(-1, 1, u.shape))
+        # directions scaled with the variation of each variable, so that the
+        # differences stay in the range where the smoothness indicators
+        # (quadratic in the coefficients) are resolved to O(h^2).  On the
+        # bounded boxes most cells are MUSCL cells, whose Barth limiter
+        # switches between face points at kinks; a cell-wise random
+        # direction moves the limiter's arguments by far more than h
+        # relative to their gaps, so there the direction is a smooth field.
+        if bc is None:
+            du = rng.normal(size=u.shape) * u.std(axis=0)
+        else:
+            x = case.array("centre").reshape(-1, 3)
+            ph = rng.uniform(0, 2 * np.pi, (5, 3))
+            du = np.stack([np.sin(1.3 * x[:, 0] + ph[v, 0]) * np.cos(0.9 * x[:, 1] + ph[v, 1]) *
+                           np.sin(1.1 * x[:, 2] + ph[v, 2]) for v in range(5)], axis=1) * u.std(axis=0)
+        _cache[name] = (case, Oracle(case), u, du, bc)
+    case, orc, u, du, bc = _cache[name]
+    if bc is not None:
+        oracle.set_boundary(*bc)
+    return case, orc, u, du, bc
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_residual_tangent_matches_differences(name):
+    case, orc, u, du, bc = setup(name)
+    r, dr = orc.residual_tl(u, du)
+    r0, _ = orc.residual(u)
+    np.testing.assert_array_equal(r, r0)
+    fd = fd_dir(lambda x: orc.residual(x)[0], u, du, 1e-6)
+    for v in range(5):
+        assert rel(dr[:, v], fd[:, v]) < 1e-6, (v, rel(dr[:, v], fd[:, v]))
+
+
+@pytest.mark.parametrize("name", ["tet-weno3-farfield", "hex-weno4-channel"])
+def test_residual_tangent_of_the_boundary_data(name):
+    """the free-stream data enter through the ghost states only"""
+    case, orc, u, du, bc = setup(name)
+    bt, far, out = bc
+    rng = np.random.default_rng(9)
+    dbc = rng.normal(size=10) * np.abs(np.r_[far, out])
+    _, dr = orc.residual_tl(u, np.zeros_like(u), dbc)
+
+    def f(t):
+        oracle.set_boundary(bt, far + t * dbc[:5], out + t * dbc[5:])
+        return orc.residual(u)[0]
+    h = 1e-6
+    fd = (f(h) - f(-h)) / (2 * h)
+    oracle.set_boundary(*bc)
+    assert np.linalg.norm(fd) > 0
+    assert rel(dr, fd) < 1e-6
+    # cells with no boundary face do not see the free stream
+    bcode = case.array("bcode").reshape(case.ncell, -1)
+    inner = ~((bcode != 0) & (bcode != 5) & (bcode != 50)).any(axis=1)
+    assert np.abs(dr[inner]).max() == 0.0
+
+
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-muscl", "hex-weno4-channel"])
+def test_rk3_tangent_matches_differences(name):
+    case, orc, u, du, bc = setup(name)
+    dt = 0.5 * orc.dt(u, 0.5)
+    uN, duN = orc.rk3_tl(u, du, 3, dt)
+    np.testing.assert_array_equal(uN, orc.rk3(u, 3, dt_fixed=dt)[0])
+    fd = fd_dir(lambda x: orc.rk3(x, 3, dt_fixed=dt)[0], u, du, 1e-6)
+    for v in range(5):
+        assert rel(duN[:, v], fd[:, v]) < 1e-6, (v, rel(duN[:, v], fd[:, v]))

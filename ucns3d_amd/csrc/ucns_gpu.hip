@@ -95,6 +95,59 @@ __device__ inline void cons2prim_d(double gamma, const double *u, double *w) {
     w[4] = (gamma - 1.0) * (u[4] - 0.5 * u[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
 }
 
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers for the adjoint: the same scalar code as the
+// primal, run on (value, one directional derivative).  Branches follow the
+// value, so the derivative is the one of the branch the primal takes.
+struct Dual {
+    double v, d;
+};
+// the Dual overloads below would otherwise hide the global math functions
+using ::fabs;
+using ::fmax;
+using ::fmin;
+using ::sqrt;
+__device__ inline double val(double x) { return x; }
+__device__ inline double val(const Dual &x) { return x.v; }
+__device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ inline Dual operator/(Dual a, Dual b) {
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+__device__ inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ inline Dual operator/(double a, Dual b) {
+    const double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+__device__ inline Dual sqrt(Dual a) {
+    const double r = ::sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+__device__ inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+__device__ inline Dual fmin(Dual a, Dual b) { return (b.v < a.v) ? b : a; }
+__device__ inline Dual fmax(Dual a, Dual b) { return (b.v > a.v) ? b : a; }
+__device__ inline Dual fmin(Dual a, double b) { return (b < a.v) ? Dual{b, 0.0} : a; }
+__device__ inline Dual fmax(Dual a, double b) { return (b > a.v) ? Dual{b, 0.0} : a; }
+
+template <class T>
+__device__ inline void cons2prim_t(double gamma, const T *u, T *w) {
+    T oo = 1.0 / u[0];
+    w[0] = u[0];
+    w[1] = u[1] * oo;
+    w[2] = u[2] * oo;
+    w[3] = u[3] * oo;
+    w[4] = (gamma - 1.0) * (u[4] - 0.5 * u[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
+}
+
 // CHECKSOL / CHECKSOLX test of one face point against the cell average
 // (src/reconstruct_p.f90:3183-3196): primitive recovery in FP64.
 __device__ inline bool jump_test(double gamma, const double *wi, const double *uq, double jump) {
@@ -1140,34 +1193,38 @@ __global__ void k_tile_p(const double *src, R *dst, int cell0, int ncells_chunk,
 // so the FP32 rounding scales with the jump across the face and not with
 // the size of the state (the pressure, the energy) as it would if U* and
 // U_K were rounded separately and subtracted.
-__device__ inline void hllc64(double gamma, const double *cl, const double *cr, double *flux) {
-    double wl[NV], wr[NV];
-    cons2prim_d(gamma, cl, wl);
-    cons2prim_d(gamma, cr, wr);
-    const double rl = wl[0], uL = wl[1], vl = wl[2], zl = wl[3], pl = wl[4], el = cl[4];
-    const double rr = wr[0], uR = wr[1], vr = wr[2], zr = wr[3], pr = wr[4], er = cr[4];
-    const double ccl = sqrt(gamma * pl / rl), ccr = sqrt(gamma * pr / rr);
-    const double srl = sqrt(rl), srr = sqrt(rr);
-    const double cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
-                            (0.5 * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
-    const double uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
-    double sl = fmin(uL - ccl, uu2 - cc2), sr = fmax(uR + ccr, uu2 + cc2);
+template <class T>
+__device__ inline void hllc_t(double gamma, const T *cl, const T *cr, T *flux) {
+    T wl[NV], wr[NV];
+    cons2prim_t(gamma, cl, wl);
+    cons2prim_t(gamma, cr, wr);
+    const T rl = wl[0], uL = wl[1], vl = wl[2], zl = wl[3], pl = wl[4], el = cl[4];
+    const T rr = wr[0], uR = wr[1], vr = wr[2], zr = wr[3], pr = wr[4], er = cr[4];
+    const T ccl = sqrt(gamma * pl / rl), ccr = sqrt(gamma * pr / rr);
+    const T srl = sqrt(rl), srr = sqrt(rr);
+    const T cc2 = sqrt((((ccl * ccl) * srl) + ((ccr * ccr) * srr)) / (srl + srr) +
+                       (0.5 * ((srl * srr) / ((srl + srr) * (srl + srr))) * ((uR - uL) * (uR - uL))));
+    const T uu2 = ((uL * srl) + (uR * srr)) / (srl + srr);
+    T sl = fmin(uL - ccl, uu2 - cc2), sr = fmax(uR + ccr, uu2 + cc2);
     sl = fmin(sl, 0.0);
     sr = fmax(sr, 0.0);
-    const double sm = (pr - pl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
-    const double fl[NV] = {rl * uL, (rl * (uL * uL)) + pl, rl * uL * vl, rl * uL * zl, uL * (el + pl)};
-    const double fr[NV] = {rr * uR, (rr * (uR * uR)) + pr, rr * uR * vr, rr * uR * zr, uR * (er + pr)};
-    const double mul = rl * ((sl - uL) / (sl - sm));
-    const double mur = rr * ((sr - uR) / (sr - sm));
-    const double lastl = (el / rl) + ((sm - uL) * (sm + ((pl) / (rl * (sl - uL)))));
-    const double lastr = (er / rr) + ((sm - uR) * (sm + ((pr) / (rr * (sr - uR)))));
-    const double usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
-    const double usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
-    const double s = copysign(1.0, sm);
+    const T sm = (pr - pl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR))) / ((rl * (sl - uL)) - (rr * (sr - uR)));
+    const T fl[NV] = {rl * uL, (rl * (uL * uL)) + pl, rl * uL * vl, rl * uL * zl, uL * (el + pl)};
+    const T fr[NV] = {rr * uR, (rr * (uR * uR)) + pr, rr * uR * vr, rr * uR * zr, uR * (er + pr)};
+    const T mul = rl * ((sl - uL) / (sl - sm));
+    const T mur = rr * ((sr - uR) / (sr - sm));
+    const T lastl = (el / rl) + ((sm - uL) * (sm + ((pl) / (rl * (sl - uL)))));
+    const T lastr = (er / rr) + ((sm - uR) * (sm + ((pr) / (rr * (sr - uR)))));
+    const T usl[NV] = {mul, mul * sm, mul * vl, mul * zl, mul * lastl};
+    const T usr[NV] = {mur, mur * sm, mur * vr, mur * zr, mur * lastr};
+    const double s = copysign(1.0, val(sm));
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         flux[k] = (((1.0 + s) / 2.0) * (fl[k] + sl * (usl[k] - cl[k]))) +
                   (((1.0 - s) / 2.0) * (fr[k] + sr * (usr[k] - cr[k])));
+}
+__device__ inline void hllc64(double gamma, const double *cl, const double *cr, double *flux) {
+    hllc_t<double>(gamma, cl, cr, flux);
 }
 
 __device__ inline void hllc32(double gamma, const double *cl, const double *cr, double *flux) {
@@ -1224,20 +1281,21 @@ struct FluxParams {
 // Euler 3D without a rotating frame, in FP64 in both precisions.
 #define UCNS_TOLSMALL 1.0e-13  // src/grid_p.f90:18
 
-__device__ inline void bc_prim2cons(double gamma, const double *s3, double *cr) {
+template <class T>
+__device__ inline void bc_prim2cons(double gamma, const T *s3, T *cr) {
     cr[0] = s3[0];
     cr[1] = s3[1] * s3[0];
     cr[2] = s3[2] * s3[0];
     cr[3] = s3[3] * s3[0];
-    const double skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
-    const double ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
+    const T skins = 0.5 * ((s3[1] * s3[1]) + (s3[2] * s3[2]) + (s3[3] * s3[3]));
+    const T ikins = s3[4] / ((gamma - 1.0) * (s3[0]));
     cr[4] = (s3[0] * (ikins)) + (s3[0] * skins);
 }
 
 // subsonic inflow (src/flow_operations.f90:2907-2915): s1 free stream, s2 interior
-__device__ inline void bc_sub_in(double gamma, const double *s1, const double *s2, double sps, double nx, double ny,
-                                 double nz, double *cr) {
-    double s3[NV];
+template <class T, class F>
+__device__ inline void bc_sub_in(double gamma, const F *s1, const T *s2, T sps, double nx, double ny, double nz, T *cr) {
+    T s3[NV];
     s3[4] = 0.5 * ((s1[4]) + (s2[4]) - (s2[0] * sps * ((nx * (s1[1] - s2[1])) + (ny * (s1[2] - s2[2])) + (nz * (s1[3] - s2[3])))));
     s3[0] = s1[0] + (s3[4] - s1[4]) / (sps * sps);
     s3[1] = s1[1] - (nx * (s1[4] - s3[4])) / (sps * s2[0]);
@@ -1247,9 +1305,10 @@ __device__ inline void bc_sub_in(double gamma, const double *s1, const double *s
 }
 
 // subsonic outflow (src/flow_operations.f90:2995-3010): s1 free stream, s2 interior
-__device__ inline void bc_sub_out(double gamma, const double *s1, const double *s2, double sps, double nx, double ny,
-                                  double nz, double *cr) {
-    double s3[NV];
+template <class T, class F>
+__device__ inline void bc_sub_out(double gamma, const F *s1, const T *s2, T sps, double nx, double ny, double nz,
+                                  T *cr) {
+    T s3[NV];
     s3[4] = s1[4];
     s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
     s3[1] = s2[1] + (nx * (s2[4] - s3[4])) / (sps * s2[0]);
@@ -1259,14 +1318,16 @@ __device__ inline void bc_sub_out(double gamma, const double *s1, const double *
 }
 
 // ROTATEF / ROTATEB with precomputed cos/sin of the face angles
-__device__ inline void rotf(const double *a, double c1, double s1, double c2, double s2, double *r) {
+template <class T>
+__device__ inline void rotf(const T *a, double c1, double s1, double c2, double s2, T *r) {
     r[0] = a[0];
     r[1] = c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3];
     r[2] = c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3];
     r[3] = (-s1) * a[1] + c1 * a[2];
     r[4] = a[4];
 }
-__device__ inline void rotb(const double *h, double c1, double s1, double c2, double s2, double *r) {
+template <class T>
+__device__ inline void rotb(const T *h, double c1, double s1, double c2, double s2, T *r) {
     r[0] = h[0];
     r[1] = c1 * s2 * h[1] + c1 * c2 * h[2] + (-s1) * h[3];
     r[2] = s1 * s2 * h[1] + s1 * c2 * h[2] + c1 * h[3];
@@ -1274,21 +1335,23 @@ __device__ inline void rotb(const double *h, double c1, double s1, double c2, do
     r[4] = h[4];
 }
 
-__device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double gamma, const double *cl, double c1,
-                                   double s1, double c2, double s2, double *cr) {
+template <class T>
+__device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double gamma, const T *cl, double c1,
+                                   double s1, double c2, double s2, T *cr) {
     const double nx = c1 * s2, ny = s1 * s2, nz = c2;
-    double f[NV], w[NV];
+    double f[NV];
+    T w[NV];
     switch (code) {
     case 1:  // inflow (src/flow_operations.f90:2880-2925)
         if (bc.boundtype == 0) {
-            for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+            for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
         } else {
-            cons2prim_d(gamma, bc.inflow, f);
-            cons2prim_d(gamma, cl, w);
-            const double sps = sqrt((gamma * w[4]) / (w[0]));
-            const double vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
-            if (vel / (sps + UCNS_TOLSMALL) > 1.0) {
-                for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+            cons2prim_t(gamma, bc.inflow, f);
+            cons2prim_t(gamma, cl, w);
+            const T sps = sqrt((gamma * w[4]) / (w[0]));
+            const T vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            if (val(vel / (sps + UCNS_TOLSMALL)) > 1.0) {
+                for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
             } else {
                 bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
             }
@@ -1298,11 +1361,11 @@ __device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double 
         if (bc.boundtype == 0) {
             for (int v = 0; v < NV; ++v) cr[v] = cl[v];
         } else {
-            cons2prim_d(gamma, bc.outflow, f);
-            cons2prim_d(gamma, cl, w);
-            const double sps = sqrt((gamma * w[4]) / (w[0]));
-            const double vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
-            if (vel / (sps + UCNS_TOLSMALL) > 1.0) {
+            cons2prim_t(gamma, bc.outflow, f);
+            cons2prim_t(gamma, cl, w);
+            const T sps = sqrt((gamma * w[4]) / (w[0]));
+            const T vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            if (val(vel / (sps + UCNS_TOLSMALL)) > 1.0) {
                 for (int v = 0; v < NV; ++v) cr[v] = cl[v];
             } else {
                 bc_sub_out(gamma, f, w, sps, nx, ny, nz, cr);
@@ -1312,7 +1375,7 @@ __device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double 
     case 3:  // symmetry (src/flow_operations.f90:3086-3131)
     case 4:  // wall, inviscid branch (src/flow_operations.f90:3132-3160)
     {
-        double rot[NV];
+        T rot[NV];
         rotf(cl, c1, s1, c2, s2, rot);
         rot[1] = -rot[1];
         rotb(rot, c1, s1, c2, s2, cr);
@@ -1320,23 +1383,23 @@ __device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double 
     }
     default:  // 6, farfield (src/flow_operations.f90:3203-3296)
     {
-        double rot[NV];
+        T rot[NV];
         rotf(cl, c1, s1, c2, s2, rot);
-        const double vnb = rot[1];
-        cons2prim_d(gamma, cl, w);
-        const double sps = sqrt((gamma * w[4]) / (w[0]));
-        if (vnb <= 0.0) {
-            if (fabs(vnb) >= sps) {
-                for (int v = 0; v < NV; ++v) cr[v] = bc.inflow[v];
+        const T vnb = rot[1];
+        cons2prim_t(gamma, cl, w);
+        const T sps = sqrt((gamma * w[4]) / (w[0]));
+        if (val(vnb) <= 0.0) {
+            if (val(fabs(vnb)) >= val(sps)) {
+                for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
             } else {
-                cons2prim_d(gamma, bc.inflow, f);
+                cons2prim_t(gamma, bc.inflow, f);
                 bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
             }
         } else {
-            if (fabs(vnb) >= sps) {
+            if (val(fabs(vnb)) >= val(sps)) {
                 for (int v = 0; v < NV; ++v) cr[v] = cl[v];
             } else {
-                cons2prim_d(gamma, bc.outflow, f);
+                cons2prim_t(gamma, bc.outflow, f);
                 bc_sub_out(gamma, f, w, sps, nx, ny, nz, cr);
             }
         }
