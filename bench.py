@@ -104,6 +104,49 @@ def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.n
     }
 
 
+def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) -> dict:
+    """adjoint / primal cost: ucns_rk3_adjoint_device over `steps` steps (its
+    forward sweep with stage checkpoints plus the reverse sweep) against
+    ucns_rk3_steps over the same steps, device-resident cotangents"""
+    import torch
+    from ucns3d_amd.solver import Solver
+
+    s = Solver(case, precision)
+    s.set_state(u0)
+    dt = s.compute_dt(cfl)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randn(case.ncell, 5, dtype=torch.float64, device="cuda", generator=g)
+    u0b = torch.empty_like(a)
+    s.rk3_steps(1, dt_fixed=dt)
+    s.set_state(u0)
+    s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)  # allocates the checkpoints
+    s.set_state(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk3_steps(steps, dt_fixed=dt)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    s.set_state(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    finite = bool(torch.isfinite(u0b).all().item())
+    s.set_state(u0)
+    s.reset_stats()
+    s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
+    st = s.stats()
+    s.close()
+    return {"steps": steps, "primal_ms_per_step": t_fwd / steps * 1e3, "adjoint_ms_per_step": t_adj / steps * 1e3,
+            "adjoint_over_primal": t_adj / t_fwd, "reverse_sweep_over_primal": (t_adj - t_fwd) / t_fwd,
+            "recon_vjp_ms": st["ms_recon_vjp"] / max(st["n_recon_vjp"], 1),
+            "flux_vjp_ms": st["ms_flux_vjp"] / max(st["n_flux_vjp"], 1),
+            "recon_ms": st["ms_recon"] / max(st["n_recon"], 1), "finite": finite,
+            "note": "adjoint = forward sweep storing every stage + reverse sweep (each stage: forward "
+                    "reconstruction, flux product, reconstruction product); dt fixed"}
+
+
 def cpu_baseline(spec_key: str, seconds: float):
     """The CPU oracle (C restatement of the reference algorithm, OpenMP) on a
     reduced mesh of the same construction, timed on this host."""
@@ -178,6 +221,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--adjoint-steps", type=int, default=2,
+                    help="steps of the adjoint / primal cost measurement on the FP64 path (0: skip)")
     ap.add_argument("--accuracy", default="",
                     help="comma-separated step counts: report the mixed path's per-variable relative L2 and Linf "
                          "against the FP64 path (and, at step 1, the CPU oracle) at each, then exit")
@@ -258,6 +303,9 @@ def main() -> int:
                         "kernel": recon_kernel_name(case, UCNS_MIXED),
                         "traffic": committed_traffic(recon_kernel_name(case, UCNS_MIXED), args.config),
                         "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps}
+    if args.adjoint_steps > 0:
+        out["adjoint"] = run_adjoint(case, UCNS_FP64, args.adjoint_steps, spec.cfl, u0)
+        log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
         log(f"cpu baseline: {out['cpu_baseline']}")

@@ -118,6 +118,8 @@ typedef struct ucns_stats_v1 {
     double bytes_recon_per_cell;                  /* algorithmic bytes per cell of the reconstruction kernel */
     int64_t n_reduced_last;                       /* cells sent to MUSCL by CHECKSOL in the last residual */
     int64_t n_firstorder_last;                    /* cells sent to first order by CHECKSOLX */
+    int64_t n_recon_vjp, n_flux_vjp;              /* launches of the residual-product kernels */
+    double ms_recon_vjp, ms_flux_vjp;             /* their summed kernel time */
 } ucns_stats_v1;
 
 /* Ghost states of boundary faces (BOUNDARYS, src/flow_operations.f90:2860-3375),
@@ -158,6 +160,28 @@ int  ucns_reconstruct(ucns_ctx *ctx, double *uleft_out);
 /* TIME_MARCHING's RK3 loop for nsteps steps.  If dt_fixed > 0 it is used for
  * every step, else dt is recomputed each step from cfl.  *t_io is advanced. */
 int  ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, double *t_io);
+/* Adjoint of the residual (no counterpart in the reference, which has no
+ * adjoint): u_bar = (dR/dU)^T w at the current state, R as ucns_residual
+ * returns it.  bc_bar (10 doubles, or NULL) receives
+ * (dR/d(inflow[0..4], outflow[0..4]))^T w for the boundary data of
+ * ucns_set_boundary.  Decisions that are discrete in the primal (CHECKSOL /
+ * CHECKSOLX flags, HLLC and boundary regimes, the Barth limiter's active face
+ * point and bound) are those of the current state. */
+int  ucns_residual_vjp(ucns_ctx *ctx, const double *w, double *u_bar, int32_t layout, double *bc_bar);
+/* Same with device pointers, AOS layout (bc_bar_dev: 10 doubles or NULL). */
+int  ucns_residual_vjp_device(ucns_ctx *ctx, const double *w_dev, double *u_bar_dev, double *bc_bar_dev);
+/* Adjoint of nsteps SSP-RK3 steps (no counterpart in the reference): runs
+ * the steps from the current state as ucns_rk3_steps does, storing every
+ * stage state on the device (3 * nsteps * ncell * 40 bytes), then sweeps back:
+ * u0_bar = (dU_N/dU_0)^T uN_bar and bc_bar (10 doubles, or NULL) =
+ * (dU_N/d(inflow, outflow))^T uN_bar.  The step sizes are those of the
+ * forward sweep, held fixed in the derivative.  The state is left at U_N and
+ * *t_io (or NULL) is advanced as by ucns_rk3_steps. */
+int  ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar,
+                      double *u0_bar, int32_t layout, double *bc_bar, double *t_io);
+/* Same with device pointers, AOS layout (bc_bar_dev: 10 doubles or NULL). */
+int  ucns_rk3_adjoint_device(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar_dev,
+                             double *u0_bar_dev, double *bc_bar_dev, double *t_io);
 int  ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out);
 int  ucns_reset_stats(ucns_ctx *ctx);
 const char *ucns_last_error(const ucns_ctx *ctx);

@@ -43,9 +43,10 @@ def small_case():
 def test_header_lists_the_path():
     fns = header_functions()
     for fn in ["ucns_create", "ucns_upload_prestore", "ucns_set_state", "ucns_get_state", "ucns_residual",
-               "ucns_reconstruct", "ucns_rk3_steps", "ucns_compute_dt", "ucns_destroy", "ucns_last_error"]:
+               "ucns_reconstruct", "ucns_rk3_steps", "ucns_compute_dt", "ucns_destroy", "ucns_last_error",
+               "ucns_residual_vjp", "ucns_residual_vjp_device", "ucns_rk3_adjoint", "ucns_rk3_adjoint_device"]:
         assert fn in fns
-    assert len(fns) == 16
+    assert len(fns) == 20
 
 
 def test_every_declared_symbol_is_exported(glib):

@@ -1,0 +1,233 @@
+"""Adjoint of the residual and of SSP-RK3 on the GPU, through the C ABI.
+
+Pinned three ways:
+  - dot-product test against the oracle's tangent-linear model (the Dual
+    instantiation of the FP64 restatement, itself pinned by central
+    differences in test_tangent.py):  <w, J du> = <J^T w, du>, for the state
+    and for the free-stream data;
+  - the same identity over several RK3 steps with stage checkpoints;
+  - central finite differences of the GPU forward itself (h = 1e-6).
+Every row layout of the reconstruction is covered: the row stream (WENO-4
+tetrahedra and hexahedra, WENO-3), the tiled layout (UCNS_STREAM=0) and the
+per-cell layout (MUSCL), on periodic and bounded boxes."""
+import contextlib
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+from ucns3d_amd.solver import Solver
+
+from test_tangent import CASES, setup
+
+pytestmark = pytest.mark.gpu
+
+
+@contextlib.contextmanager
+def stream_kernels(on: bool):
+    old = os.environ.get("UCNS_STREAM")
+    os.environ["UCNS_STREAM"] = "1" if on else "0"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["UCNS_STREAM"]
+        else:
+            os.environ["UCNS_STREAM"] = old
+
+
+def solver_for(name, precision=UCNS_FP64, stream=True):
+    case, orc, u, du, bc = setup(name)
+    with stream_kernels(stream):
+        s = Solver(case, precision)
+    if bc is not None:
+        s.set_boundary(*bc)
+    s.set_state(u)
+    return s
+
+
+def dot_gap(a, b, scale):
+    return abs(a - b) / scale
+
+
+VARIANTS = [(n, True) for n in CASES] + [("tet-weno4", False), ("hex-weno4-channel", False)]
+
+
+@pytest.mark.parametrize("name,stream", VARIANTS)
+def test_residual_vjp_dot_product(name, stream):
+    case, orc, u, du, bc = setup(name)
+    print(f"[vjp dot] {name} stream={stream} ncell={case.ncell}", flush=True)
+    s = solver_for(name, stream=stream)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=u.shape)
+    ub, bcb = s.residual_vjp(w)
+    _, jdu = orc.residual_tl(u, du)
+    lhs, rhs = float((w * jdu).sum()), float((ub * du).sum())
+    scale = np.linalg.norm(w) * np.linalg.norm(jdu)
+    print("  <w,J du>", lhs, "<J^T w,du>", rhs, "gap", dot_gap(lhs, rhs, scale))
+    assert dot_gap(lhs, rhs, scale) < 1e-12
+    # the state passed through: the forward of the product's state is the state's
+    np.testing.assert_allclose(s.get_state(), u, rtol=0, atol=0)
+    if bc is not None:
+        dbc = rng.normal(size=10) * np.abs(np.r_[bc[1], bc[2]])
+        _, jb = orc.residual_tl(u, np.zeros_like(u), dbc)
+        lhs, rhs = float((w * jb).sum()), float(bcb @ dbc)
+        scale = np.linalg.norm(w) * np.linalg.norm(jb)
+        print("  boundary data: gap", dot_gap(lhs, rhs, scale))
+        assert dot_gap(lhs, rhs, scale) < 1e-12
+    else:
+        assert np.all(bcb == 0)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-muscl", "hex-weno4-channel", "tet-weno3-farfield"])
+def test_rk3_adjoint_dot_product(name):
+    case, orc, u, du, bc = setup(name)
+    print(f"[rk3 adjoint dot] {name}", flush=True)
+    s = solver_for(name)
+    dt = 0.5 * s.compute_dt(0.5)
+    rng = np.random.default_rng(12)
+    a = rng.normal(size=u.shape)
+    u0b, bcb, t = s.rk3_adjoint(3, a, dt_fixed=dt)
+    assert abs(t - 3 * dt) < 1e-15
+    uN, duN = orc.rk3_tl(u, du, 3, dt)
+    # the state is left at U_N
+    np.testing.assert_allclose(s.get_state(), uN, rtol=1e-12, atol=1e-12 * np.abs(uN).max())
+    lhs, rhs = float((a * duN).sum()), float((u0b * du).sum())
+    scale = np.linalg.norm(a) * np.linalg.norm(duN)
+    print("  gap", dot_gap(lhs, rhs, scale))
+    assert dot_gap(lhs, rhs, scale) < 1e-11
+    if bc is not None:
+        dbc = rng.normal(size=10) * np.abs(np.r_[bc[1], bc[2]])
+        _, jb = orc.rk3_tl(u, np.zeros_like(u), 3, dt, dbc)
+        lhs, rhs = float((a * jb).sum()), float(bcb @ dbc)
+        scale = np.linalg.norm(a) * np.linalg.norm(jb)
+        print("  boundary data: gap", dot_gap(lhs, rhs, scale))
+        assert dot_gap(lhs, rhs, scale) < 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4-channel"])
+def test_rk3_adjoint_matches_differences_of_the_gpu_forward(name):
+    """J(U_0) = <c, U_N> over 5 steps; dJ/dU_0 . d against central
+    differences of the GPU's own RK3 steps"""
+    case, orc, u, du, bc = setup(name)
+    print(f"[rk3 adjoint fd] {name}", flush=True)
+    s = solver_for(name)
+    dt = 0.5 * s.compute_dt(0.5)
+    rng = np.random.default_rng(13)
+    c = rng.normal(size=u.shape)
+    g, _, _ = s.rk3_adjoint(5, c, dt_fixed=dt)
+
+    def J(x):
+        s.set_state(x)
+        s.rk3_steps(5, dt_fixed=dt)
+        return float((c * s.get_state()).sum())
+    h = 1e-6
+    fd = (J(u + h * du) - J(u - h * du)) / (2 * h)
+    an = float((g * du).sum())
+    print("  adjoint", an, "fd", fd, "rel", abs(an - fd) / abs(fd))
+    assert abs(an - fd) <= 1e-6 * abs(fd)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4"])
+def test_mixed_precision_vjp_is_close_to_fp64(name):
+    case, orc, u, du, bc = setup(name)
+    print(f"[vjp mixed] {name}", flush=True)
+    rng = np.random.default_rng(14)
+    w = rng.normal(size=u.shape)
+    s64 = solver_for(name, UCNS_FP64)
+    u64, _ = s64.residual_vjp(w)
+    s64.close()
+    s32 = solver_for(name, UCNS_MIXED)
+    u32, _ = s32.residual_vjp(w)
+    s32.close()
+    err = np.linalg.norm(u32 - u64, axis=0) / np.linalg.norm(u64, axis=0)
+    print("  rel L2 per variable", err)
+    assert np.all(np.isfinite(u32))
+    assert np.all(err[[0, 1, 2, 4]] < 1e-4), err
+
+
+# ---------------------------------------------------------------- torch harness
+
+def test_autograd_gradcheck_of_the_march():
+    """torch.autograd.gradcheck of RK3March (forward ucns_rk3_steps, backward
+    ucns_rk3_adjoint_device) on 162 WENO-3 tetrahedra: every column of the
+    Jacobian by differences against every row by the adjoint"""
+    import torch
+    from ucns3d_amd.autograd import RK3March
+    from ucns3d_amd.case import TET, Case, CaseSpec
+    from test_tangent import G
+    print("[gradcheck] tet weno3 n=3", flush=True)
+    case = Case(CaseSpec("w3", TET, 3, "weno3", jitter=0.1, seed=2))
+    u = case.init_state()
+    r = u[:, :1]
+    vel = u[:, 1:4] / r + np.array([1.5, 1.2, 1.3])
+    p = (G - 1) * (u[:, 4] - 0.5 * (u[:, 1:4] ** 2).sum(1) / u[:, 0])
+    u = np.c_[r, r * vel, p / (G - 1) + 0.5 * r[:, 0] * (vel ** 2).sum(1)]
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    dt = 0.5 * s.compute_dt(0.5)
+    u0 = torch.tensor(u, dtype=torch.float64, device="cuda", requires_grad=True)
+    ok = torch.autograd.gradcheck(lambda x: RK3March.apply(x, None, s, 2, dt), (u0,), eps=1e-6, atol=1e-6,
+                                  rtol=1e-4, nondet_tol=1e-12, fast_mode=False)
+    assert ok
+    s.close()
+
+
+def test_wall_pressure_sensitivity_to_the_inflow_mach_number():
+    """J = integrated pressure on the lower wall of a channel (inflow x-,
+    characteristic outflow x+, slip walls y, symmetry z) after 20 steps from
+    a free stream at Mach M carrying a pressure/density blob; dJ/dM through
+    the initial state and the inflow/outflow data (torch autograd over
+    RK3March) against central differences of the GPU forward at h = 1e-6"""
+    import torch
+    from ucns3d_amd.autograd import RK3March
+    from ucns3d_amd.case import HEX, Case, CaseSpec
+    print("[dJ/dMach] hex weno3 channel n=8", flush=True)
+    case = Case(CaseSpec("channel", HEX, 8, "weno3", seed=3, bc=(1, 2, 4, 4, 3, 3)))
+    g = 1.4
+    dev = "cuda"
+    x = torch.tensor(case.array("centre").reshape(-1, 3), dtype=torch.float64, device=dev)
+    nf = case.nface_max
+    bcode = case.array("bcode").reshape(-1, nf)
+    normal = case.array("normal").reshape(-1, nf, 3)
+    surf = case.array("surf").reshape(-1, nf)
+    wall = (bcode == 4) & (normal[:, :, 1] < -0.5)
+    area = torch.tensor((surf * wall).sum(1), dtype=torch.float64, device=dev)
+    assert area.sum() > 0
+    s = Solver(case, UCNS_FP64)
+
+    def state(M):
+        c = (g * 1.0 / 1.0) ** 0.5
+        blob = torch.exp(-((x - torch.tensor([2.0, 2.5, 3.0], dtype=torch.float64, device=dev)) ** 2).sum(1))
+        rho = 1.0 + 0.2 * blob
+        pr = 1.0 + 0.3 * blob
+        u = M * c * torch.ones_like(rho)
+        U0 = torch.stack([rho, rho * u, 0 * rho, 0 * rho, pr / (g - 1) + 0.5 * rho * u * u], 1)
+        fs = torch.stack([torch.ones_like(M), M * c, 0 * M, 0 * M, 1.0 / (g - 1) + 0.5 * (M * c) ** 2])
+        return U0, torch.cat([fs, fs])
+
+    def J(M, nsteps, dt):
+        U0, bc = state(M)
+        UN = RK3March.apply(U0, bc, s, nsteps, dt, 1)
+        p = (g - 1) * (UN[:, 4] - 0.5 * (UN[:, 1:4] ** 2).sum(1) / UN[:, 0])
+        return (p * area).sum()
+
+    M = torch.tensor(0.6, dtype=torch.float64, device=dev, requires_grad=True)
+    U0, bc = state(M.detach())
+    s.set_boundary(1, bc[:5].cpu().numpy(), bc[5:].cpu().numpy())
+    s.set_state(U0.cpu().numpy())
+    dt = 0.5 * s.compute_dt(0.5)
+    Jm = J(M, 20, dt)
+    Jm.backward()
+    h = 1e-6
+    with torch.no_grad():
+        fd = (J(M.detach() + h, 20, dt) - J(M.detach() - h, 20, dt)).item() / (2 * h)
+    an = M.grad.item()
+    print("  J", Jm.item(), "dJ/dM adjoint", an, "fd", fd, "rel", abs(an - fd) / abs(fd))
+    assert abs(an - fd) <= 1e-6 * abs(fd)
+    s.close()

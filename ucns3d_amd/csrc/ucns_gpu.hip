@@ -486,6 +486,7 @@ struct ReconWParams {
     int *counters;
     const unsigned char *stream;  // per-tile row stream of the DMA kernels (StreamShape)
     long long tile_bytes;
+    R *gstore;             // optional [ntile][ideg*5][8*ts]: every stencil's coefficients, for the adjoint
 };
 
 template <typename T>
@@ -563,6 +564,15 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
                                            const R (*sB)[IDEG * (IDEG + 1) / 2], R (*sG0)[IDEG * NV], int lc, int s,
                                            int cell, bool cvalid, int info, bool act) {
     const int ifca = ci_ifca(info), full = ci_full(info);
+    if (p.gstore && act) {
+        // coalesced: for each coefficient the 8 ts stencils of the tile are contiguous
+        const int TP = 8 * p.ts;
+        R *gs = p.gstore + (size_t)(cell >> 3) * IDEG * NV * TP + (lc * p.ts + s);
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gs[(size_t)(k * NV + v) * TP] = g[k][v];
+    }
     // central-stencil coefficients kept for the MUSCL fallback
     if (s == 0) {
 #pragma unroll
@@ -1309,7 +1319,7 @@ template <class T, class F>
 __device__ inline void bc_sub_out(double gamma, const F *s1, const T *s2, T sps, double nx, double ny, double nz,
                                   T *cr) {
     T s3[NV];
-    s3[4] = s1[4];
+    s3[4] = T{s1[4]};
     s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
     s3[1] = s2[1] + (nx * (s2[4] - s3[4])) / (sps * s2[0]);
     s3[2] = s2[2] + (ny * (s2[4] - s3[4])) / (sps * s2[0]);
@@ -1335,33 +1345,35 @@ __device__ inline void rotb(const T *h, double c1, double s1, double c2, double 
     r[4] = h[4];
 }
 
-template <class T>
-__device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double gamma, const T *cl, double c1,
-                                   double s1, double c2, double s2, T *cr) {
+// fin/fout: the conserved free-stream states (inflow, outflow) of the
+// boundary data, of the scalar type F (Dual when their tangent is carried)
+template <class T, class F>
+__device__ inline void ghost_state(int code, int boundtype, double gamma, const T *cl, double c1, double s1, double c2,
+                                   double s2, const F *fin, const F *fout, T *cr) {
     const double nx = c1 * s2, ny = s1 * s2, nz = c2;
-    double f[NV];
+    F f[NV];
     T w[NV];
     switch (code) {
     case 1:  // inflow (src/flow_operations.f90:2880-2925)
-        if (bc.boundtype == 0) {
-            for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
+        if (boundtype == 0) {
+            for (int v = 0; v < NV; ++v) cr[v] = T{fin[v]};
         } else {
-            cons2prim_t(gamma, bc.inflow, f);
+            cons2prim_t(gamma, fin, f);
             cons2prim_t(gamma, cl, w);
             const T sps = sqrt((gamma * w[4]) / (w[0]));
             const T vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
             if (val(vel / (sps + UCNS_TOLSMALL)) > 1.0) {
-                for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
+                for (int v = 0; v < NV; ++v) cr[v] = T{fin[v]};
             } else {
                 bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
             }
         }
         break;
     case 2:  // outflow (src/flow_operations.f90:2965-3013)
-        if (bc.boundtype == 0) {
+        if (boundtype == 0) {
             for (int v = 0; v < NV; ++v) cr[v] = cl[v];
         } else {
-            cons2prim_t(gamma, bc.outflow, f);
+            cons2prim_t(gamma, fout, f);
             cons2prim_t(gamma, cl, w);
             const T sps = sqrt((gamma * w[4]) / (w[0]));
             const T vel = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
@@ -1390,16 +1402,16 @@ __device__ inline void ghost_state(int code, const ucns_boundary_v1 &bc, double 
         const T sps = sqrt((gamma * w[4]) / (w[0]));
         if (val(vnb) <= 0.0) {
             if (val(fabs(vnb)) >= val(sps)) {
-                for (int v = 0; v < NV; ++v) cr[v] = T{bc.inflow[v]};
+                for (int v = 0; v < NV; ++v) cr[v] = T{fin[v]};
             } else {
-                cons2prim_t(gamma, bc.inflow, f);
+                cons2prim_t(gamma, fin, f);
                 bc_sub_in(gamma, f, w, sps, nx, ny, nz, cr);
             }
         } else {
             if (val(fabs(vnb)) >= val(sps)) {
                 for (int v = 0; v < NV; ++v) cr[v] = cl[v];
             } else {
-                cons2prim_t(gamma, bc.outflow, f);
+                cons2prim_t(gamma, fout, f);
                 bc_sub_out(gamma, f, w, sps, nx, ny, nz, cr);
             }
         }
@@ -1436,7 +1448,7 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
             a[v] = uL[v] + (double)ulL[q * NV + v];
             bq[v] = uR[v] + (double)ulR[q * NV + v];
         }
-        if (bnd) ghost_state(-cR, p.bc, p.gamma, a, c1, s1, c2, s2, bq);  // GET_STATES_BOUNDS
+        if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, bq);  // GET_STATES_BOUNDS
         // ROTATEF (src/grid_t.f90:4217-4258)
         const double la[NV] = {a[0], c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3],
                                c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3], (-s1) * a[1] + c1 * a[2], a[4]};
@@ -1557,6 +1569,639 @@ __global__ __launch_bounds__(256) void k_dt(DtParams p) {
     if ((threadIdx.x & (WAVE - 1)) == 0 && d < 1.0e300) atomic_min_pos(p.dt_out, d);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the residual: vector-Jacobian products w -> (dR/dU)^T w.
+//
+// The residual is R_i = sum over the faces of cell i of +-F_f, F_f the
+// quadrature of ROTATEB(HLLC(ROTATEF(a_q), ROTATEF(b_q))) w_q |f|, with face
+// states a_q = U_L + ULEFT_L(q) and b_q = U_R + ULEFT_R(q) (or the ghost
+// state of a_q).  The reverse sweep runs in two kernels:
+//   k_flux_vjp  face-parallel: F_bar = w_owner - w_neighbour, and through
+//               each quadrature point the cotangents of a_q and b_q, written
+//               to the face-point slots of the two cells (each slot belongs
+//               to exactly one face, so no atomics);
+//   k_recon_vjp tile of 8 cells per wave, lane = 8 * cell + stencil: repeats
+//               the forward decisions of the reconstruction bit for bit
+//               (same arithmetic and order), then pulls the face-point
+//               cotangents back through the extrapolation, the WENO weights
+//               and indicators or the Barth limiter, and scatters
+//               P_s^T g_bar_s to the stencil members with FP64 atomics.
+// Discrete decisions (CHECKSOL / CHECKSOLX flags, HLLC and boundary
+// regimes, the limiter's arg-min face point and arg-max/min neighbour) are
+// those of the state the product is taken at, so the result is the exact
+// derivative of the branch the primal takes.
+
+// pseudo-inverse rows and stencil members in whichever device layout the
+// forward kernels use: 0 per cell [cell][j][s][k] + [cell][s][j],
+// 1 tiled [tile][j][k][8 ts] + [tile][j][8 ts], 2 the row stream
+template <typename R>
+struct LsqView {
+    int mode, ts, tp, imax, ideg, depth, hd, pb, rd;
+    long long tile_bytes;
+    const int32_t *nbr;
+    const R *P;
+    const unsigned char *rows;
+    __device__ int member(int tile, int cell, int s, int j, int pos) const {
+        if (mode == 0) return nbr[((size_t)cell * ts + s) * imax + j];
+        if (mode == 1) return nbr[((size_t)tile * imax + j) * tp + pos];
+        const unsigned char *b = rows + (size_t)tile * tile_bytes;
+        return j < depth ? ((const int32_t *)b)[j * tp + pos]
+                         : ((const int32_t *)(b + hd + (size_t)(j - depth) * rd + pb))[pos];
+    }
+    // coefficient k of member j is row[k * kst]
+    __device__ const R *row(int tile, int cell, int s, int j, int pos, int &kst) const {
+        if (mode == 0) {
+            kst = 1;
+            return P + (((size_t)cell * imax + j) * ts + s) * ideg;
+        }
+        kst = tp;
+        if (mode == 1) return P + ((size_t)tile * imax + j) * ideg * tp + pos;
+        return (const R *)(rows + (size_t)tile * tile_bytes + hd + (size_t)j * rd) + pos;
+    }
+};
+
+template <typename R>
+struct VjpParams {
+    int ncell, ntile, tiles_per_xcd, nf, nqm, qpq, qpt, iweno;
+    double gamma, lwci1, eps;
+    const double *U;
+    LsqView<R> lsq;
+    const R *B;            // [ncell][ideg*(ideg+1)/2]
+    const R *mbar;         // [ncell][ideg]
+    const R *qp;           // [ncell][nf][nqm][3]
+    const int32_t *cinfo;
+    const int32_t *cnb;    // [ncell][nf]
+    const double *ubf;     // [ncell][nf][nqm][5] cotangents of the face-point states
+    double *ubar;          // [ncell][5] accumulated
+    const R *gload;        // the forward's stencil coefficients [ntile][ideg*5][8 ts], or null: recompute
+};
+
+__device__ inline void atomic_add_f64(double *a, double v) { unsafeAtomicAdd(a, v); }
+
+template <typename T>
+__device__ inline T lane_bcast(T v, int src) {
+    return __shfl(v, src, WAVE);
+}
+
+template <typename R, int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
+    constexpr int NP = IDEG * (IDEG + 1) / 2;
+    __shared__ double sGb[8][IDEG * NV];  // cotangent of the combined coefficients
+    __shared__ R sGh[8][IDEG * NV];       // combined coefficients sum_s omega_s G_s
+    __shared__ R sG0[8][IDEG * NV];       // central-stencil coefficients
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    const int cs = cvalid ? cell : 0;
+    const int info = cvalid ? p.cinfo[cell] : 0;
+    const int ifca = ci_ifca(info), admis = ci_admis(info), full = ci_full(info);
+    const bool act = cvalid && s < admis;
+    const int pos = lc * p.lsq.ts + s;
+    double ui[NV] = {1.0, 0.0, 0.0, 0.0, 1.0};
+    if (cvalid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
+    }
+    const int nslot = ifca * p.nqm;
+    const R *qpc = p.qp + (size_t)cs * p.nf * p.nqm * 3;
+    const R *mb = p.mbar + (size_t)cs * IDEG;
+    const double *ubc = p.ubf + (size_t)cs * p.nf * p.nqm * NV;
+    auto qvalid = [&](int q) {
+        const int f = q / p.nqm, qq = q - f * p.nqm;
+        return cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+    };
+
+    // ---- A: cotangent of the combined coefficients, sum_q phi_q ubar_q,
+    // and of the cell average through a_q = U_i + ULEFT(q) ----
+    double usum[NV] = {0, 0, 0, 0, 0};
+    {
+        double ab[IDEG][NV];
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ab[k][v] = 0.0;
+#pragma unroll 1
+        for (int q = s; q < NQL * 8; q += 8) {
+            if (!qvalid(q)) continue;
+            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, mb, phi);
+            double ub[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                ub[v] = ubc[q * NV + v];
+                usum[v] += ub[v];
+            }
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ab[k][v] += (double)phi[k] * ub[v];
+        }
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double t = grp_sum8(ab[k][v]);
+                if (s == 0) sGb[lc][k * NV + v] = t;
+            }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) usum[v] = grp_sum8(usum[v]);
+    }
+
+    // ---- B: the stencil coefficients G_s = P_s dU_s, as the forward ----
+    R g[IDEG][NV];
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[k][v] = R(0);
+    if (act && p.gload) {
+        const R *gs = p.gload + (size_t)tile * IDEG * NV * p.lsq.tp + pos;
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) g[k][v] = gs[(size_t)(k * NV + v) * p.lsq.tp];
+    } else if (act) {
+#pragma unroll 1
+        for (int j = 0; j < IMAX; ++j) {
+            const int n = p.lsq.member(tile, cell, s, j, pos);
+            R du[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) du[v] = (R)(p.U[(size_t)n * NV + v] - ui[v]);
+            int kst;
+            const R *row = p.lsq.row(tile, cell, s, j, pos, kst);
+            R pc[IDEG];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) pc[k] = row[(size_t)k * kst];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+        }
+    }
+    if (s == 0) {
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sG0[lc][k * NV + v] = g[k][v];
+    }
+
+    // ---- C: indicators, weights and combined coefficients (forward) ----
+    const R *Bc = p.B + (size_t)cs * NP;
+    R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
+    R om[NV] = {R(0), R(0), R(0), R(0), R(0)};
+    double obar[NV] = {0, 0, 0, 0, 0};
+    if (p.iweno == 1) {
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+                for (int l = 0; l < IDEG; ++l) {
+                    const R bkl = Bc[pidx(IDEG, k, l)];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
+            }
+        }
+        const R lam = (s == 0) ? (R)p.lwci1 : R(1);
+        const R eps = (R)p.eps;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R w;
+            if (sizeof(R) == 8) {
+                const R x = eps + beta[v];
+                const R x2 = x * x;
+                w = act ? lam / (x2 * x2) : R(0);
+            } else {
+                const R x = eps + beta[v];
+                const R xmin = grp_min8(act ? x : (R)INFINITY);
+                const R r = xmin / x;
+                const R r2 = r * r;
+                w = act ? lam * (r2 * r2) : R(0);
+            }
+            const R sum = grp_sum8(w);
+            om[v] = act ? w / sum : R(0);
+        }
+        __syncthreads();  // sGb
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const R gh = grp_sum8(om[v] * g[k][v]);
+                if (s == 0) sGh[lc][k * NV + v] = gh;
+                obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
+            }
+    }
+    __syncthreads();  // sGb, sGh, sG0
+
+    // ---- D: CHECKSOL (0.85) on the WENO face values (forward) ----
+    double wi[NV];
+    cons2prim_d(p.gamma, ui, wi);
+    bool reduce = cvalid;
+    if (p.iweno == 1) {
+        int trouble = 0;
+#pragma unroll 1
+        for (int q = s; q < NQL * 8; q += 8) {
+            if (!qvalid(q)) continue;
+            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, mb, phi);
+            double uq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * sGh[lc][k * NV + v];
+                uq[v] = ui[v] + (double)res;
+            }
+            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+        }
+        const int any_trouble = grp_or8(trouble);
+        reduce = cvalid && (!full || any_trouble);
+    }
+
+    // ---- E: MUSCL + Barth (forward), with the limiter's active arguments ----
+    bool first = false;
+    double slope[NV], d2s[NV];
+    int qs[NV], ext[NV];
+    R res0[NQL][NV];
+    if (reduce) {
+        double umin[NV], umax[NV];
+        int imin[NV], imax_[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            umin[v] = umax[v] = ui[v];
+            imin[v] = imax_[v] = cell;
+        }
+        for (int f = 0; f < ifca; ++f) {
+            const int n = p.cnb[(size_t)cell * p.nf + f];
+            if (n < 0) continue;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double un = p.U[(size_t)n * NV + v];
+                if (un < umin[v]) imin[v] = n;
+                if (un > umax[v]) imax_[v] = n;
+                umin[v] = fmin(umin[v], un);
+                umax[v] = fmax(umax[v], un);
+            }
+        }
+        double lmin[NV];
+        int lq[NV], lw[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            lmin[v] = 1.0e16;
+            lq[v] = -1;
+            lw[v] = 0;
+        }
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) res0[r][v] = R(0);
+            if (qvalid(q)) {
+                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                R phi[IDEG];
+                basis_at<R, IDEG>(x, mb, phi);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    R res = R(0);
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) res += phi[k] * sG0[lc][k * NV + v];
+                    res0[r][v] = res;
+                    const double d2 = (double)res;
+                    double psi;
+                    int wsel = 0;
+                    if (fabs(d2) <= 0.0)
+                        psi = 1.0;
+                    else if (d2 > 0.0)
+                        psi = fmin(1.0, (umax[v] - ui[v]) / d2);
+                    else {
+                        psi = fmin(1.0, (umin[v] - ui[v]) / d2);
+                        wsel = 1;
+                    }
+                    if (psi < lmin[v]) {
+                        lmin[v] = psi;
+                        lq[v] = q;
+                        lw[v] = wsel;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            slope[v] = grp_min8(lmin[v]);
+            // lowest lane of the group holding the minimum
+            int cand = (lmin[v] == slope[v]) ? s : 8;
+            cand = min(cand, __shfl_xor(cand, 1, WAVE));
+            cand = min(cand, __shfl_xor(cand, 2, WAVE));
+            cand = min(cand, __shfl_xor(cand, 4, WAVE));
+            const int src = lc * 8 + cand;
+            qs[v] = lane_bcast(lq[v], src);
+            const int wsel = lane_bcast(lw[v], src);
+            ext[v] = wsel ? imin[v] : imax_[v];
+            d2s[v] = 0.0;
+        }
+        // the unlimited value at the arg-min point, for the active limiters
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double mine = 0.0;
+#pragma unroll
+            for (int r = 0; r < NQL; ++r)
+                if (s + 8 * r == qs[v]) mine = (double)res0[r][v];
+            d2s[v] = grp_sum8(mine);
+        }
+        int tr2 = 0;
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            if (qvalid(q)) {
+                double uq[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) uq[v] = ui[v] + (double)(R)((double)res0[r][v] * slope[v]);
+                if (jump_test(p.gamma, wi, uq, 0.95)) tr2 = 1;
+            }
+        }
+        first = grp_or8(tr2) != 0;
+    }
+
+    // ---- F: cotangent of each lane's stencil coefficients ----
+    double gb[IDEG][NV];
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
+    double ubi[NV];  // extra cotangent of U_i from the limiter
+    bool scatter = false;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ubi[v] = 0.0;
+    if (cvalid && !first) {
+        if (reduce) {
+            // ULEFT(q) = slope * d2_q: slope_bar = sum_q ubar_q d2_q; through an
+            // active limiter slope = (u_ext - u_i) / d2_{q*}
+            double sb[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < NQL; ++r) {
+                const int q = s + 8 * r;
+                if (qvalid(q)) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sb[v] += ubc[q * NV + v] * (double)res0[r][v];
+                }
+            }
+            double cq[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                sb[v] = grp_sum8(sb[v]);
+                const bool on = slope[v] < 1.0 && qs[v] >= 0;
+                const double e = on ? sb[v] / d2s[v] : 0.0;
+                cq[v] = on ? e * slope[v] : 0.0;
+                ubi[v] = -e;
+                if (s == 0 && on) atomic_add_f64(&p.ubar[(size_t)ext[v] * NV + v], e);
+            }
+            if (s == 0) {
+                scatter = true;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) gb[k][v] = slope[v] * sGb[lc][k * NV + v];
+                    if (cq[v] != 0.0) {
+                        const int q = qs[v];
+                        R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                        R phi[IDEG];
+                        basis_at<R, IDEG>(x, mb, phi);
+#pragma unroll
+                        for (int k = 0; k < IDEG; ++k) gb[k][v] -= cq[v] * (double)phi[k];
+                    }
+                }
+            }
+        } else {
+            // omega_s = lam_s (eps+beta_s)^-4 / sum_t (...):
+            // beta_bar_s = -4 omega_s/(eps+beta_s) (obar_s - sum_t omega_t obar_t)
+            double bb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double so = grp_sum8(act ? (double)om[v] * obar[v] : 0.0);
+                bb[v] = act ? -4.0 * (double)om[v] / ((double)p.eps + (double)beta[v]) * (obar[v] - so) : 0.0;
+            }
+            if (act) {
+                scatter = true;
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) {
+                    double hg[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+                    for (int l = 0; l < IDEG; ++l) {
+                        const double bkl = (double)Bc[pidx(IDEG, k, l)];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) hg[v] += bkl * (double)g[l][v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        gb[k][v] = (double)om[v] * sGb[lc][k * NV + v] + 2.0 * bb[v] * hg[v];
+                }
+            }
+        }
+    }
+
+    // ---- G: dU_bar_j = P_j^T g_bar to the members, minus the same to U_i ----
+    double acc[NV] = {0, 0, 0, 0, 0};
+    if (scatter) {
+        // member j+1's row and index are in flight while member j is scattered
+        int kst;
+        int n = p.lsq.member(tile, cell, s, 0, pos);
+        const R *row = p.lsq.row(tile, cell, s, 0, pos, kst);
+        R pc[IDEG];
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k) pc[k] = row[(size_t)k * kst];
+#pragma unroll 1
+        for (int j = 0; j < IMAX; ++j) {
+            R pn[IDEG];
+            int nn = 0;
+            if (j + 1 < IMAX) {
+                nn = p.lsq.member(tile, cell, s, j + 1, pos);
+                const R *rn = p.lsq.row(tile, cell, s, j + 1, pos, kst);
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) pn[k] = rn[(size_t)k * kst];
+            }
+            double d[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                const double pk = (double)pc[k];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                atomic_add_f64(&p.ubar[(size_t)n * NV + v], d[v]);
+                acc[v] -= d[v];
+            }
+            n = nn;
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) pc[k] = pn[k];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double t = grp_sum8(acc[v]);
+        if (s == 0 && cvalid) atomic_add_f64(&p.ubar[(size_t)cell * NV + v], t + usum[v] + ubi[v]);
+    }
+}
+
+// Face-parallel adjoint of the HLLC flux quadrature.  The Jacobians of HLLC
+// (10 inputs) and of the ghost state (5 inputs, plus the 10 free-stream
+// values) come from forward-mode passes of the same templated code the
+// primal runs, one pass per input; F_bar is pulled back through ROTATEB
+// (whose transpose is ROTATEF) and ROTATEF (transpose ROTATEB).
+template <typename R>
+__global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double *lam, double *ubf, double *bcbar) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= p.nface) return;
+    const int cL = p.fcell[2 * f], cR = p.fcell[2 * f + 1];
+    const int s0 = p.fslot[2 * f], sR = p.fslot[2 * f + 1];
+    const int sL = s0 & 255;
+    const bool tri = (s0 >> 8) & 1;
+    const int nq = tri ? p.qpt : p.qpq;
+    const double *w = tri ? c_wt : c_wq;
+    const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
+    const double area = p.farea[f];
+    const bool bnd = cR < 0;
+    double uL[NV], uR[NV], Fb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        uL[v] = p.U[(size_t)cL * NV + v];
+        uR[v] = bnd ? 0.0 : p.U[(size_t)cR * NV + v];
+        Fb[v] = lam[(size_t)cL * NV + v] - (bnd ? 0.0 : lam[(size_t)cR * NV + v]);
+    }
+    double hb0[NV];
+    rotf(Fb, c1, s1, c2, s2, hb0);
+    const R *ulL = p.uleft + ((size_t)cL * p.nf + sL) * p.nqm * NV;
+    const R *ulR = p.uleft + ((size_t)(bnd ? cL : cR) * p.nf + (bnd ? sL : sR)) * p.nqm * NV;
+    double *obL = ubf + ((size_t)cL * p.nf + sL) * p.nqm * NV;
+    double *obR = ubf + ((size_t)(bnd ? cL : cR) * p.nf + (bnd ? sL : sR)) * p.nqm * NV;
+    double bcacc[2 * NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < nq; ++q) {
+        double a[NV], bq[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            a[v] = uL[v] + (double)ulL[q * NV + v];
+            bq[v] = uR[v] + (double)ulR[q * NV + v];
+        }
+        if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, bq);
+        double la[NV], ra[NV];
+        rotf(a, c1, s1, c2, s2, la);
+        rotf(bq, c1, s1, c2, s2, ra);
+        const double wa = w[q] * area;
+        double hb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) hb[v] = wa * hb0[v];
+        double lab[NV], rab[NV];
+#pragma unroll 1
+        for (int m = 0; m < 2 * NV; ++m) {
+            Dual L[NV], Rt[NV], F[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                L[v] = Dual{la[v], m == v ? 1.0 : 0.0};
+                Rt[v] = Dual{ra[v], m == NV + v ? 1.0 : 0.0};
+            }
+            hllc_t(p.gamma, L, Rt, F);
+            double d = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) d += hb[v] * F[v].d;
+            if (m < NV)
+                lab[m] = d;
+            else
+                rab[m - NV] = d;
+        }
+        double ab[NV], bb[NV];
+        rotb(lab, c1, s1, c2, s2, ab);
+        rotb(rab, c1, s1, c2, s2, bb);
+        if (bnd) {
+            const int code = -cR;
+#pragma unroll 1
+            for (int m = 0; m < NV; ++m) {
+                Dual A[NV], G[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) A[v] = Dual{a[v], m == v ? 1.0 : 0.0};
+                ghost_state(code, p.bc.boundtype, p.gamma, A, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, G);
+                double d = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) d += bb[v] * G[v].d;
+                ab[m] += d;
+            }
+            if (bcbar && (code == 1 || code == 2 || code == 6)) {
+#pragma unroll 1
+                for (int m = 0; m < 2 * NV; ++m) {
+                    Dual A[NV], G[NV], fi[NV], fo[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        A[v] = Dual{a[v], 0.0};
+                        fi[v] = Dual{p.bc.inflow[v], m == v ? 1.0 : 0.0};
+                        fo[v] = Dual{p.bc.outflow[v], m == NV + v ? 1.0 : 0.0};
+                    }
+                    ghost_state(code, p.bc.boundtype, p.gamma, A, c1, s1, c2, s2, fi, fo, G);
+                    double d = 0.0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) d += bb[v] * G[v].d;
+                    bcacc[m] += d;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) obR[q * NV + v] = bb[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) obL[q * NV + v] = ab[v];
+    }
+    if (bcbar && bnd) {
+#pragma unroll
+        for (int m = 0; m < 2 * NV; ++m)
+            if (bcacc[m] != 0.0) atomic_add_f64(&bcbar[m], bcacc[m]);
+    }
+}
+
+__global__ void k_zero_f64(double *p, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        p[e] = 0.0;
+}
+
+// Reverse of one Shu-Osher stage (the three stage updates are affine in
+// the stage state and linear in R):
+//   stage 3: u = 1/3 u0 + 2/3 u2 - 2/3 dt R(u2)/V   -> u2_bar, u0_bar
+//   stage 2: u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1)/V  -> u1_bar, u0_bar
+//   stage 1: u1 = u0 - dt R(u0)/V                    -> u0_bar
+// k_rk3_adj_w forms w = -c dt ybar / V (the cotangent handed to the residual
+// product) for the stage whose output cotangent is ybar; k_rk3_adj_acc then
+// adds a ybar and the product to the cotangents of the stage's inputs.
+__global__ void k_rk3_adj_w(int ncell, const double *ybar, const double *ovol, const double *dt, double c, double *w) {
+    const size_t n = (size_t)ncell * NV;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        w[e] = -(c * *dt) * ybar[e] * ovol[e / NV];
+}
+// the cotangents of one stage's inputs from its output cotangent y and the
+// residual product jt = (dR/du)^T w:
+//   stage 3: x = 2/3 y + jt (x = u2_bar), u0_bar = 1/3 y
+//   stage 2: x = 1/4 y + jt (x = u1_bar), u0_bar += 3/4 y   (x may alias y)
+//   stage 1: u0_bar += y + jt
+__global__ void k_rk3_adj_stage(int ncell, int stage, const double *y, const double *jt, double *x, double *u0) {
+    const size_t n = (size_t)ncell * NV;
+    const double to4 = 3.0 / 4.0, oo4 = 1.0 / 4.0, to3 = 2.0 / 3.0, oo3 = 1.0 / 3.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const double ye = y[e], je = jt[e];
+        if (stage == 3) {
+            x[e] = to3 * ye + je;
+            u0[e] = oo3 * ye;
+        } else if (stage == 2) {
+            u0[e] += to4 * ye;
+            x[e] = oo4 * ye + je;
+        } else {
+            u0[e] += ye + je;
+        }
+    }
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 __global__ void k_nonfinite(const double *u, size_t n, int *flag) {
@@ -1618,6 +2263,13 @@ struct ucns_ctx {
     double *U = nullptr, *U2 = nullptr, *fflux = nullptr, *rhs = nullptr, *t = nullptr, *dtfix = nullptr;
     unsigned long long *dtv = nullptr;  // [2] ping-pong dt slots
     int *counters = nullptr, *flag = nullptr;
+    // adjoint (allocated on first use): face-point cotangents, state-sized
+    // work arrays, the free-stream cotangent, stage checkpoints and step dts
+    double *adj_ubf = nullptr, *adj_lam = nullptr, *adj_ubar = nullptr, *adj_x = nullptr, *adj_a = nullptr;
+    double *adj_u0 = nullptr, *adj_bc = nullptr, *ckpt = nullptr, *dts = nullptr;
+    void *adj_g = nullptr;  // stencil coefficients of the tiled reconstruction, [ntile][ideg*5][8 ts] (R)
+    bool adj_g_on = false;  // the next tiled reconstruction stores them
+    size_t ckpt_steps = 0;
     // stats
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -1695,6 +2347,7 @@ int launch_recon_t(ucns_ctx *ctx) {
         const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
+        w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
         if (ctx->smode) {
             // (ideg, imax, face-point slots, stencils): WENO-4 tets / hexes, WENO-3 tets / hexes
             if constexpr (sizeof(R) == 8) {
@@ -1864,6 +2517,210 @@ int dt_kernel(ucns_ctx *ctx, double cfl, unsigned long long *out) {
     tstop(ctx, 3);
     tcollect(ctx, 3, &ctx->st.ms_dt, &ctx->st.n_dt);
     return UCNS_OK;
+}
+
+// ---- adjoint -----------------------------------------------------------
+void release_adjoint(ucns_ctx *ctx) {
+    dfree(ctx->adj_ubf);
+    dfree(ctx->adj_lam);
+    dfree(ctx->adj_ubar);
+    dfree(ctx->adj_x);
+    dfree(ctx->adj_a);
+    dfree(ctx->adj_u0);
+    dfree(ctx->adj_bc);
+    dfree(ctx->ckpt);
+    dfree(ctx->dts);
+    if (ctx->adj_g) (void)hipFree(ctx->adj_g);
+    ctx->adj_g = nullptr;
+    ctx->ckpt_steps = 0;
+}
+
+int ensure_adjoint(ucns_ctx *ctx) {
+    if (ctx->adj_ubf) return UCNS_OK;
+    const size_t n = (size_t)ctx->ncell * NV;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->adj_ubf, (size_t)ctx->ncell * ctx->nf * ctx->nqm * NV))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_lam, n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_ubar, n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_x, n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_a, n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_u0, n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->adj_bc, 2 * NV))) return rc;
+    if (ctx->tiled) {
+        const size_t ng = (size_t)ctx->ntile * ctx->ideg * NV * 8 * ctx->ts;
+        if (ctx->precision == UCNS_MIXED)
+            rc = dalloc(ctx, (float **)&ctx->adj_g, ng);
+        else
+            rc = dalloc(ctx, (double **)&ctx->adj_g, ng);
+        if (rc) return rc;
+    }
+    return UCNS_OK;
+}
+
+void zero_dev(ucns_ctx *ctx, double *p, size_t n) {
+    const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_f64, dim3(std::max(blocks, 1)), dim3(256), 0, ctx->stream, p, n);
+}
+
+template <typename R>
+int launch_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
+    const int nfq = ctx->nf * ctx->nqm;
+    if (ctx->ts > 8 || nfq > 56) {
+        ctx->err = "the residual product needs at most 8 stencils and 56 face points per cell";
+        return UCNS_EUNSUPPORTED;
+    }
+    // forward reconstruction of the state the product is taken at; the tiled
+    // kernels also store every stencil's coefficients for the product
+    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
+    ctx->adj_g_on = ctx->tiled && ctx->adj_g;
+    int rc = launch_recon_t<R>(ctx);
+    ctx->adj_g_on = false;
+    if (rc) return rc;
+    zero_dev(ctx, ubar, (size_t)ctx->ncell * NV);
+    FluxParams<R> fp{};
+    fp.nface = ctx->nface;
+    fp.nf = ctx->nf;
+    fp.nqm = ctx->nqm;
+    fp.qpq = ctx->qpq;
+    fp.qpt = ctx->qpt;
+    fp.gamma = ctx->gamma;
+    fp.fcell = ctx->fcell;
+    fp.fslot = ctx->fslot;
+    fp.frot = ctx->frot;
+    fp.farea = ctx->farea;
+    fp.U = ctx->U;
+    fp.uleft = (const R *)ctx->uleft;
+    fp.fflux = nullptr;
+    fp.bc = ctx->bc;
+    tstart(ctx, 1);
+    hipLaunchKernelGGL((k_flux_vjp<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
+                       ctx->adj_ubf, bcbar);
+    tstop(ctx, 1);
+    tcollect(ctx, 1, &ctx->st.ms_flux_vjp, &ctx->st.n_flux_vjp);
+    VjpParams<R> vp{};
+    vp.ncell = ctx->ncell;
+    vp.ntile = ctx->ntile;
+    vp.tiles_per_xcd = (ctx->ntile + NXCD - 1) / NXCD;
+    vp.nf = ctx->nf;
+    vp.nqm = ctx->nqm;
+    vp.qpq = ctx->qpq;
+    vp.qpt = ctx->qpt;
+    vp.iweno = ctx->iweno;
+    vp.gamma = ctx->gamma;
+    vp.lwci1 = ctx->lwci1;
+    vp.eps = ctx->eps;
+    vp.U = ctx->U;
+    const bool mixed = sizeof(R) == 4;
+    LsqView<R> &L = vp.lsq;
+    L.mode = ctx->smode ? 2 : ctx->tiled ? 1 : 0;
+    L.ts = ctx->ts;
+    L.tp = 8 * ctx->ts;
+    L.imax = ctx->imax;
+    L.ideg = ctx->ideg;
+    L.depth = stream_depth(mixed, ctx->ts);
+    L.pb = ctx->ideg * L.tp * (int)sizeof(R);
+    L.rd = L.pb + L.tp * 4;
+    L.hd = L.depth * L.tp * 4;
+    L.tile_bytes = ctx->tile_bytes;
+    L.nbr = ctx->nbr;
+    L.P = (const R *)ctx->P;
+    L.rows = ctx->rows;
+    vp.B = (const R *)ctx->B;
+    vp.mbar = (const R *)ctx->mbar;
+    vp.qp = (const R *)ctx->qp;
+    vp.cinfo = ctx->cinfo;
+    vp.cnb = ctx->cnb;
+    vp.ubf = ctx->adj_ubf;
+    vp.ubar = ubar;
+    vp.gload = (ctx->tiled && ctx->adj_g) ? (const R *)ctx->adj_g : nullptr;
+    const dim3 grid(vp.tiles_per_xcd * NXCD), block(WAVE);
+    const bool q3 = nfq <= 24;
+    tstart(ctx, 0);
+    if (ctx->ideg == 19 && ctx->imax == 39) {
+        if (q3)
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3>), grid, block, 0, ctx->stream, vp);
+        else
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7>), grid, block, 0, ctx->stream, vp);
+    } else if (ctx->ideg == 9 && ctx->imax == 19) {
+        if (q3)
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3>), grid, block, 0, ctx->stream, vp);
+        else
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7>), grid, block, 0, ctx->stream, vp);
+    } else if (ctx->ideg == 3 && ctx->imax == 8) {
+        if (q3)
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3>), grid, block, 0, ctx->stream, vp);
+        else
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7>), grid, block, 0, ctx->stream, vp);
+    } else {
+        ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
+        return UCNS_EUNSUPPORTED;
+    }
+    tstop(ctx, 0);
+    tcollect(ctx, 0, &ctx->st.ms_recon_vjp, &ctx->st.n_recon_vjp);
+    return UCNS_OK;
+}
+
+// (dR/dU)^T lam at the current state into ubar; bcbar (device, 10) += the
+// free-stream cotangent when non-null
+int residual_vjp_dev(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
+    int rc = ensure_adjoint(ctx);
+    if (rc) return rc;
+    return ctx->precision == UCNS_MIXED ? launch_vjp_t<float>(ctx, lam, ubar, bcbar)
+                                        : launch_vjp_t<double>(ctx, lam, ubar, bcbar);
+}
+
+// one SSP-RK3 step forward from ctx->U with the stage states stored
+int rk3_step_ckpt(ucns_ctx *ctx, int it, const double *dt, unsigned long long *next, double cfl) {
+    const size_t n = (size_t)ctx->ncell * NV;
+    for (int stage = 1; stage <= 3; ++stage) {
+        HIP_TRY(hipMemcpyAsync(ctx->ckpt + ((size_t)3 * it + stage - 1) * n, ctx->U, n * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        int rc = residual_kernels(ctx);
+        if (rc) return rc;
+        if (stage == 3 && next)
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, ctx->stream, next,
+                               (unsigned long long)0x7FF0000000000000ULL);
+        update_kernel(ctx, stage, dt, stage == 3 ? next : nullptr, cfl, nullptr);
+    }
+    return UCNS_OK;
+}
+
+// The reverse sweep over nsteps stored steps: abar holds the cotangent of
+// the final state on entry and of the initial state on return.
+int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
+    const size_t n = (size_t)ctx->ncell * NV;
+    const int blocks = (int)std::min<size_t>(8192, (n + 255) / 256);
+    double *const ustate = ctx->U;
+    double *a = abar, *u0 = ctx->adj_u0, *x = ctx->adj_x, *w = ctx->adj_lam, *jt = ctx->adj_ubar;
+    int rc = UCNS_OK;
+    for (int it = nsteps - 1; it >= 0 && rc == UCNS_OK; --it) {
+        const double *dt = ctx->dts + it;
+        // stage 3: u = 1/3 u0 + 2/3 u2 - 2/3 dt R(u2)/V
+        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, a, ctx->ovol, dt,
+                           2.0 / 3.0, w);
+        ctx->U = ctx->ckpt + ((size_t)3 * it + 2) * n;
+        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
+        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 3, a, jt, x, u0);
+        // stage 2: u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1)/V
+        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, x, ctx->ovol, dt,
+                           1.0 / 4.0, w);
+        ctx->U = ctx->ckpt + ((size_t)3 * it + 1) * n;
+        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
+        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 2, x, jt, x, u0);
+        // stage 1: u1 = u0 - dt R(u0)/V
+        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, x, ctx->ovol, dt, 1.0,
+                           w);
+        ctx->U = ctx->ckpt + (size_t)3 * it * n;
+        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
+        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 1, x, jt, x, u0);
+        const hipError_t e = hipMemcpyAsync(a, u0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e);
+            rc = UCNS_EHIP;
+        }
+    }
+    ctx->U = ustate;
+    return rc;
 }
 
 int state_io(ucns_ctx *ctx, const double *src, double *dst, int32_t layout, hipMemcpyKind kind, bool to_dev) {
@@ -2119,6 +2976,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     }
     HIP_TRY(hipSetDevice(ctx->device));
     release_prestore(ctx);
+    release_adjoint(ctx);  // its buffers are sized by the stencil count
     const int nc = ctx->ncell, ID = ctx->ideg, IM = ctx->imax, NNB = ctx->nnb, TSA = ctx->ts_abi;
     const int base = ctx->index_base;
     // admissible stencils, flags
@@ -2504,6 +3362,150 @@ int ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, d
     return UCNS_OK;
 }
 
+static void to_aos(const ucns_ctx *ctx, const double *src, int32_t layout, std::vector<double> &dst) {
+    const size_t n = (size_t)ctx->ncell * NV;
+    dst.resize(n);
+    if (layout == UCNS_LAYOUT_AOS)
+        std::memcpy(dst.data(), src, n * sizeof(double));
+    else
+        for (int i = 0; i < ctx->ncell; ++i)
+            for (int v = 0; v < NV; ++v) dst[(size_t)i * NV + v] = src[(size_t)v * ctx->ncell + i];
+}
+
+static void from_aos(const ucns_ctx *ctx, const std::vector<double> &src, int32_t layout, double *dst) {
+    const size_t n = (size_t)ctx->ncell * NV;
+    if (layout == UCNS_LAYOUT_AOS)
+        std::memcpy(dst, src.data(), n * sizeof(double));
+    else
+        for (int i = 0; i < ctx->ncell; ++i)
+            for (int v = 0; v < NV; ++v) dst[(size_t)v * ctx->ncell + i] = src[(size_t)i * NV + v];
+}
+
+int ucns_residual_vjp_device(ucns_ctx *ctx, const double *w_dev, double *u_bar_dev, double *bc_bar_dev) {
+    if (!ctx || !w_dev || !u_bar_dev) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = ensure_adjoint(ctx))) return rc;
+    zero_dev(ctx, ctx->adj_bc, 2 * NV);
+    if ((rc = residual_vjp_dev(ctx, w_dev, u_bar_dev, ctx->adj_bc))) return rc;
+    HIP_TRY(hipGetLastError());
+    if (bc_bar_dev)
+        HIP_TRY(hipMemcpyAsync(bc_bar_dev, ctx->adj_bc, 2 * NV * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UCNS_OK;
+}
+
+int ucns_residual_vjp(ucns_ctx *ctx, const double *w, double *u_bar, int32_t layout, double *bc_bar) {
+    if (!ctx || !w || !u_bar) return UCNS_EBADARG;
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = ensure_adjoint(ctx))) return rc;
+    const size_t n = (size_t)ctx->ncell * NV;
+    std::vector<double> h;
+    to_aos(ctx, w, layout, h);
+    HIP_TRY(hipMemcpyAsync(ctx->adj_a, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    zero_dev(ctx, ctx->adj_bc, 2 * NV);
+    if ((rc = residual_vjp_dev(ctx, ctx->adj_a, ctx->adj_x, ctx->adj_bc))) return rc;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h.data(), ctx->adj_x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double bc[2 * NV];
+    HIP_TRY(hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    from_aos(ctx, h, layout, u_bar);
+    if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
+    return UCNS_OK;
+}
+
+// forward sweep with stage checkpoints + reverse sweep; a_dev holds uN_bar
+// on entry and u0_bar on return
+static int rk3_adjoint_impl(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, double *a_dev, double *t_io) {
+    if (nsteps < 1) {
+        ctx->err = "nsteps must be >= 1";
+        return UCNS_EBADARG;
+    }
+    if (!(dt_fixed > 0) && !(cfl > 0)) {
+        ctx->err = "need cfl > 0 or dt_fixed > 0";
+        return UCNS_EBADARG;
+    }
+    int rc = ensure_adjoint(ctx);
+    if (rc) return rc;
+    const size_t n = (size_t)ctx->ncell * NV;
+    if (ctx->ckpt_steps < (size_t)nsteps) {
+        dfree(ctx->ckpt);
+        dfree(ctx->dts);
+        ctx->ckpt_steps = 0;
+        if ((rc = dalloc(ctx, &ctx->ckpt, (size_t)3 * nsteps * n))) return rc;
+        if ((rc = dalloc(ctx, &ctx->dts, (size_t)nsteps))) return rc;
+        ctx->ckpt_steps = nsteps;
+    }
+    const double t0 = t_io ? *t_io : 0.0;
+    HIP_TRY(hipMemcpyAsync(ctx->t, &t0, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const bool fixed = dt_fixed > 0;
+    if (fixed) HIP_TRY(hipMemcpyAsync(ctx->dtfix, &dt_fixed, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int cur = 0;
+    if (!fixed) dt_kernel(ctx, cfl, &ctx->dtv[cur]);
+    for (int it = 0; it < nsteps; ++it) {
+        const double *dt = fixed ? ctx->dtfix : (const double *)&ctx->dtv[cur];
+        unsigned long long *next = fixed ? nullptr : &ctx->dtv[1 - cur];
+        HIP_TRY(hipMemcpyAsync(ctx->dts + it, dt, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if ((rc = rk3_step_ckpt(ctx, it, dt, next, cfl))) return rc;
+        HIP_TRY(hipGetLastError());
+        if (!fixed) cur = 1 - cur;
+    }
+    zero_dev(ctx, ctx->adj_bc, 2 * NV);
+    if ((rc = rk3_reverse(ctx, nsteps, a_dev, ctx->adj_bc))) return rc;
+    HIP_TRY(hipGetLastError());
+    double t1 = t0;
+    HIP_TRY(hipMemcpyAsync(&t1, ctx->t, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t_io) *t_io = t1;
+    return UCNS_OK;
+}
+
+int ucns_rk3_adjoint_device(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar_dev,
+                            double *u0_bar_dev, double *bc_bar_dev, double *t_io) {
+    if (!ctx || !uN_bar_dev || !u0_bar_dev) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = ensure_adjoint(ctx))) return rc;
+    const size_t n = (size_t)ctx->ncell * NV;
+    HIP_TRY(hipMemcpyAsync(ctx->adj_a, uN_bar_dev, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = rk3_adjoint_impl(ctx, nsteps, cfl, dt_fixed, ctx->adj_a, t_io))) return rc;
+    HIP_TRY(hipMemcpyAsync(u0_bar_dev, ctx->adj_a, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (bc_bar_dev)
+        HIP_TRY(hipMemcpyAsync(bc_bar_dev, ctx->adj_bc, 2 * NV * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UCNS_OK;
+}
+
+int ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar,
+                     double *u0_bar, int32_t layout, double *bc_bar, double *t_io) {
+    if (!ctx || !uN_bar || !u0_bar) return UCNS_EBADARG;
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = ensure_adjoint(ctx))) return rc;
+    const size_t n = (size_t)ctx->ncell * NV;
+    std::vector<double> h;
+    to_aos(ctx, uN_bar, layout, h);
+    HIP_TRY(hipMemcpyAsync(ctx->adj_a, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = rk3_adjoint_impl(ctx, nsteps, cfl, dt_fixed, ctx->adj_a, t_io))) return rc;
+    HIP_TRY(hipMemcpyAsync(h.data(), ctx->adj_a, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double bc[2 * NV];
+    HIP_TRY(hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    from_aos(ctx, h, layout, u0_bar);
+    if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
+    return UCNS_OK;
+}
+
 int ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out) {
     if (!ctx || !out) return UCNS_EBADARG;
     *out = ctx->st;
@@ -2528,6 +3530,7 @@ void ucns_destroy(ucns_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     release_prestore(ctx);
+    release_adjoint(ctx);
     dfree(ctx->cinfo);
     dfree(ctx->cnb);
     dfree(ctx->cface);

@@ -61,6 +61,8 @@ class StatsV1(C.Structure):
         ("ms_recon", C.c_double), ("ms_flux", C.c_double), ("ms_update", C.c_double), ("ms_dt", C.c_double),
         ("bytes_recon_per_cell", C.c_double),
         ("n_reduced_last", C.c_int64), ("n_firstorder_last", C.c_int64),
+        ("n_recon_vjp", C.c_int64), ("n_flux_vjp", C.c_int64),
+        ("ms_recon_vjp", C.c_double), ("ms_flux_vjp", C.c_double),
     ]
 
 
@@ -86,6 +88,14 @@ GPU_API = {
     "ucns_residual": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_reconstruct": (C.c_int, [C.c_void_p, _f64p]),
     "ucns_rk3_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, _f64p]),
+    "ucns_residual_vjp": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                                    C.POINTER(C.c_double)]),
+    "ucns_residual_vjp_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ucns_rk3_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
+    "ucns_rk3_adjoint_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.POINTER(C.c_double)]),
     "ucns_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsV1)]),
     "ucns_reset_stats": (C.c_int, [C.c_void_p]),
     "ucns_last_error": (C.c_char_p, [C.c_void_p]),

@@ -5,6 +5,10 @@
     ARBITRARY_ORDER                (src/reconstruct_p.f90:2954-3015)     -> Solver.reconstruct
     CALCULATE_CFL                  (src/time_p.f90:27-191)               -> Solver.compute_dt
 
+and the adjoint of the path (no counterpart in the reference):
+    (dR/dU)^T w                                                          -> Solver.residual_vjp
+    (dU_N/dU_0)^T uN_bar over RK3 steps with stage checkpoints           -> Solver.rk3_adjoint
+
 Every call goes through libucnsgpu.so (include/ucns.h); there is no CPU
 fallback: constructing a Solver fails if the library or a GPU is missing.
 """
@@ -87,6 +91,43 @@ class Solver:
     def rk3_steps(self, nsteps: int, cfl: float = 0.5, dt_fixed: float = 0.0, t0: float = 0.0) -> float:
         t = C.c_double(t0)
         self._check("ucns_rk3_steps", self.L.ucns_rk3_steps(self._h, nsteps, cfl, dt_fixed, C.byref(t)))
+        return t.value
+
+    # -- adjoint ---------------------------------------------------------------
+    def residual_vjp(self, w: np.ndarray):
+        """(dR/dU)^T w at the current state, and (dR/d(inflow, outflow))^T w"""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ncell, 5)
+        ub = np.empty((self.ncell, 5))
+        bc = np.zeros(10)
+        self._check("ucns_residual_vjp", self.L.ucns_residual_vjp(self._h, native.f64p(w), native.f64p(ub),
+                                                                   UCNS_LAYOUT_AOS, native.f64p(bc)))
+        return ub, bc
+
+    def residual_vjp_device(self, w_ptr: int, ubar_ptr: int, bc_ptr: int | None = None) -> None:
+        self._check("ucns_residual_vjp_device",
+                    self.L.ucns_residual_vjp_device(self._h, C.c_void_p(w_ptr), C.c_void_p(ubar_ptr),
+                                                    C.c_void_p(bc_ptr) if bc_ptr else None))
+
+    def rk3_adjoint(self, nsteps: int, uN_bar: np.ndarray, cfl: float = 0.5, dt_fixed: float = 0.0,
+                    t0: float = 0.0):
+        """nsteps RK3 steps from the current state (left at U_N) and the
+        cotangents (dU_N/dU_0)^T uN_bar and (dU_N/d(inflow, outflow))^T uN_bar"""
+        a = np.ascontiguousarray(uN_bar, dtype=np.float64).reshape(self.ncell, 5)
+        u0b = np.empty((self.ncell, 5))
+        bc = np.zeros(10)
+        t = C.c_double(t0)
+        self._check("ucns_rk3_adjoint", self.L.ucns_rk3_adjoint(self._h, nsteps, cfl, dt_fixed, native.f64p(a),
+                                                                 native.f64p(u0b), UCNS_LAYOUT_AOS,
+                                                                 native.f64p(bc), C.byref(t)))
+        return u0b, bc, t.value
+
+    def rk3_adjoint_device(self, nsteps: int, uN_bar_ptr: int, u0_bar_ptr: int, bc_ptr: int | None = None,
+                           cfl: float = 0.5, dt_fixed: float = 0.0, t0: float = 0.0) -> float:
+        t = C.c_double(t0)
+        self._check("ucns_rk3_adjoint_device",
+                    self.L.ucns_rk3_adjoint_device(self._h, nsteps, cfl, dt_fixed, C.c_void_p(uN_bar_ptr),
+                                                   C.c_void_p(u0_bar_ptr), C.c_void_p(bc_ptr) if bc_ptr else None,
+                                                   C.byref(t)))
         return t.value
 
     def reset_stats(self) -> None:
