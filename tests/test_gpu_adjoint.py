@@ -155,19 +155,34 @@ def test_mixed_precision_vjp_is_close_to_fp64(name):
 
 def test_autograd_gradcheck_of_the_march():
     """torch.autograd.gradcheck of RK3March (forward ucns_rk3_steps, backward
-    ucns_rk3_adjoint_device) on 162 WENO-3 tetrahedra: every column of the
-    Jacobian by differences against every row by the adjoint"""
+    ucns_rk3_adjoint_device) on 512 WENO-3 hexahedra, every cell FULL (no
+    MUSCL cell, whose limiter a one-hot difference could switch between face
+    points): every column of the Jacobian by differences against every row
+    by the adjoint.  The faces are axis-aligned and the vortex is carried
+    with a uniform velocity larger than its own, so no face sits near
+    S_M = 0, where HLLC switches star states with a kink (on tetrahedra,
+    faces nearly parallel to the flow put such kinks within 1e-7 of the
+    state, inside the difference stencil)"""
     import torch
     from ucns3d_amd.autograd import RK3March
-    from ucns3d_amd.case import TET, Case, CaseSpec
+    from ucns3d_amd.case import HEX, Case, CaseSpec
     from test_tangent import G
-    print("[gradcheck] tet weno3 n=3", flush=True)
-    case = Case(CaseSpec("w3", TET, 3, "weno3", jitter=0.1, seed=2))
+    print("[gradcheck] hex weno3 n=8", flush=True)
+    case = Case(CaseSpec("h3", HEX, 8, "weno3", seed=2))
+    assert case.array("full").min() == 1
     u = case.init_state()
     r = u[:, :1]
     vel = u[:, 1:4] / r + np.array([1.5, 1.2, 1.3])
     p = (G - 1) * (u[:, 4] - 0.5 * (u[:, 1:4] ** 2).sum(1) / u[:, 0])
     u = np.c_[r, r * vel, p / (G - 1) + 0.5 * r[:, 0] * (vel ** 2).sum(1)]
+    # non-uniform density: with rho uniform its indicators are 0 and a 1e-6
+    # difference moves them to ~eps, where the weights are far from linear.
+    # 1% cell noise: in a smooth field the two arguments of the Einfeldt
+    # estimate S_L = min(u_L - c_L, u~ - c~) differ only by the face jump of
+    # the reconstruction, and a one-hot 1e-6 difference would flip the min
+    x = case.array("centre").reshape(-1, 3)
+    u = u * (1 + 0.02 * np.sin(0.37 * x[:, :1] + 0.71 * x[:, 1:2] + 0.53 * x[:, 2:3] + 0.2))
+    u = u * (1 + 1e-2 * np.random.default_rng(15).uniform(-1, 1, u.shape))
     s = Solver(case, UCNS_FP64)
     s.set_state(u)
     dt = 0.5 * s.compute_dt(0.5)
@@ -180,30 +195,41 @@ def test_autograd_gradcheck_of_the_march():
 
 def test_wall_pressure_sensitivity_to_the_inflow_mach_number():
     """J = integrated pressure on the lower wall of a channel (inflow x-,
-    characteristic outflow x+, slip walls y, symmetry z) after 20 steps from
-    a free stream at Mach M carrying a pressure/density blob; dJ/dM through
-    the initial state and the inflow/outflow data (torch autograd over
-    RK3March) against central differences of the GPU forward at h = 1e-6"""
+    characteristic outflow x+, slip walls y, symmetry z) after 40 steps from
+    a free stream at Mach M carrying a pressure/density blob next to the
+    wall.  dJ/dM through the initial state and the inflow/outflow data
+    (torch autograd over RK3March, i.e. ucns_rk3_adjoint_device) against
+      - the FP64 tangent-linear reference: the oracle's Dual instantiation
+        run forward with dU_0/dM and d(inflow, outflow)/dM (exact for the
+        branch the primal takes), to 1e-9;
+      - central differences of the GPU forward at h = 1e-6, to 1e-4: on
+        this mesh 94% of the cells are MUSCL cells next to the boundaries,
+        and along M the differences straddle switches of the Barth limiter's
+        active face point and of the Einfeldt min, where J has kinks"""
     import torch
     from ucns3d_amd.autograd import RK3March
     from ucns3d_amd.case import HEX, Case, CaseSpec
+    from oracle.oracle import Oracle
     print("[dJ/dMach] hex weno3 channel n=8", flush=True)
     case = Case(CaseSpec("channel", HEX, 8, "weno3", seed=3, bc=(1, 2, 4, 4, 3, 3)))
     g = 1.4
+    c = g ** 0.5  # free-stream speed of sound (rho = p = 1)
     dev = "cuda"
-    x = torch.tensor(case.array("centre").reshape(-1, 3), dtype=torch.float64, device=dev)
+    xc = case.array("centre").reshape(-1, 3)
+    x = torch.tensor(xc, dtype=torch.float64, device=dev)
     nf = case.nface_max
     bcode = case.array("bcode").reshape(-1, nf)
     normal = case.array("normal").reshape(-1, nf, 3)
     surf = case.array("surf").reshape(-1, nf)
     wall = (bcode == 4) & (normal[:, :, 1] < -0.5)
-    area = torch.tensor((surf * wall).sum(1), dtype=torch.float64, device=dev)
-    assert area.sum() > 0
+    area_np = (surf * wall).sum(1)
+    area = torch.tensor(area_np, dtype=torch.float64, device=dev)
+    assert area_np.sum() > 0
     s = Solver(case, UCNS_FP64)
 
     def state(M):
-        c = (g * 1.0 / 1.0) ** 0.5
-        blob = torch.exp(-((x - torch.tensor([2.0, 2.5, 3.0], dtype=torch.float64, device=dev)) ** 2).sum(1))
+        # a blob next to the lower wall (y = 0), upstream of the middle
+        blob = torch.exp(-2.0 * ((x - torch.tensor([1.5, 0.8, 3.0], dtype=torch.float64, device=dev)) ** 2).sum(1))
         rho = 1.0 + 0.2 * blob
         pr = 1.0 + 0.3 * blob
         u = M * c * torch.ones_like(rho)
@@ -217,17 +243,33 @@ def test_wall_pressure_sensitivity_to_the_inflow_mach_number():
         p = (g - 1) * (UN[:, 4] - 0.5 * (UN[:, 1:4] ** 2).sum(1) / UN[:, 0])
         return (p * area).sum()
 
+    nsteps = 40
     M = torch.tensor(0.6, dtype=torch.float64, device=dev, requires_grad=True)
     U0, bc = state(M.detach())
     s.set_boundary(1, bc[:5].cpu().numpy(), bc[5:].cpu().numpy())
     s.set_state(U0.cpu().numpy())
     dt = 0.5 * s.compute_dt(0.5)
-    Jm = J(M, 20, dt)
+    Jm = J(M, nsteps, dt)
     Jm.backward()
+    an = M.grad.item()
+    # tangent-linear reference on the CPU
+    Mt = torch.tensor(0.6, dtype=torch.float64, device=dev)
+    U0t, bct = state(Mt)
+    dU0 = torch.autograd.functional.jacobian(lambda m: state(m)[0], Mt).cpu().numpy()  # (ncell, 5)
+    dbc = torch.autograd.functional.jacobian(lambda m: state(m)[1], Mt).cpu().numpy()  # (10,)
+    orc = Oracle(case)
+    b = bct.cpu().numpy()
+    oracle.set_boundary(1, b[:5], b[5:])
+    uN, duN = orc.rk3_tl(U0t.cpu().numpy(), dU0, nsteps, dt, dbc)
+    r, m = uN[:, 0], uN[:, 1:4]
+    vel = m / r[:, None]
+    dpdU = (g - 1) * np.c_[0.5 * (vel ** 2).sum(1), -vel, np.ones_like(r)]
+    tl = float(((dpdU * duN).sum(1) * area_np).sum())
     h = 1e-6
     with torch.no_grad():
-        fd = (J(M.detach() + h, 20, dt) - J(M.detach() - h, 20, dt)).item() / (2 * h)
-    an = M.grad.item()
-    print("  J", Jm.item(), "dJ/dM adjoint", an, "fd", fd, "rel", abs(an - fd) / abs(fd))
-    assert abs(an - fd) <= 1e-6 * abs(fd)
+        fd = (J(M.detach() + h, nsteps, dt) - J(M.detach() - h, nsteps, dt)).item() / (2 * h)
+    print("  J", Jm.item(), "dJ/dM adjoint", an, "tangent-linear", tl, "fd", fd,
+          "rel(tl)", abs(an - tl) / abs(tl), "rel(fd)", abs(an - fd) / abs(fd))
+    assert abs(an - tl) <= 1e-9 * abs(tl)
+    assert abs(an - fd) <= 1e-4 * abs(fd)
     s.close()

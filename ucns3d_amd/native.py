@@ -139,7 +139,16 @@ def prep_lib() -> C.CDLL:
 
 
 def gpu_lib(path: str | None = None) -> C.CDLL:
-    """The HIP library; `path` selects another build of it (an absolute path)."""
+    """The HIP library; `path` selects another build of it (an absolute path).
+
+    torch (when installed) is imported first: its wheel carries its own HIP
+    runtime under the same soname, and a process that loads the system
+    runtime first cannot initialise torch's devices afterwards.  With torch
+    loaded, the library binds to the runtime already in the process."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return _load(path or "libucnsgpu.so", GPU_API)
 
 
