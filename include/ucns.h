@@ -120,6 +120,8 @@ typedef struct ucns_stats_v1 {
     int64_t n_firstorder_last;                    /* cells sent to first order by CHECKSOLX */
     int64_t n_recon_vjp, n_flux_vjp;              /* launches of the residual-product kernels */
     double ms_recon_vjp, ms_flux_vjp;             /* their summed kernel time */
+    int64_t n_recon_adj;                          /* reconstructions run by the adjoint (with the scatter) */
+    double ms_recon_adj;
 } ucns_stats_v1;
 
 /* Ghost states of boundary faces (BOUNDARYS, src/flow_operations.f90:2860-3375),

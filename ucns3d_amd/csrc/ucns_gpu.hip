@@ -40,6 +40,9 @@ namespace {
 constexpr int NV = 5;
 constexpr int WAVE = 64;
 constexpr int NXCD = 8;
+// distinct stencil members of a tile held in LDS by the residual product's scatter
+#define UCNS_TILE_UQ 512
+#define UCNS_NOSLOT 0xFFFFu
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
@@ -487,6 +490,14 @@ struct ReconWParams {
     const unsigned char *stream;  // per-tile row stream of the DMA kernels (StreamShape)
     long long tile_bytes;
     R *gstore;             // optional [ntile][ideg*5][8*ts]: every stencil's coefficients, for the adjoint
+    // scatter of a residual product riding on the row stream (k_recon_sv*):
+    // member j of lane position pos receives P_j^T gbs[.][.][pos]
+    const double *gbs;        // [ntile][ideg*5][8*ts] cotangents of the stencil coefficients
+    double *ubar;             // [ncell][5] accumulated
+    const uint16_t *slot;     // [ntile][imax][8*ts] row of the tile's member table (UCNS_NOSLOT: HBM)
+    const int32_t *uq, *ucount;
+    const int64_t *toff;      // first row of each tile's member table in tab
+    double *tab;              // [rows][5] the member tables, summed per cell by k_adj_gather
 };
 
 template <typename T>
@@ -559,12 +570,13 @@ __device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int la
     if (lane * 16 < valid) lds_load16(src + lane * 16, lds_addr(lds_dst));
 }
 
-template <typename R, int IDEG, int NQL>
+// STORE: also write every stencil's coefficients to p.gstore (the adjoint's forward pass)
+template <typename R, int IDEG, int NQL, bool STORE = false>
 __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG][NV], const double (&ui)[NV],
                                            const R (*sB)[IDEG * (IDEG + 1) / 2], R (*sG0)[IDEG * NV], int lc, int s,
                                            int cell, bool cvalid, int info, bool act) {
     const int ifca = ci_ifca(info), full = ci_full(info);
-    if (p.gstore && act) {
+    if (STORE && act) {
         // coalesced: for each coefficient the 8 ts stencils of the tile are contiguous
         const int TP = 8 * p.ts;
         R *gs = p.gstore + (size_t)(cell >> 3) * IDEG * NV * TP + (lc * p.ts + s);
@@ -959,12 +971,15 @@ struct StreamShape {
     static_assert((DEPTH - 1) * PER + NBC <= 63, "vmcnt immediate out of range");
 };
 
-template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH>
+template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH, bool SCAT = false, bool STORE = false>
 __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     using SS = StreamShape<R, IDEG, IMAX, TS, DEPTH>;
     constexpr int NP = SS::NP, TP = SS::TP;
     // ring slots, then the header's index rows
     __shared__ __attribute__((aligned(1024))) char ring[DEPTH * SS::SB + SS::HD];
+    // SCAT: the tile's member table and the rows of its member slots
+    __shared__ double sAcc[SCAT ? UCNS_TILE_UQ : 1][NV];
+    __shared__ uint16_t sSlot[SCAT ? IMAX * TP : 1];
     const int *hdr = (const int *)(ring + DEPTH * SS::SB);
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
@@ -978,11 +993,36 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
     }
+    // SCAT: this lane's coefficient cotangents, the member slots, a zeroed table
+    double gb[SCAT ? IDEG : 1][NV];
+    int nuq = 0;
+    if constexpr (SCAT) {
+        const double *gs = p.gbs + (size_t)tile * IDEG * NV * TP + (lc * TS + s);
+        const bool in = cvalid && s < TS;
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gb[k][v] = in ? gs[(size_t)(k * NV + v) * TP] : 0.0;
+        const uint16_t *st = p.slot + (size_t)tile * IMAX * TP;
+        for (int e = lane; e < IMAX * TP; e += WAVE) sSlot[e] = st[e];
+        nuq = min(p.ucount[tile], UCNS_TILE_UQ);
+        for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+    }
+    // a use of every cotangent here makes the compiler wait for their loads
+    // before the row stream starts, not inside it
+    double gsum = 0.0;
+    if constexpr (SCAT) {
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gsum += gb[k][v];
+    }
     // the cell's own data is in registers before the first row load
     asm volatile("s_waitcnt vmcnt(0)"
-                 : "+v"(info), "+v"(ui[0]), "+v"(ui[1]), "+v"(ui[2]), "+v"(ui[3]), "+v"(ui[4])
+                 : "+v"(info), "+v"(ui[0]), "+v"(ui[1]), "+v"(ui[2]), "+v"(ui[3]), "+v"(ui[4]), "+v"(gsum)
                  :
                  : "memory");
+    if constexpr (SCAT) __syncthreads();  // sSlot, sAcc
     const int admis = ci_admis(info);
     const bool act = cvalid && s < admis;
     const int rl = act ? lc * TS + s : 0;
@@ -1019,15 +1059,21 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         }
     };
 
+    // SCAT: members of the next DEPTH rows, oldest first
+    int qn[DEPTH];
+    double acc[NV] = {0, 0, 0, 0, 0};
+
     // prologue: header, then (row, states) pairs for rows 0..DEPTH-1
     dma_chunk(T, ring + DEPTH * SS::SB, lane, SS::HD);
     issue_row(0);
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(SS::NCH) : "memory");  // header landed
-    issue_states(0, hdr[rl]);
+    qn[0] = hdr[rl];
+    issue_states(0, qn[0]);
 #pragma unroll
     for (int r = 1; r < DEPTH; ++r) {
         issue_row(r);
-        issue_states(r, hdr[r * TP + rl]);
+        qn[r] = hdr[r * TP + rl];
+        issue_states(r, qn[r]);
     }
 
     // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
@@ -1054,6 +1100,29 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
             for (int k = 0; k < IDEG; ++k)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
+            if constexpr (SCAT) {
+                // dU_bar_j = P_j^T g_bar to member j (the tile's LDS table), minus the same to U_i
+                double d[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) d[v] += (double)pc[k] * gb[k][v];
+                const unsigned sl = sSlot[i * TP + rl];
+                if (sl != UCNS_NOSLOT) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) unsafeAtomicAdd(&p.ubar[(size_t)qn[0] * NV + v], d[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] -= d[v];
+            }
+        }
+        if constexpr (SCAT) {
+#pragma unroll
+            for (int r = 0; r + 1 < DEPTH; ++r) qn[r] = qn[r + 1];
         }
     };
 #pragma unroll 1
@@ -1064,6 +1133,7 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         issue_row(i + DEPTH);
         issue_states(i + DEPTH, nn);
+        if constexpr (SCAT) qn[DEPTH - 1] = nn;
     }
     // tail: the last DEPTH rows; the indicators follow into the first free slots
     [&]<int... TT>(std::integer_sequence<int, TT...>) {
@@ -1085,14 +1155,43 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         issue_b();
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
+    if constexpr (SCAT) {
+        // the cells' own share, then the tile's member table, to HBM
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double t = grp_sum8(acc[v]);
+            if (s == 0 && cvalid) unsafeAtomicAdd(&p.ubar[(size_t)cell * NV + v], t);
+        }
+        __syncthreads();
+        double *tt = p.tab + (size_t)p.toff[tile] * NV;
+        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+    }
     const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
     R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
-    recon_late<R, IDEG, NQL>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+    recon_late<R, IDEG, NQL, STORE>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
 }
 
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_s64(ReconWParams<double> p) {
     recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS)>(p);
+}
+// the same reconstruction with the scatter of a residual product on its row stream
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE) void k_recon_sv64(ReconWParams<double> p) {
+    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS), true, true>(p);
+}
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE) void k_recon_sv32(ReconWParams<float> p) {
+    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), true, true>(p);
+}
+// the reconstruction storing every stencil's coefficients for a residual product
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE) void k_recon_sg64(ReconWParams<double> p) {
+    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS), false, true>(p);
+}
+template <int IDEG, int IMAX, int NQL, int TS>
+__global__ __launch_bounds__(WAVE, 2) void k_recon_sg32(ReconWParams<float> p) {
+    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), false, true>(p);
 }
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE, 2) void k_recon_s32(ReconWParams<float> p) {
@@ -1634,6 +1733,14 @@ struct VjpParams {
     const double *ubf;     // [ncell][nf][nqm][5] cotangents of the face-point states
     double *ubar;          // [ncell][5] accumulated
     const R *gload;        // the forward's stencil coefficients [ntile][ideg*5][8 ts], or null: recompute
+    // the tile's distinct stencil members: member (j, pos) of tile t scatters
+    // to LDS row slot[t][j][pos] (UCNS_NOSLOT: straight to HBM), and row r of
+    // tile t belongs to cell uq[t][r], r < min(ucount[t], UCNS_TILE_UQ)
+    const uint16_t *slot;
+    const int32_t *uq, *ucount;
+    double *gbout;         // non-null: write each stencil's g_bar [ntile][ideg*5][8 ts] instead of scattering
+    const int64_t *toff;   // first row of each tile's member table in tab
+    double *tab;           // [rows][5] the member tables, summed per cell by k_adj_gather
 };
 
 __device__ inline void atomic_add_f64(double *a, double v) { unsafeAtomicAdd(a, v); }
@@ -1643,16 +1750,98 @@ __device__ inline T lane_bcast(T v, int src) {
     return __shfl(v, src, WAVE);
 }
 
-template <typename R, int IDEG, int IMAX, int NQL>
+// Distinct members per tile: a wave's scatter goes through an LDS table of
+// the tile's distinct member cells (a tile of 8 neighbouring cells shares
+// most of its stencil members), flushed with one HBM atomic per member and
+// variable instead of one per (stencil, member, variable).
+constexpr int HT = 4096;  // hash slots, >= 2 * the 8 * 8 * 39 member entries of a tile
+
+template <typename R, int IMAX>
+__global__ __launch_bounds__(WAVE) void k_tile_members(LsqView<R> L, int ntile, int ncell, const int32_t *cinfo,
+                                                       int32_t *uq, int32_t *ucount, uint16_t *slot) {
+    __shared__ int32_t keys[HT];
+    __shared__ uint16_t cid[HT];
+    __shared__ uint16_t hs[IMAX * 64];
+    const int lane = threadIdx.x;
+    const int tile = blockIdx.x;
+    if (tile >= ntile) return;
+    const int ts = L.ts, tp = L.tp;
+    for (int h = lane; h < HT; h += WAVE) keys[h] = -1;
+    __syncthreads();
+    // insert every (member row j, lane position) entry; linear probing
+    for (int e = lane; e < IMAX * tp; e += WAVE) {
+        const int j = e / tp, pos = e - j * tp;
+        const int lc = pos / ts, s = pos - lc * ts;
+        const int cell = tile * 8 + lc;
+        int h = 0xFFFF;
+        if (cell < ncell && s < ci_admis(cinfo[cell])) {
+            const int n = L.member(tile, cell, s, j, pos);
+            h = (int)(((unsigned)n * 2654435761u) >> 20) & (HT - 1);
+            for (int probe = 0; probe < HT; ++probe) {
+                const int old = atomicCAS(&keys[h], -1, n);
+                if (old == -1 || old == n) break;
+                h = (h + 1) & (HT - 1);
+            }
+        }
+        hs[e] = (uint16_t)h;
+    }
+    __syncthreads();
+    // compact: lane l owns hash slots [64 l, 64 l + 64)
+    constexpr int PER = HT / WAVE;
+    int cnt = 0;
+    for (int k = 0; k < PER; ++k) cnt += keys[lane * PER + k] >= 0;
+    int incl = cnt;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+    }
+    int id = incl - cnt;
+    for (int k = 0; k < PER; ++k) {
+        const int h = lane * PER + k;
+        const int key = keys[h];
+        if (key >= 0) {
+            cid[h] = id < UCNS_TILE_UQ ? (uint16_t)id : (uint16_t)UCNS_NOSLOT;
+            if (id < UCNS_TILE_UQ) uq[(size_t)tile * UCNS_TILE_UQ + id] = key;
+            ++id;
+        }
+    }
+    if (lane == WAVE - 1) ucount[tile] = incl;
+    __syncthreads();
+    for (int e = lane; e < IMAX * tp; e += WAVE) {
+        const int h = hs[e];
+        slot[(size_t)tile * IMAX * tp + e] = h == 0xFFFF ? (uint16_t)UCNS_NOSLOT : cid[h];
+    }
+}
+
+// pw(x, x^2, x^3, e) = x^e for e = 0..3
+template <typename R>
+__device__ inline R pw(R x, R x2, R x3, int e) {
+    return e == 0 ? R(1) : e == 1 ? x : e == 2 ? x2 : x3;
+}
+
+// GOUT: the cotangents g_bar of every stencil go to p.gbout element by
+// element, for the row stream's scatter (no g_bar array in registers, no
+// scatter here); otherwise this kernel scatters them itself.
+template <typename R, int IDEG, int IMAX, int NQL, bool GOUT>
 __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
+    constexpr int KL = (IDEG + 7) / 8;    // coefficients per lane in the face-point pass
     __shared__ double sGb[8][IDEG * NV];  // cotangent of the combined coefficients
     __shared__ R sGh[8][IDEG * NV];       // combined coefficients sum_s omega_s G_s
     __shared__ R sG0[8][IDEG * NV];       // central-stencil coefficients
+    __shared__ R sB[8][NP];               // the tile's indicator matrices
+    __shared__ double sAcc[GOUT ? 1 : UCNS_TILE_UQ][NV];  // scatter to the tile's distinct members
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
     if (tile >= p.ntile) return;
+    const int nuq = (!GOUT && p.slot) ? min(p.ucount[tile], UCNS_TILE_UQ) : 0;
+    for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+    if (p.iweno == 1) {
+        const int nvc = min(8, p.ncell - tile * 8);
+        const R *Bt = p.B + (size_t)tile * 8 * NP;
+        for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+    }
     const int cell = tile * 8 + lc;
     const bool cvalid = cell < p.ncell;
     const int cs = cvalid ? cell : 0;
@@ -1674,21 +1863,29 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
         return cvalid && q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
     };
 
-    // ---- A: cotangent of the combined coefficients, sum_q phi_q ubar_q,
-    // and of the cell average through a_q = U_i + ULEFT(q) ----
+    // ---- A: cotangent of the combined coefficients, sum_q phi_q ubar_q
+    // (lane s of a cell takes coefficients s, s+8, s+16 over all its face
+    // points), and of the cell average through a_q = U_i + ULEFT(q) ----
     double usum[NV] = {0, 0, 0, 0, 0};
     {
-        double ab[IDEG][NV];
+        double ab[KL][NV];
+        R mk[KL];
+        int ex[KL][3];
 #pragma unroll
-        for (int k = 0; k < IDEG; ++k)
+        for (int t = 0; t < KL; ++t) {
+            const int k = s + 8 * t;
+            mk[t] = k < IDEG ? mb[k] : R(0);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ab[k][v] = 0.0;
+            for (int d = 0; d < 3; ++d) ex[t][d] = k < IDEG ? c_mono[k][d] : 0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ab[t][v] = 0.0;
+        }
 #pragma unroll 1
-        for (int q = s; q < NQL * 8; q += 8) {
+        for (int q = 0; q < nslot; ++q) {
             if (!qvalid(q)) continue;
-            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
-            R phi[IDEG];
-            basis_at<R, IDEG>(x, mb, phi);
+            const R x0 = qpc[q * 3], x1 = qpc[q * 3 + 1], x2 = qpc[q * 3 + 2];
+            const R x0s = x0 * x0, x1s = x1 * x1, x2s = x2 * x2;
+            const R x0c = x0s * x0, x1c = x1s * x1, x2c = x2s * x2;
             double ub[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -1696,20 +1893,23 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
                 usum[v] += ub[v];
             }
 #pragma unroll
-            for (int k = 0; k < IDEG; ++k)
+            for (int t = 0; t < KL; ++t) {
+                const double ph = (double)(pw(x0, x0s, x0c, ex[t][0]) * pw(x1, x1s, x1c, ex[t][1]) *
+                                               pw(x2, x2s, x2c, ex[t][2]) - mk[t]);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) ab[k][v] += (double)phi[k] * ub[v];
+                for (int v = 0; v < NV; ++v) ab[t][v] += ph * ub[v];
+            }
         }
 #pragma unroll
-        for (int k = 0; k < IDEG; ++k)
+        for (int t = 0; t < KL; ++t) {
+            const int k = s + 8 * t;
+            if (k < IDEG) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const double t = grp_sum8(ab[k][v]);
-                if (s == 0) sGb[lc][k * NV + v] = t;
+                for (int v = 0; v < NV; ++v) sGb[lc][k * NV + v] = ab[t][v];
             }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) usum[v] = grp_sum8(usum[v]);
+        }
     }
+    __syncthreads();  // sGb, sB
 
     // ---- B: the stencil coefficients G_s = P_s dU_s, as the forward ----
     R g[IDEG][NV];
@@ -1749,7 +1949,6 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     }
 
     // ---- C: indicators, weights and combined coefficients (forward) ----
-    const R *Bc = p.B + (size_t)cs * NP;
     R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
     R om[NV] = {R(0), R(0), R(0), R(0), R(0)};
     double obar[NV] = {0, 0, 0, 0, 0};
@@ -1760,7 +1959,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
                 R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
                 for (int l = 0; l < IDEG; ++l) {
-                    const R bkl = Bc[pidx(IDEG, k, l)];
+                    const R bkl = sB[lc][pidx(IDEG, k, l)];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
                 }
@@ -1787,7 +1986,6 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
             const R sum = grp_sum8(w);
             om[v] = act ? w / sum : R(0);
         }
-        __syncthreads();  // sGb
 #pragma unroll
         for (int k = 0; k < IDEG; ++k)
 #pragma unroll
@@ -1930,85 +2128,97 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     }
 
     // ---- F: cotangent of each lane's stencil coefficients ----
-    double gb[IDEG][NV];
+    // GOUT writes them as they are formed; otherwise they are kept for G
+    double gb[GOUT ? 1 : IDEG][NV];
+    if constexpr (!GOUT) {
 #pragma unroll
-    for (int k = 0; k < IDEG; ++k)
+        for (int k = 0; k < IDEG; ++k)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
+            for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
+    }
+    double *go = nullptr;
+    if constexpr (GOUT) {
+        if (cvalid && s < p.lsq.ts) go = p.gbout + (size_t)tile * IDEG * NV * p.lsq.tp + pos;
+    }
+    auto put = [&](int k, int v, double x) {
+        if constexpr (GOUT) {
+            if (go) go[(size_t)(k * NV + v) * p.lsq.tp] = x;
+        } else {
+            gb[k][v] = x;
+        }
+    };
     double ubi[NV];  // extra cotangent of U_i from the limiter
     bool scatter = false;
 #pragma unroll
     for (int v = 0; v < NV; ++v) ubi[v] = 0.0;
-    if (cvalid && !first) {
-        if (reduce) {
-            // ULEFT(q) = slope * d2_q: slope_bar = sum_q ubar_q d2_q; through an
-            // active limiter slope = (u_ext - u_i) / d2_{q*}
-            double sb[NV] = {0, 0, 0, 0, 0};
+    if (cvalid && !first && reduce) {
+        // ULEFT(q) = slope * d2_q: slope_bar = sum_q ubar_q d2_q; through an
+        // active limiter slope = (u_ext - u_i) / d2_{q*}
+        double sb[NV] = {0, 0, 0, 0, 0};
 #pragma unroll
-            for (int r = 0; r < NQL; ++r) {
-                const int q = s + 8 * r;
-                if (qvalid(q)) {
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            if (qvalid(q)) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) sb[v] += ubc[q * NV + v] * (double)res0[r][v];
-                }
-            }
-            double cq[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                sb[v] = grp_sum8(sb[v]);
-                const bool on = slope[v] < 1.0 && qs[v] >= 0;
-                const double e = on ? sb[v] / d2s[v] : 0.0;
-                cq[v] = on ? e * slope[v] : 0.0;
-                ubi[v] = -e;
-                if (s == 0 && on) atomic_add_f64(&p.ubar[(size_t)ext[v] * NV + v], e);
-            }
-            if (s == 0) {
-                scatter = true;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-#pragma unroll
-                    for (int k = 0; k < IDEG; ++k) gb[k][v] = slope[v] * sGb[lc][k * NV + v];
-                    if (cq[v] != 0.0) {
-                        const int q = qs[v];
-                        R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
-                        R phi[IDEG];
-                        basis_at<R, IDEG>(x, mb, phi);
-#pragma unroll
-                        for (int k = 0; k < IDEG; ++k) gb[k][v] -= cq[v] * (double)phi[k];
-                    }
-                }
-            }
-        } else {
-            // omega_s = lam_s (eps+beta_s)^-4 / sum_t (...):
-            // beta_bar_s = -4 omega_s/(eps+beta_s) (obar_s - sum_t omega_t obar_t)
-            double bb[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const double so = grp_sum8(act ? (double)om[v] * obar[v] : 0.0);
-                bb[v] = act ? -4.0 * (double)om[v] / ((double)p.eps + (double)beta[v]) * (obar[v] - so) : 0.0;
-            }
-            if (act) {
-                scatter = true;
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) {
-                    double hg[NV] = {0, 0, 0, 0, 0};
-#pragma unroll
-                    for (int l = 0; l < IDEG; ++l) {
-                        const double bkl = (double)Bc[pidx(IDEG, k, l)];
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) hg[v] += bkl * (double)g[l][v];
-                    }
-#pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        gb[k][v] = (double)om[v] * sGb[lc][k * NV + v] + 2.0 * bb[v] * hg[v];
-                }
+                for (int v = 0; v < NV; ++v) sb[v] += ubc[q * NV + v] * (double)res0[r][v];
             }
         }
+        double cq[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sb[v] = grp_sum8(sb[v]);
+            const bool on = slope[v] < 1.0 && qs[v] >= 0;
+            const double e = on ? sb[v] / d2s[v] : 0.0;
+            cq[v] = on ? e * slope[v] : 0.0;
+            ubi[v] = -e;
+            if (s == 0 && on) atomic_add_f64(&p.ubar[(size_t)ext[v] * NV + v], e);
+        }
+        scatter = s == 0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R phi[IDEG];
+            const bool corr = s == 0 && cq[v] != 0.0;
+            if (corr) {
+                const int q = qs[v];
+                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                basis_at<R, IDEG>(x, mb, phi);
+            }
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+                put(k, v, s == 0 ? slope[v] * sGb[lc][k * NV + v] - (corr ? cq[v] * (double)phi[k] : 0.0) : 0.0);
+        }
+    } else if (cvalid && !first) {
+        // omega_s = lam_s (eps+beta_s)^-4 / sum_t (...):
+        // beta_bar_s = -4 omega_s/(eps+beta_s) (obar_s - sum_t omega_t obar_t)
+        double bb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double so = grp_sum8(act ? (double)om[v] * obar[v] : 0.0);
+            bb[v] = act ? -4.0 * (double)om[v] / ((double)p.eps + (double)beta[v]) * (obar[v] - so) : 0.0;
+        }
+        scatter = act;
+        // g_bar_s = omega_s gh_bar + 2 beta_bar_s B G_s
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                double hg = 0.0;
+#pragma unroll
+                for (int l = 0; l < IDEG; ++l) hg += (double)sB[lc][pidx(IDEG, k, l)] * (double)g[l][v];
+                put(k, v, act ? (double)om[v] * sGb[lc][k * NV + v] + 2.0 * bb[v] * hg : 0.0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) put(k, v, 0.0);
     }
+    if constexpr (GOUT) scatter = false;  // the row stream scatters
 
     // ---- G: dU_bar_j = P_j^T g_bar to the members, minus the same to U_i ----
     double acc[NV] = {0, 0, 0, 0, 0};
-    if (scatter) {
+    if constexpr (!GOUT) if (scatter) {
         // member j+1's row and index are in flight while member j is scattered
         int kst;
         int n = p.lsq.member(tile, cell, s, 0, pos);
@@ -2033,11 +2243,17 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
             }
+            const unsigned sl = p.slot ? p.slot[((size_t)tile * IMAX + j) * p.lsq.tp + pos] : UCNS_NOSLOT;
+            if (sl != UCNS_NOSLOT) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                atomic_add_f64(&p.ubar[(size_t)n * NV + v], d[v]);
-                acc[v] -= d[v];
+                for (int v = 0; v < NV; ++v)
+                    __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) atomic_add_f64(&p.ubar[(size_t)n * NV + v], d[v]);
             }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] -= d[v];
             n = nn;
 #pragma unroll
             for (int k = 0; k < IDEG; ++k) pc[k] = pn[k];
@@ -2047,6 +2263,41 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     for (int v = 0; v < NV; ++v) {
         const double t = grp_sum8(acc[v]);
         if (s == 0 && cvalid) atomic_add_f64(&p.ubar[(size_t)cell * NV + v], t + usum[v] + ubi[v]);
+    }
+    if (nuq > 0) {
+        __syncthreads();
+        double *tt = p.tab + (size_t)p.toff[tile] * NV;
+        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+    }
+}
+
+// ubar[n] += the rows of every tile's member table that belong to cell n
+// (CSR: rows ent[off[n]] .. ent[off[n+1]-1]); each cell is one thread's, so
+// no atomics
+__global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *off, const int64_t *ent,
+                                                    const double *tab, double *ubar) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ncell) return;
+    double a[NV] = {0, 0, 0, 0, 0};
+    for (int64_t e = off[n]; e < off[n + 1]; ++e) {
+        const double *r = tab + (size_t)ent[e] * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a[v] += r[v];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ubar[(size_t)n * NV + v] += a[v];
+}
+
+// CSR of the member tables by cell: counts, then positions
+__global__ void k_adj_csr(int ntile, const int32_t *uq, const int32_t *ucount, const int64_t *toff, int64_t *cnt,
+                          const int64_t *off, int64_t *ent) {
+    const int t = blockIdx.x;
+    if (t >= ntile) return;
+    const int nu = min(ucount[t], UCNS_TILE_UQ);
+    for (int r = threadIdx.x; r < nu; r += blockDim.x) {
+        const int n = uq[(size_t)t * UCNS_TILE_UQ + r];
+        const unsigned long long k = atomicAdd((unsigned long long *)&cnt[n], 1ull);
+        if (off) ent[off[n] + (int64_t)k] = toff[t] + r;
     }
 }
 
@@ -2267,7 +2518,14 @@ struct ucns_ctx {
     // work arrays, the free-stream cotangent, stage checkpoints and step dts
     double *adj_ubf = nullptr, *adj_lam = nullptr, *adj_ubar = nullptr, *adj_x = nullptr, *adj_a = nullptr;
     double *adj_u0 = nullptr, *adj_bc = nullptr, *ckpt = nullptr, *dts = nullptr;
-    void *adj_g = nullptr;  // stencil coefficients of the tiled reconstruction, [ntile][ideg*5][8 ts] (R)
+    void *adj_g = nullptr;  // stencil coefficients of the row-stream reconstruction, [ntile][ideg*5][8 ts] (R)
+    double *adj_gb = nullptr;  // stencil-coefficient cotangents handed to the row stream, [ntile][ideg*5][8 ts]
+    const double *sv_gb = nullptr;  // non-null: the next row-stream reconstruction also scatters sv_gb ...
+    double *sv_ubar = nullptr;      // ... into sv_ubar
+    uint16_t *adj_slot = nullptr;                      // [ntile][imax][8 ts] LDS row of each member entry
+    int32_t *adj_uq = nullptr, *adj_ucount = nullptr;  // [ntile][UCNS_TILE_UQ], [ntile]
+    int64_t *adj_toff = nullptr, *adj_off = nullptr, *adj_ent = nullptr;  // table rows per tile, CSR by cell
+    double *adj_tab = nullptr;                          // [rows][5] member tables
     bool adj_g_on = false;  // the next tiled reconstruction stores them
     size_t ckpt_steps = 0;
     // stats
@@ -2348,6 +2606,62 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
         w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
+        if (ctx->smode && ctx->sv_ubar) {
+            // reconstruction + the scatter of the previous residual product
+            w.gbs = ctx->sv_gb;
+            w.ubar = ctx->sv_ubar;
+            w.slot = ctx->adj_slot;
+            w.uq = ctx->adj_uq;
+            w.ucount = ctx->adj_ucount;
+            w.toff = ctx->adj_toff;
+            w.tab = ctx->adj_tab;
+            if constexpr (sizeof(R) == 8) {
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_sv64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_sv64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_sv64<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_sv64<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
+            } else {
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_sv32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_sv32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_sv32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_sv32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
+            }
+            return UCNS_OK;
+        }
+        if (ctx->smode && ctx->adj_g_on) {
+            if constexpr (sizeof(R) == 8) {
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_sg64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_sg64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_sg64<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_sg64<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
+            } else {
+                if (ctx->ideg == 19)
+                    if (ctx->smode == 5)
+                        hipLaunchKernelGGL((k_recon_sg32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
+                    else
+                        hipLaunchKernelGGL((k_recon_sg32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
+                else if (ctx->smode == 5)
+                    hipLaunchKernelGGL((k_recon_sg32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_sg32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
+            }
+            return UCNS_OK;
+        }
         if (ctx->smode) {
             // (ideg, imax, face-point slots, stencils): WENO-4 tets / hexes, WENO-3 tets / hexes
             if constexpr (sizeof(R) == 8) {
@@ -2532,7 +2846,49 @@ void release_adjoint(ucns_ctx *ctx) {
     dfree(ctx->dts);
     if (ctx->adj_g) (void)hipFree(ctx->adj_g);
     ctx->adj_g = nullptr;
+    dfree(ctx->adj_gb);
+    dfree(ctx->adj_slot);
+    dfree(ctx->adj_uq);
+    dfree(ctx->adj_ucount);
+    dfree(ctx->adj_toff);
+    dfree(ctx->adj_off);
+    dfree(ctx->adj_ent);
+    dfree(ctx->adj_tab);
     ctx->ckpt_steps = 0;
+}
+
+template <typename R>
+LsqView<R> lsq_view(const ucns_ctx *ctx) {
+    LsqView<R> L{};
+    L.mode = ctx->smode ? 2 : ctx->tiled ? 1 : 0;
+    L.ts = ctx->ts;
+    L.tp = 8 * ctx->ts;
+    L.imax = ctx->imax;
+    L.ideg = ctx->ideg;
+    L.depth = stream_depth(sizeof(R) == 4, ctx->ts);
+    L.pb = ctx->ideg * L.tp * (int)sizeof(R);
+    L.rd = L.pb + L.tp * 4;
+    L.hd = L.depth * L.tp * 4;
+    L.tile_bytes = ctx->tile_bytes;
+    L.nbr = ctx->nbr;
+    L.P = (const R *)ctx->P;
+    L.rows = ctx->rows;
+    return L;
+}
+
+template <typename R>
+void launch_tile_members(ucns_ctx *ctx) {
+    const LsqView<R> L = lsq_view<R>(ctx);
+    const dim3 grid(ctx->ntile), block(WAVE);
+    if (ctx->imax == 39)
+        hipLaunchKernelGGL((k_tile_members<R, 39>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
+                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
+    else if (ctx->imax == 19)
+        hipLaunchKernelGGL((k_tile_members<R, 19>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
+                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
+    else
+        hipLaunchKernelGGL((k_tile_members<R, 8>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
+                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
 }
 
 int ensure_adjoint(ucns_ctx *ctx) {
@@ -2546,7 +2902,7 @@ int ensure_adjoint(ucns_ctx *ctx) {
     if ((rc = dalloc(ctx, &ctx->adj_a, n))) return rc;
     if ((rc = dalloc(ctx, &ctx->adj_u0, n))) return rc;
     if ((rc = dalloc(ctx, &ctx->adj_bc, 2 * NV))) return rc;
-    if (ctx->tiled) {
+    if (ctx->smode) {
         const size_t ng = (size_t)ctx->ntile * ctx->ideg * NV * 8 * ctx->ts;
         if (ctx->precision == UCNS_MIXED)
             rc = dalloc(ctx, (float **)&ctx->adj_g, ng);
@@ -2554,7 +2910,67 @@ int ensure_adjoint(ucns_ctx *ctx) {
             rc = dalloc(ctx, (double **)&ctx->adj_g, ng);
         if (rc) return rc;
     }
+    if (ctx->smode) {
+        if ((rc = dalloc(ctx, &ctx->adj_gb, (size_t)ctx->ntile * ctx->ideg * NV * 8 * ctx->ts))) return rc;
+    }
+    // the tiles' distinct stencil members (once per upload)
+    if (ctx->ts <= 8 && (ctx->imax == 39 || ctx->imax == 19 || ctx->imax == 8)) {
+        if ((rc = dalloc(ctx, &ctx->adj_slot, (size_t)ctx->ntile * ctx->imax * 8 * ctx->ts))) return rc;
+        if ((rc = dalloc(ctx, &ctx->adj_uq, (size_t)ctx->ntile * UCNS_TILE_UQ))) return rc;
+        if ((rc = dalloc(ctx, &ctx->adj_ucount, (size_t)ctx->ntile))) return rc;
+        if (ctx->precision == UCNS_MIXED)
+            launch_tile_members<float>(ctx);
+        else
+            launch_tile_members<double>(ctx);
+        HIP_TRY(hipGetLastError());
+        // table rows per tile (prefix sums on the host: set-up only).  The
+        // library stream is non-blocking: every host read of what its kernels
+        // wrote waits for the stream first, and every clear is queued on it.
+        const int nt = ctx->ntile, nc = ctx->ncell;
+        std::vector<int32_t> uc(nt);
+        HIP_TRY(hipMemcpyAsync(uc.data(), ctx->adj_ucount, nt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<int64_t> toff(nt + 1, 0);
+        for (int t = 0; t < nt; ++t) toff[t + 1] = toff[t] + std::min(uc[t], UCNS_TILE_UQ);
+        const int64_t rows = toff[nt];
+        if ((rc = dalloc(ctx, &ctx->adj_toff, (size_t)nt + 1))) return rc;
+        if ((rc = dalloc(ctx, &ctx->adj_tab, (size_t)rows * NV))) return rc;
+        if ((rc = dalloc(ctx, &ctx->adj_off, (size_t)nc + 1))) return rc;
+        if ((rc = dalloc(ctx, &ctx->adj_ent, (size_t)std::max<int64_t>(rows, 1)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->adj_toff, toff.data(), (nt + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+        // CSR by cell: count, scan, fill
+        int64_t *cnt = nullptr;
+        if ((rc = dalloc(ctx, &cnt, (size_t)nc))) return rc;
+        HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
+        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->adj_uq, ctx->adj_ucount,
+                           ctx->adj_toff, cnt, (const int64_t *)nullptr, (int64_t *)nullptr);
+        std::vector<int64_t> hc(nc), off(nc + 1, 0);
+        HIP_TRY(hipMemcpyAsync(hc.data(), cnt, nc * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < nc; ++i) off[i + 1] = off[i] + hc[i];
+        if (off[nc] != rows) {  // every table row belongs to exactly one cell
+            dfree(cnt);
+            ctx->err = "member-table index: " + std::to_string(off[nc]) + " entries for " + std::to_string(rows) +
+                       " rows";
+            return UCNS_EHIP;
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->adj_off, off.data(), (nc + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
+        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->adj_uq, ctx->adj_ucount,
+                           ctx->adj_toff, cnt, (const int64_t *)ctx->adj_off, ctx->adj_ent);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dfree(cnt);
+    }
     return UCNS_OK;
+}
+
+// the member tables of the last scatter into ubar
+void adj_gather(ucns_ctx *ctx, double *ubar) {
+    hipLaunchKernelGGL(k_adj_gather, dim3((ctx->ncell + 255) / 256), dim3(256), 0, ctx->stream, ctx->ncell,
+                       ctx->adj_off, ctx->adj_ent, ctx->adj_tab, ubar);
 }
 
 void zero_dev(ucns_ctx *ctx, double *p, size_t n) {
@@ -2562,20 +2978,37 @@ void zero_dev(ucns_ctx *ctx, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(std::max(blocks, 1)), dim3(256), 0, ctx->stream, p, n);
 }
 
+// forward reconstruction of the current state for a residual product: the
+// tiled kernels also store every stencil's coefficients; with gb non-null
+// the row-stream kernel scatters that earlier product's g_bar into ubar on
+// the same pass
 template <typename R>
-int launch_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
+int vjp_recon_t(ucns_ctx *ctx, const double *gb, double *ubar) {
+    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
+    ctx->adj_g_on = ctx->smode && ctx->adj_g;
+    ctx->sv_gb = gb;
+    ctx->sv_ubar = gb ? ubar : nullptr;
+    tstart(ctx, 3);
+    const int rc = launch_recon_t<R>(ctx);
+    if (rc == UCNS_OK && gb) adj_gather(ctx, ubar);
+    tstop(ctx, 3);
+    tcollect(ctx, 3, &ctx->st.ms_recon_adj, &ctx->st.n_recon_adj);
+    ctx->adj_g_on = false;
+    ctx->sv_gb = nullptr;
+    ctx->sv_ubar = nullptr;
+    return rc;
+}
+
+// the products of one residual at the state of the last vjp_recon_t: ubar
+// (zeroed here) = the flux and reconstruction products; in row-stream mode
+// the scatter of P^T g_bar is left to the next row stream (adj_gb)
+template <typename R>
+int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
     const int nfq = ctx->nf * ctx->nqm;
     if (ctx->ts > 8 || nfq > 56) {
         ctx->err = "the residual product needs at most 8 stencils and 56 face points per cell";
         return UCNS_EUNSUPPORTED;
     }
-    // forward reconstruction of the state the product is taken at; the tiled
-    // kernels also store every stencil's coefficients for the product
-    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
-    ctx->adj_g_on = ctx->tiled && ctx->adj_g;
-    int rc = launch_recon_t<R>(ctx);
-    ctx->adj_g_on = false;
-    if (rc) return rc;
     zero_dev(ctx, ubar, (size_t)ctx->ncell * NV);
     FluxParams<R> fp{};
     fp.nface = ctx->nface;
@@ -2610,21 +3043,7 @@ int launch_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) 
     vp.lwci1 = ctx->lwci1;
     vp.eps = ctx->eps;
     vp.U = ctx->U;
-    const bool mixed = sizeof(R) == 4;
-    LsqView<R> &L = vp.lsq;
-    L.mode = ctx->smode ? 2 : ctx->tiled ? 1 : 0;
-    L.ts = ctx->ts;
-    L.tp = 8 * ctx->ts;
-    L.imax = ctx->imax;
-    L.ideg = ctx->ideg;
-    L.depth = stream_depth(mixed, ctx->ts);
-    L.pb = ctx->ideg * L.tp * (int)sizeof(R);
-    L.rd = L.pb + L.tp * 4;
-    L.hd = L.depth * L.tp * 4;
-    L.tile_bytes = ctx->tile_bytes;
-    L.nbr = ctx->nbr;
-    L.P = (const R *)ctx->P;
-    L.rows = ctx->rows;
+    vp.lsq = lsq_view<R>(ctx);
     vp.B = (const R *)ctx->B;
     vp.mbar = (const R *)ctx->mbar;
     vp.qp = (const R *)ctx->qp;
@@ -2632,31 +3051,58 @@ int launch_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) 
     vp.cnb = ctx->cnb;
     vp.ubf = ctx->adj_ubf;
     vp.ubar = ubar;
-    vp.gload = (ctx->tiled && ctx->adj_g) ? (const R *)ctx->adj_g : nullptr;
+    vp.gload = (ctx->smode && ctx->adj_g) ? (const R *)ctx->adj_g : nullptr;
+    vp.slot = ctx->adj_slot;
+    vp.uq = ctx->adj_uq;
+    vp.ucount = ctx->adj_ucount;
+    vp.gbout = ctx->smode ? ctx->adj_gb : nullptr;
+    vp.toff = ctx->adj_toff;
+    vp.tab = ctx->adj_tab;
     const dim3 grid(vp.tiles_per_xcd * NXCD), block(WAVE);
     const bool q3 = nfq <= 24;
     tstart(ctx, 0);
-    if (ctx->ideg == 19 && ctx->imax == 39) {
-        if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3>), grid, block, 0, ctx->stream, vp);
+    if (ctx->smode) {
+        // row-stream shapes: the scatter follows on the next row stream
+        if (ctx->ideg == 19 && q3)
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, true>), grid, block, 0, ctx->stream, vp);
+        else if (ctx->ideg == 19)
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7, true>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3, true>), grid, block, 0, ctx->stream, vp);
+    } else if (ctx->ideg == 19 && ctx->imax == 39) {
+        if (q3)
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, false>), grid, block, 0, ctx->stream, vp);
+        else
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7, false>), grid, block, 0, ctx->stream, vp);
     } else if (ctx->ideg == 9 && ctx->imax == 19) {
         if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3, false>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7, false>), grid, block, 0, ctx->stream, vp);
     } else if (ctx->ideg == 3 && ctx->imax == 8) {
         if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3, false>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7, false>), grid, block, 0, ctx->stream, vp);
     } else {
         ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
         return UCNS_EUNSUPPORTED;
     }
+    if (!ctx->smode && ctx->adj_slot) adj_gather(ctx, ubar);
     tstop(ctx, 0);
     tcollect(ctx, 0, &ctx->st.ms_recon_vjp, &ctx->st.n_recon_vjp);
+    return UCNS_OK;
+}
+
+// (dR/dU)^T lam at the current state into ubar; bcbar (device, 10) += the
+// free-stream cotangent when non-null
+template <typename R>
+int residual_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
+    int rc;
+    if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) return rc;
+    if ((rc = vjp_products_t<R>(ctx, lam, ubar, bcbar))) return rc;
+    // row stream: one more pass of the same state carries the scatter
+    if (ctx->smode && (rc = vjp_recon_t<R>(ctx, ctx->adj_gb, ubar))) return rc;
     return UCNS_OK;
 }
 
@@ -2665,8 +3111,8 @@ int launch_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) 
 int residual_vjp_dev(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
     int rc = ensure_adjoint(ctx);
     if (rc) return rc;
-    return ctx->precision == UCNS_MIXED ? launch_vjp_t<float>(ctx, lam, ubar, bcbar)
-                                        : launch_vjp_t<double>(ctx, lam, ubar, bcbar);
+    return ctx->precision == UCNS_MIXED ? residual_vjp_t<float>(ctx, lam, ubar, bcbar)
+                                        : residual_vjp_t<double>(ctx, lam, ubar, bcbar);
 }
 
 // one SSP-RK3 step forward from ctx->U with the stage states stored
@@ -2686,41 +3132,71 @@ int rk3_step_ckpt(ucns_ctx *ctx, int it, const double *dt, unsigned long long *n
 }
 
 // The reverse sweep over nsteps stored steps: abar holds the cotangent of
-// the final state on entry and of the initial state on return.
-int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
+// the final state on entry and of the initial state on return.  Stages are
+// reversed in the order (step n: 3, 2, 1), each at its stored state, with
+//   w = -c dt ybar / V, jt = (dR/du)^T w, and k_rk3_adj_stage combining ybar
+//   and jt into the cotangents of the stage's inputs.
+// In row-stream mode the scatter of a stage's reconstruction product rides on
+// the forward reconstruction of the next stage reversed, so each reversed
+// stage streams the pseudo-inverses once.
+template <typename R>
+int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
     const size_t n = (size_t)ctx->ncell * NV;
     const int blocks = (int)std::min<size_t>(8192, (n + 255) / 256);
     double *const ustate = ctx->U;
     double *a = abar, *u0 = ctx->adj_u0, *x = ctx->adj_x, *w = ctx->adj_lam, *jt = ctx->adj_ubar;
+    const double cst[4] = {0.0, 1.0, 1.0 / 4.0, 2.0 / 3.0};  // -w / (dt ybar / V) per stage
+    const bool pipe = ctx->smode != 0;
     int rc = UCNS_OK;
+    int pending = 0;  // stage whose jt still waits for its scatter (0: none)
+    // combine the stage whose product is complete into its input cotangents
+    auto combine = [&](int stage) {
+        const double *y = stage == 3 ? a : x;
+        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, stage, y, jt, x,
+                           u0);
+        if (stage == 1) (void)hipMemcpyAsync(a, u0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
     for (int it = nsteps - 1; it >= 0 && rc == UCNS_OK; --it) {
         const double *dt = ctx->dts + it;
-        // stage 3: u = 1/3 u0 + 2/3 u2 - 2/3 dt R(u2)/V
-        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, a, ctx->ovol, dt,
-                           2.0 / 3.0, w);
-        ctx->U = ctx->ckpt + ((size_t)3 * it + 2) * n;
-        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
-        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 3, a, jt, x, u0);
-        // stage 2: u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1)/V
-        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, x, ctx->ovol, dt,
-                           1.0 / 4.0, w);
-        ctx->U = ctx->ckpt + ((size_t)3 * it + 1) * n;
-        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
-        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 2, x, jt, x, u0);
-        // stage 1: u1 = u0 - dt R(u0)/V
-        hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, x, ctx->ovol, dt, 1.0,
-                           w);
-        ctx->U = ctx->ckpt + (size_t)3 * it * n;
-        if ((rc = residual_vjp_dev(ctx, w, jt, bcbar))) break;
-        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, 1, x, jt, x, u0);
-        const hipError_t e = hipMemcpyAsync(a, u0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        for (int stage = 3; stage >= 1 && rc == UCNS_OK; --stage) {
+            ctx->U = ctx->ckpt + ((size_t)3 * it + stage - 1) * n;
+            if (pipe) {
+                // forward reconstruction of this stage's state (+ the pending scatter)
+                if ((rc = vjp_recon_t<R>(ctx, pending ? ctx->adj_gb : nullptr, jt))) break;
+                if (pending) combine(pending);
+            } else if (pending) {
+                combine(pending);
+            }
+            const double *y = stage == 3 ? a : x;
+            hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, y, ctx->ovol, dt,
+                               cst[stage], w);
+            if (pipe) {
+                if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
+            } else {
+                if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) break;
+                if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
+            }
+            pending = stage;
+        }
+    }
+    if (rc == UCNS_OK && pending) {
+        if (pipe) rc = vjp_recon_t<R>(ctx, ctx->adj_gb, jt);  // the last scatter (the state is any stored one)
+        if (rc == UCNS_OK) combine(pending);
+    }
+    ctx->U = ustate;
+    if (rc == UCNS_OK) {
+        const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
-            ctx->err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e);
+            ctx->err = std::string("rk3 reverse: ") + hipGetErrorString(e);
             rc = UCNS_EHIP;
         }
     }
-    ctx->U = ustate;
     return rc;
+}
+
+int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
+    return ctx->precision == UCNS_MIXED ? rk3_reverse_t<float>(ctx, nsteps, abar, bcbar)
+                                        : rk3_reverse_t<double>(ctx, nsteps, abar, bcbar);
 }
 
 int state_io(ucns_ctx *ctx, const double *src, double *dst, int32_t layout, hipMemcpyKind kind, bool to_dev) {
@@ -2785,9 +3261,9 @@ void release_prestore(ucns_ctx *ctx) {
     ctx->have_prestore = false;
 }
 
+
 }  // namespace
 
-// ---------------------------------------------------------------------------
 extern "C" {
 
 int ucns_abi_version(void) { return UCNS_ABI_VERSION; }

@@ -63,6 +63,7 @@ class StatsV1(C.Structure):
         ("n_reduced_last", C.c_int64), ("n_firstorder_last", C.c_int64),
         ("n_recon_vjp", C.c_int64), ("n_flux_vjp", C.c_int64),
         ("ms_recon_vjp", C.c_double), ("ms_flux_vjp", C.c_double),
+        ("n_recon_adj", C.c_int64), ("ms_recon_adj", C.c_double),
     ]
 
 
