@@ -489,7 +489,9 @@ struct ReconWParams {
     int *counters;
     const unsigned char *stream;  // per-tile row stream of the DMA kernels (StreamShape)
     long long tile_bytes;
-    R *gstore;             // optional [ntile][ideg*5][8*ts]: every stencil's coefficients, for the adjoint
+    R *gstore;             // optional [ntile][ideg*5 + 10][8*ts]: every stencil's coefficients, then
+                           // its indicators beta[5] and weights omega[5], for the adjoint
+    int *fstore;           // with gstore: per cell, 1 = MUSCL (CHECKSOL), 2 = first order (CHECKSOLX)
     // scatter of a residual product riding on the row stream (k_recon_sv*):
     // member j of lane position pos receives P_j^T gbs[.][.][pos]
     const double *gbs;        // [ntile][ideg*5][8*ts] cotangents of the stencil coefficients
@@ -579,7 +581,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     if (STORE && act) {
         // coalesced: for each coefficient the 8 ts stencils of the tile are contiguous
         const int TP = 8 * p.ts;
-        R *gs = p.gstore + (size_t)(cell >> 3) * IDEG * NV * TP + (lc * p.ts + s);
+        R *gs = p.gstore + (size_t)(cell >> 3) * (IDEG + 2) * NV * TP + (lc * p.ts + s);
 #pragma unroll
         for (int k = 0; k < IDEG; ++k)
 #pragma unroll
@@ -632,6 +634,15 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
         const R sum = grp_sum8(w);
         om[v] = act ? w / sum : R(0);
     }
+    if (STORE && act) {
+        const int TP = 8 * p.ts;
+        R *gs = p.gstore + ((size_t)(cell >> 3) * (IDEG + 2) + IDEG) * NV * TP + (lc * p.ts + s);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gs[(size_t)v * TP] = beta[v];
+            gs[(size_t)(NV + v) * TP] = om[v];
+        }
+    }
 #pragma unroll
     for (int k = 0; k < IDEG; ++k)
 #pragma unroll
@@ -675,6 +686,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     }
     const int any_trouble = grp_or8(trouble);  // every lane takes part in the shuffle
     const bool reduce = cvalid && (!full || any_trouble);
+    bool first = false;
 
     // ---- MUSCL + Barth-Jespersen on the central polynomial (src/reconstruct_p.f90:2044-2280) ----
     // then CHECKSOLX (0.95) -> first order (src/reconstruct_p.f90:3245-3406)
@@ -742,7 +754,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
                 if (jump_test(p.gamma, wi, uq, 0.95)) tr2 = 1;
             }
         }
-        const bool first = grp_or8(tr2) != 0;
+        first = grp_or8(tr2) != 0;
 #pragma unroll
         for (int r = 0; r < NQL; ++r) {
             const int q = s + 8 * r;
@@ -756,6 +768,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
             if (first) atomicAdd(&p.counters[1], 1);
         }
     }
+    if (STORE && s == 0 && cvalid) p.fstore[cell] = (reduce ? 1 : 0) | (first ? 2 : 0);
 }
 
 template <typename R, int IDEG, int IMAX, int NQL, int D, int NCH, int DEPTH>
@@ -1732,7 +1745,9 @@ struct VjpParams {
     const int32_t *cnb;    // [ncell][nf]
     const double *ubf;     // [ncell][nf][nqm][5] cotangents of the face-point states
     double *ubar;          // [ncell][5] accumulated
-    const R *gload;        // the forward's stencil coefficients [ntile][ideg*5][8 ts], or null: recompute
+    const R *gload;        // the forward's stencil coefficients, indicators and weights
+                           // [ntile][ideg*5 + 10][8 ts], or null: recompute them
+    const int *fload;      // with gload: the forward's per-cell flags (1 MUSCL, 2 first order)
     // the tile's distinct stencil members: member (j, pos) of tile t scatters
     // to LDS row slot[t][j][pos] (UCNS_NOSLOT: straight to HBM), and row r of
     // tile t belongs to cell uq[t][r], r < min(ucount[t], UCNS_TILE_UQ)
@@ -1918,7 +1933,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) g[k][v] = R(0);
     if (act && p.gload) {
-        const R *gs = p.gload + (size_t)tile * IDEG * NV * p.lsq.tp + pos;
+        const R *gs = p.gload + (size_t)tile * (IDEG + 2) * NV * p.lsq.tp + pos;
 #pragma unroll
         for (int k = 0; k < IDEG; ++k)
 #pragma unroll
@@ -1949,79 +1964,101 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     }
 
     // ---- C: indicators, weights and combined coefficients (forward) ----
+    // With the forward's stored indicators, weights and flags (gload) only
+    // the weights' cotangent input obar_s = gh_bar . G_s is formed here;
+    // otherwise the forward is repeated with its arithmetic (the same bits).
     R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
     R om[NV] = {R(0), R(0), R(0), R(0), R(0)};
     double obar[NV] = {0, 0, 0, 0, 0};
-    if (p.iweno == 1) {
-        if (act) {
-#pragma unroll
-            for (int k = 0; k < IDEG; ++k) {
-                R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
-#pragma unroll
-                for (int l = 0; l < IDEG; ++l) {
-                    const R bkl = sB[lc][pidx(IDEG, k, l)];
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
-                }
-#pragma unroll
-                for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
-            }
-        }
-        const R lam = (s == 0) ? (R)p.lwci1 : R(1);
-        const R eps = (R)p.eps;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            R w;
-            if (sizeof(R) == 8) {
-                const R x = eps + beta[v];
-                const R x2 = x * x;
-                w = act ? lam / (x2 * x2) : R(0);
-            } else {
-                const R x = eps + beta[v];
-                const R xmin = grp_min8(act ? x : (R)INFINITY);
-                const R r = xmin / x;
-                const R r2 = r * r;
-                w = act ? lam * (r2 * r2) : R(0);
-            }
-            const R sum = grp_sum8(w);
-            om[v] = act ? w / sum : R(0);
-        }
-#pragma unroll
-        for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const R gh = grp_sum8(om[v] * g[k][v]);
-                if (s == 0) sGh[lc][k * NV + v] = gh;
-                obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
-            }
-    }
-    __syncthreads();  // sGb, sGh, sG0
-
-    // ---- D: CHECKSOL (0.85) on the WENO face values (forward) ----
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
     bool reduce = cvalid;
-    if (p.iweno == 1) {
-        int trouble = 0;
-#pragma unroll 1
-        for (int q = s; q < NQL * 8; q += 8) {
-            if (!qvalid(q)) continue;
-            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
-            R phi[IDEG];
-            basis_at<R, IDEG>(x, mb, phi);
-            double uq[NV];
+    if (p.gload) {
+        if (p.iweno == 1) {
+            if (act) {
+                const R *bo = p.gload + ((size_t)tile * (IDEG + 2) + IDEG) * NV * p.lsq.tp + pos;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    beta[v] = bo[(size_t)v * p.lsq.tp];
+                    om[v] = bo[(size_t)(NV + v) * p.lsq.tp];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
+        }
+        reduce = cvalid && (p.fload[cs] & 1);
+    } else {
+        if (p.iweno == 1) {
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) {
+                    R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+                    for (int l = 0; l < IDEG; ++l) {
+                        const R bkl = sB[lc][pidx(IDEG, k, l)];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
+                }
+            }
+            const R lam = (s == 0) ? (R)p.lwci1 : R(1);
+            const R eps = (R)p.eps;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                R res = R(0);
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) res += phi[k] * sGh[lc][k * NV + v];
-                uq[v] = ui[v] + (double)res;
+                R w;
+                if (sizeof(R) == 8) {
+                    const R x = eps + beta[v];
+                    const R x2 = x * x;
+                    w = act ? lam / (x2 * x2) : R(0);
+                } else {
+                    const R x = eps + beta[v];
+                    const R xmin = grp_min8(act ? x : (R)INFINITY);
+                    const R r = xmin / x;
+                    const R r2 = r * r;
+                    w = act ? lam * (r2 * r2) : R(0);
+                }
+                const R sum = grp_sum8(w);
+                om[v] = act ? w / sum : R(0);
             }
-            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const R gh = grp_sum8(om[v] * g[k][v]);
+                    if (s == 0) sGh[lc][k * NV + v] = gh;
+                    obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
+                }
         }
-        const int any_trouble = grp_or8(trouble);
-        reduce = cvalid && (!full || any_trouble);
+        __syncthreads();  // sGb, sGh, sG0
+
+        // ---- D: CHECKSOL (0.85) on the WENO face values (forward) ----
+        if (p.iweno == 1) {
+            int trouble = 0;
+#pragma unroll 1
+            for (int q = s; q < NQL * 8; q += 8) {
+                if (!qvalid(q)) continue;
+                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                R phi[IDEG];
+                basis_at<R, IDEG>(x, mb, phi);
+                double uq[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    R res = R(0);
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) res += phi[k] * sGh[lc][k * NV + v];
+                    uq[v] = ui[v] + (double)res;
+                }
+                if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+            }
+            const int any_trouble = grp_or8(trouble);
+            reduce = cvalid && (!full || any_trouble);
+        }
     }
+    __syncthreads();  // sG0
 
     // ---- E: MUSCL + Barth (forward), with the limiter's active arguments ----
     bool first = false;
@@ -2518,7 +2555,7 @@ struct ucns_ctx {
     // work arrays, the free-stream cotangent, stage checkpoints and step dts
     double *adj_ubf = nullptr, *adj_lam = nullptr, *adj_ubar = nullptr, *adj_x = nullptr, *adj_a = nullptr;
     double *adj_u0 = nullptr, *adj_bc = nullptr, *ckpt = nullptr, *dts = nullptr;
-    void *adj_g = nullptr;  // stencil coefficients of the row-stream reconstruction, [ntile][ideg*5][8 ts] (R)
+    void *adj_g = nullptr;  // stencil coefficients, indicators, weights of the row-stream reconstruction (R)
     double *adj_gb = nullptr;  // stencil-coefficient cotangents handed to the row stream, [ntile][ideg*5][8 ts]
     const double *sv_gb = nullptr;  // non-null: the next row-stream reconstruction also scatters sv_gb ...
     double *sv_ubar = nullptr;      // ... into sv_ubar
@@ -2527,6 +2564,8 @@ struct ucns_ctx {
     int64_t *adj_toff = nullptr, *adj_off = nullptr, *adj_ent = nullptr;  // table rows per tile, CSR by cell
     double *adj_tab = nullptr;                          // [rows][5] member tables
     bool adj_g_on = false;  // the next tiled reconstruction stores them
+    bool adj_fuse = false;  // row-stream mode: scatter on the next row stream (UCNS_ADJ_FUSE=1) or in the product
+    int *adj_flags = nullptr;  // [ncell] its MUSCL / first-order flags
     size_t ckpt_steps = 0;
     // stats
     bool timing = false;
@@ -2606,6 +2645,7 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
         w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
+        w.fstore = ctx->adj_g_on ? ctx->adj_flags : nullptr;
         if (ctx->smode && ctx->sv_ubar) {
             // reconstruction + the scatter of the previous residual product
             w.gbs = ctx->sv_gb;
@@ -2846,6 +2886,7 @@ void release_adjoint(ucns_ctx *ctx) {
     dfree(ctx->dts);
     if (ctx->adj_g) (void)hipFree(ctx->adj_g);
     ctx->adj_g = nullptr;
+    dfree(ctx->adj_flags);
     dfree(ctx->adj_gb);
     dfree(ctx->adj_slot);
     dfree(ctx->adj_uq);
@@ -2903,7 +2944,8 @@ int ensure_adjoint(ucns_ctx *ctx) {
     if ((rc = dalloc(ctx, &ctx->adj_u0, n))) return rc;
     if ((rc = dalloc(ctx, &ctx->adj_bc, 2 * NV))) return rc;
     if (ctx->smode) {
-        const size_t ng = (size_t)ctx->ntile * ctx->ideg * NV * 8 * ctx->ts;
+        const size_t ng = (size_t)ctx->ntile * (ctx->ideg + 2) * NV * 8 * ctx->ts;
+        if ((rc = dalloc(ctx, &ctx->adj_flags, (size_t)ctx->ncell))) return rc;
         if (ctx->precision == UCNS_MIXED)
             rc = dalloc(ctx, (float **)&ctx->adj_g, ng);
         else
@@ -3052,16 +3094,17 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     vp.ubf = ctx->adj_ubf;
     vp.ubar = ubar;
     vp.gload = (ctx->smode && ctx->adj_g) ? (const R *)ctx->adj_g : nullptr;
+    vp.fload = ctx->adj_flags;
     vp.slot = ctx->adj_slot;
     vp.uq = ctx->adj_uq;
     vp.ucount = ctx->adj_ucount;
-    vp.gbout = ctx->smode ? ctx->adj_gb : nullptr;
+    vp.gbout = (ctx->smode && ctx->adj_fuse) ? ctx->adj_gb : nullptr;
     vp.toff = ctx->adj_toff;
     vp.tab = ctx->adj_tab;
     const dim3 grid(vp.tiles_per_xcd * NXCD), block(WAVE);
     const bool q3 = nfq <= 24;
     tstart(ctx, 0);
-    if (ctx->smode) {
+    if (ctx->smode && ctx->adj_fuse) {
         // row-stream shapes: the scatter follows on the next row stream
         if (ctx->ideg == 19 && q3)
             hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, true>), grid, block, 0, ctx->stream, vp);
@@ -3088,7 +3131,7 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
         ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
         return UCNS_EUNSUPPORTED;
     }
-    if (!ctx->smode && ctx->adj_slot) adj_gather(ctx, ubar);
+    if (!(ctx->smode && ctx->adj_fuse) && ctx->adj_slot) adj_gather(ctx, ubar);
     tstop(ctx, 0);
     tcollect(ctx, 0, &ctx->st.ms_recon_vjp, &ctx->st.n_recon_vjp);
     return UCNS_OK;
@@ -3101,8 +3144,8 @@ int residual_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     int rc;
     if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) return rc;
     if ((rc = vjp_products_t<R>(ctx, lam, ubar, bcbar))) return rc;
-    // row stream: one more pass of the same state carries the scatter
-    if (ctx->smode && (rc = vjp_recon_t<R>(ctx, ctx->adj_gb, ubar))) return rc;
+    // fused row stream: one more pass of the same state carries the scatter
+    if (ctx->smode && ctx->adj_fuse && (rc = vjp_recon_t<R>(ctx, ctx->adj_gb, ubar))) return rc;
     return UCNS_OK;
 }
 
@@ -3146,7 +3189,7 @@ int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
     double *const ustate = ctx->U;
     double *a = abar, *u0 = ctx->adj_u0, *x = ctx->adj_x, *w = ctx->adj_lam, *jt = ctx->adj_ubar;
     const double cst[4] = {0.0, 1.0, 1.0 / 4.0, 2.0 / 3.0};  // -w / (dt ybar / V) per stage
-    const bool pipe = ctx->smode != 0;
+    const bool pipe = ctx->smode != 0 && ctx->adj_fuse;
     int rc = UCNS_OK;
     int pending = 0;  // stage whose jt still waits for its scatter (0: none)
     // combine the stage whose product is complete into its input cotangents
@@ -3486,6 +3529,8 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                                        : ((ts == 5 || ts == 7) && nfq <= 24);
         const bool stream_ok = !(st && st[0] == '0') && !ctx->ring && ctx->tiled && shape_ok;
         ctx->smode = stream_ok ? ts : 0;
+        const char *fu = std::getenv("UCNS_ADJ_FUSE");
+        ctx->adj_fuse = fu && fu[0] == '1';
     }
     ctx->ntile = (nc + 7) / 8;
     // stencil members without the cell itself
