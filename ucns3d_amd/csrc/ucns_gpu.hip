@@ -1197,6 +1197,128 @@ template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_sv32(ReconWParams<float> p) {
     recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), true, true>(p);
 }
+// Scatter of a residual product on the row stream, without the
+// reconstruction: lane (cell, stencil) holds its g_bar (p.gbs) and streams
+// the tile's pseudo-inverse rows through the same LDS ring and layout as the
+// reconstruction (no neighbour states); dU_bar_j = P_j^T g_bar goes to the
+// tile's member table (then p.tab, summed per cell by k_adj_gather) and
+// minus the same to the cell.
+template <typename R, int IDEG, int IMAX, int TS, int DEPTH>
+__device__ __forceinline__ void scatter_s_body(const ReconWParams<R> &p) {
+    using SS = StreamShape<R, IDEG, IMAX, TS, DEPTH>;
+    constexpr int TP = SS::TP, NCH = SS::NCH, SL = NCH * 1024;
+    __shared__ __attribute__((aligned(1024))) char ring[DEPTH * SL + SS::HD];
+    __shared__ double sAcc[UCNS_TILE_UQ][NV];
+    __shared__ uint16_t sSlot[IMAX * TP];
+    const int *hdr = (const int *)(ring + DEPTH * SL);
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    int info = cvalid ? p.cinfo[cell] : 0;
+    double gb[IDEG][NV];
+    {
+        const double *gs = p.gbs + (size_t)tile * IDEG * NV * TP + (lc * TS + s);
+        const bool in = cvalid && s < TS;
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gb[k][v] = in ? gs[(size_t)(k * NV + v) * TP] : 0.0;
+    }
+    const uint16_t *st = p.slot + (size_t)tile * IMAX * TP;
+    for (int e = lane; e < IMAX * TP; e += WAVE) sSlot[e] = st[e];
+    const int nuq = min(p.ucount[tile], UCNS_TILE_UQ);
+    for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+    // every cotangent is used before the row stream starts, so the compiler
+    // waits for their loads here and not inside the stream
+    double gsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) gsum += gb[k][v];
+    asm volatile("s_waitcnt vmcnt(0)" : "+v"(info), "+v"(gsum) : : "memory");
+    __syncthreads();  // sSlot, sAcc
+    const bool act = cvalid && s < ci_admis(info);
+    const int rl = act ? lc * TS + s : 0;
+    const char *T = (const char *)p.stream + (size_t)tile * SS::TILE;
+    auto issue_row = [&](int j) {
+        const char *src = T + SS::HD + (size_t)j * SS::RD;
+        char *dst = ring + (j % DEPTH) * SL;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, SS::RD - c * 1024);
+    };
+    int qn[DEPTH];  // members of the next DEPTH rows, oldest first
+    double acc[NV] = {0, 0, 0, 0, 0};
+    dma_chunk(T, ring + DEPTH * SL, lane, SS::HD);
+    issue_row(0);
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(NCH) : "memory");  // header landed
+#pragma unroll
+    for (int r = 0; r < DEPTH; ++r) qn[r] = hdr[r * TP + rl];
+#pragma unroll
+    for (int r = 1; r < DEPTH; ++r) issue_row(r);
+    auto consume = [&](int i) {
+        if (act) {
+            const R *row = (const R *)(ring + (i % DEPTH) * SL) + rl;
+            double d[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                const double pk = (double)row[k * TP];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
+            }
+            const unsigned sl = sSlot[i * TP + rl];
+            if (sl != UCNS_NOSLOT) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) unsafeAtomicAdd(&p.ubar[(size_t)qn[0] * NV + v], d[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] -= d[v];
+        }
+#pragma unroll
+        for (int r = 0; r + 1 < DEPTH; ++r) qn[r] = qn[r + 1];
+    };
+#pragma unroll 1
+    for (int i = 0; i < IMAX - DEPTH; ++i) {
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * NCH) : "memory");
+        consume(i);
+        const int nn = *(const int *)(ring + (i % DEPTH) * SL + SS::PB + rl * 4);  // members of row i+DEPTH
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        issue_row(i + DEPTH);
+        qn[DEPTH - 1] = nn;
+    }
+    [&]<int... TT>(std::integer_sequence<int, TT...>) {
+        (
+            [&] {
+                constexpr int t = TT;
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1 - t) * NCH) : "memory");
+                consume(IMAX - DEPTH + t);
+            }(),
+            ...);
+    }(std::make_integer_sequence<int, DEPTH>{});
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double t = grp_sum8(acc[v]);
+        if (s == 0 && cvalid) unsafeAtomicAdd(&p.ubar[(size_t)cell * NV + v], t);
+    }
+    __syncthreads();
+    double *tt = p.tab + (size_t)p.toff[tile] * NV;
+    for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+}
+template <int IDEG, int IMAX, int TS>
+__global__ __launch_bounds__(WAVE) void k_scatter_s64(ReconWParams<double> p) {
+    scatter_s_body<double, IDEG, IMAX, TS, stream_depth(false, TS)>(p);
+}
+template <int IDEG, int IMAX, int TS>
+__global__ __launch_bounds__(WAVE) void k_scatter_s32(ReconWParams<float> p) {
+    scatter_s_body<float, IDEG, IMAX, TS, stream_depth(true, TS)>(p);
+}
+
 // the reconstruction storing every stencil's coefficients for a residual product
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_sg64(ReconWParams<double> p) {
@@ -3024,6 +3146,47 @@ void zero_dev(ucns_ctx *ctx, double *p, size_t n) {
 // tiled kernels also store every stencil's coefficients; with gb non-null
 // the row-stream kernel scatters that earlier product's g_bar into ubar on
 // the same pass
+// P^T g_bar of the last products (adj_gb) onto the row stream, into ubar
+template <typename R>
+void launch_scatter_t(ucns_ctx *ctx, double *ubar) {
+    ReconWParams<R> w{};
+    w.ncell = ctx->ncell;
+    w.ntile = ctx->ntile;
+    w.tiles_per_xcd = (ctx->ntile + NXCD - 1) / NXCD;
+    w.ts = ctx->ts;
+    w.cinfo = ctx->cinfo;
+    w.stream = ctx->rows;
+    w.tile_bytes = ctx->tile_bytes;
+    w.gbs = ctx->adj_gb;
+    w.ubar = ubar;
+    w.slot = ctx->adj_slot;
+    w.ucount = ctx->adj_ucount;
+    w.toff = ctx->adj_toff;
+    w.tab = ctx->adj_tab;
+    const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
+    if constexpr (sizeof(R) == 8) {
+        if (ctx->ideg == 19)
+            if (ctx->smode == 5)
+                hipLaunchKernelGGL((k_scatter_s64<19, 39, 5>), grid, block, 0, ctx->stream, w);
+            else
+                hipLaunchKernelGGL((k_scatter_s64<19, 39, 7>), grid, block, 0, ctx->stream, w);
+        else if (ctx->smode == 5)
+            hipLaunchKernelGGL((k_scatter_s64<9, 19, 5>), grid, block, 0, ctx->stream, w);
+        else
+            hipLaunchKernelGGL((k_scatter_s64<9, 19, 7>), grid, block, 0, ctx->stream, w);
+    } else {
+        if (ctx->ideg == 19)
+            if (ctx->smode == 5)
+                hipLaunchKernelGGL((k_scatter_s32<19, 39, 5>), grid, block, 0, ctx->stream, w);
+            else
+                hipLaunchKernelGGL((k_scatter_s32<19, 39, 7>), grid, block, 0, ctx->stream, w);
+        else if (ctx->smode == 5)
+            hipLaunchKernelGGL((k_scatter_s32<9, 19, 5>), grid, block, 0, ctx->stream, w);
+        else
+            hipLaunchKernelGGL((k_scatter_s32<9, 19, 7>), grid, block, 0, ctx->stream, w);
+    }
+}
+
 template <typename R>
 int vjp_recon_t(ucns_ctx *ctx, const double *gb, double *ubar) {
     (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
@@ -3098,14 +3261,14 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     vp.slot = ctx->adj_slot;
     vp.uq = ctx->adj_uq;
     vp.ucount = ctx->adj_ucount;
-    vp.gbout = (ctx->smode && ctx->adj_fuse) ? ctx->adj_gb : nullptr;
+    vp.gbout = ctx->smode ? ctx->adj_gb : nullptr;
     vp.toff = ctx->adj_toff;
     vp.tab = ctx->adj_tab;
     const dim3 grid(vp.tiles_per_xcd * NXCD), block(WAVE);
     const bool q3 = nfq <= 24;
     tstart(ctx, 0);
-    if (ctx->smode && ctx->adj_fuse) {
-        // row-stream shapes: the scatter follows on the next row stream
+    if (ctx->smode) {
+        // row-stream shapes: the scatter follows on the row stream
         if (ctx->ideg == 19 && q3)
             hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, true>), grid, block, 0, ctx->stream, vp);
         else if (ctx->ideg == 19)
@@ -3131,6 +3294,7 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
         ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
         return UCNS_EUNSUPPORTED;
     }
+    if (ctx->smode && !ctx->adj_fuse) launch_scatter_t<R>(ctx, ubar);
     if (!(ctx->smode && ctx->adj_fuse) && ctx->adj_slot) adj_gather(ctx, ubar);
     tstop(ctx, 0);
     tcollect(ctx, 0, &ctx->st.ms_recon_vjp, &ctx->st.n_recon_vjp);
