@@ -1471,6 +1471,164 @@ __device__ inline void hllc64(double gamma, const double *cl, const double *cr, 
     hllc_t<double>(gamma, cl, cr, flux);
 }
 
+// Reverse mode of hllc_t (FP64): given the flux cotangent hb, the cotangents
+// of both rotated states.  The forward values repeat hllc_t; the branches
+// (Einfeldt min/max, the clamps of S_L/S_R at 0, the side sign(S_M) picks)
+// are taken as hllc_t takes them, ties included, so this is the transpose
+// of the derivative the Dual passes of hllc_t compute.
+__device__ inline void hllc_vjp(double g, const double *cl, const double *cr, const double *hb, double *clb,
+                                double *crb) {
+    const double gm = g - 1.0;
+    // ---- forward ----
+    const double rl = cl[0], ool = 1.0 / rl, uL = cl[1] * ool, vl = cl[2] * ool, zl = cl[3] * ool, el = cl[4];
+    const double rr = cr[0], oor = 1.0 / rr, uR = cr[1] * oor, vr = cr[2] * oor, zr = cr[3] * oor, er = cr[4];
+    const double ql = uL * uL + vl * vl + zl * zl, qr = uR * uR + vr * vr + zr * zr;
+    const double pl = gm * (el - 0.5 * rl * ql), pr = gm * (er - 0.5 * rr * qr);
+    const double ccl = sqrt(g * pl / rl), ccr = sqrt(g * pr / rr);
+    const double srl = sqrt(rl), srr = sqrt(rr), S = srl + srr;
+    const double du = uR - uL;
+    const double T1 = ((ccl * ccl) * srl + (ccr * ccr) * srr) / S;
+    const double cc2 = sqrt(T1 + (0.5 * ((srl * srr) / (S * S)) * (du * du)));
+    const double uu2 = ((uL * srl) + (uR * srr)) / S;
+    const double a1 = uL - ccl, a2 = uu2 - cc2, b1 = uR + ccr, b2 = uu2 + cc2;
+    const bool selL = a2 < a1, selR = b2 > b1;  // the second argument wins
+    double sl = selL ? a2 : a1, sr = selR ? b2 : b1;
+    const bool clL = 0.0 < sl, clR = 0.0 > sr;  // clamped to 0
+    if (clL) sl = 0.0;
+    if (clR) sr = 0.0;
+    const double num = pr - pl + (rl * uL * (sl - uL)) - (rr * uR * (sr - uR));
+    const double den = (rl * (sl - uL)) - (rr * (sr - uR));
+    const double sm = num / den;
+    const bool left = copysign(1.0, sm) > 0.0;
+    // ---- reverse through the chosen side K: F = F_K + S_K (U*_K - U_K) ----
+    const double rK = left ? rl : rr, uK = left ? uL : uR, vK = left ? vl : vr, zK = left ? zl : zr;
+    const double pK = left ? pl : pr, eK = left ? el : er, sK = left ? sl : sr;
+    const double *cK = left ? cl : cr;
+    const double dK = sK - uK, Q = sK - sm, mu = rK * (dK / Q);
+    const double D = rK * dK, A = sm - uK, Bq = sm + pK / D;
+    const double lastK = (eK / rK) + A * Bq;
+    const double us[NV] = {mu, mu * sm, mu * vK, mu * zK, mu * lastK};
+    double sKb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sKb += hb[k] * (us[k] - cK[k]);
+    double usb[NV], cKb[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        usb[k] = sK * hb[k];
+        cKb[k] = -sK * hb[k];
+    }
+    const double mub = usb[0] + usb[1] * sm + usb[2] * vK + usb[3] * zK + usb[4] * lastK;
+    double smb = usb[1] * mu;
+    double vKb = usb[2] * mu, zKb = usb[3] * mu;
+    const double lastb = usb[4] * mu;
+    double eKb = lastb / rK, rKb = -lastb * eK / (rK * rK);
+    const double Ab = lastb * Bq, Bqb = lastb * A;
+    smb += Ab + Bqb;
+    double uKb = -Ab;
+    double pKb = Bqb / D;
+    const double Db = -Bqb * pK / (D * D);
+    rKb += Db * dK;
+    double dKb = Db * rK;
+    rKb += mub * dK / Q;
+    dKb += mub * rK / Q;
+    const double Qb = -mub * mu / Q;
+    sKb += Qb;
+    smb -= Qb;
+    sKb += dKb;
+    uKb -= dKb;
+    // F_K = [rK uK, rK uK^2 + pK, rK uK vK, rK uK zK, uK (eK + pK)]
+    rKb += hb[0] * uK + hb[1] * uK * uK + hb[2] * uK * vK + hb[3] * uK * zK;
+    uKb += hb[0] * rK + hb[1] * 2.0 * rK * uK + hb[2] * rK * vK + hb[3] * rK * zK + hb[4] * (eK + pK);
+    pKb += hb[1] + hb[4] * uK;
+    vKb += hb[2] * rK * uK;
+    zKb += hb[3] * rK * uK;
+    eKb += hb[4] * uK;
+    // primitive cotangents of both sides
+    double rlb = 0, ulb = 0, vlb = 0, zlb = 0, plb = 0, elb = 0, slb = 0;
+    double rrb = 0, urb = 0, vrb = 0, zrb = 0, prb = 0, erb = 0, srb = 0;
+    if (left) {
+        rlb = rKb; ulb = uKb; vlb = vKb; zlb = zKb; plb = pKb; elb = eKb; slb = sKb;
+    } else {
+        rrb = rKb; urb = uKb; vrb = vKb; zrb = zKb; prb = pKb; erb = eKb; srb = sKb;
+    }
+    // S_M = num / den
+    const double numb = smb / den, denb = -smb * sm / den;
+    prb += numb;
+    plb -= numb;
+    rlb += numb * uL * (sl - uL);
+    ulb += numb * rl * (sl - 2.0 * uL);
+    slb += numb * rl * uL;
+    rrb -= numb * uR * (sr - uR);
+    urb -= numb * rr * (sr - 2.0 * uR);
+    srb -= numb * rr * uR;
+    rlb += denb * (sl - uL);
+    slb += denb * rl;
+    ulb -= denb * rl;
+    rrb -= denb * (sr - uR);
+    srb -= denb * rr;
+    urb += denb * rr;
+    // S_L, S_R: clamps and Einfeldt min / max
+    double cclb = 0, ccrb = 0, cc2b = 0, uu2b = 0;
+    if (!clL) {
+        if (selL) {
+            uu2b += slb;
+            cc2b -= slb;
+        } else {
+            ulb += slb;
+            cclb -= slb;
+        }
+    }
+    if (!clR) {
+        if (selR) {
+            uu2b += srb;
+            cc2b += srb;
+        } else {
+            urb += srb;
+            ccrb += srb;
+        }
+    }
+    ulb += uu2b * srl / S;
+    urb += uu2b * srr / S;
+    double srlb = uu2b * (uL - uu2) / S, srrb = uu2b * (uR - uu2) / S;
+    const double Xb = cc2b / (2.0 * cc2);
+    cclb += Xb * 2.0 * ccl * srl / S;
+    ccrb += Xb * 2.0 * ccr * srr / S;
+    srlb += Xb * (ccl * ccl - T1) / S;
+    srrb += Xb * (ccr * ccr - T1) / S;
+    const double S3 = S * S * S, hdu2 = 0.5 * du * du;
+    srlb += Xb * hdu2 * srr * (srr - srl) / S3;
+    srrb += Xb * hdu2 * srl * (srl - srr) / S3;
+    const double dub = Xb * ((srl * srr) / (S * S)) * du;
+    urb += dub;
+    ulb -= dub;
+    rlb += srlb / (2.0 * srl);
+    rrb += srrb / (2.0 * srr);
+    const double ylb = cclb / (2.0 * ccl), yrb = ccrb / (2.0 * ccr);
+    plb += ylb * g / rl;
+    rlb -= ylb * g * pl / (rl * rl);
+    prb += yrb * g / rr;
+    rrb -= yrb * g * pr / (rr * rr);
+    // conserved -> primitive: rho = c0, u = c1/c0 ..., p = (g-1)(c4 - c0 q/2)
+    auto back = [&](const double *c, double r_b, double u_b, double v_b, double z_b, double p_b, double e_b,
+                    double u, double v, double z, double q, double *cb) {
+        const double qb = -gm * 0.5 * c[0] * p_b;
+        const double ub = u_b + 2.0 * u * qb, vb = v_b + 2.0 * v * qb, zb = z_b + 2.0 * z * qb;
+        const double oo = 1.0 / c[0];
+        cb[0] += r_b - gm * 0.5 * q * p_b - (ub * u + vb * v + zb * z) * oo;
+        cb[1] += ub * oo;
+        cb[2] += vb * oo;
+        cb[3] += zb * oo;
+        cb[4] += gm * p_b + e_b;
+    };
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        clb[k] = left ? cKb[k] : 0.0;
+        crb[k] = left ? 0.0 : cKb[k];
+    }
+    back(cl, rlb, ulb, vlb, zlb, plb, elb, uL, vl, zl, ql, clb);
+    back(cr, rrb, urb, vrb, zrb, prb, erb, uR, vr, zr, qr, crb);
+}
+
 __device__ inline void hllc32(double gamma, const double *cl, const double *cr, double *flux) {
     double wl[NV], wr[NV];
     cons2prim_d(gamma, cl, wl);
@@ -2460,9 +2618,9 @@ __global__ void k_adj_csr(int ntile, const int32_t *uq, const int32_t *ucount, c
     }
 }
 
-// Face-parallel adjoint of the HLLC flux quadrature.  The Jacobians of HLLC
-// (10 inputs) and of the ghost state (5 inputs, plus the 10 free-stream
-// values) come from forward-mode passes of the same templated code the
+// Face-parallel adjoint of the HLLC flux quadrature: HLLC by its reverse
+// mode (hllc_vjp); the ghost state's Jacobian (5 inputs, plus the 10
+// free-stream values) from forward-mode passes of the same templated code the
 // primal runs, one pass per input; F_bar is pulled back through ROTATEB
 // (whose transpose is ROTATEF) and ROTATEF (transpose ROTATEB).
 template <typename R>
@@ -2508,23 +2666,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
 #pragma unroll
         for (int v = 0; v < NV; ++v) hb[v] = wa * hb0[v];
         double lab[NV], rab[NV];
-#pragma unroll 1
-        for (int m = 0; m < 2 * NV; ++m) {
-            Dual L[NV], Rt[NV], F[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                L[v] = Dual{la[v], m == v ? 1.0 : 0.0};
-                Rt[v] = Dual{ra[v], m == NV + v ? 1.0 : 0.0};
-            }
-            hllc_t(p.gamma, L, Rt, F);
-            double d = 0.0;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) d += hb[v] * F[v].d;
-            if (m < NV)
-                lab[m] = d;
-            else
-                rab[m - NV] = d;
-        }
+        hllc_vjp(p.gamma, la, ra, hb, lab, rab);
         double ab[NV], bb[NV];
         rotb(lab, c1, s1, c2, s2, ab);
         rotb(rab, c1, s1, c2, s2, bb);
