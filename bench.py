@@ -107,7 +107,10 @@ def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.n
 def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) -> dict:
     """adjoint / primal cost: ucns_rk3_adjoint_device over `steps` steps (its
     forward sweep with stage checkpoints plus the reverse sweep) against
-    ucns_rk3_steps over the same steps, device-resident cotangents"""
+    ucns_rk3_steps over the same steps, device-resident cotangents.  Timed
+    twice: with every stage's forward reconstruction kept for the reverse
+    sweep (the library's default when it fits in memory) and with none kept
+    (UCNS_ADJ_KEEP=0: the reverse sweep runs each reconstruction again)."""
     import torch
     from ucns3d_amd.solver import Solver
 
@@ -119,32 +122,44 @@ def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) ->
     u0b = torch.empty_like(a)
     s.rk3_steps(1, dt_fixed=dt)
     s.set_state(u0)
-    s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)  # allocates the checkpoints
+    s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)  # allocates checkpoints, kept stages
     s.set_state(u0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.rk3_steps(steps, dt_fixed=dt)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    s.set_state(u0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+
+    def adjoint() -> float:
+        s.set_state(u0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+    t_adj = adjoint()
     finite = bool(torch.isfinite(u0b).all().item())
-    s.set_state(u0)
-    s.reset_stats()
-    s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
-    st = s.stats()
+    os.environ["UCNS_ADJ_KEEP"] = "0"
+    try:
+        t_rec = adjoint()
+        s.set_state(u0)
+        s.reset_stats()
+        s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
+        st = s.stats()
+    finally:
+        del os.environ["UCNS_ADJ_KEEP"]
     s.close()
     return {"steps": steps, "primal_ms_per_step": t_fwd / steps * 1e3, "adjoint_ms_per_step": t_adj / steps * 1e3,
             "adjoint_over_primal": t_adj / t_fwd, "reverse_sweep_over_primal": (t_adj - t_fwd) / t_fwd,
+            "recompute_adjoint_ms_per_step": t_rec / steps * 1e3, "recompute_adjoint_over_primal": t_rec / t_fwd,
             "recon_vjp_ms": st["ms_recon_vjp"] / max(st["n_recon_vjp"], 1),
             "flux_vjp_ms": st["ms_flux_vjp"] / max(st["n_flux_vjp"], 1),
+            "recon_adj_ms": st["ms_recon_adj"] / max(st["n_recon_adj"], 1),
             "recon_ms": st["ms_recon"] / max(st["n_recon"], 1), "finite": finite,
-            "note": "adjoint = forward sweep storing every stage + reverse sweep (each stage: forward "
-                    "reconstruction, flux product, reconstruction product); dt fixed"}
+            "note": "adjoint = forward sweep storing every stage state (and keeping each stage's "
+                    "reconstruction) + reverse sweep (each stage: flux product, reconstruction product, "
+                    "transposed scatter); recompute = the same with the reconstruction run again per "
+                    "reversed stage; dt fixed"}
 
 
 def cpu_baseline(spec_key: str, seconds: float):

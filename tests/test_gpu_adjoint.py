@@ -26,16 +26,20 @@ pytestmark = pytest.mark.gpu
 
 
 @contextlib.contextmanager
-def stream_kernels(on: bool):
-    old = os.environ.get("UCNS_STREAM")
-    os.environ["UCNS_STREAM"] = "1" if on else "0"
+def env(name: str, value: str):
+    old = os.environ.get(name)
+    os.environ[name] = value
     try:
         yield
     finally:
         if old is None:
-            del os.environ["UCNS_STREAM"]
+            del os.environ[name]
         else:
-            os.environ["UCNS_STREAM"] = old
+            os.environ[name] = old
+
+
+def stream_kernels(on: bool):
+    return env("UCNS_STREAM", "1" if on else "0")
 
 
 def solver_for(name, precision=UCNS_FP64, stream=True):
@@ -106,6 +110,34 @@ def test_rk3_adjoint_dot_product(name):
         scale = np.linalg.norm(a) * np.linalg.norm(jb)
         print("  boundary data: gap", dot_gap(lhs, rhs, scale))
         assert dot_gap(lhs, rhs, scale) < 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("name,stream,precision", [("tet-weno4", True, UCNS_FP64), ("tet-weno4", False, UCNS_FP64),
+                                                    ("hex-muscl", True, UCNS_FP64),
+                                                    ("hex-weno4-channel", True, UCNS_FP64),
+                                                    ("tet-weno4", True, UCNS_MIXED)])
+def test_rk3_adjoint_kept_stages_agree_with_recomputed(name, stream, precision):
+    """The reverse sweep starts a stage's products from the reconstruction the
+    forward sweep kept, or runs the reconstruction again (UCNS_ADJ_KEEP caps
+    the kept stages): all, some (4 of 9) and none give the same cotangents"""
+    case, orc, u, du, bc = setup(name)
+    print(f"[rk3 adjoint keep] {name} stream={stream} precision={precision}", flush=True)
+    s = solver_for(name, precision=precision, stream=stream)
+    dt = 0.5 * s.compute_dt(0.5)
+    a = np.random.default_rng(15).normal(size=u.shape)
+    out = {}
+    for keep in ("9", "4", "0"):
+        with env("UCNS_ADJ_KEEP", keep):
+            s.set_state(u)
+            out[keep] = s.rk3_adjoint(3, a, dt_fixed=dt)
+    ref, rb, _ = out["9"]
+    for keep in ("4", "0"):
+        g, gb, _ = out[keep]
+        rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        print(f"  keep {keep}: rel {rel:.3e}")
+        assert rel <= 1e-12
+        np.testing.assert_allclose(gb, rb, rtol=1e-12, atol=1e-12 * (np.abs(rb).max() + 1e-300))
     s.close()
 
 

@@ -2831,6 +2831,11 @@ struct ucns_ctx {
     bool adj_fuse = false;  // row-stream mode: scatter on the next row stream (UCNS_ADJ_FUSE=1) or in the product
     int *adj_flags = nullptr;  // [ncell] its MUSCL / first-order flags
     size_t ckpt_steps = 0;
+    // the last stages of a march keep their forward reconstruction for the
+    // reverse sweep: face values, and in row-stream mode the stencil
+    // coefficients, indicators, weights (as adj_g) and flags
+    std::vector<void *> keep_ul, keep_g;
+    std::vector<int *> keep_f;
     // stats
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -3160,6 +3165,12 @@ void release_adjoint(ucns_ctx *ctx) {
     dfree(ctx->adj_ent);
     dfree(ctx->adj_tab);
     ctx->ckpt_steps = 0;
+    for (void *q : ctx->keep_ul) (void)hipFree(q);
+    for (void *q : ctx->keep_g) (void)hipFree(q);
+    for (int *q : ctx->keep_f) (void)hipFree(q);
+    ctx->keep_ul.clear();
+    ctx->keep_g.clear();
+    ctx->keep_f.clear();
 }
 
 template <typename R>
@@ -3464,13 +3475,83 @@ int residual_vjp_dev(ucns_ctx *ctx, const double *lam, double *ubar, double *bcb
                                         : residual_vjp_t<double>(ctx, lam, ubar, bcbar);
 }
 
-// one SSP-RK3 step forward from ctx->U with the stage states stored
-int rk3_step_ckpt(ucns_ctx *ctx, int it, const double *dt, unsigned long long *next, double cfl) {
+// How many of a march's last stages keep their forward reconstruction (at
+// most nstage): UCNS_ADJ_KEEP caps the count, UCNS_ADJ_KEEP_GB the bytes
+// (default 64 GiB, and never more than the free device memory less 4 GiB).
+// Buffers only grow; a failed allocation leaves the count at what was had.
+int adj_keep(ucns_ctx *ctx, int nstage) {
+    if (ctx->adj_fuse) return 0;  // the fused sweep recomputes every stage
+    const size_t esz = ctx->precision == UCNS_MIXED ? 4 : 8;
+    const size_t ul = (size_t)ctx->ncell * ctx->nf * ctx->nqm * NV * esz;
+    const size_t g = ctx->smode ? (size_t)ctx->ntile * (ctx->ideg + 2) * NV * 8 * ctx->ts * esz : 0;
+    const size_t fl = ctx->smode ? (size_t)ctx->ncell * sizeof(int) : 0;
+    const size_t per = ul + g + fl;
+    long want = nstage;
+    if (const char *e = std::getenv("UCNS_ADJ_KEEP")) want = std::min(want, std::max(0L, std::atol(e)));
+    double gib = 64.0;
+    if (const char *e = std::getenv("UCNS_ADJ_KEEP_GB")) gib = std::atof(e);
+    size_t budget = gib > 0 ? (size_t)(gib * (double)(1ull << 30)) : 0;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        const size_t have = fr + ctx->keep_ul.size() * per, margin = 4ull << 30;
+        budget = std::min(budget, have > margin ? have - margin : 0);
+    }
+    want = std::min<long>(want, (long)(budget / per));
+    while ((long)ctx->keep_ul.size() < want) {
+        void *a = nullptr, *b = nullptr;
+        int *c = nullptr;
+        if (hipMalloc(&a, ul) != hipSuccess) break;
+        if (ctx->smode && (hipMalloc(&b, g) != hipSuccess || hipMalloc((void **)&c, fl) != hipSuccess)) {
+            (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            break;
+        }
+        ctx->keep_ul.push_back(a);
+        ctx->keep_g.push_back(b);
+        ctx->keep_f.push_back(c);
+    }
+    (void)hipGetLastError();  // a refused allocation only lowers the count
+    return (int)std::min<long>(want, (long)ctx->keep_ul.size());
+}
+
+// Point the reconstruction's outputs (face values; stencil coefficients and
+// flags in row-stream mode) at kept stage b, and back to the context's own
+// buffers on destruction.
+struct KeptStage {
+    ucns_ctx *ctx;
+    void *ul, *g;
+    int *f;
+    KeptStage(ucns_ctx *c, int b) : ctx(c), ul(c->uleft), g(c->adj_g), f(c->adj_flags) {
+        ctx->uleft = ctx->keep_ul[b];
+        if (ctx->smode) {
+            ctx->adj_g = ctx->keep_g[b];
+            ctx->adj_flags = ctx->keep_f[b];
+        }
+    }
+    ~KeptStage() {
+        ctx->uleft = ul;
+        ctx->adj_g = g;
+        ctx->adj_flags = f;
+    }
+};
+
+// one SSP-RK3 step forward from ctx->U with the stage states stored; stages
+// 3 it + stage - 1 >= keep0 also keep their reconstruction
+int rk3_step_ckpt(ucns_ctx *ctx, int it, const double *dt, unsigned long long *next, double cfl, int keep0) {
     const size_t n = (size_t)ctx->ncell * NV;
     for (int stage = 1; stage <= 3; ++stage) {
-        HIP_TRY(hipMemcpyAsync(ctx->ckpt + ((size_t)3 * it + stage - 1) * n, ctx->U, n * sizeof(double),
-                               hipMemcpyDeviceToDevice, ctx->stream));
-        int rc = residual_kernels(ctx);
+        const int q = 3 * it + stage - 1;
+        HIP_TRY(hipMemcpyAsync(ctx->ckpt + (size_t)q * n, ctx->U, n * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        int rc;
+        if (q >= keep0) {
+            KeptStage k(ctx, q - keep0);
+            ctx->adj_g_on = ctx->smode != 0;
+            rc = residual_kernels(ctx);
+            ctx->adj_g_on = false;
+        } else {
+            rc = residual_kernels(ctx);
+        }
         if (rc) return rc;
         if (stage == 3 && next)
             hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, ctx->stream, next,
@@ -3488,8 +3569,12 @@ int rk3_step_ckpt(ucns_ctx *ctx, int it, const double *dt, unsigned long long *n
 // In row-stream mode the scatter of a stage's reconstruction product rides on
 // the forward reconstruction of the next stage reversed, so each reversed
 // stage streams the pseudo-inverses once.
+//
+// Stages q = 3 it + stage - 1 >= keep0 kept their forward reconstruction, so
+// their products start from it and only the other stages run the
+// reconstruction again.
 template <typename R>
-int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
+int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar, int keep0) {
     const size_t n = (size_t)ctx->ncell * NV;
     const int blocks = (int)std::min<size_t>(8192, (n + 255) / 256);
     double *const ustate = ctx->U;
@@ -3519,7 +3604,11 @@ int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
             const double *y = stage == 3 ? a : x;
             hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, y, ctx->ovol, dt,
                                cst[stage], w);
+            const int q = 3 * it + stage - 1;
             if (pipe) {
+                if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
+            } else if (q >= keep0) {
+                KeptStage k(ctx, q - keep0);
                 if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
             } else {
                 if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) break;
@@ -3543,9 +3632,9 @@ int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
     return rc;
 }
 
-int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar) {
-    return ctx->precision == UCNS_MIXED ? rk3_reverse_t<float>(ctx, nsteps, abar, bcbar)
-                                        : rk3_reverse_t<double>(ctx, nsteps, abar, bcbar);
+int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar, int keep0) {
+    return ctx->precision == UCNS_MIXED ? rk3_reverse_t<float>(ctx, nsteps, abar, bcbar, keep0)
+                                        : rk3_reverse_t<double>(ctx, nsteps, abar, bcbar, keep0);
 }
 
 int state_io(ucns_ctx *ctx, const double *src, double *dst, int32_t layout, hipMemcpyKind kind, bool to_dev) {
@@ -4269,6 +4358,7 @@ static int rk3_adjoint_impl(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt
         if ((rc = dalloc(ctx, &ctx->dts, (size_t)nsteps))) return rc;
         ctx->ckpt_steps = nsteps;
     }
+    const int keep0 = 3 * nsteps - adj_keep(ctx, 3 * nsteps);
     const double t0 = t_io ? *t_io : 0.0;
     HIP_TRY(hipMemcpyAsync(ctx->t, &t0, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     const bool fixed = dt_fixed > 0;
@@ -4279,12 +4369,12 @@ static int rk3_adjoint_impl(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt
         const double *dt = fixed ? ctx->dtfix : (const double *)&ctx->dtv[cur];
         unsigned long long *next = fixed ? nullptr : &ctx->dtv[1 - cur];
         HIP_TRY(hipMemcpyAsync(ctx->dts + it, dt, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        if ((rc = rk3_step_ckpt(ctx, it, dt, next, cfl))) return rc;
+        if ((rc = rk3_step_ckpt(ctx, it, dt, next, cfl, keep0))) return rc;
         HIP_TRY(hipGetLastError());
         if (!fixed) cur = 1 - cur;
     }
     zero_dev(ctx, ctx->adj_bc, 2 * NV);
-    if ((rc = rk3_reverse(ctx, nsteps, a_dev, ctx->adj_bc))) return rc;
+    if ((rc = rk3_reverse(ctx, nsteps, a_dev, ctx->adj_bc, keep0))) return rc;
     HIP_TRY(hipGetLastError());
     double t1 = t0;
     HIP_TRY(hipMemcpyAsync(&t1, ctx->t, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
