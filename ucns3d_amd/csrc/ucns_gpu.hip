@@ -2025,15 +2025,15 @@ struct VjpParams {
     const int32_t *cnb;    // [ncell][nf]
     const double *ubf;     // [ncell][nf][nqm][5] cotangents of the face-point states
     double *ubar;          // [ncell][5] accumulated
-    const R *gload;        // the forward's stencil coefficients, indicators and weights
-                           // [ntile][ideg*5 + 10][8 ts], or null: recompute them
-    const int *fload;      // with gload: the forward's per-cell flags (1 MUSCL, 2 first order)
+    const R *gload;        // row stream: the forward's stencil coefficients, indicators and
+                           // weights [ntile][ideg*5 + 10][8 ts]
+    const int *fload;      // row stream: the forward's per-cell flags (1 MUSCL, 2 first order)
     // the tile's distinct stencil members: member (j, pos) of tile t scatters
     // to LDS row slot[t][j][pos] (UCNS_NOSLOT: straight to HBM), and row r of
     // tile t belongs to cell uq[t][r], r < min(ucount[t], UCNS_TILE_UQ)
     const uint16_t *slot;
     const int32_t *uq, *ucount;
-    double *gbout;         // non-null: write each stencil's g_bar [ntile][ideg*5][8 ts] instead of scattering
+    double *gbout;         // row stream: each stencil's g_bar [ntile][ideg*5][8 ts], for the scatter
     const int64_t *toff;   // first row of each tile's member table in tab
     double *tab;           // [rows][5] the member tables, summed per cell by k_adj_gather
 };
@@ -2114,10 +2114,10 @@ __device__ inline R pw(R x, R x2, R x3, int e) {
     return e == 0 ? R(1) : e == 1 ? x : e == 2 ? x2 : x3;
 }
 
-// GOUT: the cotangents g_bar of every stencil go to p.gbout element by
-// element, for the row stream's scatter (no g_bar array in registers, no
-// scatter here); otherwise this kernel scatters them itself.
-template <typename R, int IDEG, int IMAX, int NQL, bool GOUT>
+// Reconstruction product on the tiled and per-cell layouts: the forward is
+// repeated with its own arithmetic (the same bits, so the same decisions),
+// then g_bar of each stencil is scattered through its pseudo-inverse rows.
+template <typename R, int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
     constexpr int KL = (IDEG + 7) / 8;    // coefficients per lane in the face-point pass
@@ -2125,12 +2125,12 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     __shared__ R sGh[8][IDEG * NV];       // combined coefficients sum_s omega_s G_s
     __shared__ R sG0[8][IDEG * NV];       // central-stencil coefficients
     __shared__ R sB[8][NP];               // the tile's indicator matrices
-    __shared__ double sAcc[GOUT ? 1 : UCNS_TILE_UQ][NV];  // scatter to the tile's distinct members
+    __shared__ double sAcc[UCNS_TILE_UQ][NV];  // scatter to the tile's distinct members
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
     if (tile >= p.ntile) return;
-    const int nuq = (!GOUT && p.slot) ? min(p.ucount[tile], UCNS_TILE_UQ) : 0;
+    const int nuq = p.slot ? min(p.ucount[tile], UCNS_TILE_UQ) : 0;
     for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
     if (p.iweno == 1) {
         const int nvc = min(8, p.ncell - tile * 8);
@@ -2212,13 +2212,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     for (int k = 0; k < IDEG; ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) g[k][v] = R(0);
-    if (act && p.gload) {
-        const R *gs = p.gload + (size_t)tile * (IDEG + 2) * NV * p.lsq.tp + pos;
-#pragma unroll
-        for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) g[k][v] = gs[(size_t)(k * NV + v) * p.lsq.tp];
-    } else if (act) {
+    if (act) {
 #pragma unroll 1
         for (int j = 0; j < IMAX; ++j) {
             const int n = p.lsq.member(tile, cell, s, j, pos);
@@ -2243,100 +2237,80 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
             for (int v = 0; v < NV; ++v) sG0[lc][k * NV + v] = g[k][v];
     }
 
-    // ---- C: indicators, weights and combined coefficients (forward) ----
-    // With the forward's stored indicators, weights and flags (gload) only
-    // the weights' cotangent input obar_s = gh_bar . G_s is formed here;
-    // otherwise the forward is repeated with its arithmetic (the same bits).
+    // ---- C: indicators, weights and combined coefficients (forward, with
+    // its arithmetic: the same bits) ----
     R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
     R om[NV] = {R(0), R(0), R(0), R(0), R(0)};
     double obar[NV] = {0, 0, 0, 0, 0};
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
     bool reduce = cvalid;
-    if (p.gload) {
-        if (p.iweno == 1) {
-            if (act) {
-                const R *bo = p.gload + ((size_t)tile * (IDEG + 2) + IDEG) * NV * p.lsq.tp + pos;
+    if (p.iweno == 1) {
+        if (act) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    beta[v] = bo[(size_t)v * p.lsq.tp];
-                    om[v] = bo[(size_t)(NV + v) * p.lsq.tp];
+            for (int k = 0; k < IDEG; ++k) {
+                R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+                for (int l = 0; l < IDEG; ++l) {
+                    const R bkl = sB[lc][pidx(IDEG, k, l)];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
                 }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
             }
-#pragma unroll
-            for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
         }
-        reduce = cvalid && (p.fload[cs] & 1);
-    } else {
-        if (p.iweno == 1) {
-            if (act) {
+        const R lam = (s == 0) ? (R)p.lwci1 : R(1);
+        const R eps = (R)p.eps;
 #pragma unroll
-                for (int k = 0; k < IDEG; ++k) {
-                    R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
-#pragma unroll
-                    for (int l = 0; l < IDEG; ++l) {
-                        const R bkl = sB[lc][pidx(IDEG, k, l)];
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
-                    }
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
-                }
+        for (int v = 0; v < NV; ++v) {
+            R w;
+            if (sizeof(R) == 8) {
+                const R x = eps + beta[v];
+                const R x2 = x * x;
+                w = act ? lam / (x2 * x2) : R(0);
+            } else {
+                const R x = eps + beta[v];
+                const R xmin = grp_min8(act ? x : (R)INFINITY);
+                const R r = xmin / x;
+                const R r2 = r * r;
+                w = act ? lam * (r2 * r2) : R(0);
             }
-            const R lam = (s == 0) ? (R)p.lwci1 : R(1);
-            const R eps = (R)p.eps;
+            const R sum = grp_sum8(w);
+            om[v] = act ? w / sum : R(0);
+        }
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                R w;
-                if (sizeof(R) == 8) {
-                    const R x = eps + beta[v];
-                    const R x2 = x * x;
-                    w = act ? lam / (x2 * x2) : R(0);
-                } else {
-                    const R x = eps + beta[v];
-                    const R xmin = grp_min8(act ? x : (R)INFINITY);
-                    const R r = xmin / x;
-                    const R r2 = r * r;
-                    w = act ? lam * (r2 * r2) : R(0);
-                }
-                const R sum = grp_sum8(w);
-                om[v] = act ? w / sum : R(0);
+                const R gh = grp_sum8(om[v] * g[k][v]);
+                if (s == 0) sGh[lc][k * NV + v] = gh;
+                obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
             }
-#pragma unroll
-            for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    const R gh = grp_sum8(om[v] * g[k][v]);
-                    if (s == 0) sGh[lc][k * NV + v] = gh;
-                    obar[v] += sGb[lc][k * NV + v] * (double)g[k][v];
-                }
-        }
-        __syncthreads();  // sGb, sGh, sG0
+    }
+    __syncthreads();  // sGb, sGh, sG0
 
-        // ---- D: CHECKSOL (0.85) on the WENO face values (forward) ----
-        if (p.iweno == 1) {
-            int trouble = 0;
+    // ---- D: CHECKSOL (0.85) on the WENO face values (forward) ----
+    if (p.iweno == 1) {
+        int trouble = 0;
 #pragma unroll 1
-            for (int q = s; q < NQL * 8; q += 8) {
-                if (!qvalid(q)) continue;
-                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
-                R phi[IDEG];
-                basis_at<R, IDEG>(x, mb, phi);
-                double uq[NV];
+        for (int q = s; q < NQL * 8; q += 8) {
+            if (!qvalid(q)) continue;
+            R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+            R phi[IDEG];
+            basis_at<R, IDEG>(x, mb, phi);
+            double uq[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    R res = R(0);
+            for (int v = 0; v < NV; ++v) {
+                R res = R(0);
 #pragma unroll
-                    for (int k = 0; k < IDEG; ++k) res += phi[k] * sGh[lc][k * NV + v];
-                    uq[v] = ui[v] + (double)res;
-                }
-                if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+                for (int k = 0; k < IDEG; ++k) res += phi[k] * sGh[lc][k * NV + v];
+                uq[v] = ui[v] + (double)res;
             }
-            const int any_trouble = grp_or8(trouble);
-            reduce = cvalid && (!full || any_trouble);
+            if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
         }
+        const int any_trouble = grp_or8(trouble);
+        reduce = cvalid && (!full || any_trouble);
     }
     __syncthreads();  // sG0
 
@@ -2445,25 +2419,12 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     }
 
     // ---- F: cotangent of each lane's stencil coefficients ----
-    // GOUT writes them as they are formed; otherwise they are kept for G
-    double gb[GOUT ? 1 : IDEG][NV];
-    if constexpr (!GOUT) {
+    double gb[IDEG][NV];
 #pragma unroll
-        for (int k = 0; k < IDEG; ++k)
+    for (int k = 0; k < IDEG; ++k)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
-    }
-    double *go = nullptr;
-    if constexpr (GOUT) {
-        if (cvalid && s < p.lsq.ts) go = p.gbout + (size_t)tile * IDEG * NV * p.lsq.tp + pos;
-    }
-    auto put = [&](int k, int v, double x) {
-        if constexpr (GOUT) {
-            if (go) go[(size_t)(k * NV + v) * p.lsq.tp] = x;
-        } else {
-            gb[k][v] = x;
-        }
-    };
+        for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
+    auto put = [&](int k, int v, double x) { gb[k][v] = x; };
     double ubi[NV];  // extra cotangent of U_i from the limiter
     bool scatter = false;
 #pragma unroll
@@ -2531,11 +2492,10 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
 #pragma unroll
             for (int k = 0; k < IDEG; ++k) put(k, v, 0.0);
     }
-    if constexpr (GOUT) scatter = false;  // the row stream scatters
 
     // ---- G: dU_bar_j = P_j^T g_bar to the members, minus the same to U_i ----
     double acc[NV] = {0, 0, 0, 0, 0};
-    if constexpr (!GOUT) if (scatter) {
+    if (scatter) {
         // member j+1's row and index are in flight while member j is scattered
         int kst;
         int n = p.lsq.member(tile, cell, s, 0, pos);
@@ -2585,6 +2545,300 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
         __syncthreads();
         double *tt = p.tab + (size_t)p.toff[tile] * NV;
         for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+    }
+}
+
+// Reconstruction product on the row-stream layouts (TS stencils per cell,
+// TP = 8 TS columns per tile row).  The forward kept each stencil's
+// coefficients G_s, indicators and weights ([tile][ideg*5 + 10][TP], gload)
+// and each cell's flags (fload: 1 MUSCL, 2 first order), so nothing of the
+// forward is repeated: the flags pick one of three paths per cell, and g_bar
+// of every stencil goes to p.gbout ([tile][ideg*5][TP]) for the row-stream
+// scatter.  On the WENO path every step is per variable (indicators and
+// weights are), so a lane holds one variable's column of G_s at a time; B G_s
+// runs over the packed triangle of B once per variable.
+template <typename R, int IDEG, int NQL, int TS>
+__global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
+    constexpr int NP = IDEG * (IDEG + 1) / 2;
+    constexpr int KL = (IDEG + 7) / 8;  // coefficients per lane in the face-point pass
+    constexpr int QU = 4;               // face points in flight per lane
+    constexpr int TP = 8 * TS;
+    constexpr unsigned KS = NV * TP;    // stride of one coefficient row (elements)
+    __shared__ double sGb[8][IDEG * NV];  // cotangent of the combined coefficients
+    __shared__ R sB[8][NP];               // the tile's indicator matrices
+    const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
+    const int b = blockIdx.x;
+    const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
+    if (tile >= p.ntile) return;
+    {
+        const int nvc = min(8, p.ncell - tile * 8);
+        const R *Bt = p.B + (size_t)tile * 8 * NP;
+        for (int e = lane; e < nvc * NP; e += WAVE) (&sB[0][0])[e] = Bt[e];
+    }
+    const int cell = tile * 8 + lc;
+    const bool cvalid = cell < p.ncell;
+    const int cs = cvalid ? cell : 0;
+    const int info = cvalid ? p.cinfo[cell] : 0;
+    const int fl = cvalid ? p.fload[cell] : 0;
+    const int ifca = ci_ifca(info), admis = ci_admis(info);
+    const bool act = cvalid && s < admis;
+    const int nslot = cvalid ? ifca * p.nqm : 0;
+    const R *qpc = p.qp + (size_t)cs * p.nf * p.nqm * 3;
+    const R *mb = p.mbar + (size_t)cs * IDEG;
+    const double *ubc = p.ubf + (size_t)cs * p.nf * p.nqm * NV;
+    auto qvalid = [&](int q) {
+        const int f = q / p.nqm, qq = q - f * p.nqm;
+        return q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
+    };
+
+    // ---- cotangent of the combined coefficients, sum_q phi_q ubar_q (lane
+    // s of a cell takes coefficients s, s+8, s+16 over all its face points),
+    // and of the cell average through a_q = U_i + ULEFT(q) ----
+    double usum[NV] = {0, 0, 0, 0, 0};
+    {
+        double ab[KL][NV];
+        R mk[KL];
+        int ex[KL][3];
+#pragma unroll
+        for (int t = 0; t < KL; ++t) {
+            const int k = s + 8 * t;
+            mk[t] = k < IDEG ? mb[k] : R(0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) ex[t][d] = k < IDEG ? c_mono[k][d] : 0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ab[t][v] = 0.0;
+        }
+#pragma unroll 1
+        for (int q0 = 0; q0 < nslot; q0 += QU) {
+            // QU points' loads are issued before any is used (indices stay in
+            // the cell's block: q < nslot <= nf nqm)
+            R xq[QU][3];
+            double ub[QU][NV];
+#pragma unroll
+            for (int u = 0; u < QU; ++u) {
+                const int q = min(q0 + u, nslot - 1);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) xq[u][d] = qpc[q * 3 + d];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ub[u][v] = ubc[q * NV + v];
+            }
+#pragma unroll
+            for (int u = 0; u < QU; ++u) {
+                const bool ok = q0 + u < nslot && qvalid(q0 + u);
+                const R x0 = xq[u][0], x1 = xq[u][1], x2 = xq[u][2];
+                const R x0s = x0 * x0, x1s = x1 * x1, x2s = x2 * x2;
+                const R x0c = x0s * x0, x1c = x1s * x1, x2c = x2s * x2;
+                double uv[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    uv[v] = ok ? ub[u][v] : 0.0;
+                    usum[v] += uv[v];
+                }
+#pragma unroll
+                for (int t = 0; t < KL; ++t) {
+                    const double ph = (double)(pw(x0, x0s, x0c, ex[t][0]) * pw(x1, x1s, x1c, ex[t][1]) *
+                                                   pw(x2, x2s, x2c, ex[t][2]) - mk[t]);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) ab[t][v] += ph * uv[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KL; ++t) {
+            const int k = s + 8 * t;
+            if (k < IDEG) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sGb[lc][k * NV + v] = ab[t][v];
+            }
+        }
+    }
+    __syncthreads();  // sGb, sB
+
+    const bool reduce = cvalid && (fl & 1), first = (fl & 2) != 0;
+    // the tile's blocks (uniform) and this lane's column in them; lanes past
+    // the cell's stencils read stencil 0's column and write nothing
+    const R *gt = p.gload + (size_t)tile * (IDEG + 2) * KS;
+    double *gbt = p.gbout + (size_t)tile * IDEG * KS;
+    const unsigned c0 = lc * TS, col = c0 + (s < TS ? s : 0);
+    const bool wr = cvalid && s < TS;
+    // a MUSCL cell has no use for its indicator matrix: its LDS row takes the
+    // central stencil's coefficients
+    static_assert(IDEG * NV <= NP, "central coefficients exceed the indicator row");
+    R *g0 = &sB[lc][0];
+    if (reduce && !first)
+        for (int e = s; e < IDEG * NV; e += 8) g0[e] = gt[c0 + (unsigned)e * TP];
+    __syncthreads();  // g0
+    double ubi[NV] = {0, 0, 0, 0, 0};  // extra cotangent of U_i from the limiter
+    if (cvalid && !reduce) {
+        // WENO: omega_s = lam_s (eps+beta_s)^-4 / sum_t (...), so with
+        // obar_s = gh_bar . G_s:  beta_bar_s = -4 omega_s/(eps+beta_s)
+        // (obar_s - sum_t omega_t obar_t),  g_bar_s = omega_s gh_bar + 2 beta_bar_s B G_s
+#pragma unroll 1
+        for (int v = 0; v < NV; ++v) {
+            // B is re-read from LDS per variable: kept in registers across the
+            // loop it would take NP of them
+            asm volatile("" ::: "memory");
+            const unsigned cv = col + (unsigned)v * TP;
+            R gv[IDEG];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) gv[k] = act ? gt[cv + k * KS] : R(0);
+            const double beta = act ? (double)gt[cv + IDEG * KS] : 0.0;
+            const double om = act ? (double)gt[cv + IDEG * KS + NV * TP] : 0.0;
+            double obar = 0.0;
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) obar += sGb[lc][k * NV + v] * (double)gv[k];
+            const double so = grp_sum8(act ? om * obar : 0.0);
+            const double bb = act ? -4.0 * om / ((double)p.eps + beta) * (obar - so) : 0.0;
+            double hg[IDEG];
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) hg[k] = (double)sB[lc][pidx(IDEG, k, k)] * (double)gv[k];
+            // one row of B in registers at a time: each row's products are
+            // pinned before the next row is read (else the compiler defers
+            // them to the stores and keeps all of B live)
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int l = k + 1; l < IDEG; ++l) {
+                    const double bkl = (double)sB[lc][pidx(IDEG, k, l)];
+                    hg[k] += bkl * (double)gv[l];
+                    hg[l] += bkl * (double)gv[k];
+                }
+#pragma unroll
+                for (int l = k; l < IDEG; ++l) asm volatile("" : "+v"(hg[l]));
+            }
+            if (wr) {
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k)
+                    gbt[cv + k * KS] = act ? om * sGb[lc][k * NV + v] + 2.0 * bb * hg[k] : 0.0;
+            }
+        }
+    } else if (reduce && !first) {
+        // MUSCL + Barth with the forward's arithmetic, for the limiter's
+        // active arguments: its arg-min face point and bound neighbour
+        double ui[NV], umin[NV], umax[NV];
+        int imin[NV], imax_[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            ui[v] = p.U[(size_t)cell * NV + v];
+            umin[v] = umax[v] = ui[v];
+            imin[v] = imax_[v] = cell;
+        }
+        for (int f = 0; f < ifca; ++f) {
+            const int n = p.cnb[(size_t)cell * p.nf + f];
+            if (n < 0) continue;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double un = p.U[(size_t)n * NV + v];
+                if (un < umin[v]) imin[v] = n;
+                if (un > umax[v]) imax_[v] = n;
+                umin[v] = fmin(umin[v], un);
+                umax[v] = fmax(umax[v], un);
+            }
+        }
+        double lmin[NV];
+        int lq[NV], lw[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            lmin[v] = 1.0e16;
+            lq[v] = -1;
+            lw[v] = 0;
+        }
+        R res0[NQL][NV];
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) res0[r][v] = R(0);
+            if (qvalid(q)) {
+                R x[3] = {qpc[q * 3], qpc[q * 3 + 1], qpc[q * 3 + 2]};
+                R phi[IDEG];
+                basis_at<R, IDEG>(x, mb, phi);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    R res = R(0);
+#pragma unroll
+                    for (int k = 0; k < IDEG; ++k) res += phi[k] * g0[k * NV + v];
+                    res0[r][v] = res;
+                    const double d2 = (double)res;
+                    double psi;
+                    int wsel = 0;
+                    if (fabs(d2) <= 0.0)
+                        psi = 1.0;
+                    else if (d2 > 0.0)
+                        psi = fmin(1.0, (umax[v] - ui[v]) / d2);
+                    else {
+                        psi = fmin(1.0, (umin[v] - ui[v]) / d2);
+                        wsel = 1;
+                    }
+                    if (psi < lmin[v]) {
+                        lmin[v] = psi;
+                        lq[v] = q;
+                        lw[v] = wsel;
+                    }
+                }
+            }
+        }
+        // ULEFT(q) = slope * d2_q: slope_bar = sum_q ubar_q d2_q; through an
+        // active limiter slope = (u_ext - u_i) / d2_{q*}
+        double sb[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            if (qvalid(q)) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sb[v] += ubc[q * NV + v] * (double)res0[r][v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double slope = grp_min8(lmin[v]);
+            // lowest lane of the group holding the minimum
+            int cand = (lmin[v] == slope) ? s : 8;
+            cand = min(cand, __shfl_xor(cand, 1, WAVE));
+            cand = min(cand, __shfl_xor(cand, 2, WAVE));
+            cand = min(cand, __shfl_xor(cand, 4, WAVE));
+            const int src = lc * 8 + cand;
+            const int qs = lane_bcast(lq[v], src);
+            const int wsel = lane_bcast(lw[v], src);
+            const int ext = wsel ? imin[v] : imax_[v];
+            // the unlimited value at the arg-min point
+            double mine = 0.0;
+#pragma unroll
+            for (int r = 0; r < NQL; ++r)
+                if (s + 8 * r == qs) mine = (double)res0[r][v];
+            const double d2s = grp_sum8(mine);
+            const double sbv = grp_sum8(sb[v]);
+            const bool on = slope < 1.0 && qs >= 0;
+            const double e = on ? sbv / d2s : 0.0;
+            const double cq = on ? e * slope : 0.0;
+            ubi[v] = -e;
+            if (s == 0 && on) atomic_add_f64(&p.ubar[(size_t)ext * NV + v], e);
+            if (wr) {
+                // only the central stencil (lane 0) carries a coefficient cotangent
+                R phi[IDEG];
+                const bool corr = s == 0 && cq != 0.0;
+                if (corr) {
+                    R x[3] = {qpc[qs * 3], qpc[qs * 3 + 1], qpc[qs * 3 + 2]};
+                    basis_at<R, IDEG>(x, mb, phi);
+                }
+                const unsigned cv = col + (unsigned)v * TP;
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k)
+                    gbt[cv + k * KS] = s == 0 ? slope * sGb[lc][k * NV + v] - (corr ? cq * (double)phi[k] : 0.0)
+                                              : 0.0;
+            }
+        }
+    } else if (wr) {
+        // first-order cells: no coefficient cotangent
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) gbt[col + (unsigned)v * TP + k * KS] = 0.0;
+    }
+    if (s == 0 && cvalid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) atomic_add_f64(&p.ubar[(size_t)cell * NV + v], usum[v] + ubi[v]);
     }
 }
 
@@ -3421,28 +3675,35 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     const bool q3 = nfq <= 24;
     tstart(ctx, 0);
     if (ctx->smode) {
-        // row-stream shapes: the scatter follows on the row stream
-        if (ctx->ideg == 19 && q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, true>), grid, block, 0, ctx->stream, vp);
+        // row-stream shapes: from the forward's stored coefficients; the
+        // scatter follows on the row stream
+        if (!vp.gload || !vp.fload) {
+            ctx->err = "the row-stream product needs the forward's stored coefficients";
+            return UCNS_EUNSUPPORTED;
+        }
+        if (ctx->ideg == 19 && ctx->smode == 5)
+            hipLaunchKernelGGL((k_recon_vjp_s<R, 19, 3, 5>), grid, block, 0, ctx->stream, vp);
         else if (ctx->ideg == 19)
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7, true>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp_s<R, 19, 7, 7>), grid, block, 0, ctx->stream, vp);
+        else if (ctx->smode == 5)
+            hipLaunchKernelGGL((k_recon_vjp_s<R, 9, 3, 5>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3, true>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp_s<R, 9, 3, 7>), grid, block, 0, ctx->stream, vp);
     } else if (ctx->ideg == 19 && ctx->imax == 39) {
         if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7>), grid, block, 0, ctx->stream, vp);
     } else if (ctx->ideg == 9 && ctx->imax == 19) {
         if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7>), grid, block, 0, ctx->stream, vp);
     } else if (ctx->ideg == 3 && ctx->imax == 8) {
         if (q3)
-            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3>), grid, block, 0, ctx->stream, vp);
         else
-            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7, false>), grid, block, 0, ctx->stream, vp);
+            hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7>), grid, block, 0, ctx->stream, vp);
     } else {
         ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
         return UCNS_EUNSUPPORTED;
