@@ -478,6 +478,7 @@ struct ReconWParams {
     int ncell, ntile, tiles_per_xcd, ts, nf, nqm, qpq, qpt;
     double gamma, lwci1, eps;
     const double *U;
+    const double *U48;     // [ncell][6] the state in 48-byte records (row-stream gathers)
     const int32_t *nbr;    // [ntile][imax][8*ts]
     const R *P;            // [ntile][imax][ideg][8*ts]
     const R *B;            // [ncell][ideg*(ideg+1)/2]
@@ -562,9 +563,6 @@ __device__ __forceinline__ unsigned lds_addr(const void *p) {
 __device__ __forceinline__ void lds_load16(const void *g, unsigned lds_base) {
     asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "{m0}"(lds_base) : "memory");
 }
-__device__ __forceinline__ void lds_load4(const void *g, unsigned lds_base) {
-    asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "{m0}"(lds_base) : "memory");
-}
 
 // one 1 KiB piece: lane l moves bytes [16 l, 16 l + 16) of the chunk when
 // they lie below `valid`
@@ -596,20 +594,23 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     }
     __syncthreads();
 
-    // ---- smoothness indicators beta = g^T (B g) (src/reconstruct_p.f90:995-1004) ----
+    // ---- smoothness indicators beta = g^T (B g) (src/reconstruct_p.f90:995-1004),
+    // over the packed triangle of the symmetric B:
+    //   beta = sum_k g_k (B_kk g_k + 2 sum_{l>k} B_kl g_l) ----
     R beta[NV] = {R(0), R(0), R(0), R(0), R(0)};
     if (act) {
 #pragma unroll
         for (int k = 0; k < IDEG; ++k) {
             R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
-            for (int l = 0; l < IDEG; ++l) {
+            for (int l = k + 1; l < IDEG; ++l) {
                 const R bkl = sB[lc][pidx(IDEG, k, l)];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
             }
+            const R bkk = sB[lc][pidx(IDEG, k, k)];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * bg[v];
+            for (int v = 0; v < NV; ++v) beta[v] += g[k][v] * (bkk * g[k][v] + R(2) * bg[v]);
         }
     }
     // ---- weights (src/reconstruct_p.f90:1031-1049) and the weighted sum ----
@@ -947,10 +948,11 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
 // member indices of row j + DEPTH ([8 ts] int32), so the indices of a row
 // are in LDS one ring turn before its states are fetched.
 //
-// LDS slot of a row: the row block as streamed, then the 40-byte states of
-// the lanes' members as four lane-ordered pieces (16, 16, 4, 4 bytes); the
-// two 4-byte pieces sit in the slack of the row's last 1 KiB chunk when it
-// has room.  After the loop the slots hold the tile's indicators and the
+// LDS slot of a row: the row block as streamed, then the member states of
+// its TP columns as 48-byte records (5 doubles and a pad, gathered from a
+// padded copy of the state in 16-byte pieces: consecutive lanes take
+// consecutive pieces, so a record is one or two cache lines and a row is
+// NSI gathers).  After the loop the slots hold the tile's indicators and the
 // central coefficients.
 constexpr int stream_depth(bool mixed, int ts) { return ts == 5 ? (mixed ? 3 : 4) : (mixed ? 2 : 3); }
 
@@ -962,12 +964,12 @@ struct StreamShape {
     static constexpr int RD = PB + TP * 4;                 // row bytes in HBM
     static constexpr int HD = DEPTH * TP * 4;              // tile header bytes
     static constexpr int NCH = (RD + 1023) / 1024;         // 1 KiB loads per row
-    static constexpr bool CD_IN_P = NCH * 1024 - RD >= 512;
-    static constexpr int UA = NCH * 1024, UB = UA + 1024;
-    static constexpr int UC = CD_IN_P ? RD : UB + 1024;
-    static constexpr int UD = UC + 256;
-    static constexpr int SB = CD_IN_P ? UB + 1024 : UD + 256;  // slot bytes
-    static constexpr int PER = NCH + 4;                    // loads per row
+    // the row's member states: the TP columns' 48-byte records, three
+    // 16-byte pieces each, gathered by NSI loads of 64 pieces
+    static constexpr int NSI = (3 * TP + 63) / 64;
+    static constexpr int UA = NCH * 1024;
+    static constexpr int SB = UA + NSI * 1024;             // slot bytes
+    static constexpr int PER = NCH + NSI;                  // loads per row
     static constexpr int BBYTES = 8 * NP * (int)sizeof(R);
     static constexpr int NBC = (BBYTES + 1023) / 1024;
     static constexpr int NB = (BBYTES + SB - 1) / SB;
@@ -1048,15 +1050,23 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
 #pragma unroll
         for (int c = 0; c < SS::NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, SS::RD - c * 1024);
     };
-    // lane 0 of cell 0 of every tile is active, so these four loads are issued in every wave
-    auto issue_states = [&](int j, int n) {
-        char *dst = ring + (j % DEPTH) * SS::SB;
-        if (act) {
-            const char *src = (const char *)(p.U + (size_t)n * NV);
-            lds_load16(src, lds_addr(dst + SS::UA));
-            lds_load16(src + 16, lds_addr(dst + SS::UB));
-            lds_load4(src + 32, lds_addr(dst + SS::UC));
-            lds_load4(src + 36, lds_addr(dst + SS::UD));
+    // piece q = lane + 64 c of a row is bytes [16 (q % 3), +16) of column
+    // q / 3's record; pieces past the TP columns re-read cell 0.  Every lane
+    // issues every gather, so each row adds exactly NSI loads to the count.
+    auto piece_cells = [&](const int *cols, int (&ix)[SS::NSI]) {
+#pragma unroll
+        for (int c = 0; c < SS::NSI; ++c) {
+            const int q = lane + 64 * c;
+            ix[c] = q < 3 * TP ? cols[q / 3] : 0;
+        }
+    };
+    auto issue_states = [&](int j, const int (&ix)[SS::NSI]) {
+        char *dst = ring + (j % DEPTH) * SS::SB + SS::UA;
+#pragma unroll
+        for (int c = 0; c < SS::NSI; ++c) {
+            const int q = lane + 64 * c;
+            const int part = q < 3 * TP ? q % 3 : 0;
+            lds_load16(p.U48 + (size_t)ix[c] * 6 + 2 * part, lds_addr(dst + c * 1024));
         }
     };
     // every chunk is issued, also past the last cell's block (lane 0 then
@@ -1080,13 +1090,13 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     dma_chunk(T, ring + DEPTH * SS::SB, lane, SS::HD);
     issue_row(0);
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(SS::NCH) : "memory");  // header landed
-    qn[0] = hdr[rl];
-    issue_states(0, qn[0]);
 #pragma unroll
-    for (int r = 1; r < DEPTH; ++r) {
-        issue_row(r);
+    for (int r = 0; r < DEPTH; ++r) {
+        if (r > 0) issue_row(r);
         qn[r] = hdr[r * TP + rl];
-        issue_states(r, qn[r]);
+        int ix[SS::NSI];
+        piece_cells(hdr + r * TP, ix);
+        issue_states(r, ix);
     }
 
     // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497) ----
@@ -1098,11 +1108,10 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     auto consume = [&](int i) {
         if (act) {
             const char *slot = ring + (i % DEPTH) * SS::SB;
-            const double2 a = *(const double2 *)(slot + SS::UA + lane * 16);
-            const double2 c = *(const double2 *)(slot + SS::UB + lane * 16);
-            const unsigned lo = *(const unsigned *)(slot + SS::UC + lane * 4);
-            const unsigned hi = *(const unsigned *)(slot + SS::UD + lane * 4);
-            const double u4 = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+            const double *st = (const double *)(slot + SS::UA + rl * 48);
+            const double2 a = *(const double2 *)st;
+            const double2 c = *(const double2 *)(st + 2);
+            const double u4 = st[4];
             const R du[NV] = {(R)(a.x - ui[0]), (R)(a.y - ui[1]), (R)(c.x - ui[2]), (R)(c.y - ui[3]),
                               (R)(u4 - ui[4])};
             const R *row = (const R *)slot + rl;
@@ -1142,10 +1151,14 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     for (int i = 0; i < IMAX - DEPTH; ++i) {
         asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * SS::PER) : "memory");
         consume(i);
-        const int nn = *(const int *)(ring + (i % DEPTH) * SS::SB + SS::PB + rl * 4);  // members of row i+DEPTH
+        // members of row i+DEPTH, read before that row overwrites this slot
+        const int *cols = (const int *)(ring + (i % DEPTH) * SS::SB + SS::PB);
+        const int nn = cols[rl];
+        int ix[SS::NSI];
+        piece_cells(cols, ix);
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         issue_row(i + DEPTH);
-        issue_states(i + DEPTH, nn);
+        issue_states(i + DEPTH, ix);
         if constexpr (SCAT) qn[DEPTH - 1] = nn;
     }
     // tail: the last DEPTH rows; the indicators follow into the first free slots
@@ -2968,6 +2981,14 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     }
 }
 
+// the state [ncell][5] as 48-byte records [ncell][6] (the sixth value zero)
+__global__ __launch_bounds__(256) void k_pack48(size_t n48, const double *U, double *U48) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n48) return;
+    const size_t i = e / 6, v = e - i * 6;
+    U48[e] = v < NV ? U[i * NV + v] : 0.0;
+}
+
 __global__ void k_zero_f64(double *p, size_t n) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         p[e] = 0.0;
@@ -3064,6 +3085,7 @@ struct ucns_ctx {
     int32_t *nbr = nullptr;
     void *P = nullptr, *B = nullptr, *mbar = nullptr, *qp = nullptr, *uleft = nullptr;
     unsigned char *rows = nullptr;  // row stream of the DMA reconstruction kernels
+    double *u48 = nullptr;          // [ncell][6] the state in 48-byte records, for their gathers
     long long tile_bytes = 0;
     // state (device)
     double *U = nullptr, *U2 = nullptr, *fflux = nullptr, *rhs = nullptr, *t = nullptr, *dtfix = nullptr;
@@ -3167,6 +3189,13 @@ int launch_recon_t(ucns_ctx *ctx) {
         const dim3 grid(w.tiles_per_xcd * NXCD), block(WAVE);
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
+        if (ctx->smode) {
+            // the row-stream kernels gather the member states as 48-byte records
+            const size_t n48 = (size_t)ctx->ncell * 6;
+            hipLaunchKernelGGL(k_pack48, dim3((unsigned)((n48 + 255) / 256)), dim3(256), 0, ctx->stream, n48, ctx->U,
+                               ctx->u48);
+            w.U48 = ctx->u48;
+        }
         w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
         w.fstore = ctx->adj_g_on ? ctx->adj_flags : nullptr;
         if (ctx->smode && ctx->sv_ubar) {
@@ -3549,10 +3578,6 @@ void zero_dev(ucns_ctx *ctx, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(std::max(blocks, 1)), dim3(256), 0, ctx->stream, p, n);
 }
 
-// forward reconstruction of the current state for a residual product: the
-// tiled kernels also store every stencil's coefficients; with gb non-null
-// the row-stream kernel scatters that earlier product's g_bar into ubar on
-// the same pass
 // P^T g_bar of the last products (adj_gb) onto the row stream, into ubar
 template <typename R>
 void launch_scatter_t(ucns_ctx *ctx, double *ubar) {
@@ -3954,6 +3979,7 @@ void release_prestore(ucns_ctx *ctx) {
     dfree(ctx->nbr);
     dfree(ctx->P);
     dfree(ctx->rows);
+    dfree(ctx->u48);
     dfree(ctx->B);
     dfree(ctx->mbar);
     dfree(ctx->qp);
@@ -4231,6 +4257,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     if (ctx->smode) {
         ctx->tile_bytes = sHD + (long long)IM * sRD;
         if ((rc = dalloc(ctx, &ctx->rows, (size_t)ctx->ntile * ctx->tile_bytes))) return rc;
+        if ((rc = dalloc(ctx, &ctx->u48, (size_t)nc * 6))) return rc;
         // member-index rows: row r holds members j = r (zero past imax)
         const size_t rows = (size_t)sD + IM;
         std::vector<int32_t> idx((size_t)ctx->ntile * rows * sTP, 0);
