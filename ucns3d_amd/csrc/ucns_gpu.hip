@@ -954,7 +954,7 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
 // consecutive pieces, so a record is one or two cache lines and a row is
 // NSI gathers).  After the loop the slots hold the tile's indicators and the
 // central coefficients.
-constexpr int stream_depth(bool mixed, int ts) { return ts == 5 ? (mixed ? 3 : 4) : (mixed ? 2 : 3); }
+constexpr int stream_depth(bool mixed, int ts) { return ts == 5 ? 4 : (mixed ? 2 : 3); }
 
 template <typename R, int IDEG, int IMAX, int TS, int DEPTH>
 struct StreamShape {
