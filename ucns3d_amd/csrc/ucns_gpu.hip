@@ -1846,12 +1846,35 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
     const R *ulL = p.uleft + ((size_t)cL * p.nf + sL) * p.nqm * NV;
     const R *ulR = p.uleft + ((size_t)(bnd ? cL : cR) * p.nf + (bnd ? sL : sR)) * p.nqm * NV;
     double acc[NV] = {0, 0, 0, 0, 0};
+    // QB points' deviations are loaded before any is used (indices clamped
+    // to the face's nq points; the surplus is not evaluated)
+    constexpr int QB = 3;
+    R dl[QB][NV], dr[QB][NV];
     for (int q = 0; q < nq; ++q) {
+        const int qb = q % QB;
+        if (qb == 0) {
+#pragma unroll
+            for (int u = 0; u < QB; ++u) {
+                const int qq = min(q + u, nq - 1);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    dl[u][v] = ulL[qq * NV + v];
+                    dr[u][v] = ulR[qq * NV + v];
+                }
+            }
+        }
         double a[NV], bq[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            a[v] = uL[v] + (double)ulL[q * NV + v];
-            bq[v] = uR[v] + (double)ulR[q * NV + v];
+            R xl = dl[0][v], xr = dr[0][v];
+#pragma unroll
+            for (int u = 1; u < QB; ++u)
+                if (qb == u) {
+                    xl = dl[u][v];
+                    xr = dr[u][v];
+                }
+            a[v] = uL[v] + (double)xl;
+            bq[v] = uR[v] + (double)xr;
         }
         if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, bq);  // GET_STATES_BOUNDS
         // ROTATEF (src/grid_t.f90:4217-4258)
