@@ -1921,17 +1921,28 @@ __global__ __launch_bounds__(256) void k_update(UpdateParams p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double dtl = 1.0e300;
     if (i < p.ncell) {
+        // every face's flux is loaded before the first is summed (face
+        // slots past nf, and slots with no face, read face 0 and add nothing)
+        constexpr int NFM = 6;
+        int cf[NFM];
+#pragma unroll
+        for (int f = 0; f < NFM; ++f) cf[f] = f < p.nf ? p.cface[(size_t)i * p.nf + f] : 0;
+        double F[NFM][NV];
+#pragma unroll
+        for (int f = 0; f < NFM; ++f) {
+            const size_t id = cf[f] != 0 ? (size_t)(abs(cf[f]) - 1) : 0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) F[f][v] = p.fflux[id * NV + v];
+        }
         double r[NV] = {0, 0, 0, 0, 0};
-        for (int f = 0; f < p.nf; ++f) {
-            const int cf = p.cface[(size_t)i * p.nf + f];
-            if (cf == 0) continue;
-            const double *F = p.fflux + (size_t)(abs(cf) - 1) * NV;
-            if (cf > 0) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) r[v] = r[v] + F[v];
-            } else {
+        for (int f = 0; f < NFM; ++f) {
+            if (cf[f] > 0) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) r[v] = r[v] - F[v];
+                for (int v = 0; v < NV; ++v) r[v] = r[v] + F[f][v];
+            } else if (cf[f] < 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) r[v] = r[v] - F[f][v];
             }
         }
         if (p.rhs_out) {
@@ -2940,12 +2951,34 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     double *obL = ubf + ((size_t)cL * p.nf + sL) * p.nqm * NV;
     double *obR = ubf + ((size_t)(bnd ? cL : cR) * p.nf + (bnd ? sL : sR)) * p.nqm * NV;
     double bcacc[2 * NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // QB points' deviations are loaded before any is used, as in k_flux
+    constexpr int QB = 3;
+    R dl[QB][NV], dr[QB][NV];
     for (int q = 0; q < nq; ++q) {
+        const int qb = q % QB;
+        if (qb == 0) {
+#pragma unroll
+            for (int u = 0; u < QB; ++u) {
+                const int qq = min(q + u, nq - 1);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    dl[u][v] = ulL[qq * NV + v];
+                    dr[u][v] = ulR[qq * NV + v];
+                }
+            }
+        }
         double a[NV], bq[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            a[v] = uL[v] + (double)ulL[q * NV + v];
-            bq[v] = uR[v] + (double)ulR[q * NV + v];
+            R xl = dl[0][v], xr = dr[0][v];
+#pragma unroll
+            for (int u = 1; u < QB; ++u)
+                if (qb == u) {
+                    xl = dl[u][v];
+                    xr = dr[u][v];
+                }
+            a[v] = uL[v] + (double)xl;
+            bq[v] = uR[v] + (double)xr;
         }
         if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, bq);
         double la[NV], ra[NV];
@@ -4085,6 +4118,7 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
     }
     if (any_quad) nqm = std::max(nqm, s->qp_quad);
     if (any_tri) nqm = std::max(nqm, s->qp_triangle);
+    if (nf > 6) return bad(UCNS_EUNSUPPORTED, "at most 6 faces per cell");
     ctx->nf = nf;
     ctx->nqm = nqm;
     if (nf * nqm > 6 * 16) return bad(UCNS_EUNSUPPORTED, "too many face Gauss points per cell");
