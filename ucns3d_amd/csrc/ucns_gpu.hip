@@ -43,6 +43,10 @@ constexpr int NXCD = 8;
 // distinct stencil members of a tile held in LDS by the residual product's scatter
 #define UCNS_TILE_UQ 512
 #define UCNS_NOSLOT 0xFFFFu
+// rows of a tile's member table: 8 neighbouring cells share about 343
+// distinct members with WENO-4 on tets (5 stencils), 373 with WENO-3 on
+// hexes, 660 with WENO-4 on hexes (7 stencils of 39 members)
+constexpr int tile_uq(int ts, int imax) { return ts > 5 && imax > 19 ? 768 : UCNS_TILE_UQ; }
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
@@ -498,6 +502,7 @@ struct ReconWParams {
     const double *gbs;        // [ntile][ideg*5][8*ts] cotangents of the stencil coefficients
     double *ubar;             // [ncell][5] accumulated
     const uint16_t *slot;     // [ntile][imax][8*ts] row of the tile's member table (UCNS_NOSLOT: HBM)
+    int uq_cap;               // rows of a member table at most (tile_uq(ts, imax))
     const int32_t *uq, *ucount;
     const int64_t *toff;      // first row of each tile's member table in tab
     double *tab;              // [rows][5] the member tables, summed per cell by k_adj_gather
@@ -1020,8 +1025,8 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
             for (int v = 0; v < NV; ++v) gb[k][v] = in ? gs[(size_t)(k * NV + v) * TP] : 0.0;
         const uint16_t *st = p.slot + (size_t)tile * IMAX * TP;
         for (int e = lane; e < IMAX * TP; e += WAVE) sSlot[e] = st[e];
-        nuq = min(p.ucount[tile], UCNS_TILE_UQ);
-        for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+        nuq = min(p.ucount[tile], p.uq_cap);
+        for (int e = lane; e < min(nuq, UCNS_TILE_UQ) * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
     }
     // a use of every cotangent here makes the compiler wait for their loads
     // before the row stream starts, not inside it
@@ -1129,8 +1134,10 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
                 for (int k = 0; k < IDEG; ++k)
 #pragma unroll
                     for (int v = 0; v < NV; ++v) d[v] += (double)pc[k] * gb[k][v];
+                // rows past this kernel's LDS table go straight to HBM (and are
+                // written as zeros below)
                 const unsigned sl = sSlot[i * TP + rl];
-                if (sl != UCNS_NOSLOT) {
+                if (sl < UCNS_TILE_UQ) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v)
                         __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1190,7 +1197,7 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         }
         __syncthreads();
         double *tt = p.tab + (size_t)p.toff[tile] * NV;
-        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = e < UCNS_TILE_UQ * NV ? (&sAcc[0][0])[e] : 0.0;
     }
     const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
     R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
@@ -1220,8 +1227,9 @@ template <typename R, int IDEG, int IMAX, int TS, int DEPTH>
 __device__ __forceinline__ void scatter_s_body(const ReconWParams<R> &p) {
     using SS = StreamShape<R, IDEG, IMAX, TS, DEPTH>;
     constexpr int TP = SS::TP, NCH = SS::NCH, SL = NCH * 1024;
+    constexpr int UQ = tile_uq(TS, IMAX);  // LDS table rows
     __shared__ __attribute__((aligned(1024))) char ring[DEPTH * SL + SS::HD];
-    __shared__ double sAcc[UCNS_TILE_UQ][NV];
+    __shared__ double sAcc[UQ][NV];
     __shared__ uint16_t sSlot[IMAX * TP];
     const int *hdr = (const int *)(ring + DEPTH * SL);
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
@@ -1242,8 +1250,8 @@ __device__ __forceinline__ void scatter_s_body(const ReconWParams<R> &p) {
     }
     const uint16_t *st = p.slot + (size_t)tile * IMAX * TP;
     for (int e = lane; e < IMAX * TP; e += WAVE) sSlot[e] = st[e];
-    const int nuq = min(p.ucount[tile], UCNS_TILE_UQ);
-    for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+    const int nuq = min(p.ucount[tile], p.uq_cap);
+    for (int e = lane; e < min(nuq, UQ) * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
     // every cotangent is used before the row stream starts, so the compiler
     // waits for their loads here and not inside the stream
     double gsum = 0.0;
@@ -1282,7 +1290,7 @@ __device__ __forceinline__ void scatter_s_body(const ReconWParams<R> &p) {
                 for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
             }
             const unsigned sl = sSlot[i * TP + rl];
-            if (sl != UCNS_NOSLOT) {
+            if (sl < (unsigned)UQ) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v)
                     __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1321,7 +1329,7 @@ __device__ __forceinline__ void scatter_s_body(const ReconWParams<R> &p) {
     }
     __syncthreads();
     double *tt = p.tab + (size_t)p.toff[tile] * NV;
-    for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+    for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = e < UQ * NV ? (&sAcc[0][0])[e] : 0.0;
 }
 template <int IDEG, int IMAX, int TS>
 __global__ __launch_bounds__(WAVE) void k_scatter_s64(ReconWParams<double> p) {
@@ -2076,10 +2084,12 @@ struct VjpParams {
                            // weights [ntile][ideg*5 + 10][8 ts]
     const int *fload;      // row stream: the forward's per-cell flags (1 MUSCL, 2 first order)
     // the tile's distinct stencil members: member (j, pos) of tile t scatters
-    // to LDS row slot[t][j][pos] (UCNS_NOSLOT: straight to HBM), and row r of
-    // tile t belongs to cell uq[t][r], r < min(ucount[t], UCNS_TILE_UQ)
+    // to table row slot[t][j][pos] (UCNS_NOSLOT: straight to HBM), and row r
+    // of tile t belongs to cell uq[t][r], r < min(ucount[t], uq_cap); rows
+    // past a kernel's LDS table also go straight to HBM and are written as 0
     const uint16_t *slot;
     const int32_t *uq, *ucount;
+    int uq_cap;
     double *gbout;         // row stream: each stencil's g_bar [ntile][ideg*5][8 ts], for the scatter
     const int64_t *toff;   // first row of each tile's member table in tab
     double *tab;           // [rows][5] the member tables, summed per cell by k_adj_gather
@@ -2100,7 +2110,7 @@ constexpr int HT = 4096;  // hash slots, >= 2 * the 8 * 8 * 39 member entries of
 
 template <typename R, int IMAX>
 __global__ __launch_bounds__(WAVE) void k_tile_members(LsqView<R> L, int ntile, int ncell, const int32_t *cinfo,
-                                                       int32_t *uq, int32_t *ucount, uint16_t *slot) {
+                                                       int cap, int32_t *uq, int32_t *ucount, uint16_t *slot) {
     __shared__ int32_t keys[HT];
     __shared__ uint16_t cid[HT];
     __shared__ uint16_t hs[IMAX * 64];
@@ -2142,8 +2152,8 @@ __global__ __launch_bounds__(WAVE) void k_tile_members(LsqView<R> L, int ntile, 
         const int h = lane * PER + k;
         const int key = keys[h];
         if (key >= 0) {
-            cid[h] = id < UCNS_TILE_UQ ? (uint16_t)id : (uint16_t)UCNS_NOSLOT;
-            if (id < UCNS_TILE_UQ) uq[(size_t)tile * UCNS_TILE_UQ + id] = key;
+            cid[h] = id < cap ? (uint16_t)id : (uint16_t)UCNS_NOSLOT;
+            if (id < cap) uq[(size_t)tile * cap + id] = key;
             ++id;
         }
     }
@@ -2177,8 +2187,8 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
     if (tile >= p.ntile) return;
-    const int nuq = p.slot ? min(p.ucount[tile], UCNS_TILE_UQ) : 0;
-    for (int e = lane; e < nuq * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
+    const int nuq = p.slot ? min(p.ucount[tile], p.uq_cap) : 0;
+    for (int e = lane; e < min(nuq, UCNS_TILE_UQ) * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
     if (p.iweno == 1) {
         const int nvc = min(8, p.ncell - tile * 8);
         const R *Bt = p.B + (size_t)tile * 8 * NP;
@@ -2568,7 +2578,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
                 for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
             }
             const unsigned sl = p.slot ? p.slot[((size_t)tile * IMAX + j) * p.lsq.tp + pos] : UCNS_NOSLOT;
-            if (sl != UCNS_NOSLOT) {
+            if (sl < UCNS_TILE_UQ) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v)
                     __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -2591,7 +2601,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     if (nuq > 0) {
         __syncthreads();
         double *tt = p.tab + (size_t)p.toff[tile] * NV;
-        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = (&sAcc[0][0])[e];
+        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = e < UCNS_TILE_UQ * NV ? (&sAcc[0][0])[e] : 0.0;
     }
 }
 
@@ -2907,13 +2917,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *of
 }
 
 // CSR of the member tables by cell: counts, then positions
-__global__ void k_adj_csr(int ntile, const int32_t *uq, const int32_t *ucount, const int64_t *toff, int64_t *cnt,
-                          const int64_t *off, int64_t *ent) {
+__global__ void k_adj_csr(int ntile, int cap, const int32_t *uq, const int32_t *ucount, const int64_t *toff,
+                          int64_t *cnt, const int64_t *off, int64_t *ent) {
     const int t = blockIdx.x;
     if (t >= ntile) return;
-    const int nu = min(ucount[t], UCNS_TILE_UQ);
+    const int nu = min(ucount[t], cap);
     for (int r = threadIdx.x; r < nu; r += blockDim.x) {
-        const int n = uq[(size_t)t * UCNS_TILE_UQ + r];
+        const int n = uq[(size_t)t * cap + r];
         const unsigned long long k = atomicAdd((unsigned long long *)&cnt[n], 1ull);
         if (off) ent[off[n] + (int64_t)k] = toff[t] + r;
     }
@@ -3156,7 +3166,8 @@ struct ucns_ctx {
     const double *sv_gb = nullptr;  // non-null: the next row-stream reconstruction also scatters sv_gb ...
     double *sv_ubar = nullptr;      // ... into sv_ubar
     uint16_t *adj_slot = nullptr;                      // [ntile][imax][8 ts] LDS row of each member entry
-    int32_t *adj_uq = nullptr, *adj_ucount = nullptr;  // [ntile][UCNS_TILE_UQ], [ntile]
+    int32_t *adj_uq = nullptr, *adj_ucount = nullptr;  // [ntile][uq_cap], [ntile]
+    int uq_cap = UCNS_TILE_UQ;                         // member-table rows per tile (tile_uq(ts, imax))
     int64_t *adj_toff = nullptr, *adj_off = nullptr, *adj_ent = nullptr;  // table rows per tile, CSR by cell
     double *adj_tab = nullptr;                          // [rows][5] member tables
     bool adj_g_on = false;  // the next tiled reconstruction stores them
@@ -3259,6 +3270,7 @@ int launch_recon_t(ucns_ctx *ctx) {
             w.gbs = ctx->sv_gb;
             w.ubar = ctx->sv_ubar;
             w.slot = ctx->adj_slot;
+            w.uq_cap = ctx->uq_cap;
             w.uq = ctx->adj_uq;
             w.ucount = ctx->adj_ucount;
             w.toff = ctx->adj_toff;
@@ -3537,13 +3549,13 @@ void launch_tile_members(ucns_ctx *ctx) {
     const dim3 grid(ctx->ntile), block(WAVE);
     if (ctx->imax == 39)
         hipLaunchKernelGGL((k_tile_members<R, 39>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
-                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
+                           ctx->cinfo, ctx->uq_cap, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
     else if (ctx->imax == 19)
         hipLaunchKernelGGL((k_tile_members<R, 19>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
-                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
+                           ctx->cinfo, ctx->uq_cap, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
     else
         hipLaunchKernelGGL((k_tile_members<R, 8>), grid, block, 0, ctx->stream, L, ctx->ntile, ctx->ncell,
-                           ctx->cinfo, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
+                           ctx->cinfo, ctx->uq_cap, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
 }
 
 int ensure_adjoint(ucns_ctx *ctx) {
@@ -3572,7 +3584,8 @@ int ensure_adjoint(ucns_ctx *ctx) {
     // the tiles' distinct stencil members (once per upload)
     if (ctx->ts <= 8 && (ctx->imax == 39 || ctx->imax == 19 || ctx->imax == 8)) {
         if ((rc = dalloc(ctx, &ctx->adj_slot, (size_t)ctx->ntile * ctx->imax * 8 * ctx->ts))) return rc;
-        if ((rc = dalloc(ctx, &ctx->adj_uq, (size_t)ctx->ntile * UCNS_TILE_UQ))) return rc;
+        ctx->uq_cap = tile_uq(ctx->ts, ctx->imax);
+        if ((rc = dalloc(ctx, &ctx->adj_uq, (size_t)ctx->ntile * ctx->uq_cap))) return rc;
         if ((rc = dalloc(ctx, &ctx->adj_ucount, (size_t)ctx->ntile))) return rc;
         if (ctx->precision == UCNS_MIXED)
             launch_tile_members<float>(ctx);
@@ -3588,7 +3601,7 @@ int ensure_adjoint(ucns_ctx *ctx) {
                                ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         std::vector<int64_t> toff(nt + 1, 0);
-        for (int t = 0; t < nt; ++t) toff[t + 1] = toff[t] + std::min(uc[t], UCNS_TILE_UQ);
+        for (int t = 0; t < nt; ++t) toff[t + 1] = toff[t] + std::min(uc[t], ctx->uq_cap);
         const int64_t rows = toff[nt];
         if ((rc = dalloc(ctx, &ctx->adj_toff, (size_t)nt + 1))) return rc;
         if ((rc = dalloc(ctx, &ctx->adj_tab, (size_t)rows * NV))) return rc;
@@ -3600,8 +3613,8 @@ int ensure_adjoint(ucns_ctx *ctx) {
         int64_t *cnt = nullptr;
         if ((rc = dalloc(ctx, &cnt, (size_t)nc))) return rc;
         HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
-        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->adj_uq, ctx->adj_ucount,
-                           ctx->adj_toff, cnt, (const int64_t *)nullptr, (int64_t *)nullptr);
+        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->uq_cap, ctx->adj_uq,
+                           ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)nullptr, (int64_t *)nullptr);
         std::vector<int64_t> hc(nc), off(nc + 1, 0);
         HIP_TRY(hipMemcpyAsync(hc.data(), cnt, nc * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -3615,8 +3628,8 @@ int ensure_adjoint(ucns_ctx *ctx) {
         HIP_TRY(hipMemcpyAsync(ctx->adj_off, off.data(), (nc + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                                ctx->stream));
         HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
-        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->adj_uq, ctx->adj_ucount,
-                           ctx->adj_toff, cnt, (const int64_t *)ctx->adj_off, ctx->adj_ent);
+        hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->uq_cap, ctx->adj_uq,
+                           ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)ctx->adj_off, ctx->adj_ent);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         dfree(cnt);
     }
@@ -3648,6 +3661,7 @@ void launch_scatter_t(ucns_ctx *ctx, double *ubar) {
     w.gbs = ctx->adj_gb;
     w.ubar = ubar;
     w.slot = ctx->adj_slot;
+    w.uq_cap = ctx->uq_cap;
     w.ucount = ctx->adj_ucount;
     w.toff = ctx->adj_toff;
     w.tab = ctx->adj_tab;
@@ -3747,6 +3761,7 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     vp.gload = (ctx->smode && ctx->adj_g) ? (const R *)ctx->adj_g : nullptr;
     vp.fload = ctx->adj_flags;
     vp.slot = ctx->adj_slot;
+    vp.uq_cap = ctx->uq_cap;
     vp.uq = ctx->adj_uq;
     vp.ucount = ctx->adj_ucount;
     vp.gbout = ctx->smode ? ctx->adj_gb : nullptr;
