@@ -482,7 +482,7 @@ struct ReconWParams {
     int ncell, ntile, tiles_per_xcd, ts, nf, nqm, qpq, qpt;
     double gamma, lwci1, eps;
     const double *U;
-    const double *U48;     // [ncell][6] the state in 48-byte records (row-stream gathers)
+    const double *U64;     // [ncell][8] the state in 64-byte records (row-stream gathers)
     const int32_t *nbr;    // [ntile][imax][8*ts]
     const R *P;            // [ntile][imax][ideg][8*ts]
     const R *B;            // [ncell][ideg*(ideg+1)/2]
@@ -954,10 +954,10 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
 // are in LDS one ring turn before its states are fetched.
 //
 // LDS slot of a row: the row block as streamed, then the member states of
-// its TP columns as 48-byte records (5 doubles and a pad, gathered from a
-// padded copy of the state in 16-byte pieces: consecutive lanes take
-// consecutive pieces, so a record is one or two cache lines and a row is
-// NSI gathers).  After the loop the slots hold the tile's indicators and the
+// its TP columns as 48-byte records (5 doubles and a pad), gathered in
+// 16-byte pieces from a copy of the state padded to 64-byte records, so a
+// record is one cache line; consecutive lanes take consecutive pieces and a
+// row is NSI gathers.  After the loop the slots hold the tile's indicators and the
 // central coefficients.
 constexpr int stream_depth(bool mixed, int ts) { return ts == 5 ? 4 : (mixed ? 2 : 3); }
 
@@ -1071,7 +1071,7 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         for (int c = 0; c < SS::NSI; ++c) {
             const int q = lane + 64 * c;
             const int part = q < 3 * TP ? q % 3 : 0;
-            lds_load16(p.U48 + (size_t)ix[c] * 6 + 2 * part, lds_addr(dst + c * 1024));
+            lds_load16(p.U64 + (size_t)ix[c] * 8 + 2 * part, lds_addr(dst + c * 1024));
         }
     };
     // every chunk is issued, also past the last cell's block (lane 0 then
@@ -3047,12 +3047,12 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     }
 }
 
-// the state [ncell][5] as 48-byte records [ncell][6] (the sixth value zero)
-__global__ __launch_bounds__(256) void k_pack48(size_t n48, const double *U, double *U48) {
+// the state [ncell][5] as 64-byte records [ncell][8] (the last three zero)
+__global__ __launch_bounds__(256) void k_pack64(size_t n64, const double *U, double *U64) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n48) return;
-    const size_t i = e / 6, v = e - i * 6;
-    U48[e] = v < NV ? U[i * NV + v] : 0.0;
+    if (e >= n64) return;
+    const size_t i = e / 8, v = e - i * 8;
+    U64[e] = v < NV ? U[i * NV + v] : 0.0;
 }
 
 __global__ void k_zero_f64(double *p, size_t n) {
@@ -3151,7 +3151,7 @@ struct ucns_ctx {
     int32_t *nbr = nullptr;
     void *P = nullptr, *B = nullptr, *mbar = nullptr, *qp = nullptr, *uleft = nullptr;
     unsigned char *rows = nullptr;  // row stream of the DMA reconstruction kernels
-    double *u48 = nullptr;          // [ncell][6] the state in 48-byte records, for their gathers
+    double *u64 = nullptr;          // [ncell][8] the state in 64-byte records, for their gathers
     long long tile_bytes = 0;
     // state (device)
     double *U = nullptr, *U2 = nullptr, *fflux = nullptr, *rhs = nullptr, *t = nullptr, *dtfix = nullptr;
@@ -3257,11 +3257,11 @@ int launch_recon_t(ucns_ctx *ctx) {
         w.stream = ctx->rows;
         w.tile_bytes = ctx->tile_bytes;
         if (ctx->smode) {
-            // the row-stream kernels gather the member states as 48-byte records
-            const size_t n48 = (size_t)ctx->ncell * 6;
-            hipLaunchKernelGGL(k_pack48, dim3((unsigned)((n48 + 255) / 256)), dim3(256), 0, ctx->stream, n48, ctx->U,
-                               ctx->u48);
-            w.U48 = ctx->u48;
+            // the row-stream kernels gather the member states from 64-byte records
+            const size_t n64 = (size_t)ctx->ncell * 8;
+            hipLaunchKernelGGL(k_pack64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, ctx->stream, n64, ctx->U,
+                               ctx->u64);
+            w.U64 = ctx->u64;
         }
         w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
         w.fstore = ctx->adj_g_on ? ctx->adj_flags : nullptr;
@@ -4050,7 +4050,7 @@ void release_prestore(ucns_ctx *ctx) {
     dfree(ctx->nbr);
     dfree(ctx->P);
     dfree(ctx->rows);
-    dfree(ctx->u48);
+    dfree(ctx->u64);
     dfree(ctx->B);
     dfree(ctx->mbar);
     dfree(ctx->qp);
@@ -4329,7 +4329,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     if (ctx->smode) {
         ctx->tile_bytes = sHD + (long long)IM * sRD;
         if ((rc = dalloc(ctx, &ctx->rows, (size_t)ctx->ntile * ctx->tile_bytes))) return rc;
-        if ((rc = dalloc(ctx, &ctx->u48, (size_t)nc * 6))) return rc;
+        if ((rc = dalloc(ctx, &ctx->u64, (size_t)nc * 8))) return rc;
         // member-index rows: row r holds members j = r (zero past imax)
         const size_t rows = (size_t)sD + IM;
         std::vector<int32_t> idx((size_t)ctx->ntile * rows * sTP, 0);
