@@ -2617,8 +2617,8 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
 template <typename R, int IDEG, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
-    constexpr int KL = (IDEG + 7) / 8;  // coefficients per lane in the face-point pass
-    constexpr int QU = 4;               // face points in flight per lane
+    constexpr int KL = (IDEG + 3) / 4;  // coefficients per lane in the face-point pass
+    constexpr int QU = 3;               // face points in flight per lane (and group)
     constexpr int TP = 8 * TS;
     constexpr unsigned KS = NV * TP;    // stride of one coefficient row (elements)
     __shared__ double sGb[8][IDEG * NV];  // cotangent of the combined coefficients
@@ -2648,17 +2648,20 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
         return q < nslot && qq < (ci_tri(info, f) ? p.qpt : p.qpq);
     };
 
-    // ---- cotangent of the combined coefficients, sum_q phi_q ubar_q (lane
-    // s of a cell takes coefficients s, s+8, s+16 over all its face points),
-    // and of the cell average through a_q = U_i + ULEFT(q) ----
+    // ---- cotangent of the combined coefficients, sum_q phi_q ubar_q, and of
+    // the cell average through a_q = U_i + ULEFT(q).  The 8 lanes of a cell
+    // are two groups of 4: group g takes the face points of parity g, lane i
+    // of a group the coefficients i, i+4, i+8, ...; the groups' sums meet by
+    // one exchange ----
     double usum[NV] = {0, 0, 0, 0, 0};
     {
+        const int g = s >> 2, il = s & 3;
         double ab[KL][NV];
         R mk[KL];
         int ex[KL][3];
 #pragma unroll
         for (int t = 0; t < KL; ++t) {
-            const int k = s + 8 * t;
+            const int k = il + 4 * t;
             mk[t] = k < IDEG ? mb[k] : R(0);
 #pragma unroll
             for (int d = 0; d < 3; ++d) ex[t][d] = k < IDEG ? c_mono[k][d] : 0;
@@ -2666,14 +2669,14 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
             for (int v = 0; v < NV; ++v) ab[t][v] = 0.0;
         }
 #pragma unroll 1
-        for (int q0 = 0; q0 < nslot; q0 += QU) {
+        for (int q0 = 0; q0 < nslot; q0 += 2 * QU) {
             // QU points' loads are issued before any is used (indices stay in
             // the cell's block: q < nslot <= nf nqm)
             R xq[QU][3];
             double ub[QU][NV];
 #pragma unroll
             for (int u = 0; u < QU; ++u) {
-                const int q = min(q0 + u, nslot - 1);
+                const int q = min(q0 + 2 * u + g, nslot - 1);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) xq[u][d] = qpc[q * 3 + d];
 #pragma unroll
@@ -2681,7 +2684,8 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
             }
 #pragma unroll
             for (int u = 0; u < QU; ++u) {
-                const bool ok = q0 + u < nslot && qvalid(q0 + u);
+                const int q = q0 + 2 * u + g;
+                const bool ok = q < nslot && qvalid(q);
                 const R x0 = xq[u][0], x1 = xq[u][1], x2 = xq[u][2];
                 const R x0s = x0 * x0, x1s = x1 * x1, x2s = x2 * x2;
                 const R x0c = x0s * x0, x1c = x1s * x1, x2c = x2s * x2;
@@ -2701,11 +2705,19 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
             }
         }
 #pragma unroll
-        for (int t = 0; t < KL; ++t) {
-            const int k = s + 8 * t;
-            if (k < IDEG) {
+        for (int v = 0; v < NV; ++v) {
+            usum[v] += __shfl_xor(usum[v], 4, WAVE);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sGb[lc][k * NV + v] = ab[t][v];
+            for (int t = 0; t < KL; ++t) ab[t][v] += __shfl_xor(ab[t][v], 4, WAVE);
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int t = 0; t < KL; ++t) {
+                const int k = il + 4 * t;
+                if (k < IDEG) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sGb[lc][k * NV + v] = ab[t][v];
+                }
             }
         }
     }
