@@ -2919,10 +2919,25 @@ __global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *of
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= ncell) return;
     double a[NV] = {0, 0, 0, 0, 0};
-    for (int64_t e = off[n]; e < off[n + 1]; ++e) {
-        const double *r = tab + (size_t)ent[e] * NV;
+    // GU rows in flight: their indices, then the rows, are loaded before any
+    // is added (the sum keeps the entries' order)
+    constexpr int GU = 4;
+    const int64_t e0 = off[n], e1 = off[n + 1];
+    for (int64_t e = e0; e < e1; e += GU) {
+        int64_t rr[GU];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a[v] += r[v];
+        for (int u = 0; u < GU; ++u) rr[u] = ent[min(e + u, e1 - 1)];
+        double x[GU][NV];
+#pragma unroll
+        for (int u = 0; u < GU; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) x[u][v] = tab[(size_t)rr[u] * NV + v];
+#pragma unroll
+        for (int u = 0; u < GU; ++u)
+            if (e + u < e1) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) a[v] += x[u][v];
+            }
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) ubar[(size_t)n * NV + v] += a[v];
