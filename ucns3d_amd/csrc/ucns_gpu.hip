@@ -2961,11 +2961,15 @@ __global__ void k_adj_csr(int ntile, int cap, const int32_t *uq, const int32_t *
 // free-stream values) from forward-mode passes of the same templated code the
 // primal runs, one pass per input; F_bar is pulled back through ROTATEB
 // (whose transpose is ROTATEF) and ROTATEF (transpose ROTATEB).
-template <typename R>
+// BND: the launch over boundary faces (ghost-state Jacobians by forward
+// mode); the other launch takes the interior and periodic faces, whose
+// kernel then carries none of the boundary code's registers
+template <typename R, bool BND>
 __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double *lam, double *ubf, double *bcbar) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= p.nface) return;
     const int cL = p.fcell[2 * f], cR = p.fcell[2 * f + 1];
+    if ((cR < 0) != BND) return;
     const int s0 = p.fslot[2 * f], sR = p.fslot[2 * f + 1];
     const int sL = s0 & 255;
     const bool tri = (s0 >> 8) & 1;
@@ -2973,7 +2977,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     const double *w = tri ? c_wt : c_wq;
     const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
     const double area = p.farea[f];
-    const bool bnd = cR < 0;
+    constexpr bool bnd = BND;
     double uL[NV], uR[NV], Fb[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -3168,6 +3172,7 @@ struct ucns_ctx {
     bool ring = false;  // UCNS_RING=1 selects the LDS-ring kernels (experimental)
     int smode = 0;      // stencils per cell of the row-stream kernel in use (0: none)
     bool needs_bc = false, have_bc = false;  // boundary codes 1/2/6 present; ucns_set_boundary called
+    bool has_bnd_faces = false;              // any face with a ghost state (codes 1/2/3/4/6)
     ucns_boundary_v1 bc{};
     int ntile = 0;
     // mesh (device)
@@ -3760,7 +3765,10 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     fp.fflux = nullptr;
     fp.bc = ctx->bc;
     tstart(ctx, 1);
-    hipLaunchKernelGGL((k_flux_vjp<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
+    hipLaunchKernelGGL((k_flux_vjp<R, false>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
+                       ctx->adj_ubf, bcbar);
+    if (ctx->has_bnd_faces)
+        hipLaunchKernelGGL((k_flux_vjp<R, true>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
                        ctx->adj_ubf, bcbar);
     tstop(ctx, 1);
     tcollect(ctx, 1, &ctx->st.ms_flux_vjp, &ctx->st.n_flux_vjp);
@@ -4180,6 +4188,7 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
             if (bc == 1 || bc == 2 || bc == 3 || bc == 4 || bc == 6) {
                 // boundary face: owner only, the right state is a ghost (BOUNDARYS)
                 if (bc != 3 && bc != 4) ctx->needs_bc = true;
+                ctx->has_bnd_faces = true;
                 cnb[(size_t)i * nf + f] = -1;
                 const int id = (int)farea.size();
                 fcell.push_back(i);
