@@ -178,7 +178,11 @@ int  ucns_residual_vjp_device(ucns_ctx *ctx, const double *w_dev, double *u_bar_
  * u0_bar = (dU_N/dU_0)^T uN_bar and bc_bar (10 doubles, or NULL) =
  * (dU_N/d(inflow, outflow))^T uN_bar.  The step sizes are those of the
  * forward sweep, held fixed in the derivative.  The state is left at U_N and
- * *t_io (or NULL) is advanced as by ucns_rk3_steps. */
+ * *t_io (or NULL) is advanced as by ucns_rk3_steps.  The last stages of the
+ * forward sweep also keep their reconstruction for the reverse sweep, as
+ * many as fit in the environment's UCNS_ADJ_KEEP_GB (default 64 GiB, at most
+ * the free device memory less 4 GiB) and UCNS_ADJ_KEEP (a stage count); the
+ * context holds those buffers until ucns_destroy or ucns_upload_prestore. */
 int  ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar,
                       double *u0_bar, int32_t layout, double *bc_bar, double *t_io);
 /* Same with device pointers, AOS layout (bc_bar_dev: 10 doubles or NULL). */
