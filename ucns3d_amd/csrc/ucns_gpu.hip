@@ -970,9 +970,10 @@ struct StreamShape {
     static constexpr int HD = DEPTH * TP * 4;              // tile header bytes
     static constexpr int NCH = (RD + 1023) / 1024;         // 1 KiB loads per row
     // the row's member states: the TP columns' 48-byte records, three
-    // 16-byte pieces each, gathered by NSI loads of 64 pieces
+    // 16-byte pieces each, gathered by NSI loads of 64 pieces; they start
+    // right after the row block (its last 1 KiB load writes only to RD)
     static constexpr int NSI = (3 * TP + 63) / 64;
-    static constexpr int UA = NCH * 1024;
+    static constexpr int UA = (RD + 15) / 16 * 16;
     static constexpr int SB = UA + NSI * 1024;             // slot bytes
     static constexpr int PER = NCH + NSI;                  // loads per row
     static constexpr int BBYTES = 8 * NP * (int)sizeof(R);
