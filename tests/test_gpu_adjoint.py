@@ -120,19 +120,22 @@ def test_rk3_adjoint_dot_product(name):
 def test_rk3_adjoint_kept_stages_agree_with_recomputed(name, stream, precision):
     """The reverse sweep starts a stage's products from the reconstruction the
     forward sweep kept, or runs the reconstruction again (UCNS_ADJ_KEEP caps
-    the kept stages): all, some (4 of 9) and none give the same cotangents"""
+    the kept stages, UCNS_ADJ_KEEP_GB their bytes): all, some (4 of 9) and
+    none give the same cotangents"""
     case, orc, u, du, bc = setup(name)
     print(f"[rk3 adjoint keep] {name} stream={stream} precision={precision}", flush=True)
     s = solver_for(name, precision=precision, stream=stream)
     dt = 0.5 * s.compute_dt(0.5)
     a = np.random.default_rng(15).normal(size=u.shape)
     out = {}
-    for keep in ("9", "4", "0"):
-        with env("UCNS_ADJ_KEEP", keep):
+    settings = {"9": ("UCNS_ADJ_KEEP", "9"), "4": ("UCNS_ADJ_KEEP", "4"), "0": ("UCNS_ADJ_KEEP", "0"),
+                "bytes": ("UCNS_ADJ_KEEP_GB", "1e-9")}
+    for key, (var, val) in settings.items():
+        with env(var, val):
             s.set_state(u)
-            out[keep] = s.rk3_adjoint(3, a, dt_fixed=dt)
+            out[key] = s.rk3_adjoint(3, a, dt_fixed=dt)
     ref, rb, _ = out["9"]
-    for keep in ("4", "0"):
+    for keep in ("4", "0", "bytes"):
         g, gb, _ = out[keep]
         rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
         print(f"  keep {keep}: rel {rel:.3e}")
