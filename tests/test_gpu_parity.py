@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, PRESETS, TET, Case, CaseSpec, reduced
+from ucns3d_amd.case import HEX, PRESETS, PRISM, TET, Case, CaseSpec, reduced
 from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
@@ -22,6 +22,10 @@ CASES = {
     "tet-weno4-6": lambda: reduced("c2", 6),
     "hex-weno3-10": lambda: CaseSpec("hex-weno3-10", HEX, 10, "weno3", seed=11),
     "tet-weno3-5": lambda: CaseSpec("tet-weno3-5", TET, 5, "weno3", jitter=0.1, seed=12),
+    # prisms: 2 triangular + 3 quadrilateral faces, 6 stencils (the tiled kernels)
+    "prism-weno3-6": lambda: CaseSpec("prism-weno3-6", PRISM, 6, "weno3", jitter=0.1, seed=13),
+    "prism-weno4-6": lambda: CaseSpec("prism-weno4-6", PRISM, 6, "weno4", jitter=0.1, seed=14),
+    "prism-muscl-6": lambda: CaseSpec("prism-muscl-6", PRISM, 6, "muscl", jitter=0.1, seed=15),
 }
 
 _cache = {}
@@ -47,8 +51,12 @@ def valid_uleft_mask(case):
     """face points that exist (triangles use fewer than the padded count)"""
     nqm = max(case.qp_quad, case.qp_tri)
     mask = np.zeros((case.ncell, case.nface_max, nqm), dtype=bool)
-    faces = 6 if case.shape == 1 else 4
-    nq = case.qp_quad if case.shape == 1 else case.qp_tri
+    if case.shape == PRISM:
+        mask[:, :2, :case.qp_tri] = True   # the two triangles
+        mask[:, 2:5, :case.qp_quad] = True  # the three quadrilaterals
+        return mask
+    faces = 6 if case.shape == HEX else 4
+    nq = case.qp_quad if case.shape == HEX else case.qp_tri
     mask[:, :faces, :nq] = True
     return mask
 

@@ -17,13 +17,14 @@ from . import native
 
 HEX = 1
 TET = 2
+PRISM = 3
 
 
 @dataclass(frozen=True)
 class CaseSpec:
     """Mesh and scheme choices of one configuration."""
     name: str
-    shape: int            # HEX or TET (6 Kuhn tets per hex)
+    shape: int            # HEX, TET (6 Kuhn tets per hex) or PRISM (2 per hex)
     n: int                # hexes per direction
     scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2) or "weno4" (IORDER 3)
     jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
@@ -51,7 +52,7 @@ class CaseSpec:
 
     @property
     def ncell(self) -> int:
-        return self.n ** 3 * (6 if self.shape == TET else 1)
+        return self.n ** 3 * {HEX: 1, TET: 6, PRISM: 2}[self.shape]
 
 
 # The benchmark configurations (sizes as specified) and reduced versions of

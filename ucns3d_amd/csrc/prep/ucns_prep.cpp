@@ -30,20 +30,26 @@ int fail(int code, const std::string &msg) {
 const int HEX_FACES[6][4] = {{1, 0, 3, 2}, {4, 5, 6, 7}, {5, 1, 2, 6},
                              {7, 3, 0, 4}, {6, 2, 3, 7}, {4, 0, 1, 5}};
 const int TET_FACES[4][3] = {{1, 3, 0}, {1, 0, 2}, {1, 2, 3}, {2, 0, 3}};
+// prism: faces 1-2 triangles (top, bottom), 3-5 quadrilaterals
+const int PRISM_FACES[5][4] = {{4, 5, 3, -1}, {1, 0, 2, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}};
 // Sub-tetrahedra of DECOMPOSE3, src/grid_t.f90:1027-1074.
 const int HEX_TETS[6][4] = {{0, 5, 7, 4}, {0, 1, 7, 5}, {1, 6, 7, 5},
                             {0, 7, 2, 3}, {0, 7, 1, 2}, {1, 7, 6, 2}};
 const int TET_TETS[1][4] = {{0, 1, 2, 3}};
+// prism (DECOMPOSE3 element type 4, src/grid_t.f90:1065-1069)
+const int PRISM_TETS[3][4] = {{0, 1, 2, 5}, {0, 1, 5, 4}, {0, 4, 5, 3}};
 
-inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : 4; }
-inline int nnodes_of(int shape) { return shape == UCNSP_HEX ? 8 : 4; }
-inline int face_nv(int shape, int) { return shape == UCNSP_HEX ? 4 : 3; }
-inline int face_node(int shape, int f, int k) {
-    return shape == UCNSP_HEX ? HEX_FACES[f][k] : TET_FACES[f][k];
+inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_PRISM ? 5 : 4; }
+inline int nnodes_of(int shape) { return shape == UCNSP_HEX ? 8 : shape == UCNSP_PRISM ? 6 : 4; }
+inline int face_nv(int shape, int f) {
+    return shape == UCNSP_HEX ? 4 : shape == UCNSP_PRISM ? (f < 2 ? 3 : 4) : 3;
 }
-inline int nsubtets(int shape) { return shape == UCNSP_HEX ? 6 : 1; }
+inline int face_node(int shape, int f, int k) {
+    return shape == UCNSP_HEX ? HEX_FACES[f][k] : shape == UCNSP_PRISM ? PRISM_FACES[f][k] : TET_FACES[f][k];
+}
+inline int nsubtets(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_PRISM ? 3 : 1; }
 inline const int *subtet(int shape, int t) {
-    return shape == UCNSP_HEX ? HEX_TETS[t] : TET_TETS[t];
+    return shape == UCNSP_HEX ? HEX_TETS[t] : shape == UCNSP_PRISM ? PRISM_TETS[t] : TET_TETS[t];
 }
 
 // ----------------------------------------------------------------------------
@@ -1321,7 +1327,7 @@ void gen_box(ucnsp_case &c) {
             }
     const int64_t nhex = (int64_t)nx * ny * nz;
     c.shape = o.shape;
-    c.ncell = (o.shape == UCNSP_HEX) ? nhex : 6 * nhex;
+    c.ncell = (o.shape == UCNSP_HEX) ? nhex : (o.shape == UCNSP_PRISM) ? 2 * nhex : 6 * nhex;
     c.cnodes.assign(c.ncell * 8, -1);
     c.cshape.assign(c.ncell, o.shape);
     for (int k = 0; k < nz; ++k)
@@ -1332,6 +1338,12 @@ void gen_box(ucnsp_case &c) {
                                 nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
                 if (o.shape == UCNSP_HEX) {
                     for (int q = 0; q < 8; ++q) c.cnodes[h * 8 + q] = (int32_t)v[q];
+                } else if (o.shape == UCNSP_PRISM) {
+                    // two prisms per hex, split along the bottom face's 0-2 diagonal;
+                    // bottom triangle counter-clockwise seen from +z, top above it
+                    const int pr[2][6] = {{0, 1, 2, 4, 5, 6}, {0, 2, 3, 4, 6, 7}};
+                    for (int t = 0; t < 2; ++t)
+                        for (int q = 0; q < 6; ++q) c.cnodes[(2 * h + t) * 8 + q] = (int32_t)v[pr[t][q]];
                 } else {
                     // Kuhn split: 6 tets along the 000-111 diagonal, one per axis order
                     const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -1401,7 +1413,8 @@ int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n
 int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
-    if (o.shape != UCNSP_HEX && o.shape != UCNSP_TET) return fail(UCNS_EUNSUPPORTED, "shape must be hex(1) or tet(2)");
+    if (o.shape != UCNSP_HEX && o.shape != UCNSP_TET && o.shape != UCNSP_PRISM)
+        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2) or prism(3)");
     if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
     for (int d = 0; d < 3; ++d) {
         const int a = o.bc[2 * d], b = o.bc[2 * d + 1];

@@ -32,9 +32,11 @@ extern "C" {
 /* element shapes (IELEM%ISHAPE) */
 #define UCNSP_HEX    1
 #define UCNSP_TET    2
+#define UCNSP_PRISM  3
 
 typedef struct ucnsp_opts {
-    int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex */
+    int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex;
+                               UCNSP_PRISM: 2 prisms per hex */
     int32_t nx, ny, nz;
     double  lx, ly, lz;     /* periodic box [0,lx]x[0,ly]x[0,lz] */
     double  jitter;         /* interior vertices moved by U(-j,j)*spacing per coordinate */
