@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, PRESETS, PRISM, TET, Case, CaseSpec, reduced
+from ucns3d_amd.case import HEX, PRESETS, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
 from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
@@ -26,6 +26,10 @@ CASES = {
     "prism-weno3-6": lambda: CaseSpec("prism-weno3-6", PRISM, 6, "weno3", jitter=0.1, seed=13),
     "prism-weno4-6": lambda: CaseSpec("prism-weno4-6", PRISM, 6, "weno4", jitter=0.1, seed=14),
     "prism-muscl-6": lambda: CaseSpec("prism-muscl-6", PRISM, 6, "muscl", jitter=0.1, seed=15),
+    # pyramids: 1 quadrilateral + 4 triangular faces (6 per hex, apex at its centre)
+    "pyramid-weno3-4": lambda: CaseSpec("pyramid-weno3-4", PYRAMID, 4, "weno3", jitter=0.1, seed=17),
+    "pyramid-weno4-4": lambda: CaseSpec("pyramid-weno4-4", PYRAMID, 4, "weno4", jitter=0.1, seed=18),
+    "pyramid-muscl-4": lambda: CaseSpec("pyramid-muscl-4", PYRAMID, 4, "muscl", jitter=0.1, seed=19),
 }
 
 _cache = {}
@@ -54,6 +58,10 @@ def valid_uleft_mask(case):
     if case.shape == PRISM:
         mask[:, :2, :case.qp_tri] = True   # the two triangles
         mask[:, 2:5, :case.qp_quad] = True  # the three quadrilaterals
+        return mask
+    if case.shape == PYRAMID:
+        mask[:, :1, :case.qp_quad] = True   # the base
+        mask[:, 1:5, :case.qp_tri] = True   # the four triangles
         return mask
     faces = 6 if case.shape == HEX else 4
     nq = case.qp_quad if case.shape == HEX else case.qp_tri

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, PRISM, TET, Case, CaseSpec, reduced
+from ucns3d_amd.case import HEX, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
 
 from test_oracle import G, prim2cons, random_state
 
@@ -84,6 +84,7 @@ CASES = {
     "hex-weno4-channel": lambda: (CaseSpec("channel", HEX, 12, "weno4", seed=3, bc=(1, 2, 4, 4, 3, 3)),
                                   (1, prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4), prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4))),
     "prism-weno3": lambda: (CaseSpec("prism", PRISM, 5, "weno3", jitter=0.1, seed=16), None),
+    "pyramid-weno3": lambda: (CaseSpec("pyramid", PYRAMID, 4, "weno3", jitter=0.1, seed=17), None),
 }
 _cache = {}
 

@@ -18,13 +18,14 @@ from . import native
 HEX = 1
 TET = 2
 PRISM = 3
+PYRAMID = 4
 
 
 @dataclass(frozen=True)
 class CaseSpec:
     """Mesh and scheme choices of one configuration."""
     name: str
-    shape: int            # HEX, TET (6 Kuhn tets per hex) or PRISM (2 per hex)
+    shape: int            # HEX, TET (6 Kuhn tets per hex), PRISM (2 per hex) or PYRAMID (6 per hex)
     n: int                # hexes per direction
     scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2) or "weno4" (IORDER 3)
     jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
@@ -52,7 +53,7 @@ class CaseSpec:
 
     @property
     def ncell(self) -> int:
-        return self.n ** 3 * {HEX: 1, TET: 6, PRISM: 2}[self.shape]
+        return self.n ** 3 * {HEX: 1, TET: 6, PRISM: 2, PYRAMID: 6}[self.shape]
 
 
 # The benchmark configurations (sizes as specified) and reduced versions of

@@ -32,24 +32,38 @@ const int HEX_FACES[6][4] = {{1, 0, 3, 2}, {4, 5, 6, 7}, {5, 1, 2, 6},
 const int TET_FACES[4][3] = {{1, 3, 0}, {1, 0, 2}, {1, 2, 3}, {2, 0, 3}};
 // prism: faces 1-2 triangles (top, bottom), 3-5 quadrilaterals
 const int PRISM_FACES[5][4] = {{4, 5, 3, -1}, {1, 0, 2, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}};
+// pyramid (src/grid_p.f90:2100-2124): face 1 the quadrilateral base, 2-5 triangles to the apex (node 5)
+const int PYR_FACES[5][4] = {{3, 2, 1, 0}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {3, 0, 4, -1}};
 // Sub-tetrahedra of DECOMPOSE3, src/grid_t.f90:1027-1074.
 const int HEX_TETS[6][4] = {{0, 5, 7, 4}, {0, 1, 7, 5}, {1, 6, 7, 5},
                             {0, 7, 2, 3}, {0, 7, 1, 2}, {1, 7, 6, 2}};
 const int TET_TETS[1][4] = {{0, 1, 2, 3}};
 // prism (DECOMPOSE3 element type 4, src/grid_t.f90:1065-1069)
 const int PRISM_TETS[3][4] = {{0, 1, 2, 5}, {0, 1, 5, 4}, {0, 4, 5, 3}};
+// pyramid (DECOMPOSE3 element type 3, src/grid_t.f90:1058-1060)
+const int PYR_TETS[2][4] = {{0, 1, 2, 4}, {0, 2, 3, 4}};
 
-inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_PRISM ? 5 : 4; }
-inline int nnodes_of(int shape) { return shape == UCNSP_HEX ? 8 : shape == UCNSP_PRISM ? 6 : 4; }
+inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_TET ? 4 : 5; }
+inline int nnodes_of(int shape) {
+    return shape == UCNSP_HEX ? 8 : shape == UCNSP_PRISM ? 6 : shape == UCNSP_PYRAMID ? 5 : 4;
+}
 inline int face_nv(int shape, int f) {
-    return shape == UCNSP_HEX ? 4 : shape == UCNSP_PRISM ? (f < 2 ? 3 : 4) : 3;
+    return shape == UCNSP_HEX ? 4 : shape == UCNSP_PRISM ? (f < 2 ? 3 : 4) : shape == UCNSP_PYRAMID ? (f == 0 ? 4 : 3) : 3;
 }
 inline int face_node(int shape, int f, int k) {
-    return shape == UCNSP_HEX ? HEX_FACES[f][k] : shape == UCNSP_PRISM ? PRISM_FACES[f][k] : TET_FACES[f][k];
+    return shape == UCNSP_HEX     ? HEX_FACES[f][k]
+           : shape == UCNSP_PRISM ? PRISM_FACES[f][k]
+           : shape == UCNSP_PYRAMID ? PYR_FACES[f][k]
+                                    : TET_FACES[f][k];
 }
-inline int nsubtets(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_PRISM ? 3 : 1; }
+inline int nsubtets(int shape) {
+    return shape == UCNSP_HEX ? 6 : shape == UCNSP_PRISM ? 3 : shape == UCNSP_PYRAMID ? 2 : 1;
+}
 inline const int *subtet(int shape, int t) {
-    return shape == UCNSP_HEX ? HEX_TETS[t] : shape == UCNSP_PRISM ? PRISM_TETS[t] : TET_TETS[t];
+    return shape == UCNSP_HEX     ? HEX_TETS[t]
+           : shape == UCNSP_PRISM ? PRISM_TETS[t]
+           : shape == UCNSP_PYRAMID ? PYR_TETS[t]
+                                    : TET_TETS[t];
 }
 
 // ----------------------------------------------------------------------------
@@ -1326,6 +1340,25 @@ void gen_box(ucnsp_case &c) {
                 c.canon[n] = (int32_t)(ci + (nx + 1) * (cj + (int64_t)(ny + 1) * ck));
             }
     const int64_t nhex = (int64_t)nx * ny * nz;
+    const int64_t ngrid = c.nnode;
+    if (o.shape == UCNSP_PYRAMID) {
+        // one apex node per hex, at the mean of its corners
+        c.nnode += nhex;
+        c.xyz.resize(c.nnode * 3);
+        c.canon.resize(c.nnode);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k), n = ngrid + h;
+                    double m[3] = {0, 0, 0};
+                    for (int b = 0; b < 8; ++b) {
+                        const int64_t q = nid(i + (b & 1), j + ((b >> 1) & 1), k + (b >> 2));
+                        for (int d = 0; d < 3; ++d) m[d] += 0.125 * c.xyz[q * 3 + d];
+                    }
+                    for (int d = 0; d < 3; ++d) c.xyz[n * 3 + d] = m[d];
+                    c.canon[n] = (int32_t)n;
+                }
+    }
     c.shape = o.shape;
     c.ncell = (o.shape == UCNSP_HEX) ? nhex : (o.shape == UCNSP_PRISM) ? 2 * nhex : 6 * nhex;
     c.cnodes.assign(c.ncell * 8, -1);
@@ -1338,6 +1371,14 @@ void gen_box(ucnsp_case &c) {
                                 nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
                 if (o.shape == UCNSP_HEX) {
                     for (int q = 0; q < 8; ++q) c.cnodes[h * 8 + q] = (int32_t)v[q];
+                } else if (o.shape == UCNSP_PYRAMID) {
+                    // six pyramids per hex: each hex face is a base (reversed, so
+                    // that its normal points to the apex), the hex centre the apex
+                    for (int f = 0; f < 6; ++f) {
+                        const int64_t cell = h * 6 + f;
+                        for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)v[HEX_FACES[f][3 - q]];
+                        c.cnodes[cell * 8 + 4] = (int32_t)(ngrid + h);
+                    }
                 } else if (o.shape == UCNSP_PRISM) {
                     // two prisms per hex, split along the bottom face's 0-2 diagonal;
                     // bottom triangle counter-clockwise seen from +z, top above it
@@ -1413,8 +1454,8 @@ int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n
 int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
-    if (o.shape != UCNSP_HEX && o.shape != UCNSP_TET && o.shape != UCNSP_PRISM)
-        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2) or prism(3)");
+    if (o.shape < UCNSP_HEX || o.shape > UCNSP_PYRAMID)
+        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3) or pyramid(4)");
     if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
     for (int d = 0; d < 3; ++d) {
         const int a = o.bc[2 * d], b = o.bc[2 * d + 1];
