@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, PRESETS, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
+from ucns3d_amd.case import HEX, HYBRID, PRESETS, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
 from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
@@ -30,6 +30,9 @@ CASES = {
     "pyramid-weno3-4": lambda: CaseSpec("pyramid-weno3-4", PYRAMID, 4, "weno3", jitter=0.1, seed=17),
     "pyramid-weno4-4": lambda: CaseSpec("pyramid-weno4-4", PYRAMID, 4, "weno4", jitter=0.1, seed=18),
     "pyramid-muscl-4": lambda: CaseSpec("pyramid-muscl-4", PYRAMID, 4, "muscl", jitter=0.1, seed=19),
+    # hexes, pyramids and prisms in one mesh (thirds of x)
+    "hybrid-weno3-6": lambda: CaseSpec("hybrid-weno3-6", HYBRID, 6, "weno3", jitter=0.1, seed=20),
+    "hybrid-weno4-6": lambda: CaseSpec("hybrid-weno4-6", HYBRID, 6, "weno4", jitter=0.1, seed=21),
 }
 
 _cache = {}
@@ -51,21 +54,19 @@ def rel_l2(a, b):
     return num / np.where(den > 0, den, 1.0)
 
 
+# face node counts of each element type (HEX, TET, PRISM, PYRAMID), in face order
+FACE_NV = {HEX: [4] * 6, TET: [3] * 4, PRISM: [3, 3, 4, 4, 4], PYRAMID: [4, 3, 3, 3, 3]}
+
+
 def valid_uleft_mask(case):
     """face points that exist (triangles use fewer than the padded count)"""
     nqm = max(case.qp_quad, case.qp_tri)
     mask = np.zeros((case.ncell, case.nface_max, nqm), dtype=bool)
-    if case.shape == PRISM:
-        mask[:, :2, :case.qp_tri] = True   # the two triangles
-        mask[:, 2:5, :case.qp_quad] = True  # the three quadrilaterals
-        return mask
-    if case.shape == PYRAMID:
-        mask[:, :1, :case.qp_quad] = True   # the base
-        mask[:, 1:5, :case.qp_tri] = True   # the four triangles
-        return mask
-    faces = 6 if case.shape == HEX else 4
-    nq = case.qp_quad if case.shape == HEX else case.qp_tri
-    mask[:, :faces, :nq] = True
+    shapes = case.array("cell_shape")
+    for sh, nvs in FACE_NV.items():
+        cells = shapes == sh
+        for f, nv in enumerate(nvs):
+            mask[cells, f, :(case.qp_quad if nv == 4 else case.qp_tri)] = True
     return mask
 
 

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
+from ucns3d_amd.case import HEX, HYBRID, PRISM, PYRAMID, TET, Case, CaseSpec, reduced
 
 from test_oracle import G, prim2cons, random_state
 
@@ -85,6 +85,7 @@ CASES = {
                                   (1, prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4), prim2cons(1.0, 0.5, 0.0, 0.0, 100 / 1.4))),
     "prism-weno3": lambda: (CaseSpec("prism", PRISM, 5, "weno3", jitter=0.1, seed=16), None),
     "pyramid-weno3": lambda: (CaseSpec("pyramid", PYRAMID, 4, "weno3", jitter=0.1, seed=17), None),
+    "hybrid-weno4": lambda: (CaseSpec("hybrid", HYBRID, 6, "weno4", jitter=0.1, seed=21), None),
 }
 _cache = {}
 

@@ -19,13 +19,14 @@ HEX = 1
 TET = 2
 PRISM = 3
 PYRAMID = 4
+HYBRID = 5  # hexes, pyramids and prisms in thirds of x
 
 
 @dataclass(frozen=True)
 class CaseSpec:
     """Mesh and scheme choices of one configuration."""
     name: str
-    shape: int            # HEX, TET (6 Kuhn tets per hex), PRISM (2 per hex) or PYRAMID (6 per hex)
+    shape: int            # HEX, TET (6 Kuhn tets per hex), PRISM (2 per hex), PYRAMID (6 per hex) or HYBRID
     n: int                # hexes per direction
     scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2) or "weno4" (IORDER 3)
     jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
@@ -53,6 +54,8 @@ class CaseSpec:
 
     @property
     def ncell(self) -> int:
+        if self.shape == HYBRID:
+            return self.n ** 2 * sum(1 if i < self.n // 3 else 6 if i < 2 * self.n // 3 else 2 for i in range(self.n))
         return self.n ** 3 * {HEX: 1, TET: 6, PRISM: 2, PYRAMID: 6}[self.shape]
 
 

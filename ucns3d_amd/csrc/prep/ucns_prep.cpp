@@ -1341,7 +1341,17 @@ void gen_box(ucnsp_case &c) {
             }
     const int64_t nhex = (int64_t)nx * ny * nz;
     const int64_t ngrid = c.nnode;
-    if (o.shape == UCNSP_PYRAMID) {
+    // element type of the hexes in column i: one type, or for the hybrid box
+    // hexahedra, pyramids and prisms in thirds of x (their shared faces are
+    // the hexes' quadrilaterals, so the mesh is conforming)
+    auto etype = [&](int i) {
+        return o.shape != UCNSP_HYBRID ? o.shape
+               : i < nx / 3            ? UCNSP_HEX
+               : i < 2 * nx / 3        ? UCNSP_PYRAMID
+                                       : UCNSP_PRISM;
+    };
+    auto per_hex = [](int t) { return t == UCNSP_HEX ? 1 : t == UCNSP_PRISM ? 2 : 6; };
+    if (o.shape == UCNSP_PYRAMID || o.shape == UCNSP_HYBRID) {
         // one apex node per hex, at the mean of its corners
         c.nnode += nhex;
         c.xyz.resize(c.nnode * 3);
@@ -1360,31 +1370,37 @@ void gen_box(ucnsp_case &c) {
                 }
     }
     c.shape = o.shape;
-    c.ncell = (o.shape == UCNSP_HEX) ? nhex : (o.shape == UCNSP_PRISM) ? 2 * nhex : 6 * nhex;
+    c.ncell = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) c.ncell += per_hex(etype(i));
     c.cnodes.assign(c.ncell * 8, -1);
     c.cshape.assign(c.ncell, o.shape);
+    int64_t cell0 = 0;  // first cell of the current hex
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k);
                 int64_t v[8] = {nid(i, j, k),     nid(i + 1, j, k),     nid(i + 1, j + 1, k),     nid(i, j + 1, k),
                                 nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
-                if (o.shape == UCNSP_HEX) {
-                    for (int q = 0; q < 8; ++q) c.cnodes[h * 8 + q] = (int32_t)v[q];
-                } else if (o.shape == UCNSP_PYRAMID) {
+                const int et = etype(i);
+                for (int q = 0; q < per_hex(et); ++q) c.cshape[cell0 + q] = et;
+                if (et == UCNSP_HEX) {
+                    for (int q = 0; q < 8; ++q) c.cnodes[cell0 * 8 + q] = (int32_t)v[q];
+                } else if (et == UCNSP_PYRAMID) {
                     // six pyramids per hex: each hex face is a base (reversed, so
                     // that its normal points to the apex), the hex centre the apex
                     for (int f = 0; f < 6; ++f) {
-                        const int64_t cell = h * 6 + f;
+                        const int64_t cell = cell0 + f;
                         for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)v[HEX_FACES[f][3 - q]];
                         c.cnodes[cell * 8 + 4] = (int32_t)(ngrid + h);
                     }
-                } else if (o.shape == UCNSP_PRISM) {
+                } else if (et == UCNSP_PRISM) {
                     // two prisms per hex, split along the bottom face's 0-2 diagonal;
                     // bottom triangle counter-clockwise seen from +z, top above it
                     const int pr[2][6] = {{0, 1, 2, 4, 5, 6}, {0, 2, 3, 4, 6, 7}};
                     for (int t = 0; t < 2; ++t)
-                        for (int q = 0; q < 6; ++q) c.cnodes[(2 * h + t) * 8 + q] = (int32_t)v[pr[t][q]];
+                        for (int q = 0; q < 6; ++q) c.cnodes[(cell0 + t) * 8 + q] = (int32_t)v[pr[t][q]];
                 } else {
                     // Kuhn split: 6 tets along the 000-111 diagonal, one per axis order
                     const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -1418,10 +1434,11 @@ void gen_box(ucnsp_case &c) {
                         double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
                                      e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
                         if (det < 0) std::swap(tv[1], tv[2]);
-                        int64_t cell = h * 6 + t;
+                        int64_t cell = cell0 + t;
                         for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)tv[q];
                     }
                 }
+                cell0 += per_hex(et);
             }
 }
 
@@ -1454,8 +1471,8 @@ int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n
 int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
-    if (o.shape < UCNSP_HEX || o.shape > UCNSP_PYRAMID)
-        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3) or pyramid(4)");
+    if (o.shape < UCNSP_HEX || o.shape > UCNSP_HYBRID)
+        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3), pyramid(4) or hybrid(5)");
     if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
     for (int d = 0; d < 3; ++d) {
         const int a = o.bc[2 * d], b = o.bc[2 * d + 1];

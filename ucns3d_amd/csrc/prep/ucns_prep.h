@@ -34,11 +34,13 @@ extern "C" {
 #define UCNSP_TET    2
 #define UCNSP_PRISM  3
 #define UCNSP_PYRAMID 4
+#define UCNSP_HYBRID 5
 
 typedef struct ucnsp_opts {
     int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex;
                                UCNSP_PRISM: 2 prisms per hex; UCNSP_PYRAMID: 6 pyramids
-                               per hex (its faces as bases, its centre the apex) */
+                               per hex (its faces as bases, its centre the apex);
+                               UCNSP_HYBRID: hexes, pyramids, prisms in thirds of x */
     int32_t nx, ny, nz;
     double  lx, ly, lz;     /* periodic box [0,lx]x[0,ly]x[0,lz] */
     double  jitter;         /* interior vertices moved by U(-j,j)*spacing per coordinate */
