@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle
 from oracle.oracle import Oracle
-from ucns3d_amd.case import HEX, TET, Case, CaseSpec
+from ucns3d_amd.case import HEX, HYBRID, TET, Case, CaseSpec
 from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
@@ -36,6 +36,12 @@ BOXES = {
     # closed box, MUSCL
     "hex-muscl-walls": (CaseSpec("walls", HEX, 8, "muscl", seed=6, bc=(4,) * 6), 0,
                         prim2cons(1, 0, 0, 0, P0), prim2cons(1, 0, 0, 0, P0)),
+    # the adjoint channel's construction: hexahedra, pyramids and prisms, a lower-wall bump
+    # (a_k ~ U(0, 0.01)), WENO-3, subsonic inflow/outflow and slip walls
+    "hybrid-weno3-bump": (CaseSpec("bump", HYBRID, 6, "weno3", seed=7, bc=(1, 2, 4, 4, 3, 3),
+                                   bump=tuple(np.random.default_rng(22).uniform(0, 0.01, 8))), 1,
+                          prim2cons(1.0, 0.6 * math.sqrt(G * P0), 0.0, 0.0, P0),
+                          prim2cons(1.0, 0.6 * math.sqrt(G * P0), 0.0, 0.0, P0)),
 }
 _cache = {}
 

@@ -38,6 +38,7 @@ class CaseSpec:
     lwci1: float = 10000.0   # wenocentralweight of tests/taylor_green_vortex/UCNS3D.DAT
     gamma: float = 1.4
     bc: tuple = (0, 0, 0, 0, 0, 0)  # codes of the x-, x+, y-, y+, z-, z+ sides; 0 = periodic axis
+    bump: tuple = ()      # lower-wall bump coefficients a_1..a_8 (channel configuration)
 
     @property
     def iorder(self) -> int:
@@ -83,7 +84,8 @@ class Case:
         o = native.PrepOpts(shape=spec.shape, nx=spec.n, ny=spec.n, nz=spec.n,
                             lx=spec.length, ly=spec.length, lz=spec.length, jitter=spec.jitter, seed=spec.seed,
                             iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads,
-                            bc=(C.c_int32 * 6)(*spec.bc))
+                            bc=(C.c_int32 * 6)(*spec.bc),
+                            bump=(C.c_double * 8)(*(tuple(spec.bump) + (0.0,) * (8 - len(spec.bump)))))
         h = C.c_void_p()
         rc = lib.ucnsp_build(C.byref(o), C.byref(h))
         if rc != 0:

@@ -1309,6 +1309,23 @@ void build_prestore(ucnsp_case &c) {
         if (!ok[k]) c.full[k / TS] = 0;  // NaN stencils disable FULL (src/prestore.f90:526-529)
 }
 
+// The lower-wall bump of the channel configuration, faded out towards y = ly;
+// applied after the box sides are classified (on the undeformed box).
+void apply_bump(ucnsp_case &c) {
+    const ucnsp_opts &o = c.o;
+    bool bumped = false;
+    for (int q = 0; q < 8; ++q) bumped = bumped || o.bump[q] != 0.0;
+    if (!bumped) return;
+    for (int64_t n = 0; n < c.nnode; ++n) {
+        double *p = &c.xyz[n * 3];
+        const double xh = 4.0 * p[0] / o.lx, zh = p[2] / o.lz;
+        if (xh < 1.0 || xh > 3.0) continue;
+        double w = 0.0;
+        for (int q = 0; q < 8; ++q) w += o.bump[q] * std::sin((q + 1) * M_PI * (xh - 1.0) / 2.0);
+        p[1] += o.ly * (1.0 - p[1] / o.ly) * w * std::sin(M_PI * zh);
+    }
+}
+
 void gen_box(ucnsp_case &c) {
     const ucnsp_opts &o = c.o;
     const int nx = o.nx, ny = o.ny, nz = o.nz;
@@ -1506,6 +1523,7 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
                 delete c;
                 return fail(UCNS_EBADARG, "unmatched face off the box sides");
             }
+    apply_bump(*c);
     build_geometry(*c);
     lap("geometry");
     // scheme sizes (DETERMINE_SIZE, src/grid_p.f90:6097-6288; parameters.f90:1270-1394)

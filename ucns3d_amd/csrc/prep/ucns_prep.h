@@ -53,6 +53,8 @@ typedef struct ucnsp_opts {
     int32_t bc[6];          /* boundary code of the x-, x+, y-, y+, z-, z+ sides; 0 on both sides of an
                                axis makes it periodic (code 5 on the paired faces).  Codes as IBOUND%ICODE:
                                1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield */
+    double  bump[8];        /* lower-wall bump a_k: with x^ = 4x/lx, z^ = z/lz, the nodes move by
+                               dy = ly (1 - y/ly) sum_k a_k sin(k pi (x^-1)/2) sin(pi z^) for 1 <= x^ <= 3 */
 } ucnsp_opts;
 
 typedef struct ucnsp_case ucnsp_case;

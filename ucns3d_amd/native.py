@@ -72,7 +72,7 @@ class PrepOpts(C.Structure):
         ("shape", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double), ("jitter", C.c_double),
         ("seed", C.c_uint64), ("iorder", C.c_int32), ("iweno", C.c_int32), ("extf", C.c_int32),
-        ("weight_lsqr", C.c_int32), ("nthreads", C.c_int32), ("bc", C.c_int32 * 6),
+        ("weight_lsqr", C.c_int32), ("nthreads", C.c_int32), ("bc", C.c_int32 * 6), ("bump", C.c_double * 8),
     ]
 
 
