@@ -86,7 +86,16 @@ CASES = {
     "prism-weno3": lambda: (CaseSpec("prism", PRISM, 5, "weno3", jitter=0.1, seed=16), None),
     "pyramid-weno3": lambda: (CaseSpec("pyramid", PYRAMID, 4, "weno3", jitter=0.1, seed=17), None),
     "hybrid-weno4": lambda: (CaseSpec("hybrid", HYBRID, 6, "weno4", jitter=0.1, seed=21), None),
+    # the adjoint channel's construction: bumped lower wall, inflow/outflow, slip walls
+    "hybrid-weno3-bump": lambda: (CaseSpec("bump", HYBRID, 6, "weno3", seed=7, bc=(1, 2, 4, 4, 3, 3),
+                                           bump=tuple(np.random.default_rng(22).uniform(0, 0.01, 8))),
+                                  (1, prim2cons(1.0, 0.6 * math.sqrt(1.4 * 100 / 1.4), 0.0, 0.0, 100 / 1.4),
+                                   prim2cons(1.0, 0.6 * math.sqrt(1.4 * 100 / 1.4), 0.0, 0.0, 100 / 1.4))),
 }
+# cases kept out of the difference test: on the bumped channel 473 of 648
+# cells take the Barth limiter, whose arg-min face point moves between
+# points at kinks of the primal
+FD_KINKED = {"hybrid-weno3-bump"}
 _cache = {}
 
 
@@ -134,6 +143,9 @@ def setup(name):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_residual_tangent_matches_differences(name):
+    if name in FD_KINKED:
+        pytest.skip("most cells are MUSCL cells: central differences straddle limiter switches "
+                    "(the GPU product is checked against the tangent model by the dot-product test)")
     case, orc, u, du, bc = setup(name)
     r, dr = orc.residual_tl(u, du)
     r0, _ = orc.residual(u)
