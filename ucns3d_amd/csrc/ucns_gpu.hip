@@ -2915,7 +2915,7 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp_s(VjpParams<R> p) {
 // ubar[n] += the rows of every tile's member table that belong to cell n
 // (CSR: rows ent[off[n]] .. ent[off[n+1]-1]); each cell is one thread's, so
 // no atomics
-__global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *off, const int64_t *ent,
+__global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *off, const int32_t *ent,
                                                     const double *tab, double *ubar) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= ncell) return;
@@ -2925,7 +2925,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *of
     constexpr int GU = 4;
     const int64_t e0 = off[n], e1 = off[n + 1];
     for (int64_t e = e0; e < e1; e += GU) {
-        int64_t rr[GU];
+        int32_t rr[GU];
 #pragma unroll
         for (int u = 0; u < GU; ++u) rr[u] = ent[min(e + u, e1 - 1)];
         double x[GU][NV];
@@ -2946,14 +2946,14 @@ __global__ __launch_bounds__(256) void k_adj_gather(int ncell, const int64_t *of
 
 // CSR of the member tables by cell: counts, then positions
 __global__ void k_adj_csr(int ntile, int cap, const int32_t *uq, const int32_t *ucount, const int64_t *toff,
-                          int64_t *cnt, const int64_t *off, int64_t *ent) {
+                          int64_t *cnt, const int64_t *off, int32_t *ent) {
     const int t = blockIdx.x;
     if (t >= ntile) return;
     const int nu = min(ucount[t], cap);
     for (int r = threadIdx.x; r < nu; r += blockDim.x) {
         const int n = uq[(size_t)t * cap + r];
         const unsigned long long k = atomicAdd((unsigned long long *)&cnt[n], 1ull);
-        if (off) ent[off[n] + (int64_t)k] = toff[t] + r;
+        if (off) ent[off[n] + (int64_t)k] = (int32_t)(toff[t] + r);
     }
 }
 
@@ -3201,7 +3201,8 @@ struct ucns_ctx {
     uint16_t *adj_slot = nullptr;                      // [ntile][imax][8 ts] LDS row of each member entry
     int32_t *adj_uq = nullptr, *adj_ucount = nullptr;  // [ntile][uq_cap], [ntile]
     int uq_cap = UCNS_TILE_UQ;                         // member-table rows per tile (tile_uq(ts, imax))
-    int64_t *adj_toff = nullptr, *adj_off = nullptr, *adj_ent = nullptr;  // table rows per tile, CSR by cell
+    int64_t *adj_toff = nullptr, *adj_off = nullptr;  // table rows per tile, CSR offsets by cell
+    int32_t *adj_ent = nullptr;                       // CSR entries: table rows (fewer than 2^31)
     double *adj_tab = nullptr;                          // [rows][5] member tables
     bool adj_g_on = false;  // the next tiled reconstruction stores them
     bool adj_fuse = false;  // row-stream mode: scatter on the next row stream (UCNS_ADJ_FUSE=1) or in the product
@@ -3636,6 +3637,10 @@ int ensure_adjoint(ucns_ctx *ctx) {
         std::vector<int64_t> toff(nt + 1, 0);
         for (int t = 0; t < nt; ++t) toff[t + 1] = toff[t] + std::min(uc[t], ctx->uq_cap);
         const int64_t rows = toff[nt];
+        if (rows >= ((int64_t)1 << 31)) {
+            ctx->err = "member tables exceed 2^31 rows";
+            return UCNS_EUNSUPPORTED;
+        }
         if ((rc = dalloc(ctx, &ctx->adj_toff, (size_t)nt + 1))) return rc;
         if ((rc = dalloc(ctx, &ctx->adj_tab, (size_t)rows * NV))) return rc;
         if ((rc = dalloc(ctx, &ctx->adj_off, (size_t)nc + 1))) return rc;
@@ -3647,7 +3652,7 @@ int ensure_adjoint(ucns_ctx *ctx) {
         if ((rc = dalloc(ctx, &cnt, (size_t)nc))) return rc;
         HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
         hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->uq_cap, ctx->adj_uq,
-                           ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)nullptr, (int64_t *)nullptr);
+                           ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)nullptr, (int32_t *)nullptr);
         std::vector<int64_t> hc(nc), off(nc + 1, 0);
         HIP_TRY(hipMemcpyAsync(hc.data(), cnt, nc * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
