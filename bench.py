@@ -50,12 +50,9 @@ def recon_bytes_per_cell(case, precision: int) -> float:
 
 def recon_kernel_name(case, precision: int) -> str:
     """the reconstruction kernel the library launches for this case: the
-    row-stream kernels for tetrahedra (5 stencils) and hexahedra (7), the
-    register-prefetch ones when UCNS_STREAM=0"""
+    row-stream kernels for tetrahedra (5 stencils) and hexahedra (7)"""
     tet = case.shape == 2
     ts, nql = (5, 3) if tet else (7, 7)
-    if os.environ.get("UCNS_STREAM") == "0" or os.environ.get("UCNS_RING") == "1":
-        return f"k_recon_{'w32' if precision else 'w64'}<19,39,{nql}>"
     return f"k_recon_{'s32' if precision else 's64'}<19,39,{nql},{ts}>"
 
 
@@ -63,8 +60,6 @@ def committed_traffic(kernel: str, config: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by the copy calibration) and committed under
     profiles/ as 'traffic <kernel> <config> <bytes>' lines; None if absent"""
-    if os.environ.get("UCNS_RING") == "1":
-        return None
     d = os.path.join(REPO, "profiles")
     for fn in sorted(os.listdir(d)) if os.path.isdir(d) else []:
         if not fn.endswith(".txt"):
@@ -109,8 +104,9 @@ def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) ->
     forward sweep with stage checkpoints plus the reverse sweep) against
     ucns_rk3_steps over the same steps, device-resident cotangents.  Timed
     twice: with every stage's forward reconstruction kept for the reverse
-    sweep (the library's default when it fits in memory) and with none kept
-    (UCNS_ADJ_KEEP=0: the reverse sweep runs each reconstruction again)."""
+    sweep (the library's default, as many as fit in its memory budget) and
+    with none kept (a context with adj_keep_stages < 0: every reversed stage
+    runs its reconstruction again)."""
     import torch
     from ucns3d_amd.solver import Solver
 
@@ -139,15 +135,15 @@ def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) ->
         return time.perf_counter() - t0
     t_adj = adjoint()
     finite = bool(torch.isfinite(u0b).all().item())
-    os.environ["UCNS_ADJ_KEEP"] = "0"
-    try:
-        t_rec = adjoint()
-        s.set_state(u0)
-        s.reset_stats()
-        s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
-        st = s.stats()
-    finally:
-        del os.environ["UCNS_ADJ_KEEP"]
+    s.close()
+    s = Solver(case, precision, adj_keep_stages=-1)
+    s.set_state(u0)
+    s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)  # allocates checkpoints
+    t_rec = adjoint()
+    s.set_state(u0)
+    s.reset_stats()
+    s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
+    st = s.stats()
     s.close()
     return {"steps": steps, "primal_ms_per_step": t_fwd / steps * 1e3, "adjoint_ms_per_step": t_adj / steps * 1e3,
             "adjoint_over_primal": t_adj / t_fwd, "reverse_sweep_over_primal": (t_adj - t_fwd) / t_fwd,

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define UCNS_ABI_VERSION 1
+#define UCNS_ABI_VERSION 2
 
 /* return codes */
 #define UCNS_OK             0
@@ -54,10 +54,22 @@ extern "C" {
 #define UCNS_LAYOUT_AOS  0   /* u[i*5+v]     = U_C(i)%VAL(1,v)   */
 #define UCNS_LAYOUT_SOA  1   /* u[v*ncell+i] = U_C(i)%VAL(1,v)   */
 
+/* cell order inside the library (ucns_scheme_v2.cell_order).  Every array
+ * crossing the ABI stays in the caller's order; the library permutes on the
+ * way in and out. */
+#define UCNS_ORDER_CALLER  0   /* the caller's order as given */
+#define UCNS_ORDER_RCM     1   /* reverse Cuthill-McKee on the face graph */
+#define UCNS_ORDER_MORTON  2   /* Morton (Z-order) curve of the cell centroids (needs ucns_mesh_v2.centroid) */
+
+/* reconstruction kernel family (ucns_scheme_v2.kernels) */
+#define UCNS_KERNELS_AUTO      0   /* row-stream kernels where the stencil shape allows */
+#define UCNS_KERNELS_REGISTER  1   /* register-prefetch kernels only */
+
 /* Mesh topology and geometry, one partition (the whole mesh).
  * Mirrors IELEM(N,i)%{IFCA,INEIGH,INEIGHN,IBOUNDS->ICODE,TYPES_FACES,
- * FACEANGLEX,FACEANGLEY,SURF,TOTVOLUME,MINEDGE} (src/declarations.f90:644-712). */
-typedef struct ucns_mesh_v1 {
+ * FACEANGLEX,FACEANGLEY,SURF,TOTVOLUME,MINEDGE,XXC/YYC/ZZC}
+ * (src/declarations.f90:644-712). */
+typedef struct ucns_mesh_v2 {
     int32_t abi_version;        /* UCNS_ABI_VERSION */
     int32_t ncell;              /* KMAXE */
     int32_t index_base;         /* 0 or 1 */
@@ -72,10 +84,13 @@ typedef struct ucns_mesh_v1 {
     const double *surf;         /* [ncell*nface_max] face area */
     const double *totvolume;    /* [ncell] */
     const double *minedge;      /* [ncell] 3V/sum(S), src/grid_t.f90:1122 */
-} ucns_mesh_v1;
+    const double *centroid;     /* [ncell][3] cell centroids IELEM%XXC,YYC,ZZC; may be NULL
+                                   unless cell_order is UCNS_ORDER_MORTON */
+} ucns_mesh_v2;
 
-/* Scheme switches (the UCNS3D.DAT fields on the path, src/parameters.f90:215-287). */
-typedef struct ucns_scheme_v1 {
+/* Scheme switches (the UCNS3D.DAT fields on the path, src/parameters.f90:215-287),
+ * then the library's own options, which have no UCNS3D.DAT counterpart. */
+typedef struct ucns_scheme_v2 {
     int32_t abi_version;
     int32_t precision;          /* UCNS_FP64 | UCNS_MIXED */
     int32_t iweno;              /* 1 = WENO (spatiladiscret 3), -1 = MUSCL (2) */
@@ -95,7 +110,17 @@ typedef struct ucns_scheme_v1 {
     double weno_eps;            /* 10E-12 = 1e-11 in the reference */
     const double *weights_quad; /* [qp_quad], sum 1 */
     const double *weights_tri;  /* [qp_triangle], sum 1 */
-} ucns_scheme_v1;
+    /* library options */
+    int32_t cell_order;         /* UCNS_ORDER_CALLER | UCNS_ORDER_RCM | UCNS_ORDER_MORTON */
+    int32_t kernels;            /* UCNS_KERNELS_AUTO | UCNS_KERNELS_REGISTER */
+    int32_t adj_keep_stages;    /* ucns_rk3_adjoint keeps the forward reconstruction of at most
+                                   this many of a march's last stages; 0: no count limit (the
+                                   byte limit alone); < 0: none (every reversed stage runs its
+                                   reconstruction again) */
+    int32_t reserved;           /* 0 */
+    double adj_keep_gib;        /* and of at most this many GiB of them (never more than the free
+                                   device memory less 4 GiB); <= 0 selects 64 */
+} ucns_scheme_v2;
 
 /* Per-cell precomputed reconstruction data (PRESTORE_RECONSTRUCTION3 and
  * MEMORY_FAST outputs, src/prestore.f90:167-815, src/reconstruct_p.f90:45-302).
@@ -140,7 +165,7 @@ typedef struct ucns_boundary_v1 {
 typedef struct ucns_ctx ucns_ctx;
 
 /* Set up one partition on the current HIP device (topology, faces, scheme). */
-int  ucns_create(const ucns_mesh_v1 *mesh, const ucns_scheme_v1 *scheme, ucns_ctx **out);
+int  ucns_create(const ucns_mesh_v2 *mesh, const ucns_scheme_v2 *scheme, ucns_ctx **out);
 /* Upload PRESTORE data (pseudo-inverses, indicators, qp coordinates ...). */
 int  ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre);
 /* Free-stream data of the boundary codes 1, 2 and 6 (needed before the first
@@ -157,6 +182,8 @@ int  ucns_compute_dt(ucns_ctx *ctx, double cfl, double *dt_out);
 /* CALL_FLUX_SUBROUTINES_3D on the current state: RHS(i)%VAL(1:5) (face-flux
  * sums, not divided by the volume).  rhs_out may be NULL (result stays on device). */
 int  ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout);
+/* Same, with rhs_dev a device pointer (no host staging). */
+int  ucns_residual_device(ucns_ctx *ctx, double *rhs_dev, int32_t layout);
 /* ARBITRARY_ORDER only: ILOCAL_RECON3(i)%ULEFT(1:5,L,NGP) as [ncell][nface_max][nqp_max][5]. */
 int  ucns_reconstruct(ucns_ctx *ctx, double *uleft_out);
 /* TIME_MARCHING's RK3 loop for nsteps steps.  If dt_fixed > 0 it is used for
@@ -180,15 +207,18 @@ int  ucns_residual_vjp_device(ucns_ctx *ctx, const double *w_dev, double *u_bar_
  * forward sweep, held fixed in the derivative.  The state is left at U_N and
  * *t_io (or NULL) is advanced as by ucns_rk3_steps.  The last stages of the
  * forward sweep also keep their reconstruction for the reverse sweep, as
- * many as fit in the environment's UCNS_ADJ_KEEP_GB (default 64 GiB, at most
- * the free device memory less 4 GiB) and UCNS_ADJ_KEEP (a stage count); the
- * context holds those buffers until ucns_destroy or ucns_upload_prestore. */
+ * many as the scheme's adj_keep_gib and adj_keep_stages allow; the context
+ * holds those buffers until ucns_destroy or ucns_upload_prestore. */
 int  ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar,
                       double *u0_bar, int32_t layout, double *bc_bar, double *t_io);
 /* Same with device pointers, AOS layout (bc_bar_dev: 10 doubles or NULL). */
 int  ucns_rk3_adjoint_device(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar_dev,
                              double *u0_bar_dev, double *bc_bar_dev, double *t_io);
 int  ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out);
+/* Fault injection for tests of the error paths (no counterpart in the
+ * reference): the nth device allocation the context makes from now on
+ * fails with UCNS_ENOMEM (0 disarms). */
+int  ucns_debug_fail_alloc(ucns_ctx *ctx, int32_t nth);
 int  ucns_reset_stats(ucns_ctx *ctx);
 const char *ucns_last_error(const ucns_ctx *ctx);
 void ucns_destroy(ucns_ctx *ctx);

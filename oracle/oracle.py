@@ -47,6 +47,12 @@ def lib() -> C.CDLL:
         L.oracle_barth.restype = C.c_double
         L.oracle_set_boundary.argtypes = [vp]
         L.oracle_set_boundary.restype = None
+        L.oracle_set_face_order.argtypes = [C.c_int]
+        L.oracle_set_face_order.restype = None
+        L.oracle_set_uleft_form.argtypes = [C.c_int]
+        L.oracle_set_uleft_form.restype = None
+        L.oracle_set_face_once.argtypes = [C.c_int]
+        L.oracle_set_face_once.restype = None
         L.oracle_ghost.argtypes = [C.c_int, C.c_double, f64, C.c_double, C.c_double, f64]
         L.oracle_ghost.restype = None
         L.oracle_residual_tl.argtypes = [vp, vp, vp, f64, f64, f64, f64, f64]
@@ -171,6 +177,26 @@ def set_boundary(boundtype, inflow, outflow):
     b = native.BoundaryV1(abi_version=native.UCNS_ABI_VERSION, boundtype=boundtype,
                           inflow=(C.c_double * 5)(*inflow), outflow=(C.c_double * 5)(*outflow))
     lib().oracle_set_boundary(C.byref(b))
+
+
+def set_face_order(reversed_: bool) -> None:
+    """sum each cell's faces in reverse slot order (process-wide): the same
+    residual in exact arithmetic, rounded differently"""
+    lib().oracle_set_face_order(1 if reversed_ else 0)
+
+
+def set_uleft_form(mean_first: bool) -> None:
+    """form each WENO face value as U_i + sum_s w_s res_s instead of the
+    reference's sum_s w_s (U_i + res_s) (process-wide): equal in exact
+    arithmetic (the weights sum to 1), rounded differently"""
+    lib().oracle_set_uleft_form(1 if mean_first else 0)
+
+
+def set_face_once(once: bool) -> None:
+    """solve each interior face once from its owner and negate it for the
+    neighbour, instead of from both sides with each side's own face angles
+    and area (process-wide): equal in exact arithmetic, rounded differently"""
+    lib().oracle_set_face_once(1 if once else 0)
 
 
 def ghost(code, gamma, cl, a1, a2):

@@ -303,8 +303,8 @@ static void basis_rec(const double *x, int ideg, const double *integ, double vol
 }
 
 struct ctx_t {
-    const ucns_mesh_v1 *m;
-    const ucns_scheme_v1 *s;
+    const ucns_mesh_v2 *m;
+    const ucns_scheme_v2 *s;
     const ucns_prestore_v1 *p;
     int nf, nqm, ideg, nnb, imax, ts, base;
 };
@@ -312,6 +312,12 @@ struct ctx_t {
 static int nqp_of(const ctx_t *c, int64_t i, int f) {
     return c->m->types_faces[i * c->nf + f] == 5 ? c->s->qp_quad : c->s->qp_triangle;
 }
+
+/* Form each face value as U_i + sum_s omega_s res_s instead of the
+ * reference's sum_s omega_s (U_i + res_s) (oracle_set_uleft_form): equal in
+ * exact arithmetic since the weights sum to 1, rounded differently.  Test
+ * hook for measuring the reference arithmetic's own rounding sensitivity. */
+static int g_uleft_mean_first = 0;
 
 template <class T>
 static T *uleft_at(const ctx_t *c, T *ul, int64_t i, int f, int q) {
@@ -402,8 +408,17 @@ static void weno_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) 
                 for (int v = 0; v < NV; ++v) {
                     T res = T(0.0);
                     for (int k = 0; k < ideg; ++k) res = res + phi[k] * grad[((size_t)ll * ideg + k) * NV + v];
-                    o[v] = o[v] + (ui[v] + res) * omega[ll][v];
+                    if (g_uleft_mean_first)
+                        o[v] = o[v] + res * omega[ll][v];
+                    else
+                        o[v] = o[v] + (ui[v] + res) * omega[ll][v];
                 }
+            }
+    if (g_uleft_mean_first)
+        for (int f = 0; f < c->m->ifca[i]; ++f)
+            for (int q = 0; q < nqp_of(c, i, f); ++q) {
+                T *o = uleft_at(c, ul, i, f, q);
+                for (int v = 0; v < NV; ++v) o[v] = ui[v] + o[v];
             }
 }
 
@@ -441,8 +456,11 @@ static void muscl_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w)
     T umin[NV], umax[NV], slope[NV];
     for (int v = 0; v < NV; ++v) umin[v] = umax[v] = ui[v];
     for (int f = 0; f < c->m->ifca[i]; ++f) {
+        /* strict bounds (EXTENDED_BOUNDS=0, FIND_BOUNDS src/reconstruct_p.f90:1847-1885): every face
+         * neighbour of an interior cell; of a cell with boundary faces, the interior neighbours and
+         * the periodic ones with code 5 (1871), not those of code 50 nor the other boundary codes */
         int bc = c->m->bcode[i * nf + f];
-        if (bc != 0 && bc != 5 && bc != 50) continue;  /* non-periodic boundary faces add no neighbour */
+        if (bc != 0 && bc != 5) continue;
         const T *un = u + (size_t)(c->m->ineigh[i * nf + f] - c->base) * NV;
         for (int v = 0; v < NV; ++v) {
             umin[v] = fmin(umin[v], un[v]);
@@ -499,7 +517,7 @@ static void lsq_central(const ctx_t *c, const T *u, int64_t i, Work<T> &w) {
         }
 }
 
-static void init_ctx(ctx_t *c, const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p) {
+static void init_ctx(ctx_t *c, const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p) {
     c->m = m;
     c->s = s;
     c->p = p;
@@ -515,7 +533,7 @@ static void init_ctx(ctx_t *c, const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, c
 /* ARBITRARY_ORDER (reconstruct_p.f90:2954-3015) for all cells -> ULEFT.
  * counters (optional): [0] cells sent to MUSCL, [1] cells set to first order. */
 template <class T>
-static int reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *u, T *ul,
+static int reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const T *u, T *ul,
                        int64_t *counters) {
     ctx_t c;
     init_ctx(&c, m, s, p);
@@ -567,16 +585,29 @@ static int reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucn
     return UCNS_OK;
 }
 
+/* Sum each cell's faces in reverse slot order (oracle_set_face_order): the
+ * same residual in exact arithmetic with another rounding of each cell's
+ * face sum, for measuring how far the reference's own arithmetic moves a
+ * result under a reordering of one sum. */
+static int g_faces_reversed = 0;
+/* Solve each interior face once, from its owner (the lower cell id; equal
+ * ids: the lower slot), and give the neighbour the negated flux
+ * (oracle_set_face_once), instead of solving it from both sides with each
+ * side's own face angles and area as the reference does: equal in exact
+ * arithmetic, rounded differently. */
+static int g_face_once = 0;
+
 /* CALCULATE_FLUXESHI_CONVECTIVE, cell based, every face solved from both sides */
 template <class T>
-static int fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *ul, T *rhs) {
+static int fluxes(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const T *ul, T *rhs) {
     ctx_t c;
     init_ctx(&c, m, s, p);
     const int64_t nc = m->ncell;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nc; ++i) {
         T r[NV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
-        for (int f = 0; f < m->ifca[i]; ++f) {
+        for (int ff = 0; ff < m->ifca[i]; ++ff) {
+            const int f = g_faces_reversed ? m->ifca[i] - 1 - ff : ff;
             const int64_t slot = i * c.nf + f;
             const double a1 = m->faceanglex[slot], a2 = m->faceangley[slot];
             const int iqp = nqp_of(&c, i, f);
@@ -586,6 +617,23 @@ static int fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_pre
             const int64_t j = bnd ? 0 : m->ineigh[slot] - c.base;
             const int fj = bnd ? 0 : m->ineighn[slot] - c.base;
             T god[NV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
+            if (g_face_once && !bnd && !(i < j || (i == j && f < fj))) {
+                /* the owner's flux through this face, negated */
+                const int64_t so = j * c.nf + fj;
+                const double b1 = m->faceanglex[so], b2 = m->faceangley[so];
+                const double *wo = m->types_faces[so] == 5 ? s->weights_quad : s->weights_tri;
+                T go[NV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
+                for (int q = 0; q < iqp; ++q) {
+                    T clr[NV], crr[NV], h[NV], rh[NV];
+                    rotatef(crr, uleft_at(&c, (T *)ul, i, f, q), b1, b2);
+                    rotatef(clr, uleft_at(&c, (T *)ul, j, fj, q), b1, b2);
+                    hllc(s->gamma, clr, crr, h);
+                    rotateb(rh, h, b1, b2);
+                    for (int v = 0; v < NV; ++v) go[v] = go[v] + (rh[v] * (wo[q] * m->surf[so]));
+                }
+                for (int v = 0; v < NV; ++v) r[v] = r[v] - go[v];
+                continue;
+            }
             for (int q = 0; q < iqp; ++q) {
                 const T *cl = uleft_at(&c, (T *)ul, i, f, q);
                 T gst[NV];
@@ -610,13 +658,13 @@ static int fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_pre
     return UCNS_OK;
 }
 
-static size_t uleft_size(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s) {
+static size_t uleft_size(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s) {
     int nqm = s->qp_quad > s->qp_triangle ? s->qp_quad : s->qp_triangle;
     return (size_t)m->ncell * m->nface_max * nqm * NV;
 }
 
 template <class T>
-static int residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const T *u, T *rhs,
+static int residual(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const T *u, T *rhs,
                     int64_t *counters) {
     std::vector<T> ul(uleft_size(m, s), T(0.0));
     int rc = reconstruct(m, s, p, u, ul.data(), counters);
@@ -625,7 +673,7 @@ static int residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_p
 }
 
 /* CALCULATE_CFL, Euler branch (time_p.f90:46, 67-116) */
-static void cfl_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, double cfl, double *dt_out) {
+static void cfl_dt(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const double *u, double cfl, double *dt_out) {
     const double ccfl = cfl / 3.0;
     double dt = 1.0e16;
     const int64_t nc = m->ncell;
@@ -650,7 +698,7 @@ static void values(const Dual *u, double *out, size_t n) {
  * dt_fixed <= 0 takes dt from the CFL condition of each step's state (its
  * derivative is not carried: the tangent runs use dt_fixed). */
 template <class T>
-static int rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, T *u, int32_t nsteps,
+static int rk3_steps(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, T *u, int32_t nsteps,
                      double cfl, double dt_fixed, double *t_io) {
     const int64_t nc = m->ncell;
     const size_t n = (size_t)nc * NV;
@@ -722,32 +770,35 @@ extern "C" {
 void oracle_rotatef(double *rot, const double *vec, double a1, double a2) { rotatef(rot, vec, a1, a2); }
 void oracle_rotateb(double *vec, const double *rot, double a1, double a2) { rotateb(vec, rot, a1, a2); }
 void oracle_set_boundary(const ucns_boundary_v1 *bc) { g_bc = *bc; }
+void oracle_set_face_order(int reversed) { g_faces_reversed = reversed ? 1 : 0; }
+void oracle_set_uleft_form(int mean_first) { g_uleft_mean_first = mean_first ? 1 : 0; }
+void oracle_set_face_once(int once) { g_face_once = once ? 1 : 0; }
 void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2, double *cr) {
     ghost(code, gamma, cl, a1, a2, cr);
 }
 void oracle_hllc(double gamma, const double *cl, const double *cr, double *flux) { hllc(gamma, cl, cr, flux); }
 
-int oracle_reconstruct(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+int oracle_reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const double *u,
                        double *ul, int64_t *counters) {
     return reconstruct(m, s, p, u, ul, counters);
 }
 
-int oracle_fluxes(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *ul,
+int oracle_fluxes(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const double *ul,
                   double *rhs) {
     return fluxes(m, s, p, ul, rhs);
 }
 
-int oracle_residual(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+int oracle_residual(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const double *u,
                     double *rhs, int64_t *counters) {
     return residual(m, s, p, u, rhs, counters);
 }
 
-int oracle_dt(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const double *u, double cfl, double *dt_out) {
+int oracle_dt(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const double *u, double cfl, double *dt_out) {
     cfl_dt(m, s, u, cfl, dt_out);
     return UCNS_OK;
 }
 
-int oracle_rk3_steps(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, double *u,
+int oracle_rk3_steps(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, double *u,
                      int32_t nsteps, double cfl, double dt_fixed, double *t_io) {
     return rk3_steps(m, s, p, u, nsteps, cfl, dt_fixed, t_io);
 }
@@ -766,7 +817,7 @@ double oracle_barth(double ui, double umin, double umax, double usol) { return b
 /* ---- tangent-linear model: the Dual instantiation of the same code ----
  * du: [ncell][5] state tangent; dbc (or null): tangent of the boundary
  * data inflow[0..4], outflow[0..4] set by oracle_set_boundary. */
-int oracle_residual_tl(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, const double *u,
+int oracle_residual_tl(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, const double *u,
                        const double *du, const double *dbc, double *rhs, double *drhs) {
     const size_t n = (size_t)m->ncell * NV;
     set_dbc(dbc);
@@ -782,7 +833,7 @@ int oracle_residual_tl(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucn
 
 /* nsteps SSP-RK3 steps of fixed dt with the tangent carried along:
  * u, du in/out */
-int oracle_rk3_tl(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, const ucns_prestore_v1 *p, double *u, double *du,
+int oracle_rk3_tl(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_prestore_v1 *p, double *u, double *du,
                   const double *dbc, int32_t nsteps, double dt_fixed) {
     if (!(dt_fixed > 0.0)) return UCNS_EBADARG;
     const size_t n = (size_t)m->ncell * NV;

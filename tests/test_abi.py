@@ -46,7 +46,7 @@ def test_header_lists_the_path():
                "ucns_reconstruct", "ucns_rk3_steps", "ucns_compute_dt", "ucns_destroy", "ucns_last_error",
                "ucns_residual_vjp", "ucns_residual_vjp_device", "ucns_rk3_adjoint", "ucns_rk3_adjoint_device"]:
         assert fn in fns
-    assert len(fns) == 20
+    assert len(fns) == 22
 
 
 def test_every_declared_symbol_is_exported(glib):
@@ -59,6 +59,8 @@ def test_every_declared_symbol_is_exported(glib):
 def test_constants_match_bindings(glib):
     m = header_macros()
     assert m["UCNS_ABI_VERSION"] == native.UCNS_ABI_VERSION == glib.ucns_abi_version()
+    for k in ("UCNS_ORDER_CALLER", "UCNS_ORDER_RCM", "UCNS_ORDER_MORTON", "UCNS_KERNELS_AUTO", "UCNS_KERNELS_REGISTER"):
+        assert m[k] == getattr(native, k), k
     assert m["UCNS_FP64"] == native.UCNS_FP64 and m["UCNS_MIXED"] == native.UCNS_MIXED
     assert m["UCNS_LAYOUT_AOS"] == native.UCNS_LAYOUT_AOS and m["UCNS_LAYOUT_SOA"] == native.UCNS_LAYOUT_SOA
     codes = [m[k] for k in ("UCNS_EBADARG", "UCNS_EUNSUPPORTED", "UCNS_ENOMEM", "UCNS_EHIP", "UCNS_ENONFINITE",
@@ -69,7 +71,7 @@ def test_constants_match_bindings(glib):
 def test_struct_layouts_match_header():
     """ctypes mirrors: field count and order of each struct in the header."""
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
-    for cname, py in [("ucns_mesh_v1", native.MeshV1), ("ucns_scheme_v1", native.SchemeV1),
+    for cname, py in [("ucns_mesh_v2", native.MeshV2), ("ucns_scheme_v2", native.SchemeV2),
                       ("ucns_prestore_v1", native.PrestoreV1), ("ucns_stats_v1", native.StatsV1),
                       ("ucns_boundary_v1", native.BoundaryV1)]:
         body = re.search(r"typedef struct %s \{(.*?)\}" % cname, text, flags=re.S).group(1)
@@ -120,6 +122,12 @@ def test_create_rejects_bad_arguments(glib, small_case):
         (None, lambda s: setattr(s, "limiter", 2), -2, "Barth"),
         (None, lambda s: setattr(s, "qp_quad", 0), -1, "quadrature"),
         (None, lambda s: setattr(s, "idegfree", 9), -2, "idegfree"),
+        (None, lambda s: setattr(s, "cell_order", 7), -1, "cell_order"),
+        (lambda m: setattr(m, "centroid", None), lambda s: setattr(s, "cell_order", native.UCNS_ORDER_MORTON), -1,
+         "centroids"),
+        (None, lambda s: setattr(s, "kernels", 3), -1, "kernels"),
+        (None, lambda s: setattr(s, "reserved", 1), -1, "reserved"),
+        (None, lambda s: setattr(s, "adj_keep_gib", float("nan")), -1, "adj_keep_gib"),
     ]
     for mm, ms, want, needle in cases:
         rc, h = attempt(mm, ms)
@@ -139,6 +147,8 @@ def test_null_context_is_rejected(glib):
     assert glib.ucns_get_state_device(None, None, 0) == -1
     assert glib.ucns_compute_dt(None, 0.5, C.byref(d)) == -1
     assert glib.ucns_residual(None, None, 0) == -1
+    assert glib.ucns_residual_device(None, None, 0) == -1
+    assert glib.ucns_debug_fail_alloc(None, 1) == -1
     assert glib.ucns_reconstruct(None, None) == -1
     assert glib.ucns_rk3_steps(None, 1, 0.5, 0.0, None) == -1
     assert glib.ucns_stats(None, C.byref(st)) == -1
@@ -156,6 +166,32 @@ def test_prep_library_exports_and_errors():
     h = C.c_void_p()
     assert lib.ucnsp_build(C.byref(o), C.byref(h)) != 0
     assert lib.ucnsp_last_error().decode()
+
+
+def test_prep_rejects_a_bump_it_cannot_apply():
+    """a lower-wall bump needs bounded y sides (periodic partners would not
+    move together), and one that folds the cells next to the wall is refused
+    with the cell named"""
+    lib = native.prep_lib()
+
+    def build(bc, a):
+        o = native.PrepOpts(shape=1, nx=8, ny=4, nz=4, lx=4, ly=1, lz=1, jitter=0, seed=1, iorder=1, iweno=-1,
+                            extf=3, weight_lsqr=0, nthreads=1, bc=(C.c_int32 * 6)(*bc),
+                            bump=(C.c_double * 8)(*([a] + [0.0] * 7)))
+        h = C.c_void_p()
+        rc = lib.ucnsp_build(C.byref(o), C.byref(h))
+        if rc == 0:
+            lib.ucnsp_free(h)
+        return rc, lib.ucnsp_last_error().decode()
+
+    rc, msg = build((1, 2, 0, 0, 3, 3), 0.01)
+    assert rc == -1 and "bounded y" in msg
+    rc, msg = build((1, 2, 4, 4, 3, 3), 0.01)
+    assert rc == 0, msg
+    # slope ~0.9 along x over cells 0.5 long and 0.25 high: the wall nodes of
+    # a cell differ in height by more than the cell, folding sub-tetrahedra
+    rc, msg = build((1, 2, 4, 4, 3, 3), 0.6)
+    assert rc == -1 and "inverts cell" in msg, msg
 
 
 def test_oracle_library_exports():

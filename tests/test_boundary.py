@@ -187,3 +187,55 @@ def test_uniform_flow_is_preserved(shape, scheme, bc, boundtype):
     rhs, _ = orc.residual(u)
     scale = np.abs(phys_flux(U0, np.array([1.0, 0, 0]))).max() * case.array("surf").max()
     assert np.abs(rhs).max() <= 1e-12 * scale
+
+
+def code50_structs(case):
+    """the case's structs with the periodic faces whose outward normal is +x
+    carrying code 50 (their partners keep 5), as a host numbering periodic
+    pairs that way passes them; returns (holder, structs)"""
+    m, s, p = case.structs()
+    nf = case.nface_max
+    bcode = case.array("bcode").copy().reshape(-1, nf)
+    normal = case.array("normal").reshape(-1, nf, 3)
+    plus_x = (bcode == 5) & (normal[:, :, 0] > 0.9)
+    bcode[plus_x] = 50
+    bcode = np.ascontiguousarray(bcode.reshape(-1)).astype(np.int32)
+    from ucns3d_amd import native
+    m.bcode = native.i32p(bcode)
+    return (bcode, m, s, p), (m, s, p), int(plus_x.sum())
+
+
+def test_code50_neighbours_stay_out_of_the_muscl_bounds():
+    """FIND_BOUNDS with strict bounds (src/reconstruct_p.f90:1847-1885)
+    takes, for a cell with boundary faces, the interior neighbours and the
+    periodic neighbours of code 5 (1871), not those of code 50.  On a
+    periodic MUSCL box whose x+ periodic faces carry 50: the limited face
+    values of those cells stay within the bounds without the x+ neighbour
+    (and some leave the bounds that include it), the fluxes still pair
+    every face (sum of residuals 0), and the other cells are unchanged."""
+    case = Case(CaseSpec("muscl-50", HEX, 6, "muscl", seed=5))
+    orc5 = Oracle(case)
+    holder, (m, s, p), n50 = code50_structs(case)
+    assert n50 == 36
+    orc50 = Oracle(case)
+    orc50.m = m
+    rng = np.random.default_rng(4)
+    u = np.stack([prim2cons(1 + 0.3 * rng.uniform(), 0.1, 0.2, -0.1, 1 + 0.3 * rng.uniform())
+                  for _ in range(case.ncell)])
+    ul5, _ = orc5.reconstruct(u)
+    ul50, _ = orc50.reconstruct(u)
+    nf = 6
+    bcode = holder[0].reshape(-1, nf)
+    ineigh = case.array("ineigh").reshape(-1, nf)
+    has50 = (bcode == 50).any(1)
+    np.testing.assert_array_equal(ul5[~has50], ul50[~has50])
+    q = ul50[:, :nf, :case.qp_quad, :]
+    keep = (bcode == 0) | (bcode == 5)
+    nb = np.where(keep, ineigh, np.arange(case.ncell)[:, None])
+    lo = np.minimum(u[:, None, :], u[nb]).min(1)
+    hi = np.maximum(u[:, None, :], u[nb]).max(1)
+    tol = 1e-12 * np.abs(u).max(0)
+    assert np.all(q >= lo[:, None, None, :] - tol) and np.all(q <= hi[:, None, None, :] + tol)
+    assert not np.array_equal(ul5[has50], ul50[has50])
+    r50, _ = orc50.residual(u)
+    assert np.all(np.abs(r50.sum(0)) <= 1e-12 * np.abs(r50).sum(0))

@@ -8,16 +8,13 @@ Pinned three ways:
   - the same identity over several RK3 steps with stage checkpoints;
   - central finite differences of the GPU forward itself (h = 1e-6).
 Every row layout of the reconstruction is covered: the row stream (WENO-4
-tetrahedra and hexahedra, WENO-3), the tiled layout (UCNS_STREAM=0) and the
-per-cell layout (MUSCL), on periodic and bounded boxes."""
-import contextlib
-import os
-
+tetrahedra and hexahedra, WENO-3), the tiled layout (UCNS_KERNELS_REGISTER)
+and the per-cell layout (MUSCL), on periodic and bounded boxes."""
 import numpy as np
 import pytest
 
 from oracle import oracle
-from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+from ucns3d_amd.native import UCNS_FP64, UCNS_KERNELS_AUTO, UCNS_KERNELS_REGISTER, UCNS_MIXED
 from ucns3d_amd.solver import Solver
 
 from test_tangent import CASES, setup
@@ -25,27 +22,9 @@ from test_tangent import CASES, setup
 pytestmark = pytest.mark.gpu
 
 
-@contextlib.contextmanager
-def env(name: str, value: str):
-    old = os.environ.get(name)
-    os.environ[name] = value
-    try:
-        yield
-    finally:
-        if old is None:
-            del os.environ[name]
-        else:
-            os.environ[name] = old
-
-
-def stream_kernels(on: bool):
-    return env("UCNS_STREAM", "1" if on else "0")
-
-
-def solver_for(name, precision=UCNS_FP64, stream=True):
+def solver_for(name, precision=UCNS_FP64, stream=True, **opts):
     case, orc, u, du, bc = setup(name)
-    with stream_kernels(stream):
-        s = Solver(case, precision)
+    s = Solver(case, precision, kernels=UCNS_KERNELS_AUTO if stream else UCNS_KERNELS_REGISTER, **opts)
     if bc is not None:
         s.set_boundary(*bc)
     s.set_state(u)
@@ -119,21 +98,22 @@ def test_rk3_adjoint_dot_product(name):
                                                     ("tet-weno4", True, UCNS_MIXED)])
 def test_rk3_adjoint_kept_stages_agree_with_recomputed(name, stream, precision):
     """The reverse sweep starts a stage's products from the reconstruction the
-    forward sweep kept, or runs the reconstruction again (UCNS_ADJ_KEEP caps
-    the kept stages, UCNS_ADJ_KEEP_GB their bytes): all, some (4 of 9) and
-    none give the same cotangents"""
+    forward sweep kept, or runs the reconstruction again (the scheme's
+    adj_keep_stages caps the kept stages, adj_keep_gib their bytes): all,
+    some (4 of 9) and none give the same cotangents"""
     case, orc, u, du, bc = setup(name)
     print(f"[rk3 adjoint keep] {name} stream={stream} precision={precision}", flush=True)
-    s = solver_for(name, precision=precision, stream=stream)
-    dt = 0.5 * s.compute_dt(0.5)
     a = np.random.default_rng(15).normal(size=u.shape)
     out = {}
-    settings = {"9": ("UCNS_ADJ_KEEP", "9"), "4": ("UCNS_ADJ_KEEP", "4"), "0": ("UCNS_ADJ_KEEP", "0"),
-                "bytes": ("UCNS_ADJ_KEEP_GB", "1e-9")}
-    for key, (var, val) in settings.items():
-        with env(var, val):
-            s.set_state(u)
-            out[key] = s.rk3_adjoint(3, a, dt_fixed=dt)
+    settings = {"9": dict(adj_keep_stages=9), "4": dict(adj_keep_stages=4), "0": dict(adj_keep_stages=-1),
+                "bytes": dict(adj_keep_gib=1e-9)}
+    dt = None
+    for key, opts in settings.items():
+        s = solver_for(name, precision=precision, stream=stream, **opts)
+        if dt is None:
+            dt = 0.5 * s.compute_dt(0.5)
+        out[key] = s.rk3_adjoint(3, a, dt_fixed=dt)
+        s.close()
     ref, rb, _ = out["9"]
     for keep in ("4", "0", "bytes"):
         g, gb, _ = out[keep]

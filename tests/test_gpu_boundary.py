@@ -130,3 +130,35 @@ def test_mixed_vs_fp64(name):
     print("  rel L2 mixed vs FP64 oracle", err)
     assert np.all(err <= 1e-5), err
     s.close()
+
+
+@pytest.mark.parametrize("scheme", ["muscl", "weno4"])
+def test_code50_periodic_faces(scheme):
+    """periodic faces numbered 5 on one side and 50 on the other (the x+
+    side here): code-50 neighbours stay out of the MUSCL bounds
+    (src/reconstruct_p.f90:1871) while both codes pair their fluxes; GPU
+    against the oracle given the same structs, one step <= 1e-12"""
+    from test_boundary import code50_structs
+    case = Case(CaseSpec(f"{scheme}-50", HEX, 8 if scheme == "muscl" else 12, scheme, seed=5))
+    holder, st, n50 = code50_structs(case)
+    print(f"[code 50] {scheme} ncell={case.ncell} faces={n50}", flush=True)
+    orc = Oracle(case)
+    orc.m = st[0]
+    u = case.init_state()
+    x = case.array("centre").reshape(-1, 3)
+    u = u * (1 + 0.05 * np.sin(0.37 * x[:, :1] + 0.71 * x[:, 1:2] + 0.53 * x[:, 2:3] + 0.2))
+    s = Solver(case, UCNS_FP64, structs=st)
+    s.set_state(u)
+    ul = s.reconstruct()
+    ul_ref, cnt = orc.reconstruct(u)
+    st_gpu = s.stats()
+    assert st_gpu["n_reduced_last"] == cnt[0]
+    assert scheme != "muscl" or cnt[0] == case.ncell
+    s.rk3_steps(1, cfl=0.5)
+    u1_ref, _ = orc.rk3(u, 1, cfl=0.5)
+    err = np.sqrt(((s.get_state() - u1_ref) ** 2).sum(0)) / np.sqrt((u1_ref ** 2).sum(0))
+    print("  rel L2", err)
+    assert np.all(err <= 1e-12)
+    m = np.isfinite(ul_ref)
+    assert np.abs(ul[m] - ul_ref[m]).max() <= 1e-12 * np.abs(ul_ref[m]).max()
+    s.close()

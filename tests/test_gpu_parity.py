@@ -105,6 +105,31 @@ def test_residual_fp64(name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_residual_fp64_against_the_oracle_in_the_gpu_arithmetic(name):
+    """the residual against the oracle making the GPU path's two choices
+    that round differently from the reference (face values as
+    U_i + sum_s w_s res_s; each face solved once, negated for the
+    neighbour): 1e-12 relative L2 per variable"""
+    from oracle import oracle
+    case, orc, u = get(name)
+    print(f"[residual, GPU arithmetic] {name}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_state(u)
+    r_gpu = s.residual()
+    s.close()
+    oracle.set_uleft_form(True)
+    oracle.set_face_once(True)
+    try:
+        r_ref, _ = orc.residual(u)
+    finally:
+        oracle.set_uleft_form(False)
+        oracle.set_face_once(False)
+    err = rel_l2(r_gpu, r_ref)
+    print("  rel L2", err)
+    assert np.all(err <= 1e-12), err
+
+
+@pytest.mark.parametrize("name", list(CASES))
 def test_one_step_fp64(name):
     case, orc, u = get(name)
     print(f"[rk3 x1] {name}", flush=True)

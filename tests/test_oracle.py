@@ -419,6 +419,93 @@ def test_checksol_threshold(name):
     assert checked > 30
 
 
+def test_checksolx_threshold():
+    """CHECKSOLX sends a limited (MUSCL) cell to first order when a face-point
+    density departs from the cell's by 95% or more (src/reconstruct_p.f90:
+    3245-3406, the 0.95 at 3258).
+
+    Density rho = 1 + A f along x with f = -1, 0, 4 in three slabs of cells
+    (then back to -1 across the periodic seam), zero velocity, uniform
+    pressure.  The least-squares slopes are linear in A and the Barth
+    factor is a ratio of differences, so every limited face deviation is A
+    times the one of a small-A run (K).  In the middle slab (rho = 1) the
+    steep unlimited slope overshoots the low neighbour and Barth stops the
+    face value exactly on it: the largest relative jump is A itself, so the
+    cells stay MUSCL for A < 0.95 (also between 0.85 and 0.95) and go first
+    order above it.  Everywhere else a cell is a local extremum among equal
+    neighbours and stays flat.  Over amplitudes sweeping 0.30-0.99 the
+    oracle's first-order count equals the cells whose predicted jump
+    A K / rho reaches 0.95, and the count the same prediction gives with
+    0.5 or 0.99 differs from it at some amplitude, so a threshold moved to
+    either value fails."""
+    case, orc = get("hex-muscl")
+    m = Mesh(case)
+    n = case.spec.n
+    assert case.array("full").max() == 0  # every cell takes the MUSCL path
+    slab = np.floor(m.centre[:, 0] / (m.L[0] / n)).astype(int)
+    f = np.where(slab < n // 2 - 1, -1.0, np.where(slab == n // 2 - 1, 0.0, 4.0))
+    nq, nf = case.qp_quad, 6
+
+    def state(A):
+        u = np.zeros((case.ncell, 5))
+        u[:, 0] = 1.0 + A * f
+        u[:, 4] = 2.5
+        return u
+    a0 = 1e-3
+    ul, cnt = orc.reconstruct(state(a0))
+    assert cnt[1] == 0
+    K = np.abs(ul[:, :nf, :nq, 0] - state(a0)[:, None, None, 0]).reshape(case.ncell, -1).max(1) / a0
+    middle = f == 0.0
+    np.testing.assert_allclose(K[middle], 1.0, rtol=1e-9)   # the face value lands on the low neighbour
+    assert np.all(K[~middle] == 0.0)
+    checked, differs = 0, {0.5: False, 0.99: False}
+    for A in np.linspace(0.30, 0.99, 70):
+        jump = A * K / (1.0 + A * f)
+        if any(np.any(np.abs(jump - t) < 1e-6) for t in (0.5, 0.95, 0.99)):
+            continue  # on a threshold: not called
+        _, cnt = orc.reconstruct(state(A))
+        want = int((jump >= 0.95).sum())
+        assert cnt[1] == want, (A, cnt[1], want)
+        for t in differs:
+            differs[t] |= int((jump >= t).sum()) != want
+        checked += 1
+    assert checked > 60
+    assert all(differs.values()), differs
+    # the band the reference keeps at MUSCL: limited jumps of 85-95%
+    for A in (0.86, 0.90, 0.94):
+        _, cnt = orc.reconstruct(state(A))
+        assert cnt[1] == 0, A
+    for A in (0.951, 0.97):
+        _, cnt = orc.reconstruct(state(A))
+        assert cnt[1] == middle.sum(), A
+
+
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4", "hex-muscl"])
+def test_arithmetic_variants_agree_with_the_reference_order(name):
+    """the oracle's switches that change only rounding (faces summed in
+    reverse, face values formed as U_i + sum w_s res_s, each face solved
+    once): the residual stays within 1e-10 of the reference order, a
+    uniform state keeps a zero residual (1e-12), and the residuals still sum
+    to zero over the periodic mesh (1e-11 of their magnitude: solved from
+    both sides, a face's two fluxes cancel only to rounding)"""
+    case, orc = get(name)
+    u = case.init_state()
+    r0, c0 = orc.residual(u)
+    uf = np.tile(prim2cons(1.2, 0.3, -0.2, 0.1, 3.0), (case.ncell, 1))
+    for setter in (oracle.set_face_order, oracle.set_uleft_form, oracle.set_face_once):
+        setter(True)
+        try:
+            r1, c1 = orc.residual(u)
+            rf, _ = orc.residual(uf)
+        finally:
+            setter(False)
+        assert list(c1) == list(c0)
+        err = np.sqrt(((r1 - r0) ** 2).sum(0)) / np.sqrt((r0 ** 2).sum(0))
+        assert np.all(err <= 1e-10), (setter.__name__, err)
+        assert np.abs(rf).max() <= 1e-12 * 3.0
+        assert np.all(np.abs(r1.sum(0)) <= 1e-11 * np.abs(r1).sum(0))
+
+
 # ---------------------------------------------------------------- convergence of the residual
 
 def wave_averages(m, k, u0):

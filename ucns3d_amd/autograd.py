@@ -34,9 +34,10 @@ class Residual(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u, solver):
         u = _dev(u)
+        rhs = torch.empty_like(u)
         _sync()
         solver.set_state_device(u.data_ptr())
-        rhs = torch.from_numpy(solver.residual()).to(u.device)
+        solver.residual_device(rhs.data_ptr())
         ctx.solver = solver
         ctx.save_for_backward(u)
         return rhs
@@ -56,8 +57,9 @@ class RK3March(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, bc, solver, nsteps: int, dt: float, boundtype: int = 1):
         u0 = _dev(u0)
+        b = None
         if bc is not None:
-            b = bc.detach().cpu().double().numpy()
+            b = bc.detach().cpu().double().numpy().copy()
             solver.set_boundary(boundtype, b[:5], b[5:])
         _sync()
         solver.set_state_device(u0.data_ptr())
@@ -66,6 +68,9 @@ class RK3March(torch.autograd.Function):
         solver.get_state_device(uN.data_ptr())
         ctx.solver, ctx.nsteps, ctx.dt, ctx.has_bc = solver, nsteps, dt, bc is not None
         ctx.bc_device = bc.device if bc is not None else None
+        # the free-stream data of this forward: a later forward on the same
+        # solver may have set others before this backward runs
+        ctx.bc_host, ctx.boundtype = b, boundtype
         ctx.save_for_backward(u0)
         return uN
 
@@ -76,6 +81,8 @@ class RK3March(torch.autograd.Function):
         u0b = torch.empty_like(u0)
         bcb = torch.zeros(10, dtype=torch.float64, device=u0.device)
         _sync()
+        if ctx.bc_host is not None:
+            ctx.solver.set_boundary(ctx.boundtype, ctx.bc_host[:5], ctx.bc_host[5:])
         ctx.solver.set_state_device(u0.data_ptr())
         ctx.solver.rk3_adjoint_device(ctx.nsteps, g.data_ptr(), u0b.data_ptr(), bcb.data_ptr(), dt_fixed=ctx.dt)
         return u0b, (bcb.to(ctx.bc_device) if ctx.has_bc else None), None, None, None, None

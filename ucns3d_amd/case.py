@@ -114,7 +114,7 @@ class Case:
         return np.ctypeslib.as_array(buf)
 
     def structs(self, precision: int = native.UCNS_FP64):
-        m, s, p = native.MeshV1(), native.SchemeV1(), native.PrestoreV1()
+        m, s, p = native.MeshV2(), native.SchemeV2(), native.PrestoreV1()
         rc = self._lib.ucnsp_fill(self._h, precision, self.spec.gamma, self.spec.lwci1, C.byref(m), C.byref(s),
                                   C.byref(p))
         if rc != 0:

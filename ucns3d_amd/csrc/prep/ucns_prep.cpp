@@ -1326,6 +1326,37 @@ void apply_bump(ucnsp_case &c) {
     }
 }
 
+// signed volumes (x6) of every cell's DECOMPOSE3 sub-tetrahedra, [cell][6]
+std::vector<double> subtet_orientation(const ucnsp_case &c) {
+    std::vector<double> o(c.ncell * 6, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < c.ncell; ++i) {
+        const int sh = c.cshape[i];
+        double v[8][3];
+        for (int k = 0; k < nnodes_of(sh); ++k) node_xyz(c, c.cnodes[i * 8 + k], v[k]);
+        for (int t = 0; t < nsubtets(sh); ++t) {
+            const int *tt = subtet(sh, t);
+            const double *a = v[tt[0]], *b = v[tt[1]], *cc = v[tt[2]], *d = v[tt[3]];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const double e2[3] = {cc[0] - a[0], cc[1] - a[1], cc[2] - a[2]};
+            const double e3[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+            o[i * 6 + t] = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                           e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+        }
+    }
+    return o;
+}
+
+// the first cell one of whose sub-tetrahedra the bump turned inside out or
+// flattened (its signed volume changed sign or vanished), or -1
+int64_t first_inverted_cell(const ucnsp_case &c, const std::vector<double> &before) {
+    const std::vector<double> after = subtet_orientation(c);
+    for (int64_t i = 0; i < c.ncell; ++i)
+        for (int t = 0; t < nsubtets(c.cshape[i]); ++t)
+            if (!(before[i * 6 + t] * after[i * 6 + t] > 0.0)) return i;
+    return -1;
+}
+
 void gen_box(ucnsp_case &c) {
     const ucnsp_opts &o = c.o;
     const int nx = o.nx, ny = o.ny, nz = o.nz;
@@ -1502,6 +1533,10 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
     if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
     if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");
+    bool bumped = false;
+    for (int q = 0; q < 8; ++q) bumped = bumped || o.bump[q] != 0.0;
+    if (bumped && o.bc[2] == 0)
+        return fail(UCNS_EBADARG, "a lower-wall bump needs a bounded y axis (the y sides cannot be periodic)");
     if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
     ucnsp_case *c = new ucnsp_case();
     c->o = o;
@@ -1523,7 +1558,17 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
                 delete c;
                 return fail(UCNS_EBADARG, "unmatched face off the box sides");
             }
+    std::vector<double> orient;
+    if (bumped) orient = subtet_orientation(*c);
     apply_bump(*c);
+    if (bumped) {
+        const int64_t bad = first_inverted_cell(*c, orient);
+        if (bad >= 0) {
+            delete c;
+            return fail(UCNS_EBADARG, "the lower-wall bump inverts cell " + std::to_string(bad) +
+                                          " (a sub-tetrahedron of non-positive volume): smaller a_k or more cells");
+        }
+    }
     build_geometry(*c);
     lap("geometry");
     // scheme sizes (DETERMINE_SIZE, src/grid_p.f90:6097-6288; parameters.f90:1270-1394)
@@ -1578,7 +1623,7 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     return UCNS_OK;
 }
 
-int ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1, ucns_mesh_v1 *m, ucns_scheme_v1 *s,
+int ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1, ucns_mesh_v2 *m, ucns_scheme_v2 *s,
                ucns_prestore_v1 *p) {
     if (!c || !m || !s || !p) return fail(UCNS_EBADARG, "null argument");
     std::memset(m, 0, sizeof(*m));
@@ -1598,6 +1643,7 @@ int ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci
     m->surf = c->surf.data();
     m->totvolume = c->vol.data();
     m->minedge = c->minedge.data();
+    m->centroid = c->centre.data();
     s->abi_version = UCNS_ABI_VERSION;
     s->precision = precision;
     s->iweno = c->o.iweno;

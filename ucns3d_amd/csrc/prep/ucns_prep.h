@@ -63,7 +63,7 @@ int  ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out);
 /* Fill the boundary structs with pointers into the case's storage (valid
  * until ucnsp_free).  precision/gamma/lwci1 are the caller's scheme choices. */
 int  ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1,
-                ucns_mesh_v1 *mesh, ucns_scheme_v1 *scheme, ucns_prestore_v1 *pre);
+                ucns_mesh_v2 *mesh, ucns_scheme_v2 *scheme, ucns_prestore_v1 *pre);
 /* Named internal arrays for tests and drivers: "xyz", "cell_nodes",
  * "centre", "ihexl", ... ; dtype 0=int32, 1=float64. */
 int  ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *count, int32_t *dtype);

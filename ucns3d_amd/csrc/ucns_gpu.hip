@@ -65,9 +65,6 @@ __constant__ int8_t c_mono[34][3] = {
     {4, 0, 0}, {3, 1, 0}, {2, 2, 0}, {1, 3, 0}, {0, 4, 0}, {3, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 3, 1},
     {2, 0, 2}, {1, 1, 2}, {0, 2, 2}, {1, 0, 3}, {0, 1, 3}, {0, 0, 4}};
 
-// face quadrature weights (quad, triangle), sum 1
-__constant__ double c_wq[64];
-__constant__ double c_wt[64];
 
 // per-cell info word
 //   bits 0-2 ifca, 3-5 admis, 6 full, 8-13 face k is a triangle
@@ -534,29 +531,6 @@ __host__ __device__ constexpr int pidx(int ideg, int k, int l) {
     return k <= l ? k * ideg - k * (k - 1) / 2 + (l - k) : l * ideg - l * (l - 1) / 2 + (k - l);
 }
 
-// Shape of the LDS ring that streams a tile's pseudo-inverse rows: NCH
-// chunks of 1 KiB per row (one global_load_lds_dwordx4 each), DEPTH rows in
-// flight.  After the LSQ loop the ring is reused for the tile's indicator
-// matrices and the central-stencil coefficients.  For tetrahedra the
-// indicators are fetched into the slots that fall free during the last
-// DEPTH rows, so their latency hides behind the end of the stream.
-template <typename R, int IDEG, int IMAX, int NCH, int DEPTH>
-struct RingShape {
-    static constexpr int NP = IDEG * (IDEG + 1) / 2;
-    static constexpr int SLOTB = (NCH > 0 ? NCH : 1) * 1024;
-    static constexpr int BBYTES = 8 * NP * (int)sizeof(R);
-    static constexpr int NBC = (BBYTES + 1023) / 1024;          // indicator chunks
-    static constexpr int NB = (BBYTES + SLOTB - 1) / SLOTB;     // ring slots they take
-    static constexpr int S0 = (IMAX - DEPTH) % DEPTH;           // first slot freed in the tail
-    static constexpr bool TAILB = NCH > 0 && S0 + NB <= DEPTH && NB < DEPTH;
-    static constexpr int BSLOT = TAILB ? S0 : 0;
-    static constexpr int GSLOT = TAILB ? (S0 + NB) % DEPTH : NB;
-    static constexpr int BYTES = NCH > 0 ? DEPTH * SLOTB : 1;
-    static_assert(NCH == 0 || GSLOT < DEPTH, "ring too shallow for the late data");
-    static_assert(NCH == 0 || 8 * IDEG * NV * (int)sizeof(R) <= SLOTB, "central coefficients exceed a slot");
-    static_assert(NCH == 0 || (DEPTH - 1) * NCH + NBC <= 63, "vmcnt immediate out of range");
-};
-
 // Direct global->LDS loads (global_load_lds_dword / _dwordx4): lane l's 4 or
 // 16 bytes from its own global address land at LDS byte (M0 + 4 l) or
 // (M0 + 16 l).  Issued as inline asm so that the compiler neither counts nor
@@ -777,17 +751,12 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     if (STORE && s == 0 && cvalid) p.fstore[cell] = (reduce ? 1 : 0) | (first ? 2 : 0);
 }
 
-template <typename R, int IDEG, int IMAX, int NQL, int D, int NCH, int DEPTH>
+template <typename R, int IDEG, int IMAX, int NQL, int D>
 __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
-    using RS = RingShape<R, IDEG, IMAX, NCH, DEPTH>;
-    constexpr bool RING = NCH > 0;
-    __shared__ __attribute__((aligned(1024))) char ring[RS::BYTES];
-    __shared__ R sBx[RING ? 1 : 8][RING ? 1 : NP];
-    __shared__ R sG0x[RING ? 1 : 8][RING ? 1 : IDEG * NV];
-    __shared__ int sN[RING ? 1 : IMAX * 64];  // stencil members of the tile, [j][pos]
-    R (*sB)[NP] = RING ? (R (*)[NP])(ring + RS::BSLOT * RS::SLOTB) : (R (*)[NP])&sBx[0][0];
-    R (*sG0)[IDEG * NV] = RING ? (R (*)[IDEG * NV])(ring + RS::GSLOT * RS::SLOTB) : (R (*)[IDEG * NV])&sG0x[0][0];
+    __shared__ R sB[8][NP];
+    __shared__ R sG0[8][IDEG * NV];
+    __shared__ int sN[IMAX * 64];  // stencil members of the tile, [j][pos]
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
     const int tile = (b % NXCD) * p.tiles_per_xcd + b / NXCD;
@@ -809,80 +778,7 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
     for (int k = 0; k < IDEG; ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) g[k][v] = R(0);
-    if constexpr (RING) {
-        // ---- LSQ: g_s = P_s dU_s (src/gradients.f90:253-497), rows streamed
-        // through the LDS ring by direct global->LDS loads ----
-        const int rowb = IDEG * TP * (int)sizeof(R);
-        const char *Pt = (const char *)(p.P + (size_t)tile * IMAX * IDEG * TP);
-        auto issue_row = [&](int j) {
-            const char *src = Pt + (size_t)j * rowb;
-            char *dst = ring + (j % DEPTH) * RS::SLOTB;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, rowb - c * 1024);
-        };
-        auto issue_b = [&]() {
-            const char *src = (const char *)(p.B + (size_t)tile * 8 * NP);
-            char *dst = (char *)&sB[0][0];
-            const int bb = nvc * NP * (int)sizeof(R);
-#pragma unroll
-            for (int c = 0; c < RS::NBC; ++c) dma_chunk(src + c * 1024, dst + c * 1024, lane, bb - c * 1024);
-        };
-#pragma unroll
-        for (int j = 0; j < DEPTH; ++j) issue_row(j);
-        const int32_t *nt = p.nbr + (size_t)tile * IMAX * TP + (act ? pos : 0);
-        double ub[NV] = {0, 0, 0, 0, 0};
-        int nnext = 0;
-        if (act) {
-            const int n0 = nt[0];
-            nnext = nt[TP];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)n0 * NV + v];
-        }
-        const int rl = act ? pos : 0;
-        // one member: dU from the prefetched state, prefetch the next state,
-        // wait for the row, accumulate
-        auto member = [&](int j, auto wait) {
-            R du[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) du[v] = (R)(ub[v] - ui[v]);
-            if (act && j + 1 < IMAX) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) ub[v] = p.U[(size_t)nnext * NV + v];
-                if (j + 2 < IMAX) nnext = nt[(size_t)(j + 2) * TP];
-            }
-            wait();
-            if (act) {
-                const R *row = (const R *)(ring + (j % DEPTH) * RS::SLOTB) + rl;
-                R pc[IDEG];
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k) pc[k] = row[k * TP];
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
-            }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        };
-        // steady state: DEPTH-1 rows behind row j are in flight
-#pragma unroll 1
-        for (int j = 0; j <= IMAX - DEPTH; ++j) {
-            member(j, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"((DEPTH - 1) * NCH) : "memory"); });
-            if (j + DEPTH < IMAX) issue_row(j + DEPTH);
-        }
-        if (RS::TAILB && RS::NB == 1) issue_b();
-        // tail: rows IMAX-DEPTH+1 .. IMAX-1, nothing new but the indicators
-        [&]<int... T>(std::integer_sequence<int, T...>) {
-            (
-                [&] {
-                    constexpr int t = T + 1;
-                    constexpr int pend = (DEPTH - 1 - t) * NCH + ((RS::TAILB && RS::NB <= t) ? RS::NBC : 0);
-                    member(IMAX - DEPTH + t, [] { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(pend) : "memory"); });
-                    if constexpr (RS::TAILB && RS::NB == t + 1) issue_b();
-                }(),
-                ...);
-        }(std::make_integer_sequence<int, DEPTH - 1>{});
-        if constexpr (!RS::TAILB) issue_b();
-    } else {
+    {
         // indicators and stencil members of the tile -> LDS (contiguous blocks)
         {
             const R *Bt = p.B + (size_t)tile * 8 * NP;
@@ -936,7 +832,6 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
             }
         }
     }
-    if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
     recon_late<R, IDEG, NQL>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
 }
 
@@ -992,15 +887,12 @@ struct StreamShape {
     static_assert((DEPTH - 1) * PER + NBC <= 63, "vmcnt immediate out of range");
 };
 
-template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH, bool SCAT = false, bool STORE = false>
+template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH, bool STORE = false>
 __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     using SS = StreamShape<R, IDEG, IMAX, TS, DEPTH>;
     constexpr int NP = SS::NP, TP = SS::TP;
     // ring slots, then the header's index rows
     __shared__ __attribute__((aligned(1024))) char ring[DEPTH * SS::SB + SS::HD];
-    // SCAT: the tile's member table and the rows of its member slots
-    __shared__ double sAcc[SCAT ? UCNS_TILE_UQ : 1][NV];
-    __shared__ uint16_t sSlot[SCAT ? IMAX * TP : 1];
     const int *hdr = (const int *)(ring + DEPTH * SS::SB);
     const int lane = threadIdx.x, lc = lane >> 3, s = lane & 7;
     const int b = blockIdx.x;
@@ -1014,36 +906,11 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) ui[v] = p.U[(size_t)cell * NV + v];
     }
-    // SCAT: this lane's coefficient cotangents, the member slots, a zeroed table
-    double gb[SCAT ? IDEG : 1][NV];
-    int nuq = 0;
-    if constexpr (SCAT) {
-        const double *gs = p.gbs + (size_t)tile * IDEG * NV * TP + (lc * TS + s);
-        const bool in = cvalid && s < TS;
-#pragma unroll
-        for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) gb[k][v] = in ? gs[(size_t)(k * NV + v) * TP] : 0.0;
-        const uint16_t *st = p.slot + (size_t)tile * IMAX * TP;
-        for (int e = lane; e < IMAX * TP; e += WAVE) sSlot[e] = st[e];
-        nuq = min(p.ucount[tile], p.uq_cap);
-        for (int e = lane; e < min(nuq, UCNS_TILE_UQ) * NV; e += WAVE) (&sAcc[0][0])[e] = 0.0;
-    }
-    // a use of every cotangent here makes the compiler wait for their loads
-    // before the row stream starts, not inside it
-    double gsum = 0.0;
-    if constexpr (SCAT) {
-#pragma unroll
-        for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) gsum += gb[k][v];
-    }
     // the cell's own data is in registers before the first row load
     asm volatile("s_waitcnt vmcnt(0)"
-                 : "+v"(info), "+v"(ui[0]), "+v"(ui[1]), "+v"(ui[2]), "+v"(ui[3]), "+v"(ui[4]), "+v"(gsum)
+                 : "+v"(info), "+v"(ui[0]), "+v"(ui[1]), "+v"(ui[2]), "+v"(ui[3]), "+v"(ui[4])
                  :
                  : "memory");
-    if constexpr (SCAT) __syncthreads();  // sSlot, sAcc
     const int admis = ci_admis(info);
     const bool act = cvalid && s < admis;
     const int rl = act ? lc * TS + s : 0;
@@ -1088,9 +955,6 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         }
     };
 
-    // SCAT: members of the next DEPTH rows, oldest first
-    int qn[DEPTH];
-    double acc[NV] = {0, 0, 0, 0, 0};
 
     // prologue: header, then (row, states) pairs for rows 0..DEPTH-1
     dma_chunk(T, ring + DEPTH * SS::SB, lane, SS::HD);
@@ -1099,7 +963,6 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
 #pragma unroll
     for (int r = 0; r < DEPTH; ++r) {
         if (r > 0) issue_row(r);
-        qn[r] = hdr[r * TP + rl];
         int ix[SS::NSI];
         piece_cells(hdr + r * TP, ix);
         issue_states(r, ix);
@@ -1128,31 +991,6 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
             for (int k = 0; k < IDEG; ++k)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) g[k][v] += pc[k] * du[v];
-            if constexpr (SCAT) {
-                // dU_bar_j = P_j^T g_bar to member j (the tile's LDS table), minus the same to U_i
-                double d[NV] = {0, 0, 0, 0, 0};
-#pragma unroll
-                for (int k = 0; k < IDEG; ++k)
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) d[v] += (double)pc[k] * gb[k][v];
-                // rows past this kernel's LDS table go straight to HBM (and are
-                // written as zeros below)
-                const unsigned sl = sSlot[i * TP + rl];
-                if (sl < UCNS_TILE_UQ) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        __hip_atomic_fetch_add(&sAcc[sl][v], d[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) unsafeAtomicAdd(&p.ubar[(size_t)qn[0] * NV + v], d[v]);
-                }
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] -= d[v];
-            }
-        }
-        if constexpr (SCAT) {
-#pragma unroll
-            for (int r = 0; r + 1 < DEPTH; ++r) qn[r] = qn[r + 1];
         }
     };
 #pragma unroll 1
@@ -1161,13 +999,11 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         consume(i);
         // members of row i+DEPTH, read before that row overwrites this slot
         const int *cols = (const int *)(ring + (i % DEPTH) * SS::SB + SS::PB);
-        const int nn = cols[rl];
         int ix[SS::NSI];
         piece_cells(cols, ix);
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         issue_row(i + DEPTH);
         issue_states(i + DEPTH, ix);
-        if constexpr (SCAT) qn[DEPTH - 1] = nn;
     }
     // tail: the last DEPTH rows; the indicators follow into the first free slots
     [&]<int... TT>(std::integer_sequence<int, TT...>) {
@@ -1189,17 +1025,6 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         issue_b();
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
-    if constexpr (SCAT) {
-        // the cells' own share, then the tile's member table, to HBM
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double t = grp_sum8(acc[v]);
-            if (s == 0 && cvalid) unsafeAtomicAdd(&p.ubar[(size_t)cell * NV + v], t);
-        }
-        __syncthreads();
-        double *tt = p.tab + (size_t)p.toff[tile] * NV;
-        for (int e = lane; e < nuq * NV; e += WAVE) tt[e] = e < UCNS_TILE_UQ * NV ? (&sAcc[0][0])[e] : 0.0;
-    }
     const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
     R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
     recon_late<R, IDEG, NQL, STORE>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
@@ -1208,15 +1033,6 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_s64(ReconWParams<double> p) {
     recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS)>(p);
-}
-// the same reconstruction with the scatter of a residual product on its row stream
-template <int IDEG, int IMAX, int NQL, int TS>
-__global__ __launch_bounds__(WAVE) void k_recon_sv64(ReconWParams<double> p) {
-    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS), true, true>(p);
-}
-template <int IDEG, int IMAX, int NQL, int TS>
-__global__ __launch_bounds__(WAVE) void k_recon_sv32(ReconWParams<float> p) {
-    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), true, true>(p);
 }
 // Scatter of a residual product on the row stream, without the
 // reconstruction: lane (cell, stencil) holds its g_bar (p.gbs) and streams
@@ -1344,11 +1160,11 @@ __global__ __launch_bounds__(WAVE) void k_scatter_s32(ReconWParams<float> p) {
 // the reconstruction storing every stencil's coefficients for a residual product
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE) void k_recon_sg64(ReconWParams<double> p) {
-    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS), false, true>(p);
+    recon_s_body<double, IDEG, IMAX, NQL, TS, stream_depth(false, TS), true>(p);
 }
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE, 2) void k_recon_sg32(ReconWParams<float> p) {
-    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), false, true>(p);
+    recon_s_body<float, IDEG, IMAX, NQL, TS, stream_depth(true, TS), true>(p);
 }
 template <int IDEG, int IMAX, int NQL, int TS>
 __global__ __launch_bounds__(WAVE, 2) void k_recon_s32(ReconWParams<float> p) {
@@ -1403,23 +1219,12 @@ __global__ void k_stream_idx(const int32_t *idx, unsigned char *stream, int ntil
 #endif
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE) void k_recon_w64(ReconWParams<double> p) {
-    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64, 0, 1>(p);
+    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64>(p);
 }
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE, UCNS_WAVES32) void k_recon_w32(ReconWParams<float> p) {
-    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32, 0, 1>(p);
+    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32>(p);
 }
-// ring variants: NCH = ceil(19 * 8 ts * sizeof(R) / 1024) for the stencil
-// count ts they are instantiated for
-template <int IDEG, int IMAX, int NQL, int NCH, int DEPTH>
-__global__ __launch_bounds__(WAVE) void k_recon_r64(ReconWParams<double> p) {
-    recon_w_body<double, IDEG, IMAX, NQL, 1, NCH, DEPTH>(p);
-}
-template <int IDEG, int IMAX, int NQL, int NCH, int DEPTH>
-__global__ __launch_bounds__(WAVE, 2) void k_recon_r32(ReconWParams<float> p) {
-    recon_w_body<float, IDEG, IMAX, NQL, 1, NCH, DEPTH>(p);
-}
-
 // [cells][ts_in][imax][ideg] (FP64, ABI) -> tiled [tile][imax][ideg][8*ts] (R)
 template <typename R>
 __global__ void k_tile_p(const double *src, R *dst, int cell0, int ncells_chunk, int ncell, int ts_in, int ts,
@@ -1697,6 +1502,8 @@ struct FluxParams {
     const double *U;        // [ncell][5] cell averages
     const R *uleft;         // face-point deviations from the cell average
     double *fflux;          // [nface][5] outward from the owner
+    const double *wq;       // [qpq] quadrilateral-face weights of this context, sum 1
+    const double *wt;       // [qpt] triangular-face weights, sum 1
     ucns_boundary_v1 bc;    // free-stream data of the boundary codes
 };
 
@@ -1842,7 +1649,7 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
     const int sL = s0 & 255;
     const bool tri = (s0 >> 8) & 1;
     const int nq = tri ? p.qpt : p.qpq;
-    const double *w = tri ? c_wt : c_wq;
+    const double *w = tri ? p.wt : p.wq;
     const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
     const double area = p.farea[f];
     const bool bnd = cR < 0;  // boundary face: -code
@@ -2975,7 +2782,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     const int sL = s0 & 255;
     const bool tri = (s0 >> 8) & 1;
     const int nq = tri ? p.qpt : p.qpq;
-    const double *w = tri ? c_wt : c_wq;
+    const double *w = tri ? p.wt : p.wq;
     const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
     const double area = p.farea[f];
     constexpr bool bnd = BND;
@@ -3129,6 +2936,28 @@ __global__ void k_rk3_adj_stage(int ncell, int stage, const double *y, const dou
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// Per-cell 5-vectors across the ABI: the caller's cell o = perm[c] (o = c
+// without a permutation) in the caller's layout <-> the library's cell c in
+// AoS.  src and dst never alias.
+__global__ __launch_bounds__(256) void k_cells_in(int ncell, const int32_t *perm, const double *src, int layout,
+                                                  double *dst) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const size_t o = perm ? (size_t)perm[c] : (size_t)c;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        dst[(size_t)c * NV + v] = layout == UCNS_LAYOUT_AOS ? src[o * NV + v] : src[(size_t)v * ncell + o];
+}
+__global__ __launch_bounds__(256) void k_cells_out(int ncell, const int32_t *perm, const double *src, int layout,
+                                                   double *dst) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const size_t o = perm ? (size_t)perm[c] : (size_t)c;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        dst[layout == UCNS_LAYOUT_AOS ? o * NV + v : (size_t)v * ncell + o] = src[(size_t)c * NV + v];
+}
+
 __global__ void k_nonfinite(const double *u, size_t n, int *flag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
@@ -3170,7 +2999,6 @@ struct ucns_ctx {
     double gamma = 1.4, lwci1 = 1.0, eps = 1e-11;
     bool have_prestore = false, have_state = false;
     bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
-    bool ring = false;  // UCNS_RING=1 selects the LDS-ring kernels (experimental)
     int smode = 0;      // stencils per cell of the row-stream kernel in use (0: none)
     bool needs_bc = false, have_bc = false;  // boundary codes 1/2/6 present; ucns_set_boundary called
     bool has_bnd_faces = false;              // any face with a ghost state (codes 1/2/3/4/6)
@@ -3179,7 +3007,18 @@ struct ucns_ctx {
     // mesh (device)
     int32_t *cinfo = nullptr, *cnb = nullptr, *cface = nullptr, *fcell = nullptr, *fslot = nullptr;
     double *frot = nullptr, *farea = nullptr, *ovol = nullptr, *minedge = nullptr;
+    double *wquad = nullptr;  // [128] face weights: [0, 64) quadrilateral, [64, 128) triangle
     std::vector<int32_t> h_cinfo_base;  // ifca + face types (admis/full added at upload)
+    // cell order: library cell c is the caller's cell h_perm[c] (both empty:
+    // the caller's order); h_iperm is the inverse
+    int cell_order = UCNS_ORDER_CALLER;
+    std::vector<int32_t> h_perm, h_iperm;
+    int32_t *perm = nullptr;    // device copy of h_perm, or null
+    double *stage = nullptr;    // [ncell][5] staging buffer of host transfers
+    // library options of the scheme
+    int kernels = UCNS_KERNELS_AUTO;
+    int adj_keep_stages = 0;
+    double adj_keep_gib = 64.0;
     // prestore (device)
     int32_t *nbr = nullptr;
     void *P = nullptr, *B = nullptr, *mbar = nullptr, *qp = nullptr, *uleft = nullptr;
@@ -3196,8 +3035,6 @@ struct ucns_ctx {
     double *adj_u0 = nullptr, *adj_bc = nullptr, *ckpt = nullptr, *dts = nullptr;
     void *adj_g = nullptr;  // stencil coefficients, indicators, weights of the row-stream reconstruction (R)
     double *adj_gb = nullptr;  // stencil-coefficient cotangents handed to the row stream, [ntile][ideg*5][8 ts]
-    const double *sv_gb = nullptr;  // non-null: the next row-stream reconstruction also scatters sv_gb ...
-    double *sv_ubar = nullptr;      // ... into sv_ubar
     uint16_t *adj_slot = nullptr;                      // [ntile][imax][8 ts] LDS row of each member entry
     int32_t *adj_uq = nullptr, *adj_ucount = nullptr;  // [ntile][uq_cap], [ntile]
     int uq_cap = UCNS_TILE_UQ;                         // member-table rows per tile (tile_uq(ts, imax))
@@ -3205,8 +3042,9 @@ struct ucns_ctx {
     int32_t *adj_ent = nullptr;                       // CSR entries: table rows (fewer than 2^31)
     double *adj_tab = nullptr;                          // [rows][5] member tables
     bool adj_g_on = false;  // the next tiled reconstruction stores them
-    bool adj_fuse = false;  // row-stream mode: scatter on the next row stream (UCNS_ADJ_FUSE=1) or in the product
     int *adj_flags = nullptr;  // [ncell] its MUSCL / first-order flags
+    bool adj_ready = false;  // every adjoint buffer and member table is in place
+    int fail_alloc_in = 0;   // > 0: the fail_alloc_in-th device allocation from now fails (ucns_debug_fail_alloc)
     size_t ckpt_steps = 0;
     // the last stages of a march keep their forward reconstruction for the
     // reverse sweep: face values, and in row-stream mode the stencil
@@ -3223,6 +3061,11 @@ namespace {
 
 template <typename T>
 int dalloc(ucns_ctx *ctx, T **p, size_t n) {
+    if (ctx->fail_alloc_in > 0 && --ctx->fail_alloc_in == 0) {
+        *p = nullptr;
+        ctx->err = "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed (injected)";
+        return UCNS_ENOMEM;
+    }
     hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
     if (e != hipSuccess) {
         ctx->err = std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " bytes): " + hipGetErrorString(e);
@@ -3299,39 +3142,6 @@ int launch_recon_t(ucns_ctx *ctx) {
         }
         w.gstore = ctx->adj_g_on ? (R *)ctx->adj_g : nullptr;
         w.fstore = ctx->adj_g_on ? ctx->adj_flags : nullptr;
-        if (ctx->smode && ctx->sv_ubar) {
-            // reconstruction + the scatter of the previous residual product
-            w.gbs = ctx->sv_gb;
-            w.ubar = ctx->sv_ubar;
-            w.slot = ctx->adj_slot;
-            w.uq_cap = ctx->uq_cap;
-            w.uq = ctx->adj_uq;
-            w.ucount = ctx->adj_ucount;
-            w.toff = ctx->adj_toff;
-            w.tab = ctx->adj_tab;
-            if constexpr (sizeof(R) == 8) {
-                if (ctx->ideg == 19)
-                    if (ctx->smode == 5)
-                        hipLaunchKernelGGL((k_recon_sv64<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
-                    else
-                        hipLaunchKernelGGL((k_recon_sv64<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
-                else if (ctx->smode == 5)
-                    hipLaunchKernelGGL((k_recon_sv64<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
-                else
-                    hipLaunchKernelGGL((k_recon_sv64<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
-            } else {
-                if (ctx->ideg == 19)
-                    if (ctx->smode == 5)
-                        hipLaunchKernelGGL((k_recon_sv32<19, 39, 3, 5>), grid, block, 0, ctx->stream, w);
-                    else
-                        hipLaunchKernelGGL((k_recon_sv32<19, 39, 7, 7>), grid, block, 0, ctx->stream, w);
-                else if (ctx->smode == 5)
-                    hipLaunchKernelGGL((k_recon_sv32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
-                else
-                    hipLaunchKernelGGL((k_recon_sv32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
-            }
-            return UCNS_OK;
-        }
         if (ctx->smode && ctx->adj_g_on) {
             if constexpr (sizeof(R) == 8) {
                 if (ctx->ideg == 19)
@@ -3378,24 +3188,6 @@ int launch_recon_t(ucns_ctx *ctx) {
                     hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
                 else
                     hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
-            }
-            return UCNS_OK;
-        }
-        // pseudo-inverse rows streamed through an LDS ring (tetrahedra: 5
-        // stencils, hexahedra: 7)
-        const bool ring_ok = ctx->ring && ctx->ideg == 19 && ((ctx->ts == 5 && ctx->nf * ctx->nqm <= 24) ||
-                                               (ctx->ts == 7 && ctx->nf * ctx->nqm > 24));
-        if (ring_ok) {
-            if constexpr (sizeof(R) == 8) {
-                if (ctx->ts == 5)
-                    hipLaunchKernelGGL((k_recon_r64<19, 39, 3, 6, 6>), grid, block, 0, ctx->stream, w);
-                else
-                    hipLaunchKernelGGL((k_recon_r64<19, 39, 7, 9, 4>), grid, block, 0, ctx->stream, w);
-            } else {
-                if (ctx->ts == 5)
-                    hipLaunchKernelGGL((k_recon_r32<19, 39, 3, 3, 6>), grid, block, 0, ctx->stream, w);
-                else
-                    hipLaunchKernelGGL((k_recon_r32<19, 39, 7, 5, 4>), grid, block, 0, ctx->stream, w);
             }
             return UCNS_OK;
         }
@@ -3447,6 +3239,8 @@ void launch_flux_t(ucns_ctx *ctx) {
     p.U = ctx->U;
     p.uleft = (const R *)ctx->uleft;
     p.fflux = ctx->fflux;
+    p.wq = ctx->wquad;
+    p.wt = ctx->wquad + 64;
     p.bc = ctx->bc;
     hipLaunchKernelGGL((k_flux<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, p);
 }
@@ -3550,6 +3344,7 @@ void release_adjoint(ucns_ctx *ctx) {
     dfree(ctx->adj_ent);
     dfree(ctx->adj_tab);
     ctx->ckpt_steps = 0;
+    ctx->adj_ready = false;
     for (void *q : ctx->keep_ul) (void)hipFree(q);
     for (void *q : ctx->keep_g) (void)hipFree(q);
     for (int *q : ctx->keep_f) (void)hipFree(q);
@@ -3592,8 +3387,14 @@ void launch_tile_members(ucns_ctx *ctx) {
                            ctx->cinfo, ctx->uq_cap, ctx->adj_uq, ctx->adj_ucount, ctx->adj_slot);
 }
 
-int ensure_adjoint(ucns_ctx *ctx) {
-    if (ctx->adj_ubf) return UCNS_OK;
+// frees a temporary device buffer on every exit of the scope
+template <typename T>
+struct DevTemp {
+    T *p = nullptr;
+    ~DevTemp() { dfree(p); }
+};
+
+int ensure_adjoint_steps(ucns_ctx *ctx) {
     const size_t n = (size_t)ctx->ncell * NV;
     int rc;
     if ((rc = dalloc(ctx, &ctx->adj_ubf, (size_t)ctx->ncell * ctx->nf * ctx->nqm * NV))) return rc;
@@ -3648,8 +3449,9 @@ int ensure_adjoint(ucns_ctx *ctx) {
         HIP_TRY(hipMemcpyAsync(ctx->adj_toff, toff.data(), (nt + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                                ctx->stream));
         // CSR by cell: count, scan, fill
-        int64_t *cnt = nullptr;
-        if ((rc = dalloc(ctx, &cnt, (size_t)nc))) return rc;
+        DevTemp<int64_t> cnt_buf;
+        if ((rc = dalloc(ctx, &cnt_buf.p, (size_t)nc))) return rc;
+        int64_t *cnt = cnt_buf.p;
         HIP_TRY(hipMemsetAsync(cnt, 0, nc * sizeof(int64_t), ctx->stream));
         hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->uq_cap, ctx->adj_uq,
                            ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)nullptr, (int32_t *)nullptr);
@@ -3658,7 +3460,6 @@ int ensure_adjoint(ucns_ctx *ctx) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i < nc; ++i) off[i + 1] = off[i] + hc[i];
         if (off[nc] != rows) {  // every table row belongs to exactly one cell
-            dfree(cnt);
             ctx->err = "member-table index: " + std::to_string(off[nc]) + " entries for " + std::to_string(rows) +
                        " rows";
             return UCNS_EHIP;
@@ -3669,8 +3470,23 @@ int ensure_adjoint(ucns_ctx *ctx) {
         hipLaunchKernelGGL(k_adj_csr, dim3(nt), dim3(256), 0, ctx->stream, nt, ctx->uq_cap, ctx->adj_uq,
                            ctx->adj_ucount, ctx->adj_toff, cnt, (const int64_t *)ctx->adj_off, ctx->adj_ent);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        dfree(cnt);
     }
+    return UCNS_OK;
+}
+
+// The adjoint's buffers and member tables, set up on first use.  A context
+// counts as set up only once every step has succeeded; a failed step frees
+// what the earlier ones allocated, so the next call starts over and returns
+// the error again instead of running kernels on missing buffers.
+int ensure_adjoint(ucns_ctx *ctx) {
+    if (ctx->adj_ready) return UCNS_OK;
+    release_adjoint(ctx);
+    const int rc = ensure_adjoint_steps(ctx);
+    if (rc != UCNS_OK) {
+        release_adjoint(ctx);
+        return rc;
+    }
+    ctx->adj_ready = true;
     return UCNS_OK;
 }
 
@@ -3727,26 +3543,24 @@ void launch_scatter_t(ucns_ctx *ctx, double *ubar) {
     }
 }
 
+// the forward reconstruction of the current state for a residual product:
+// ULEFT, and in row-stream mode every stencil's coefficients, indicators,
+// weights and the cell flags
 template <typename R>
-int vjp_recon_t(ucns_ctx *ctx, const double *gb, double *ubar) {
+int vjp_recon_t(ucns_ctx *ctx) {
     (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
     ctx->adj_g_on = ctx->smode && ctx->adj_g;
-    ctx->sv_gb = gb;
-    ctx->sv_ubar = gb ? ubar : nullptr;
     tstart(ctx, 3);
     const int rc = launch_recon_t<R>(ctx);
-    if (rc == UCNS_OK && gb) adj_gather(ctx, ubar);
     tstop(ctx, 3);
     tcollect(ctx, 3, &ctx->st.ms_recon_adj, &ctx->st.n_recon_adj);
     ctx->adj_g_on = false;
-    ctx->sv_gb = nullptr;
-    ctx->sv_ubar = nullptr;
     return rc;
 }
 
 // the products of one residual at the state of the last vjp_recon_t: ubar
-// (zeroed here) = the flux and reconstruction products; in row-stream mode
-// the scatter of P^T g_bar is left to the next row stream (adj_gb)
+// (zeroed here) = the flux product, the reconstruction product and, in
+// row-stream mode, the scatter of P^T g_bar (adj_gb) through the member tables
 template <typename R>
 int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
     const int nfq = ctx->nf * ctx->nqm;
@@ -3769,6 +3583,8 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     fp.U = ctx->U;
     fp.uleft = (const R *)ctx->uleft;
     fp.fflux = nullptr;
+    fp.wq = ctx->wquad;
+    fp.wt = ctx->wquad + 64;
     fp.bc = ctx->bc;
     tstart(ctx, 1);
     hipLaunchKernelGGL((k_flux_vjp<R, false>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
@@ -3845,8 +3661,8 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
         ctx->err = "no residual product for idegfree=" + std::to_string(ctx->ideg);
         return UCNS_EUNSUPPORTED;
     }
-    if (ctx->smode && !ctx->adj_fuse) launch_scatter_t<R>(ctx, ubar);
-    if (!(ctx->smode && ctx->adj_fuse) && ctx->adj_slot) adj_gather(ctx, ubar);
+    if (ctx->smode) launch_scatter_t<R>(ctx, ubar);
+    if (ctx->adj_slot) adj_gather(ctx, ubar);
     tstop(ctx, 0);
     tcollect(ctx, 0, &ctx->st.ms_recon_vjp, &ctx->st.n_recon_vjp);
     return UCNS_OK;
@@ -3857,11 +3673,8 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
 template <typename R>
 int residual_vjp_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar) {
     int rc;
-    if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) return rc;
-    if ((rc = vjp_products_t<R>(ctx, lam, ubar, bcbar))) return rc;
-    // fused row stream: one more pass of the same state carries the scatter
-    if (ctx->smode && ctx->adj_fuse && (rc = vjp_recon_t<R>(ctx, ctx->adj_gb, ubar))) return rc;
-    return UCNS_OK;
+    if ((rc = vjp_recon_t<R>(ctx))) return rc;
+    return vjp_products_t<R>(ctx, lam, ubar, bcbar);
 }
 
 // (dR/dU)^T lam at the current state into ubar; bcbar (device, 10) += the
@@ -3874,21 +3687,20 @@ int residual_vjp_dev(ucns_ctx *ctx, const double *lam, double *ubar, double *bcb
 }
 
 // How many of a march's last stages keep their forward reconstruction (at
-// most nstage): UCNS_ADJ_KEEP caps the count, UCNS_ADJ_KEEP_GB the bytes
-// (default 64 GiB, and never more than the free device memory less 4 GiB).
+// most nstage): the scheme's adj_keep_stages caps the count (0: no cap,
+// < 0: none), adj_keep_gib the bytes (never more than the free device memory
+// less 4 GiB).
 // Buffers only grow; a failed allocation leaves the count at what was had.
 int adj_keep(ucns_ctx *ctx, int nstage) {
-    if (ctx->adj_fuse) return 0;  // the fused sweep recomputes every stage
     const size_t esz = ctx->precision == UCNS_MIXED ? 4 : 8;
     const size_t ul = (size_t)ctx->ncell * ctx->nf * ctx->nqm * NV * esz;
     const size_t g = ctx->smode ? (size_t)ctx->ntile * (ctx->ideg + 2) * NV * 8 * ctx->ts * esz : 0;
     const size_t fl = ctx->smode ? (size_t)ctx->ncell * sizeof(int) : 0;
     const size_t per = ul + g + fl;
     long want = nstage;
-    if (const char *e = std::getenv("UCNS_ADJ_KEEP")) want = std::min(want, std::max(0L, std::atol(e)));
-    double gib = 64.0;
-    if (const char *e = std::getenv("UCNS_ADJ_KEEP_GB")) gib = std::atof(e);
-    size_t budget = gib > 0 ? (size_t)(gib * (double)(1ull << 30)) : 0;
+    if (ctx->adj_keep_stages < 0) return 0;
+    if (ctx->adj_keep_stages > 0) want = std::min<long>(want, ctx->adj_keep_stages);
+    size_t budget = (size_t)(ctx->adj_keep_gib * (double)(1ull << 30));
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
         const size_t have = fr + ctx->keep_ul.size() * per, margin = 4ull << 30;
@@ -3978,46 +3790,27 @@ int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar, int ke
     double *const ustate = ctx->U;
     double *a = abar, *u0 = ctx->adj_u0, *x = ctx->adj_x, *w = ctx->adj_lam, *jt = ctx->adj_ubar;
     const double cst[4] = {0.0, 1.0, 1.0 / 4.0, 2.0 / 3.0};  // -w / (dt ybar / V) per stage
-    const bool pipe = ctx->smode != 0 && ctx->adj_fuse;
     int rc = UCNS_OK;
-    int pending = 0;  // stage whose jt still waits for its scatter (0: none)
-    // combine the stage whose product is complete into its input cotangents
-    auto combine = [&](int stage) {
-        const double *y = stage == 3 ? a : x;
-        hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, stage, y, jt, x,
-                           u0);
-        if (stage == 1) (void)hipMemcpyAsync(a, u0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
-    };
     for (int it = nsteps - 1; it >= 0 && rc == UCNS_OK; --it) {
         const double *dt = ctx->dts + it;
         for (int stage = 3; stage >= 1 && rc == UCNS_OK; --stage) {
             ctx->U = ctx->ckpt + ((size_t)3 * it + stage - 1) * n;
-            if (pipe) {
-                // forward reconstruction of this stage's state (+ the pending scatter)
-                if ((rc = vjp_recon_t<R>(ctx, pending ? ctx->adj_gb : nullptr, jt))) break;
-                if (pending) combine(pending);
-            } else if (pending) {
-                combine(pending);
-            }
             const double *y = stage == 3 ? a : x;
             hipLaunchKernelGGL(k_rk3_adj_w, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, y, ctx->ovol, dt,
                                cst[stage], w);
             const int q = 3 * it + stage - 1;
-            if (pipe) {
-                if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
-            } else if (q >= keep0) {
+            if (q >= keep0) {
                 KeptStage k(ctx, q - keep0);
                 if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
             } else {
-                if ((rc = vjp_recon_t<R>(ctx, nullptr, nullptr))) break;
+                if ((rc = vjp_recon_t<R>(ctx))) break;
                 if ((rc = vjp_products_t<R>(ctx, w, jt, bcbar))) break;
             }
-            pending = stage;
+            // this stage's product is complete: fold it into its input cotangents
+            hipLaunchKernelGGL(k_rk3_adj_stage, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ncell, stage, y, jt, x,
+                               u0);
+            if (stage == 1) (void)hipMemcpyAsync(a, u0, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
         }
-    }
-    if (rc == UCNS_OK && pending) {
-        if (pipe) rc = vjp_recon_t<R>(ctx, ctx->adj_gb, jt);  // the last scatter (the state is any stored one)
-        if (rc == UCNS_OK) combine(pending);
     }
     ctx->U = ustate;
     if (rc == UCNS_OK) {
@@ -4033,44 +3826,6 @@ int rk3_reverse_t(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar, int ke
 int rk3_reverse(ucns_ctx *ctx, int nsteps, double *abar, double *bcbar, int keep0) {
     return ctx->precision == UCNS_MIXED ? rk3_reverse_t<float>(ctx, nsteps, abar, bcbar, keep0)
                                         : rk3_reverse_t<double>(ctx, nsteps, abar, bcbar, keep0);
-}
-
-int state_io(ucns_ctx *ctx, const double *src, double *dst, int32_t layout, hipMemcpyKind kind, bool to_dev) {
-    const size_t n = (size_t)ctx->ncell * NV;
-    if (layout == UCNS_LAYOUT_AOS) {
-        if (to_dev)
-            HIP_TRY(hipMemcpyAsync(ctx->U, src, n * sizeof(double), kind, ctx->stream));
-        else
-            HIP_TRY(hipMemcpyAsync(dst, ctx->U, n * sizeof(double), kind, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        return UCNS_OK;
-    }
-    if (layout != UCNS_LAYOUT_SOA) {
-        ctx->err = "unknown layout";
-        return UCNS_EBADARG;
-    }
-    // SoA <-> AoS through host memory (layout conversion is not on the hot path)
-    std::vector<double> a(n), b(n);
-    if (to_dev) {
-        if (kind == hipMemcpyDeviceToDevice) {
-            HIP_TRY(hipMemcpy(b.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
-        } else {
-            std::memcpy(b.data(), src, n * sizeof(double));
-        }
-        for (int i = 0; i < ctx->ncell; ++i)
-            for (int v = 0; v < NV; ++v) a[(size_t)i * NV + v] = b[(size_t)v * ctx->ncell + i];
-        HIP_TRY(hipMemcpy(ctx->U, a.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-        HIP_TRY(hipMemcpy(a.data(), ctx->U, n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < ctx->ncell; ++i)
-            for (int v = 0; v < NV; ++v) b[(size_t)v * ctx->ncell + i] = a[(size_t)i * NV + v];
-        if (kind == hipMemcpyDeviceToDevice) {
-            HIP_TRY(hipMemcpy(dst, b.data(), n * sizeof(double), hipMemcpyHostToDevice));
-        } else {
-            std::memcpy(dst, b.data(), n * sizeof(double));
-        }
-    }
-    return UCNS_OK;
 }
 
 template <typename R>
@@ -4099,6 +3854,228 @@ void release_prestore(ucns_ctx *ctx) {
 }
 
 
+// ---- cell order -------------------------------------------------------------
+// The library may hold the cells in an order of its own, so that a tile of 8
+// consecutive cells is a compact cluster whatever order the caller's
+// partitioner produced.  perm[c] = the caller's cell of library cell c.
+
+// reverse Cuthill-McKee on the face graph (interior and periodic faces):
+// per connected component, breadth-first from a pseudo-peripheral cell of
+// least degree, neighbours in increasing degree, then the whole order reversed
+void order_rcm(const ucns_mesh_v2 *m, std::vector<int32_t> &perm) {
+    const int nc = m->ncell, nfa = m->nface_max, base = m->index_base;
+    std::vector<int32_t> deg(nc, 0);
+    auto nbr = [&](int i, int f) -> int {
+        if (f >= m->ifca[i]) return -1;
+        const int bc = m->bcode[(size_t)i * nfa + f];
+        if (bc != 0 && bc != 5 && bc != 50) return -1;
+        const int j = m->ineigh[(size_t)i * nfa + f] - base;
+        return j == i ? -1 : j;
+    };
+    for (int i = 0; i < nc; ++i)
+        for (int f = 0; f < nfa; ++f) deg[i] += nbr(i, f) >= 0;
+    std::vector<int32_t> level(nc, -1), q;
+    q.reserve(nc);
+    // breadth-first levels from r over the cells not yet ordered; returns the
+    // cell of least degree in the last level and the number of levels
+    std::vector<char> done(nc, 0);
+    std::vector<int32_t> touched;
+    auto bfs_far = [&](int r, int &nlev) {
+        touched.clear();
+        std::vector<int32_t> frontier{r}, next;
+        level[r] = 0;
+        touched.push_back(r);
+        nlev = 1;
+        int far = r;
+        while (!frontier.empty()) {
+            next.clear();
+            for (int i : frontier)
+                for (int f = 0; f < nfa; ++f) {
+                    const int j = nbr(i, f);
+                    if (j >= 0 && !done[j] && level[j] < 0) {
+                        level[j] = level[i] + 1;
+                        touched.push_back(j);
+                        next.push_back(j);
+                    }
+                }
+            if (next.empty()) {
+                far = frontier[0];
+                for (int i : frontier)
+                    if (deg[i] < deg[far] || (deg[i] == deg[far] && i < far)) far = i;
+            } else {
+                ++nlev;
+            }
+            frontier.swap(next);
+        }
+        for (int i : touched) level[i] = -1;
+        return far;
+    };
+    perm.clear();
+    perm.reserve(nc);
+    std::vector<int32_t> by_deg(nc);
+    for (int i = 0; i < nc; ++i) by_deg[i] = i;
+    std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    std::vector<int32_t> cand;
+    for (int seed : by_deg) {
+        if (done[seed]) continue;
+        int r = seed, nlev = 0;
+        int far = bfs_far(r, nlev);
+        for (int it = 0; it < 8; ++it) {  // pseudo-peripheral cell (George and Liu)
+            int nl2 = 0;
+            const int far2 = bfs_far(far, nl2);
+            if (nl2 <= nlev) break;
+            r = far;
+            nlev = nl2;
+            far = far2;
+        }
+        size_t head = perm.size();
+        perm.push_back(r);
+        done[r] = 1;
+        while (head < perm.size()) {
+            const int i = perm[head++];
+            cand.clear();
+            for (int f = 0; f < nfa; ++f) {
+                const int j = nbr(i, f);
+                if (j >= 0 && !done[j]) {
+                    done[j] = 1;
+                    cand.push_back(j);
+                }
+            }
+            std::sort(cand.begin(), cand.end(),
+                      [&](int a, int b) { return deg[a] < deg[b] || (deg[a] == deg[b] && a < b); });
+            perm.insert(perm.end(), cand.begin(), cand.end());
+        }
+    }
+    std::reverse(perm.begin(), perm.end());
+}
+
+// Morton (Z-order) curve of the centroids: 21 bits per axis over the
+// bounding box, ties by the caller's index
+void order_morton(const ucns_mesh_v2 *m, std::vector<int32_t> &perm) {
+    const int nc = m->ncell;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < nc; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], m->centroid[(size_t)3 * i + d]);
+            hi[d] = std::max(hi[d], m->centroid[(size_t)3 * i + d]);
+        }
+    auto spread = [](uint64_t x) {  // 21 bits -> every third bit of 63
+        x &= 0x1FFFFFull;
+        x = (x | x << 32) & 0x1F00000000FFFFull;
+        x = (x | x << 16) & 0x1F0000FF0000FFull;
+        x = (x | x << 8) & 0x100F00F00F00F00Full;
+        x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    std::vector<uint64_t> key(nc);
+    for (int i = 0; i < nc; ++i) {
+        uint64_t k = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double ext = hi[d] - lo[d];
+            const double t = ext > 0 ? (m->centroid[(size_t)3 * i + d] - lo[d]) / ext : 0.0;
+            const uint64_t g = (uint64_t)std::min(2097151.0, std::max(0.0, std::floor(t * 2097152.0)));
+            k |= spread(g) << (2 - d);
+        }
+        key[i] = k;
+    }
+    perm.resize(nc);
+    for (int i = 0; i < nc; ++i) perm[i] = i;
+    std::sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+}
+
+// the caller's cell of library cell i, and the library cell of caller cell o
+inline size_t src_cell(const ucns_ctx *ctx, int i) { return ctx->h_perm.empty() ? (size_t)i : (size_t)ctx->h_perm[i]; }
+inline int lib_cell(const ucns_ctx *ctx, int o) { return ctx->h_iperm.empty() ? o : ctx->h_iperm[o]; }
+
+// caller-order 5-vectors (host or device memory, either layout) -> the
+// library's AoS device array dst
+int cells_in(ucns_ctx *ctx, const double *src, bool host, int32_t layout, double *dst) {
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) {
+        ctx->err = "unknown layout";
+        return UCNS_EBADARG;
+    }
+    const size_t n = (size_t)ctx->ncell * NV;
+    if (host) {
+        HIP_TRY(hipMemcpyAsync(ctx->stage, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        src = ctx->stage;
+    }
+    hipLaunchKernelGGL(k_cells_in, dim3((ctx->ncell + 255) / 256), dim3(256), 0, ctx->stream, ctx->ncell, ctx->perm,
+                       src, layout, dst);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UCNS_OK;
+}
+
+// the library's AoS device array src -> caller-order 5-vectors
+int cells_out(ucns_ctx *ctx, const double *src, int32_t layout, double *dst, bool host) {
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) {
+        ctx->err = "unknown layout";
+        return UCNS_EBADARG;
+    }
+    const size_t n = (size_t)ctx->ncell * NV;
+    hipLaunchKernelGGL(k_cells_out, dim3((ctx->ncell + 255) / 256), dim3(256), 0, ctx->stream, ctx->ncell, ctx->perm,
+                       src, layout, host ? ctx->stage : dst);
+    HIP_TRY(hipGetLastError());
+    if (host) HIP_TRY(hipMemcpyAsync(dst, ctx->stage, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UCNS_OK;
+}
+
+// The caller's mesh with its cells in the library's order (per-cell rows
+// moved, neighbour ids renumbered); the vectors own the storage the struct
+// points into.
+struct PermutedMesh {
+    ucns_mesh_v2 m{};
+    std::vector<int32_t> ifca, ineigh, ineighn, bcode, types;
+    std::vector<double> angx, angy, surf, vol, minedge;
+    void build(const ucns_mesh_v2 *src, const std::vector<int32_t> &perm, const std::vector<int32_t> &iperm) {
+        const int nc = src->ncell, nfa = src->nface_max, base = src->index_base;
+        const size_t nfc = (size_t)nc * nfa;
+        ifca.resize(nc);
+        vol.resize(nc);
+        minedge.resize(nc);
+        ineigh.resize(nfc);
+        ineighn.resize(nfc);
+        bcode.resize(nfc);
+        types.resize(nfc);
+        angx.resize(nfc);
+        angy.resize(nfc);
+        surf.resize(nfc);
+        for (int c = 0; c < nc; ++c) {
+            const size_t o = perm[c];
+            ifca[c] = src->ifca[o];
+            vol[c] = src->totvolume[o];
+            minedge[c] = src->minedge[o];
+            for (int f = 0; f < nfa; ++f) {
+                const size_t d = (size_t)c * nfa + f, sl = o * nfa + f;
+                const int bc = src->bcode[sl];
+                const int j = src->ineigh[sl] - base;
+                const bool link = f < src->ifca[o] && (bc == 0 || bc == 5 || bc == 50) && j >= 0 && j < nc;
+                ineigh[d] = link ? iperm[j] + base : src->ineigh[sl];
+                ineighn[d] = src->ineighn[sl];
+                bcode[d] = bc;
+                types[d] = src->types_faces[sl];
+                angx[d] = src->faceanglex[sl];
+                angy[d] = src->faceangley[sl];
+                surf[d] = src->surf[sl];
+            }
+        }
+        m = *src;
+        m.ifca = ifca.data();
+        m.ineigh = ineigh.data();
+        m.ineighn = ineighn.data();
+        m.bcode = bcode.data();
+        m.types_faces = types.data();
+        m.faceanglex = angx.data();
+        m.faceangley = angy.data();
+        m.surf = surf.data();
+        m.totvolume = vol.data();
+        m.minedge = minedge.data();
+        m.centroid = nullptr;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -4107,10 +4084,11 @@ int ucns_abi_version(void) { return UCNS_ABI_VERSION; }
 
 const char *ucns_last_error(const ucns_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) {
+int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) {
     if (!out) return UCNS_EBADARG;
     *out = nullptr;
     if (!m || !s) return UCNS_EBADARG;
+    PermutedMesh pm;
     ucns_ctx *ctx = new ucns_ctx();
     *out = ctx;
     auto bad = [&](int code, const std::string &msg) {
@@ -4134,6 +4112,18 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
     if (s->qp_quad < 1 || s->qp_quad > 64 || s->qp_triangle < 1 || s->qp_triangle > 64 || !s->weights_quad ||
         !s->weights_tri)
         return bad(UCNS_EBADARG, "bad face quadrature");
+    if (s->cell_order != UCNS_ORDER_CALLER && s->cell_order != UCNS_ORDER_RCM && s->cell_order != UCNS_ORDER_MORTON)
+        return bad(UCNS_EBADARG, "cell_order must be UCNS_ORDER_CALLER, _RCM or _MORTON");
+    if (s->cell_order == UCNS_ORDER_MORTON && !m->centroid)
+        return bad(UCNS_EBADARG, "UCNS_ORDER_MORTON needs the mesh centroids");
+    if (s->kernels != UCNS_KERNELS_AUTO && s->kernels != UCNS_KERNELS_REGISTER)
+        return bad(UCNS_EBADARG, "kernels must be UCNS_KERNELS_AUTO or UCNS_KERNELS_REGISTER");
+    if (s->reserved != 0) return bad(UCNS_EBADARG, "reserved must be 0");
+    if (!std::isfinite(s->adj_keep_gib)) return bad(UCNS_EBADARG, "adj_keep_gib must be finite");
+    ctx->cell_order = s->cell_order;
+    ctx->kernels = s->kernels;
+    ctx->adj_keep_stages = s->adj_keep_stages;
+    ctx->adj_keep_gib = s->adj_keep_gib > 0 ? s->adj_keep_gib : 64.0;
     ctx->ncell = m->ncell;
     ctx->nf_abi = m->nface_max;
     ctx->index_base = m->index_base;
@@ -4153,9 +4143,23 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
         return bad(UCNS_EUNSUPPORTED,
                    "supported (idegfree, inumneighbours): (19, 40) WENO-4, (9, 20) WENO-3 and (3, 9) MUSCL");
     if (ctx->iweno == -1 && ctx->ideg != 3) return bad(UCNS_EUNSUPPORTED, "MUSCL path is linear (idegfree 3)");
+    if (m->ncell > 0 && m->ifca)
+        for (int i = 0; i < m->ncell; ++i)
+            if (m->ifca[i] < 4 || m->ifca[i] > m->nface_max)
+                return bad(UCNS_EBADARG, "ifca out of range at cell " + std::to_string(i));
     HIP_TRY(hipGetDevice(&ctx->device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+    if (ctx->cell_order != UCNS_ORDER_CALLER) {
+        if (ctx->cell_order == UCNS_ORDER_RCM)
+            order_rcm(m, ctx->h_perm);
+        else
+            order_morton(m, ctx->h_perm);
+        ctx->h_iperm.assign(m->ncell, 0);
+        for (int c = 0; c < m->ncell; ++c) ctx->h_iperm[ctx->h_perm[c]] = c;
+        pm.build(m, ctx->h_perm, ctx->h_iperm);
+        m = &pm.m;
+    }
 
     const int nc = m->ncell, nfa = m->nface_max, base = m->index_base;
     // faces in use, Gauss points per face type present
@@ -4218,7 +4222,9 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
             const size_t slj = (size_t)j * nfa + fj;
             if (m->ineigh[slj] - base != i || m->ineighn[slj] - base != f)
                 return bad(UCNS_EBADARG, "face neighbours are not symmetric at cell " + std::to_string(i));
-            cnb[(size_t)i * nf + f] = j;
+            // MUSCL bounds take the neighbours of interior faces and of periodic
+            // faces of code 5, not of code 50 (FIND_BOUNDS, src/reconstruct_p.f90:1871)
+            cnb[(size_t)i * nf + f] = bc == 50 ? -1 : j;
             const bool owner = (i < j) || (i == j && f < fj);
             if (!owner) continue;
             const int id = (int)farea.size();
@@ -4253,8 +4259,13 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
         (rc = dalloc(ctx, &ctx->U2, (size_t)nc * NV)) || (rc = dalloc(ctx, &ctx->rhs, (size_t)nc * NV)) ||
         (rc = dalloc(ctx, &ctx->fflux, (size_t)ctx->nface * NV)) || (rc = dalloc(ctx, &ctx->t, 1)) ||
         (rc = dalloc(ctx, &ctx->dtfix, 1)) || (rc = dalloc(ctx, &ctx->dtv, 2)) ||
-        (rc = dalloc(ctx, &ctx->counters, 2)) || (rc = dalloc(ctx, &ctx->flag, 1)))
+        (rc = dalloc(ctx, &ctx->counters, 2)) || (rc = dalloc(ctx, &ctx->flag, 1)) ||
+        (rc = dalloc(ctx, &ctx->stage, (size_t)nc * NV)))
         return rc;
+    if (!ctx->h_perm.empty()) {
+        if ((rc = dalloc(ctx, &ctx->perm, nc))) return rc;
+        HIP_TRY(hipMemcpy(ctx->perm, ctx->h_perm.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     const size_t ul_elems = (size_t)nc * nf * nqm * NV;
     rc = ctx->precision == UCNS_MIXED ? dalloc(ctx, (float **)&ctx->uleft, ul_elems)
                                       : dalloc(ctx, (double **)&ctx->uleft, ul_elems);
@@ -4272,8 +4283,9 @@ int ucns_create(const ucns_mesh_v1 *m, const ucns_scheme_v1 *s, ucns_ctx **out) 
     double wq[64] = {0}, wt[64] = {0};
     for (int q = 0; q < s->qp_quad; ++q) wq[q] = s->weights_quad[q];
     for (int q = 0; q < s->qp_triangle; ++q) wt[q] = s->weights_tri[q];
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wq), wq, sizeof(wq)));
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wt), wt, sizeof(wt)));
+    if ((rc = dalloc(ctx, &ctx->wquad, 128))) return rc;
+    HIP_TRY(hipMemcpy(ctx->wquad, wq, sizeof(wq), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->wquad + 64, wt, sizeof(wt), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(ctx->t, 0, sizeof(double)));
     return UCNS_OK;
 }
@@ -4298,13 +4310,14 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     int ts = 1;
     std::vector<int32_t> cinfo = ctx->h_cinfo_base;
     for (int i = 0; i < nc; ++i) {
-        const int ad = pre->admis[i];
+        const size_t o = src_cell(ctx, i);
+        const int ad = pre->admis[o];
         if (ad < 1 || ad > TSA) {
-            ctx->err = "admis out of range at cell " + std::to_string(i);
+            ctx->err = "admis out of range at cell " + std::to_string(o);
             return UCNS_EBADARG;
         }
         ts = std::max(ts, ad);
-        cinfo[i] |= (ad << 3) | ((pre->full[i] ? 1 : 0) << 6);
+        cinfo[i] |= (ad << 3) | ((pre->full[o] ? 1 : 0) << 6);
     }
     if (ctx->iweno == -1 && ts != 1) {
         ctx->err = "MUSCL uses the central stencil only (admis must be 1)";
@@ -4315,37 +4328,35 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     ctx->tiled = (ctx->iweno == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) && ts <= 8 &&
                   ctx->nf * ctx->nqm <= 56);
     {
-        const char *e = std::getenv("UCNS_RING");
-        ctx->ring = e && e[0] == '1';
         // row-stream kernels: tetrahedra (5 stencils, 24 face points) and
-        // hexahedra (7 stencils, 54 face points); UCNS_STREAM=0 opts out
-        const char *st = std::getenv("UCNS_STREAM");
+        // hexahedra (7 stencils, 54 face points), unless the scheme asks for
+        // the register-prefetch kernels
         const int nfq = ctx->nf * ctx->nqm;
         const bool shape_ok = ID == 19 ? ((ts == 5 && nfq <= 24) || (ts == 7 && nfq > 24))
                                        : ((ts == 5 || ts == 7) && nfq <= 24);
-        const bool stream_ok = !(st && st[0] == '0') && !ctx->ring && ctx->tiled && shape_ok;
+        const bool stream_ok = ctx->kernels == UCNS_KERNELS_AUTO && ctx->tiled && shape_ok;
         ctx->smode = stream_ok ? ts : 0;
-        const char *fu = std::getenv("UCNS_ADJ_FUSE");
-        ctx->adj_fuse = fu && fu[0] == '1';
     }
     ctx->ntile = (nc + 7) / 8;
     // stencil members without the cell itself
     //   tiled: [tile][imax][8*ts], lane position 8-cell-tile local
     //   plain: [cell][ts][imax]
     std::vector<int32_t> nbr(ctx->smode ? (size_t)0 : ctx->tiled ? (size_t)ctx->ntile * IM * 8 * ts : (size_t)nc * ts * IM, 0);
-    for (int i = 0; i < nc; ++i)
-        for (int s = 0; s < pre->admis[i]; ++s) {
-            const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
-            if (row[0] - base != i) {
-                ctx->err = "IHEXL(s,1) must be the cell itself (cell " + std::to_string(i) + ")";
+    for (int i = 0; i < nc; ++i) {
+        const size_t o = src_cell(ctx, i);
+        for (int s = 0; s < pre->admis[o]; ++s) {
+            const int32_t *row = pre->ihexl + (o * TSA + s) * NNB;
+            if ((size_t)(row[0] - base) != o) {
+                ctx->err = "IHEXL(s,1) must be the cell itself (cell " + std::to_string(o) + ")";
                 return UCNS_EBADARG;
             }
             for (int k = 0; k < IM; ++k) {
-                const int j = row[k + 1] - base;
-                if (j < 0 || j >= nc) {
-                    ctx->err = "stencil member out of range at cell " + std::to_string(i);
+                const int jo = row[k + 1] - base;
+                if (jo < 0 || jo >= nc) {
+                    ctx->err = "stencil member out of range at cell " + std::to_string(o);
                     return UCNS_EBADARG;
                 }
+                const int j = lib_cell(ctx, jo);
                 if (ctx->smode) {
                     continue;  // carried by the row stream
                 } else if (ctx->tiled) {
@@ -4356,6 +4367,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                 }
             }
         }
+    }
     int rc = dalloc(ctx, &ctx->nbr, std::max<size_t>(nbr.size(), 1));
     if (rc) return rc;
     if (!nbr.empty())
@@ -4375,12 +4387,14 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         // member-index rows: row r holds members j = r (zero past imax)
         const size_t rows = (size_t)sD + IM;
         std::vector<int32_t> idx((size_t)ctx->ntile * rows * sTP, 0);
-        for (int i = 0; i < nc; ++i)
-            for (int s = 0; s < pre->admis[i]; ++s) {
-                const int32_t *row = pre->ihexl + ((size_t)i * TSA + s) * NNB;
+        for (int i = 0; i < nc; ++i) {
+            const size_t o = src_cell(ctx, i);
+            for (int s = 0; s < pre->admis[o]; ++s) {
+                const int32_t *row = pre->ihexl + (o * TSA + s) * NNB;
                 for (int r = 0; r < IM; ++r)
-                    idx[((size_t)(i / 8) * rows + r) * sTP + (size_t)(i % 8) * ts + s] = row[r + 1] - base;
+                    idx[((size_t)(i / 8) * rows + r) * sTP + (size_t)(i % 8) * ts + s] = lib_cell(ctx, row[r + 1] - base);
             }
+        }
         int32_t *didx = nullptr;
         if ((rc = dalloc(ctx, &didx, idx.size()))) return rc;
         HIP_TRY(hipMemcpy(didx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -4398,16 +4412,23 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         if (hipHostMalloc((void **)&host_pin, chunk_cells * per_cell_abi * sizeof(double), hipHostMallocDefault) !=
             hipSuccess)
             host_pin = nullptr;
+        std::vector<double> host_buf;  // pageable staging when pinned memory is refused and cells move
+        if (!host_pin && !ctx->h_perm.empty()) host_buf.resize(chunk_cells * per_cell_abi);
+        double *gather = host_pin ? host_pin : host_buf.data();
         for (size_t c0 = 0; c0 < (size_t)nc; c0 += chunk_cells) {
             const size_t cn = std::min(chunk_cells, (size_t)nc - c0);
-            const double *src = pre->invmat + c0 * per_cell_abi;
             const size_t bytes = cn * per_cell_abi * sizeof(double);
-            if (host_pin) {
+            const double *src = pre->invmat + c0 * per_cell_abi;
+            if (!ctx->h_perm.empty()) {  // the chunk's cells from their places in the caller's order
+                for (size_t c = 0; c < cn; ++c)
+                    std::memcpy(gather + c * per_cell_abi, pre->invmat + src_cell(ctx, (int)(c0 + c)) * per_cell_abi,
+                                per_cell_abi * sizeof(double));
+                src = gather;
+            } else if (host_pin) {
                 std::memcpy(host_pin, src, bytes);
-                HIP_TRY(hipMemcpyAsync(stage, host_pin, bytes, hipMemcpyHostToDevice, ctx->stream));
-            } else {
-                HIP_TRY(hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+                src = host_pin;
             }
+            HIP_TRY(hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, ctx->stream));
             if (ctx->smode) {
                 const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
                 if (mixed)
@@ -4445,7 +4466,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         const size_t np = (size_t)ID * (ID + 1) / 2;
         std::vector<double> bp((size_t)nc * np);
         for (int i = 0; i < nc; ++i) {
-            const double *Bi = pre->indicator + (size_t)i * ID * ID;
+            const double *Bi = pre->indicator + src_cell(ctx, i) * ID * ID;
             size_t e = (size_t)i * np;
             for (int k = 0; k < ID; ++k)
                 for (int l = k; l < ID; ++l) bp[e++] = Bi[(size_t)k * ID + l];
@@ -4462,8 +4483,9 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     {
         std::vector<double> mb((size_t)nc * ID);
         for (int i = 0; i < nc; ++i) {
-            const double oov = 1.0 / pre->volume11[i];
-            for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[(size_t)i * ID + d] * oov;
+            const size_t o = src_cell(ctx, i);
+            const double oov = 1.0 / pre->volume11[o];
+            for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[o * ID + d] * oov;
         }
         if ((rc = upload_real_any(ctx, &ctx->mbar, mb))) return rc;
     }
@@ -4476,7 +4498,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                 for (int g = 0; g < nqm && g < nqa; ++g)
                     for (int d = 0; d < 3; ++d)
                         q[(((size_t)i * nf + f) * nqm + g) * 3 + d] =
-                            pre->qpoints[(((size_t)i * nfa + f) * nqa + g) * 3 + d];
+                            pre->qpoints[((src_cell(ctx, i) * nfa + f) * nqa + g) * 3 + d];
         if ((rc = upload_real_any(ctx, &ctx->qp, q))) return rc;
     }
     ctx->have_prestore = true;
@@ -4513,7 +4535,7 @@ int ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc) {
 int ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
-    int rc = state_io(ctx, u, nullptr, layout, hipMemcpyHostToDevice, true);
+    int rc = cells_in(ctx, u, true, layout, ctx->U);
     if (rc == 0) ctx->have_state = true;
     return rc;
 }
@@ -4521,13 +4543,13 @@ int ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout) {
 int ucns_get_state(ucns_ctx *ctx, double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
-    return state_io(ctx, nullptr, u, layout, hipMemcpyDeviceToHost, false);
+    return cells_out(ctx, ctx->U, layout, u, true);
 }
 
 int ucns_set_state_device(ucns_ctx *ctx, const double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
-    int rc = state_io(ctx, u, nullptr, layout, hipMemcpyDeviceToDevice, true);
+    int rc = cells_in(ctx, u, false, layout, ctx->U);
     if (rc == 0) ctx->have_state = true;
     return rc;
 }
@@ -4535,7 +4557,7 @@ int ucns_set_state_device(ucns_ctx *ctx, const double *u, int32_t layout) {
 int ucns_get_state_device(ucns_ctx *ctx, double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
-    return state_io(ctx, nullptr, u, layout, hipMemcpyDeviceToDevice, false);
+    return cells_out(ctx, ctx->U, layout, u, false);
 }
 
 static int ready(ucns_ctx *ctx) {
@@ -4592,12 +4614,14 @@ int ucns_reconstruct(ucns_ctx *ctx, double *uleft_out) {
     HIP_TRY(hipMemcpy(u.data(), ctx->U, u.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int nfa = ctx->nf_abi, nqa = ctx->nqm_abi;
     std::memset(uleft_out, 0, sizeof(double) * (size_t)ctx->ncell * nfa * nqa * NV);
-    for (int i = 0; i < ctx->ncell; ++i)
+    for (int i = 0; i < ctx->ncell; ++i) {
+        const size_t o = src_cell(ctx, i);
         for (int f = 0; f < ctx->nf; ++f)
             for (int q = 0; q < ctx->nqm; ++q)
                 for (int v = 0; v < NV; ++v)
-                    uleft_out[(((size_t)i * nfa + f) * nqa + q) * NV + v] =
+                    uleft_out[((o * nfa + f) * nqa + q) * NV + v] =
                         u[(size_t)i * NV + v] + d[(((size_t)i * ctx->nf + f) * ctx->nqm + q) * NV + v];
+    }
     int cnt[2];
     HIP_TRY(hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
     ctx->st.n_reduced_last = cnt[0];
@@ -4605,7 +4629,7 @@ int ucns_reconstruct(ucns_ctx *ctx, double *uleft_out) {
     return UCNS_OK;
 }
 
-int ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout) {
+static int residual_to(ucns_ctx *ctx, double *rhs_out, int32_t layout, bool host) {
     if (!ctx) return UCNS_EBADARG;
     if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
     int rc = ready(ctx);
@@ -4619,17 +4643,15 @@ int ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout) {
     HIP_TRY(hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
     ctx->st.n_reduced_last = cnt[0];
     ctx->st.n_firstorder_last = cnt[1];
-    if (rhs_out) {
-        const size_t n = (size_t)ctx->ncell * NV;
-        std::vector<double> a(n);
-        HIP_TRY(hipMemcpy(a.data(), ctx->rhs, n * sizeof(double), hipMemcpyDeviceToHost));
-        if (layout == UCNS_LAYOUT_AOS)
-            std::memcpy(rhs_out, a.data(), n * sizeof(double));
-        else
-            for (int i = 0; i < ctx->ncell; ++i)
-                for (int v = 0; v < NV; ++v) rhs_out[(size_t)v * ctx->ncell + i] = a[(size_t)i * NV + v];
-    }
+    if (rhs_out) return cells_out(ctx, ctx->rhs, layout, rhs_out, host);
     return UCNS_OK;
+}
+
+int ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout) { return residual_to(ctx, rhs_out, layout, true); }
+
+int ucns_residual_device(ucns_ctx *ctx, double *rhs_dev, int32_t layout) {
+    if (!rhs_dev) return UCNS_EBADARG;
+    return residual_to(ctx, rhs_dev, layout, false);
 }
 
 int ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, double *t_io) {
@@ -4680,39 +4702,20 @@ int ucns_rk3_steps(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, d
     return UCNS_OK;
 }
 
-static void to_aos(const ucns_ctx *ctx, const double *src, int32_t layout, std::vector<double> &dst) {
-    const size_t n = (size_t)ctx->ncell * NV;
-    dst.resize(n);
-    if (layout == UCNS_LAYOUT_AOS)
-        std::memcpy(dst.data(), src, n * sizeof(double));
-    else
-        for (int i = 0; i < ctx->ncell; ++i)
-            for (int v = 0; v < NV; ++v) dst[(size_t)i * NV + v] = src[(size_t)v * ctx->ncell + i];
-}
-
-static void from_aos(const ucns_ctx *ctx, const std::vector<double> &src, int32_t layout, double *dst) {
-    const size_t n = (size_t)ctx->ncell * NV;
-    if (layout == UCNS_LAYOUT_AOS)
-        std::memcpy(dst, src.data(), n * sizeof(double));
-    else
-        for (int i = 0; i < ctx->ncell; ++i)
-            for (int v = 0; v < NV; ++v) dst[(size_t)v * ctx->ncell + i] = src[(size_t)i * NV + v];
-}
-
 int ucns_residual_vjp_device(ucns_ctx *ctx, const double *w_dev, double *u_bar_dev, double *bc_bar_dev) {
     if (!ctx || !w_dev || !u_bar_dev) return UCNS_EBADARG;
     int rc = ready(ctx);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     if ((rc = ensure_adjoint(ctx))) return rc;
+    if ((rc = cells_in(ctx, w_dev, false, UCNS_LAYOUT_AOS, ctx->adj_a))) return rc;
     zero_dev(ctx, ctx->adj_bc, 2 * NV);
-    if ((rc = residual_vjp_dev(ctx, w_dev, u_bar_dev, ctx->adj_bc))) return rc;
+    if ((rc = residual_vjp_dev(ctx, ctx->adj_a, ctx->adj_x, ctx->adj_bc))) return rc;
     HIP_TRY(hipGetLastError());
     if (bc_bar_dev)
         HIP_TRY(hipMemcpyAsync(bc_bar_dev, ctx->adj_bc, 2 * NV * sizeof(double), hipMemcpyDeviceToDevice,
                                ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return UCNS_OK;
+    return cells_out(ctx, ctx->adj_x, UCNS_LAYOUT_AOS, u_bar_dev, false);
 }
 
 int ucns_residual_vjp(ucns_ctx *ctx, const double *w, double *u_bar, int32_t layout, double *bc_bar) {
@@ -4722,18 +4725,13 @@ int ucns_residual_vjp(ucns_ctx *ctx, const double *w, double *u_bar, int32_t lay
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     if ((rc = ensure_adjoint(ctx))) return rc;
-    const size_t n = (size_t)ctx->ncell * NV;
-    std::vector<double> h;
-    to_aos(ctx, w, layout, h);
-    HIP_TRY(hipMemcpyAsync(ctx->adj_a, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = cells_in(ctx, w, true, layout, ctx->adj_a))) return rc;
     zero_dev(ctx, ctx->adj_bc, 2 * NV);
     if ((rc = residual_vjp_dev(ctx, ctx->adj_a, ctx->adj_x, ctx->adj_bc))) return rc;
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(h.data(), ctx->adj_x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     double bc[2 * NV];
     HIP_TRY(hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    from_aos(ctx, h, layout, u_bar);
+    if ((rc = cells_out(ctx, ctx->adj_x, layout, u_bar, true))) return rc;
     if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
     return UCNS_OK;
 }
@@ -4792,15 +4790,12 @@ int ucns_rk3_adjoint_device(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     if ((rc = ensure_adjoint(ctx))) return rc;
-    const size_t n = (size_t)ctx->ncell * NV;
-    HIP_TRY(hipMemcpyAsync(ctx->adj_a, uN_bar_dev, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = cells_in(ctx, uN_bar_dev, false, UCNS_LAYOUT_AOS, ctx->adj_a))) return rc;
     if ((rc = rk3_adjoint_impl(ctx, nsteps, cfl, dt_fixed, ctx->adj_a, t_io))) return rc;
-    HIP_TRY(hipMemcpyAsync(u0_bar_dev, ctx->adj_a, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     if (bc_bar_dev)
         HIP_TRY(hipMemcpyAsync(bc_bar_dev, ctx->adj_bc, 2 * NV * sizeof(double), hipMemcpyDeviceToDevice,
                                ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return UCNS_OK;
+    return cells_out(ctx, ctx->adj_a, UCNS_LAYOUT_AOS, u0_bar_dev, false);
 }
 
 int ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar,
@@ -4811,16 +4806,11 @@ int ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed,
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     if ((rc = ensure_adjoint(ctx))) return rc;
-    const size_t n = (size_t)ctx->ncell * NV;
-    std::vector<double> h;
-    to_aos(ctx, uN_bar, layout, h);
-    HIP_TRY(hipMemcpyAsync(ctx->adj_a, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = cells_in(ctx, uN_bar, true, layout, ctx->adj_a))) return rc;
     if ((rc = rk3_adjoint_impl(ctx, nsteps, cfl, dt_fixed, ctx->adj_a, t_io))) return rc;
-    HIP_TRY(hipMemcpyAsync(h.data(), ctx->adj_a, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     double bc[2 * NV];
     HIP_TRY(hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    from_aos(ctx, h, layout, u0_bar);
+    if ((rc = cells_out(ctx, ctx->adj_a, layout, u0_bar, true))) return rc;
     if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
     return UCNS_OK;
 }
@@ -4828,6 +4818,12 @@ int ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed,
 int ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out) {
     if (!ctx || !out) return UCNS_EBADARG;
     *out = ctx->st;
+    return UCNS_OK;
+}
+
+int ucns_debug_fail_alloc(ucns_ctx *ctx, int32_t nth) {
+    if (!ctx || nth < 0) return UCNS_EBADARG;
+    ctx->fail_alloc_in = nth;
     return UCNS_OK;
 }
 
@@ -4859,6 +4855,9 @@ void ucns_destroy(ucns_ctx *ctx) {
     dfree(ctx->farea);
     dfree(ctx->ovol);
     dfree(ctx->minedge);
+    dfree(ctx->wquad);
+    dfree(ctx->stage);
+    dfree(ctx->perm);
     dfree(ctx->U);
     dfree(ctx->U2);
     dfree(ctx->rhs);

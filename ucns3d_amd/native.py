@@ -1,6 +1,6 @@
 """ctypes bindings of the C-ABI in include/ucns.h and of the host preprocessor.
 
-The structs below mirror ucns_mesh_v1 / ucns_scheme_v1 / ucns_prestore_v1
+The structs below mirror ucns_mesh_v2 / ucns_scheme_v2 / ucns_prestore_v1
 field for field.  Loading fails loudly when a library has not been built.
 """
 from __future__ import annotations
@@ -11,26 +11,32 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 
-UCNS_ABI_VERSION = 1
+UCNS_ABI_VERSION = 2
 UCNS_OK = 0
 UCNS_FP64 = 0
 UCNS_MIXED = 1
 UCNS_LAYOUT_AOS = 0
 UCNS_LAYOUT_SOA = 1
+UCNS_ORDER_CALLER = 0
+UCNS_ORDER_RCM = 1
+UCNS_ORDER_MORTON = 2
+UCNS_KERNELS_AUTO = 0
+UCNS_KERNELS_REGISTER = 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
 
-class MeshV1(C.Structure):
+class MeshV2(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("ncell", C.c_int32), ("index_base", C.c_int32), ("nface_max", C.c_int32),
         ("ifca", _i32p), ("ineigh", _i32p), ("ineighn", _i32p), ("bcode", _i32p), ("types_faces", _i32p),
         ("faceanglex", _f64p), ("faceangley", _f64p), ("surf", _f64p), ("totvolume", _f64p), ("minedge", _f64p),
+        ("centroid", _f64p),
     ]
 
 
-class SchemeV1(C.Structure):
+class SchemeV2(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("precision", C.c_int32), ("iweno", C.c_int32), ("iorder", C.c_int32),
         ("idegfree", C.c_int32), ("inumneighbours", C.c_int32), ("typesten", C.c_int32), ("ees", C.c_int32),
@@ -38,6 +44,8 @@ class SchemeV1(C.Structure):
         ("qp_quad", C.c_int32), ("qp_triangle", C.c_int32),
         ("gamma", C.c_double), ("lwci1", C.c_double), ("weno_eps", C.c_double),
         ("weights_quad", _f64p), ("weights_tri", _f64p),
+        ("cell_order", C.c_int32), ("kernels", C.c_int32), ("adj_keep_stages", C.c_int32), ("reserved", C.c_int32),
+        ("adj_keep_gib", C.c_double),
     ]
 
 
@@ -78,7 +86,7 @@ class PrepOpts(C.Structure):
 
 # every entry point declared in include/ucns.h: name -> (restype, argtypes)
 GPU_API = {
-    "ucns_create": (C.c_int, [C.POINTER(MeshV1), C.POINTER(SchemeV1), C.POINTER(C.c_void_p)]),
+    "ucns_create": (C.c_int, [C.POINTER(MeshV2), C.POINTER(SchemeV2), C.POINTER(C.c_void_p)]),
     "ucns_upload_prestore": (C.c_int, [C.c_void_p, C.POINTER(PrestoreV1)]),
     "ucns_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(BoundaryV1)]),
     "ucns_set_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
@@ -87,6 +95,7 @@ GPU_API = {
     "ucns_get_state_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "ucns_compute_dt": (C.c_int, [C.c_void_p, C.c_double, _f64p]),
     "ucns_residual": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
+    "ucns_residual_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "ucns_reconstruct": (C.c_int, [C.c_void_p, _f64p]),
     "ucns_rk3_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, _f64p]),
     "ucns_residual_vjp": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
@@ -99,6 +108,7 @@ GPU_API = {
                                           C.c_void_p, C.POINTER(C.c_double)]),
     "ucns_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsV1)]),
     "ucns_reset_stats": (C.c_int, [C.c_void_p]),
+    "ucns_debug_fail_alloc": (C.c_int, [C.c_void_p, C.c_int32]),
     "ucns_last_error": (C.c_char_p, [C.c_void_p]),
     "ucns_destroy": (None, [C.c_void_p]),
     "ucns_abi_version": (C.c_int, []),
@@ -107,7 +117,7 @@ GPU_API = {
 PREP_API = {
     "ucnsp_build": (C.c_int, [C.POINTER(PrepOpts), C.POINTER(C.c_void_p)]),
     "ucnsp_fill": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double,
-                             C.POINTER(MeshV1), C.POINTER(SchemeV1), C.POINTER(PrestoreV1)]),
+                             C.POINTER(MeshV2), C.POINTER(SchemeV2), C.POINTER(PrestoreV1)]),
     "ucnsp_array": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
                               C.POINTER(C.c_int32)]),
     "ucnsp_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -29,11 +29,23 @@ class UcnsError(RuntimeError):
 
 
 class Solver:
-    def __init__(self, case, precision: int = UCNS_FP64, lib_path: str | None = None):
+    """One library context over a Case.
+
+    cell_order, kernels, adj_keep_stages and adj_keep_gib are the library
+    options of ucns_scheme_v2 (UCNS_ORDER_*, UCNS_KERNELS_*; 0 = defaults);
+    `structs` replaces the Case's (mesh, scheme, prestore) structs, e.g. with
+    a caller order of the cells of one's own."""
+
+    def __init__(self, case, precision: int = UCNS_FP64, lib_path: str | None = None, cell_order: int = 0,
+                 kernels: int = 0, adj_keep_stages: int = 0, adj_keep_gib: float = 0.0, structs=None):
         self.case = case
         self.precision = precision
         self.L = native.gpu_lib(lib_path)
-        self._m, self._s, self._p = case.structs(precision)
+        self._m, self._s, self._p = structs if structs is not None else case.structs(precision)
+        self._s.cell_order = cell_order
+        self._s.kernels = kernels
+        self._s.adj_keep_stages = adj_keep_stages
+        self._s.adj_keep_gib = adj_keep_gib
         h = C.c_void_p()
         rc = self.L.ucns_create(C.byref(self._m), C.byref(self._s), C.byref(h))
         self._h = h
@@ -77,6 +89,12 @@ class Solver:
         rhs = np.empty((self.ncell, 5))
         self._check("ucns_residual", self.L.ucns_residual(self._h, native.f64p(rhs), UCNS_LAYOUT_AOS))
         return rhs
+
+    def residual_device(self, ptr: int) -> None:
+        self._check("ucns_residual_device", self.L.ucns_residual_device(self._h, C.c_void_p(ptr), UCNS_LAYOUT_AOS))
+
+    def debug_fail_alloc(self, nth: int) -> None:
+        self._check("ucns_debug_fail_alloc", self.L.ucns_debug_fail_alloc(self._h, nth))
 
     def reconstruct(self) -> np.ndarray:
         ul = np.empty(self.uleft_shape)
