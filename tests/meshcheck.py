@@ -1,8 +1,17 @@
 """Test-side geometry written independently of the preprocessor: cell
-vertices, quadrature over tetrahedra (collapsed Gauss-Legendre) and
-trilinear hexahedra (tensor Gauss-Legendre), local-frame maps and exact
-cell averages of polynomials.  Used to pin the preprocessor's PRESTORE
-arrays and the oracle against closed forms."""
+vertices, quadrature over tetrahedra (collapsed Gauss-Legendre), trilinear
+hexahedra (tensor Gauss-Legendre), prisms and pyramids (their
+sub-tetrahedra), local-frame maps and exact cell averages of polynomials.
+Used to pin the preprocessor's PRESTORE arrays and the oracle against
+closed forms.
+
+A cell whose quadrilaterals are not planar (jittered prisms, pyramids and
+hexahedra, the bumped channel wall) is defined by the sub-tetrahedra of the
+reference's DECOMPOSE3 (src/grid_t.f90:1027-1074), so its rule integrates
+over those; a hexahedron that is a parallelepiped takes the trilinear map,
+and a second, different split of prisms and pyramids (ALT_TETS) agrees with
+DECOMPOSE3 wherever the faces are planar, which tests the tables
+themselves."""
 from __future__ import annotations
 
 import itertools
@@ -10,7 +19,15 @@ import math
 
 import numpy as np
 
-from ucns3d_amd.case import HEX, TET
+from ucns3d_amd.case import HEX, PRISM, PYRAMID, TET
+
+NNODES = {HEX: 8, TET: 4, PRISM: 6, PYRAMID: 5}
+NFACES = {HEX: 6, TET: 4, PRISM: 5, PYRAMID: 5}
+# prism: nodes 0-2 the bottom triangle, 3-5 the top (3 above 0);
+# pyramid: 0-3 the base, 4 the apex
+SUBTETS = {PRISM: [(0, 1, 2, 5), (0, 1, 5, 4), (0, 4, 5, 3)], PYRAMID: [(0, 1, 2, 4), (0, 2, 3, 4)],
+           HEX: [(0, 5, 7, 4), (0, 1, 7, 5), (1, 6, 7, 5), (0, 7, 2, 3), (0, 7, 1, 2), (1, 7, 6, 2)]}
+ALT_TETS = {PRISM: [(0, 1, 2, 3), (1, 2, 3, 4), (2, 3, 4, 5)], PYRAMID: [(0, 1, 3, 4), (1, 2, 3, 4)]}
 
 def monomial_exponents(ideg: int):
     """exponents of the reference's monomial basis (BASIS_REC, src/basis.f90:27-102):
@@ -89,10 +106,10 @@ class Mesh:
         self.frame = case.array("frame").reshape(nc, 12)
 
     def nfaces(self, i):
-        return 6 if self.shape[i] == HEX else 4
+        return NFACES[int(self.shape[i])]
 
     def verts(self, i, shift=None):
-        n = 8 if self.shape[i] == HEX else 4
+        n = NNODES[int(self.shape[i])]
         v = self.xyz[self.cnodes[i, :n]].copy()
         if shift is not None:
             v += shift
@@ -105,13 +122,28 @@ class Mesh:
         L = np.where(self.L > 0, self.L, 1.0)
         return np.where(self.L > 0, -np.round(d / L) * L, 0.0)
 
-    def cell_rule(self, i, n=4, shift=None):
-        """global quadrature points and weights (sum = volume) of cell i"""
+    def cell_rule(self, i, n=4, shift=None, alt=False):
+        """global quadrature points and weights (sum = volume) of cell i;
+        alt: prisms and pyramids through the other split"""
         v = self.verts(i, shift)
-        if self.shape[i] == TET:
+        sh = int(self.shape[i])
+        if sh == TET:
             p, w = tet_rule(n)
             A = np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]], axis=1)
             return v[0] + p @ A.T, w * abs(np.linalg.det(A))
+        if sh == HEX:
+            e = [v[1] - v[0], v[3] - v[0], v[4] - v[0]]
+            corners = v[0] + HEX_CORNERS @ np.stack(e)
+            affine = np.abs(corners - v).max() <= 1e-12 * np.abs(v).max()
+        if sh in SUBTETS and not (sh == HEX and affine):
+            p, w = tet_rule(n)
+            xs, ws = [], []
+            for t in (ALT_TETS if alt else SUBTETS)[sh]:
+                a = v[list(t)]
+                A = np.stack([a[1] - a[0], a[2] - a[0], a[3] - a[0]], axis=1)
+                xs.append(a[0] + p @ A.T)
+                ws.append(w * abs(np.linalg.det(A)))
+            return np.concatenate(xs), np.concatenate(ws)
         g, gw = gauss_legendre01(n)
         xi = np.array(list(itertools.product(g, g, g)))
         ww = np.array([a * b * c for a, b, c in itertools.product(gw, gw, gw)])
@@ -119,12 +151,19 @@ class Mesh:
         return x, ww * dj
 
     def rules(self, n=4):
-        """quadrature points [nc, nq, 3] and weights [nc, nq] of every cell
-        (meshes of one cell shape)"""
+        """quadrature points [nc, nq, 3] and weights [nc, nq] of every cell;
+        cells with fewer points than the most are padded with zero weights"""
         key = ("rules", n)
         if key not in self.__dict__:
             pts, wts = zip(*(self.cell_rule(i, n) for i in range(self.nc)))
-            self.__dict__[key] = (np.stack(pts), np.stack(wts))
+            nq = max(len(w) for w in wts)
+            P = np.zeros((self.nc, nq, 3))
+            W = np.zeros((self.nc, nq))
+            for i, (x, w) in enumerate(zip(pts, wts)):
+                P[i, :len(w)] = x
+                P[i, len(w):] = x[0]
+                W[i, :len(w)] = w
+            self.__dict__[key] = (P, W)
         return self.__dict__[key]
 
     def member_means(self, i, members, exps, n=4):

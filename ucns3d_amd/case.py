@@ -20,6 +20,7 @@ TET = 2
 PRISM = 3
 PYRAMID = 4
 HYBRID = 5  # hexes, pyramids and prisms in thirds of x
+CHANNEL = 6  # prism wall layers, a pyramid/tetrahedron interface layer, tetrahedra above
 
 
 @dataclass(frozen=True)
@@ -39,6 +40,8 @@ class CaseSpec:
     gamma: float = 1.4
     bc: tuple = (0, 0, 0, 0, 0, 0)  # codes of the x-, x+, y-, y+, z-, z+ sides; 0 = periodic axis
     bump: tuple = ()      # lower-wall bump coefficients a_1..a_8 (channel configuration)
+    box: tuple = ()       # (nx, ny, nz, lx, ly, lz) for a box other than n^3 cubes of side `length`
+    mach: float = 0.0     # > 0: the initial state is a uniform flow along x at this Mach number, rho = p = 1
 
     @property
     def iorder(self) -> int:
@@ -54,10 +57,24 @@ class CaseSpec:
         return 3 if self.scheme == "muscl" else 2
 
     @property
+    def dims(self):
+        """(nx, ny, nz, lx, ly, lz)"""
+        return tuple(self.box) if self.box else (self.n, self.n, self.n, self.length, self.length, self.length)
+
+    @property
     def ncell(self) -> int:
+        nx, ny, nz = self.dims[:3]
         if self.shape == HYBRID:
-            return self.n ** 2 * sum(1 if i < self.n // 3 else 6 if i < 2 * self.n // 3 else 2 for i in range(self.n))
-        return self.n ** 3 * {HEX: 1, TET: 6, PRISM: 2, PYRAMID: 6}[self.shape]
+            return ny * nz * sum(1 if i < nx // 3 else 6 if i < 2 * nx // 3 else 2 for i in range(nx))
+        if self.shape == CHANNEL:
+            layers = max(1, min(10, ny // 3))
+            return nx * nz * (2 * layers + 7 + 6 * (ny - layers - 1))
+        return nx * ny * nz * {HEX: 1, TET: 6, PRISM: 2, PYRAMID: 6}[self.shape]
+
+    def freestream(self):
+        """conserved state of the uniform flow along x at Mach `mach`, rho = p = 1"""
+        u = self.mach * math.sqrt(self.gamma)
+        return np.array([1.0, u, 0.0, 0.0, 1.0 / (self.gamma - 1.0) + 0.5 * u * u])
 
 
 # The benchmark configurations (sizes as specified) and reduced versions of
@@ -68,21 +85,33 @@ PRESETS = {
                    steps=200),
     "c3": CaseSpec("c3: 128^3 hex + 1e-3 density noise, WENO-4, HLLC, SSP-RK3, mixed", HEX, 128, "weno4", seed=3,
                    noise=1e-3, steps=500),
+    # [0,4]x[0,1]x[0,1], 120 x 30 x 30 hexes split into prisms (10 wall
+    # layers), pyramids and tetrahedra (one interface layer) and tetrahedra:
+    # 507,600 cells; lower-wall bump a_k ~ U(0, 0.01) (seed 4); inflow x-,
+    # characteristic outflow x+, slip walls elsewhere; uniform M 0.6, p = rho = 1
+    "c4": CaseSpec("c4: 120x30x30 bump channel (prism wall layers, pyramid/tet interface, tets), WENO-3, M0.6",
+                   CHANNEL, 30, "weno3", seed=4, steps=50, bc=(1, 2, 4, 4, 4, 4),
+                   bump=tuple(float(a) for a in np.random.default_rng(4).uniform(0.0, 0.01, 8)),
+                   box=(120, 30, 30, 4.0, 1.0, 1.0), mach=0.6),
 }
 
 
 def reduced(key: str, n: int) -> CaseSpec:
     s = PRESETS[key]
+    box = ()
+    if s.box:  # the same aspect: 4n x n x n cells
+        box = (4 * n, n, n) + tuple(s.box[3:])
     return CaseSpec(f"{key}-reduced-{n}", s.shape, n, s.scheme, s.jitter, s.seed, s.noise, s.cfl, s.steps,
-                    s.length, s.lwci1, s.gamma, s.bc)
+                    s.length, s.lwci1, s.gamma, s.bc, s.bump, box, s.mach)
 
 
 class Case:
     def __init__(self, spec: CaseSpec, nthreads: int = 0):
         self.spec = spec
         lib = native.prep_lib()
-        o = native.PrepOpts(shape=spec.shape, nx=spec.n, ny=spec.n, nz=spec.n,
-                            lx=spec.length, ly=spec.length, lz=spec.length, jitter=spec.jitter, seed=spec.seed,
+        nx, ny, nz, lx, ly, lz = spec.dims
+        o = native.PrepOpts(shape=spec.shape, nx=nx, ny=ny, nz=nz,
+                            lx=lx, ly=ly, lz=lz, jitter=spec.jitter, seed=spec.seed,
                             iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads,
                             bc=(C.c_int32 * 6)(*spec.bc),
                             bump=(C.c_double * 8)(*(tuple(spec.bump) + (0.0,) * (8 - len(spec.bump)))))
@@ -122,6 +151,8 @@ class Case:
         return m, s, p
 
     def init_state(self) -> np.ndarray:
+        if self.spec.mach > 0:
+            return np.tile(self.spec.freestream(), (self.ncell, 1))
         u = np.empty((self.ncell, 5), dtype=np.float64)
         rc = self._lib.ucnsp_init_vortex(self._h, self.spec.gamma, self.spec.noise, self.spec.seed,
                                          native.f64p(u))

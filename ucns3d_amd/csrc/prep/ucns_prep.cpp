@@ -43,6 +43,13 @@ const int PRISM_TETS[3][4] = {{0, 1, 2, 5}, {0, 1, 5, 4}, {0, 4, 5, 3}};
 // pyramid (DECOMPOSE3 element type 3, src/grid_t.f90:1058-1060)
 const int PYR_TETS[2][4] = {{0, 1, 2, 4}, {0, 2, 3, 4}};
 
+// hexes of the channel's interface layer (pyramids and tetrahedra), an
+// element kind of the generator only: its cells are UCNSP_PYRAMID and UCNSP_TET
+constexpr int UCNSP_INTERFACE = 100;
+// prism layers of the channel: 10 (the configuration's wall region), fewer
+// on coarse boxes so that tetrahedra fill at least two thirds of y
+inline int channel_layers(int ny) { return std::max(1, std::min(10, ny / 3)); }
+
 inline int nfaces_of(int shape) { return shape == UCNSP_HEX ? 6 : shape == UCNSP_TET ? 4 : 5; }
 inline int nnodes_of(int shape) {
     return shape == UCNSP_HEX ? 8 : shape == UCNSP_PRISM ? 6 : shape == UCNSP_PYRAMID ? 5 : 4;
@@ -1389,18 +1396,26 @@ void gen_box(ucnsp_case &c) {
             }
     const int64_t nhex = (int64_t)nx * ny * nz;
     const int64_t ngrid = c.nnode;
-    // element type of the hexes in column i: one type, or for the hybrid box
-    // hexahedra, pyramids and prisms in thirds of x (their shared faces are
-    // the hexes' quadrilaterals, so the mesh is conforming)
-    auto etype = [&](int i) {
+    // element type of hex (i, j): one type; for the hybrid box hexahedra,
+    // pyramids and prisms in thirds of x (their shared faces are the hexes'
+    // quadrilaterals, so the mesh is conforming); for the channel prisms in
+    // the wall layers j < channel_layers(ny), the interface layer of
+    // pyramids and tetrahedra (UCNSP_INTERFACE) and Kuhn tetrahedra above
+    const int layers = channel_layers(ny);
+    auto etype = [&](int i, int j) {
+        if (o.shape == UCNSP_CHANNEL) return j < layers ? UCNSP_PRISM : j == layers ? UCNSP_INTERFACE : UCNSP_TET;
         return o.shape != UCNSP_HYBRID ? o.shape
                : i < nx / 3            ? UCNSP_HEX
                : i < 2 * nx / 3        ? UCNSP_PYRAMID
                                        : UCNSP_PRISM;
     };
-    auto per_hex = [](int t) { return t == UCNSP_HEX ? 1 : t == UCNSP_PRISM ? 2 : 6; };
-    if (o.shape == UCNSP_PYRAMID || o.shape == UCNSP_HYBRID) {
-        // one apex node per hex, at the mean of its corners
+    auto per_hex = [](int t) {
+        return t == UCNSP_HEX ? 1 : t == UCNSP_PRISM ? 2 : t == UCNSP_INTERFACE ? 7 : 6;
+    };
+    auto has_apex = [&](int i, int j) { const int t = etype(i, j); return t == UCNSP_PYRAMID || t == UCNSP_INTERFACE; };
+    if (o.shape == UCNSP_PYRAMID || o.shape == UCNSP_HYBRID || o.shape == UCNSP_CHANNEL) {
+        // one apex node per hex that holds pyramids (the node slot of every
+        // hex is reserved), at the mean of its corners
         c.nnode += nhex;
         c.xyz.resize(c.nnode * 3);
         c.canon.resize(c.nnode);
@@ -1408,20 +1423,24 @@ void gen_box(ucnsp_case &c) {
             for (int j = 0; j < ny; ++j)
                 for (int i = 0; i < nx; ++i) {
                     const int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k), n = ngrid + h;
+                    c.canon[n] = (int32_t)n;
+                    if (!has_apex(i, j)) {
+                        for (int d = 0; d < 3; ++d) c.xyz[n * 3 + d] = 0.0;
+                        continue;
+                    }
                     double m[3] = {0, 0, 0};
                     for (int b = 0; b < 8; ++b) {
                         const int64_t q = nid(i + (b & 1), j + ((b >> 1) & 1), k + (b >> 2));
                         for (int d = 0; d < 3; ++d) m[d] += 0.125 * c.xyz[q * 3 + d];
                     }
                     for (int d = 0; d < 3; ++d) c.xyz[n * 3 + d] = m[d];
-                    c.canon[n] = (int32_t)n;
                 }
     }
     c.shape = o.shape;
     c.ncell = 0;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) c.ncell += per_hex(etype(i));
+            for (int i = 0; i < nx; ++i) c.ncell += per_hex(etype(i, j));
     c.cnodes.assign(c.ncell * 8, -1);
     c.cshape.assign(c.ncell, o.shape);
     int64_t cell0 = 0;  // first cell of the current hex
@@ -1431,9 +1450,44 @@ void gen_box(ucnsp_case &c) {
                 int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k);
                 int64_t v[8] = {nid(i, j, k),     nid(i + 1, j, k),     nid(i + 1, j + 1, k),     nid(i, j + 1, k),
                                 nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
-                const int et = etype(i);
+                const int et = etype(i, j);
                 for (int q = 0; q < per_hex(et); ++q) c.cshape[cell0 + q] = et;
-                if (et == UCNSP_HEX) {
+                // a tetrahedron from four nodes, numbered with positive orientation
+                auto add_tet = [&](int64_t cell, int64_t a, int64_t b, int64_t cc, int64_t d) {
+                    double x[4][3];
+                    const int64_t tv0[4] = {a, b, cc, d};
+                    for (int q = 0; q < 4; ++q)
+                        for (int e = 0; e < 3; ++e) x[q][e] = c.xyz[tv0[q] * 3 + e];
+                    double e1[3], e2[3], e3[3];
+                    for (int e = 0; e < 3; ++e) {
+                        e1[e] = x[1][e] - x[0][e];
+                        e2[e] = x[2][e] - x[0][e];
+                        e3[e] = x[3][e] - x[0][e];
+                    }
+                    const double det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                                       e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+                    int64_t tv[4] = {a, b, cc, d};
+                    if (det < 0) std::swap(tv[1], tv[2]);
+                    c.cshape[cell] = UCNSP_TET;
+                    for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)tv[q];
+                };
+                if (et == UCNSP_INTERFACE) {
+                    // between the prism layers (quadrilateral faces at y = j)
+                    // and the Kuhn tetrahedra (triangles at y = j + 1): five
+                    // pyramids on the hex faces other than y = j + 1, whose
+                    // quadrilateral is split along the diagonal 3-6 (the one
+                    // the Kuhn split of the hex above takes, through its
+                    // corner (0,0,0)) into two tetrahedra with the same apex
+                    const int pyr_faces[5] = {5, 0, 1, 2, 3};  // y = j, z = k, z = k+1, x = i+1, x = i
+                    for (int t = 0; t < 5; ++t) {
+                        const int64_t cell = cell0 + t;
+                        c.cshape[cell] = UCNSP_PYRAMID;
+                        for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)v[HEX_FACES[pyr_faces[t]][3 - q]];
+                        c.cnodes[cell * 8 + 4] = (int32_t)(ngrid + h);
+                    }
+                    add_tet(cell0 + 5, v[3], v[2], v[6], ngrid + h);
+                    add_tet(cell0 + 6, v[3], v[6], v[7], ngrid + h);
+                } else if (et == UCNSP_HEX) {
                     for (int q = 0; q < 8; ++q) c.cnodes[cell0 * 8 + q] = (int32_t)v[q];
                 } else if (et == UCNSP_PYRAMID) {
                     // six pyramids per hex: each hex face is a base (reversed, so
@@ -1519,8 +1573,8 @@ int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n
 int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
-    if (o.shape < UCNSP_HEX || o.shape > UCNSP_HYBRID)
-        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3), pyramid(4) or hybrid(5)");
+    if (o.shape < UCNSP_HEX || o.shape > UCNSP_CHANNEL)
+        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3), pyramid(4), hybrid(5) or channel(6)");
     if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
     for (int d = 0; d < 3; ++d) {
         const int a = o.bc[2 * d], b = o.bc[2 * d + 1];

@@ -35,12 +35,16 @@ extern "C" {
 #define UCNSP_PRISM  3
 #define UCNSP_PYRAMID 4
 #define UCNSP_HYBRID 5
+#define UCNSP_CHANNEL 6
 
 typedef struct ucnsp_opts {
     int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex;
                                UCNSP_PRISM: 2 prisms per hex; UCNSP_PYRAMID: 6 pyramids
                                per hex (its faces as bases, its centre the apex);
-                               UCNSP_HYBRID: hexes, pyramids, prisms in thirds of x */
+                               UCNSP_HYBRID: hexes, pyramids, prisms in thirds of x;
+                               UCNSP_CHANNEL: 2 prisms per hex in the wall layers
+                               j < min(10, ny/3), then one layer of 5 pyramids + 2 tets
+                               per hex, then 6 Kuhn tets per hex */
     int32_t nx, ny, nz;
     double  lx, ly, lz;     /* periodic box [0,lx]x[0,ly]x[0,lz] */
     double  jitter;         /* interior vertices moved by U(-j,j)*spacing per coordinate */

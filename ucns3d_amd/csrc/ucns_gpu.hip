@@ -2935,6 +2935,10 @@ __global__ void k_rk3_adj_stage(int ncell, int stage, const double *y, const dou
 }
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_zero_i32(int *p, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = 0;
+}
 
 // Per-cell 5-vectors across the ABI: the caller's cell o = perm[c] (o = c
 // without a permutation) in the caller's layout <-> the library's cell c in
@@ -2986,6 +2990,7 @@ template <typename T>
 int dalloc(ucns_ctx *ctx, T **p, size_t n);
 
 }  // namespace
+
 
 // ---------------------------------------------------------------------------
 struct ucns_ctx {
@@ -3263,7 +3268,7 @@ void tcollect(ucns_ctx *ctx, int k, double *acc, int64_t *n) {
 
 // reconstruction + face fluxes of the current state (ULEFT and face fluxes on device)
 int residual_kernels(ucns_ctx *ctx) {
-    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 2);
     tstart(ctx, 0);
     int rc = ctx->precision == UCNS_MIXED ? launch_recon_t<float>(ctx) : launch_recon_t<double>(ctx);
     if (rc) return rc;
@@ -3548,7 +3553,7 @@ void launch_scatter_t(ucns_ctx *ctx, double *ubar) {
 // weights and the cell flags
 template <typename R>
 int vjp_recon_t(ucns_ctx *ctx) {
-    (void)hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 2);
     ctx->adj_g_on = ctx->smode && ctx->adj_g;
     tstart(ctx, 3);
     const int rc = launch_recon_t<R>(ctx);
@@ -4594,7 +4599,7 @@ int ucns_reconstruct(ucns_ctx *ctx, double *uleft_out) {
     int rc = ready(ctx);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 2);
     rc = ctx->precision == UCNS_MIXED ? launch_recon_t<float>(ctx) : launch_recon_t<double>(ctx);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
