@@ -184,6 +184,15 @@ int  ucns_compute_dt(ucns_ctx *ctx, double cfl, double *dt_out);
 int  ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout);
 /* Same, with rhs_dev a device pointer (no host staging). */
 int  ucns_residual_device(ucns_ctx *ctx, double *rhs_dev, int32_t layout);
+/* ucns_residual_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (the face fluxes and troubled-cell counters) is a static device
+ * array of 256 MiB per GPU, held from the library's first use on that GPU
+ * until the process ends; meshes with more than about 6.7 million faces are
+ * rejected with UCNS_EUNSUPPORTED.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (from several threads or contexts) race. */
+int  ucns_residual_noalloc(ucns_ctx *ctx, double *rhs_dev, int32_t layout);
 /* ARBITRARY_ORDER only: ILOCAL_RECON3(i)%ULEFT(1:5,L,NGP) as [ncell][nface_max][nqp_max][5]. */
 int  ucns_reconstruct(ucns_ctx *ctx, double *uleft_out);
 /* TIME_MARCHING's RK3 loop for nsteps steps.  If dt_fixed > 0 it is used for

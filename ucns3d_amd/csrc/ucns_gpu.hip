@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -2989,8 +2990,43 @@ __global__ void k_relayout_p(const double *src, R *dst, int ncells, int ts_in, i
 template <typename T>
 int dalloc(ucns_ctx *ctx, T **p, size_t n);
 
+// Scratch of ucns_residual_noalloc: its troubled-cell counters (the first
+// 256 bytes) and face fluxes, in a file-scope device array that every GPU the
+// library runs on holds from the library's first use on it until the process
+// ends, so that the call allocates nothing.  In host code the array's name is
+// a stand-in, not a device address: scratch_on() takes each device's own
+// address once, with hipGetSymbolAddress, and only that address is used.
+constexpr size_t UCNS_SCRATCH_BYTES = (size_t)256 << 20;  // 256 MiB: 6.7 M faces
+constexpr size_t UCNS_SCRATCH_HEAD = 256;                  // counters
+
 }  // namespace
 
+static __device__ __align__(256) unsigned char scratch[UCNS_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[256] = {};
+
+// the address of `scratch` on the current device (taken once per device)
+int scratch_on(int device, void **out, std::string &err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= 256) {
+        err = "device ordinal out of range for the scratch table";
+        return UCNS_EUNSUPPORTED;
+    }
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return UCNS_EHIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return UCNS_OK;
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 struct ucns_ctx {
@@ -3020,6 +3056,7 @@ struct ucns_ctx {
     std::vector<int32_t> h_perm, h_iperm;
     int32_t *perm = nullptr;    // device copy of h_perm, or null
     double *stage = nullptr;    // [ncell][5] staging buffer of host transfers
+    void *scratch = nullptr;    // this device's address of the static scratch array
     // library options of the scheme
     int kernels = UCNS_KERNELS_AUTO;
     int adj_keep_stages = 0;
@@ -4155,6 +4192,10 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     HIP_TRY(hipGetDevice(&ctx->device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+    {
+        const int rc0 = scratch_on(ctx->device, &ctx->scratch, ctx->err);  // with the device current
+        if (rc0) return rc0;
+    }
     if (ctx->cell_order != UCNS_ORDER_CALLER) {
         if (ctx->cell_order == UCNS_ORDER_RCM)
             order_rcm(m, ctx->h_perm);
@@ -4653,6 +4694,49 @@ static int residual_to(ucns_ctx *ctx, double *rhs_out, int32_t layout, bool host
 }
 
 int ucns_residual(ucns_ctx *ctx, double *rhs_out, int32_t layout) { return residual_to(ctx, rhs_out, layout, true); }
+
+int ucns_residual_noalloc(ucns_ctx *ctx, double *rhs_dev, int32_t layout) {
+    if (!ctx || !rhs_dev) return UCNS_EBADARG;
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
+    int rc = ready(ctx);
+    if (rc) return rc;
+    const size_t need = UCNS_SCRATCH_HEAD + (size_t)ctx->nface * NV * sizeof(double);
+    if (need > UCNS_SCRATCH_BYTES) {
+        ctx->err = "ucns_residual_noalloc: " + std::to_string(ctx->nface) + " faces need " + std::to_string(need) +
+                   " bytes of scratch, more than the library's " + std::to_string(UCNS_SCRATCH_BYTES) +
+                   " (ucns_residual_device has no such limit)";
+        return UCNS_EUNSUPPORTED;
+    }
+    if (!ctx->scratch) {
+        ctx->err = "ucns_residual_noalloc: no scratch address for this device";
+        return UCNS_ESTATE;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    // the same kernels as ucns_residual_device, with their counters and face
+    // fluxes in the static scratch (the counters are cleared by a kernel at
+    // the start of every reconstruction; every face flux is written before
+    // it is read)
+    int *const own_counters = ctx->counters;
+    double *const own_fflux = ctx->fflux;
+    ctx->counters = (int *)ctx->scratch;
+    ctx->fflux = (double *)((unsigned char *)ctx->scratch + UCNS_SCRATCH_HEAD);
+    rc = residual_kernels(ctx);
+    if (rc == UCNS_OK) update_kernel(ctx, 0, nullptr, nullptr, 0.0, ctx->rhs);
+    int cnt[2] = {0, 0};
+    if (rc == UCNS_OK && hipMemcpyAsync(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = UCNS_EHIP;
+    if (rc == UCNS_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = UCNS_EHIP;
+    ctx->counters = own_counters;
+    ctx->fflux = own_fflux;
+    if (rc == UCNS_EHIP) {
+        ctx->err = std::string("ucns_residual_noalloc: ") + hipGetErrorString(hipGetLastError());
+        return rc;
+    }
+    if (rc) return rc;
+    ctx->st.n_reduced_last = cnt[0];
+    ctx->st.n_firstorder_last = cnt[1];
+    return cells_out(ctx, ctx->rhs, layout, rhs_dev, false);
+}
 
 int ucns_residual_device(ucns_ctx *ctx, double *rhs_dev, int32_t layout) {
     if (!rhs_dev) return UCNS_EBADARG;

@@ -96,6 +96,7 @@ GPU_API = {
     "ucns_compute_dt": (C.c_int, [C.c_void_p, C.c_double, _f64p]),
     "ucns_residual": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_residual_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "ucns_residual_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "ucns_reconstruct": (C.c_int, [C.c_void_p, _f64p]),
     "ucns_rk3_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, _f64p]),
     "ucns_residual_vjp": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
