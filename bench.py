@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: cell-updates/s per SSP-RK3 stage of the WENO-4 HLLC residual
-path on one MI355X, FP64 and mixed FP32/FP64.
+path on one MI355X, FP64 and mixed FP32/FP64, and the adjoint's cost.
 
 A "step" is one SSP-RK3 time step (three residual evaluations + stage
 updates) over the whole mesh; the state, the PRESTORE arrays and the face
 data are resident in HBM before the timed region starts.  `value` is the
 FP64 path on the default workload (64^3 hexes split into 6 jittered
 tetrahedra each, 1,572,864 cells); the mixed path runs on the same mesh and
-is reported beside it.
+is reported beside it.  The same line carries a block for the 128^3-hex
+WENO-4 mesh (FP64, mixed, and the adjoint over a march long enough that the
+kept reconstructions run out of their memory budget) and one for the
+507,600-cell mixed-element channel (WENO-3, FP64 and mixed, the adjoint over
+its 50-step march).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1|c4] [--extra c3,c4]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -32,28 +37,42 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def face_points(case) -> np.ndarray:
+    """Gauss points per cell (qp_quad on quadrilaterals, qp_tri on triangles)"""
+    nv = (case.array("face_nodes").reshape(case.ncell, -1, 4) >= 0).sum(2)
+    return np.where(nv == 4, case.qp_quad, np.where(nv == 3, case.qp_tri, 0)).sum(1)
+
+
 def recon_bytes_per_cell(case, precision: int) -> float:
-    """Algorithmic bytes one cell costs the reconstruction kernel: stencil
-    pseudo-inverses, packed indicator, stencil indices, face Gauss points,
-    basis means and the cell's own state (neighbour re-reads and the ULEFT
-    round trip are implementation overheads and not counted)."""
+    """Algorithmic bytes one cell costs the reconstruction kernel, averaged
+    over the cells: stencil pseudo-inverses, packed indicator, stencil
+    indices, face Gauss points, basis means and the cell's own state
+    (neighbour re-reads and the ULEFT round trip are implementation
+    overheads and not counted)."""
     b = 4 if precision == 1 else 8
     adm = case.array("admis").astype(np.float64).mean()
-    faces = 6 if case.shape == 1 else 4
-    nqp = faces * (case.qp_quad if case.shape == 1 else case.qp_tri)
+    nqp = face_points(case).mean()
     by = adm * case.imax * case.ideg * b
     if case.iweno == 1:
         by += case.ideg * (case.ideg + 1) / 2 * b
     by += adm * case.imax * 4 + nqp * 3 * b + case.ideg * b + 5 * 8
-    return by
+    return float(by)
 
 
 def recon_kernel_name(case, precision: int) -> str:
-    """the reconstruction kernel the library launches for this case: the
-    row-stream kernels for tetrahedra (5 stencils) and hexahedra (7)"""
-    tet = case.shape == 2
-    ts, nql = (5, 3) if tet else (7, 7)
-    return f"k_recon_{'s32' if precision else 's64'}<19,39,{nql},{ts}>"
+    """the reconstruction kernel the library launches: the row-stream
+    kernels for 5 or 7 stencils (tetrahedra, hexahedra), the
+    register-prefetch ones for other stencil counts"""
+    ts = int(case.array("admis").max())
+    nv = (case.array("face_nodes").reshape(case.ncell, -1, 4) >= 0).sum(2)
+    nf = int((nv > 0).sum(1).max())
+    nqm = max(case.qp_quad if (nv == 4).any() else 0, case.qp_tri if (nv == 3).any() else 0)
+    nfq = nf * nqm
+    stream = ((ts == 5 and nfq <= 24) or (ts == 7 and nfq > 24)) if case.ideg == 19 else (ts in (5, 7) and nfq <= 24)
+    nql = 3 if nfq <= 24 else 7
+    if stream:
+        return f"k_recon_{'s32' if precision else 's64'}<{case.ideg},{case.imax},{nql},{ts}>"
+    return f"k_recon_{'w32' if precision else 'w64'}<{case.ideg},{case.imax},{nql}>"
 
 
 def committed_traffic(kernel: str, config: str):
@@ -71,12 +90,20 @@ def committed_traffic(kernel: str, config: str):
     return None
 
 
+def make_solver(case, precision: int, **opts):
+    from ucns3d_amd.solver import Solver
+    s = Solver(case, precision, **opts)
+    if case.spec.mach > 0:  # the channel: characteristic inflow/outflow at the free stream
+        fs = case.spec.freestream()
+        s.set_boundary(1, fs, fs)
+    return s
+
+
 def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.ndarray):
     import torch
-    from ucns3d_amd.solver import Solver
 
     t0 = time.time()
-    s = Solver(case, precision)
+    s = make_solver(case, precision)
     s.set_state(u0)
     log(f"  upload {'mixed' if precision else 'fp64'}: {time.time() - t0:.1f}s")
     if warmup > 0:
@@ -94,23 +121,38 @@ def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.n
     rhs = s.residual()
     s.close()
     cons = float(np.abs(rhs.sum(axis=0)).max() / max(np.abs(rhs).sum(axis=0).max(), 1e-300))
-    return {
-        "elapsed": elapsed, "t_sim": t_sim, "u": u, "stats": st, "conservation": cons,
-    }
+    return {"elapsed": elapsed, "t_sim": t_sim, "u": u, "stats": st, "conservation": cons}
+
+
+def path_block(case, key: str, precision: int, r: dict, steps: int) -> dict:
+    """throughput, kernel times and the reconstruction kernel's roofline"""
+    st = r["stats"]
+    value = case.ncell * 3 * steps / r["elapsed"]
+    rk_ms = st["ms_recon"] / max(st["n_recon"], 1)
+    by = recon_bytes_per_cell(case, precision)
+    achieved = by * case.ncell / (rk_ms * 1e-3) / 1e9
+    kernel = recon_kernel_name(case, precision)
+    return {"value": value, "ms_per_step": r["elapsed"] / steps * 1e3,
+            "kernels_ms": {"recon": rk_ms, "flux": st["ms_flux"] / max(st["n_flux"], 1),
+                           "update": st["ms_update"] / max(st["n_update"], 1)},
+            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": committed_traffic(kernel, key), "algorithmic_bytes": by * case.ncell,
+                         "bytes_per_cell": by, "kernel_ms": rk_ms},
+            "step_roofline_frac": by * case.ncell * 3 / (r["elapsed"] / steps) / (HBM_PEAK_GBS * 1e9)}
 
 
 def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) -> dict:
     """adjoint / primal cost: ucns_rk3_adjoint_device over `steps` steps (its
     forward sweep with stage checkpoints plus the reverse sweep) against
     ucns_rk3_steps over the same steps, device-resident cotangents.  Timed
-    twice: with every stage's forward reconstruction kept for the reverse
-    sweep (the library's default, as many as fit in its memory budget) and
-    with none kept (a context with adj_keep_stages < 0: every reversed stage
-    runs its reconstruction again)."""
+    twice: with the library's default, which keeps the forward
+    reconstruction of as many of the last stages as fit in its memory
+    budget (64 GiB), and with none kept (a context with adj_keep_stages < 0:
+    every reversed stage runs its reconstruction again)."""
     import torch
-    from ucns3d_amd.solver import Solver
 
-    s = Solver(case, precision)
+    s = make_solver(case, precision)
     s.set_state(u0)
     dt = s.compute_dt(cfl)
     g = torch.Generator(device="cuda").manual_seed(5)
@@ -136,7 +178,7 @@ def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) ->
     t_adj = adjoint()
     finite = bool(torch.isfinite(u0b).all().item())
     s.close()
-    s = Solver(case, precision, adj_keep_stages=-1)
+    s = make_solver(case, precision, adj_keep_stages=-1)
     s.set_state(u0)
     s.rk3_adjoint_device(steps, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)  # allocates checkpoints
     t_rec = adjoint()
@@ -145,40 +187,56 @@ def run_adjoint(case, precision: int, steps: int, cfl: float, u0: np.ndarray) ->
     s.rk3_adjoint_device(1, a.data_ptr(), u0b.data_ptr(), dt_fixed=dt)
     st = s.stats()
     s.close()
-    return {"steps": steps, "primal_ms_per_step": t_fwd / steps * 1e3, "adjoint_ms_per_step": t_adj / steps * 1e3,
-            "adjoint_over_primal": t_adj / t_fwd, "reverse_sweep_over_primal": (t_adj - t_fwd) / t_fwd,
+    return {"steps": steps, "precision": "mixed" if precision else "fp64",
+            "primal_ms_per_step": t_fwd / steps * 1e3, "adjoint_ms_per_step": t_adj / steps * 1e3,
+            "adjoint_over_primal": t_adj / t_fwd,
             "recompute_adjoint_ms_per_step": t_rec / steps * 1e3, "recompute_adjoint_over_primal": t_rec / t_fwd,
             "recon_vjp_ms": st["ms_recon_vjp"] / max(st["n_recon_vjp"], 1),
             "flux_vjp_ms": st["ms_flux_vjp"] / max(st["n_flux_vjp"], 1),
             "recon_adj_ms": st["ms_recon_adj"] / max(st["n_recon_adj"], 1),
             "recon_ms": st["ms_recon"] / max(st["n_recon"], 1), "finite": finite,
-            "note": "adjoint = forward sweep storing every stage state (and keeping each stage's "
-                    "reconstruction) + reverse sweep (each stage: flux product, reconstruction product, "
-                    "transposed scatter); recompute = the same with the reconstruction run again per "
-                    "reversed stage; dt fixed"}
+            "note": "adjoint = forward sweep storing every stage state (and keeping the last stages' "
+                    "reconstructions within the memory budget) + reverse sweep (each stage: flux product, "
+                    "reconstruction product, transposed scatter); recompute = every reversed stage runs "
+                    "its reconstruction again; dt fixed"}
 
 
-def cpu_baseline(spec_key: str, seconds: float):
-    """The CPU oracle (C restatement of the reference algorithm, OpenMP) on a
-    reduced mesh of the same construction, timed on this host."""
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, u0: np.ndarray, n1: int, n2: int):
+    """The CPU oracle (C++ restatement of the reference algorithm, OpenMP on
+    the host's cores) on the same mesh and state as the GPU run, timed by
+    differencing: (t(n2 steps) - t(n1 steps)) / (3 (n2 - n1)) per stage,
+    which leaves out set-up.  The steps run in one sequence, so t(n1) is a
+    prefix of t(n2)."""
     from oracle.oracle import Oracle
-    from ucns3d_amd.case import Case, reduced
 
-    n = 24 if spec_key == "c2" else 40
-    case = Case(reduced(spec_key, n))
     orc = Oracle(case)
-    u = case.init_state()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    steps = 0
+    dt = orc.dt(u0, case.spec.cfl)
+    u = u0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds and steps < 50:
-        u, _ = orc.rk3(u, 1, cfl=case.spec.cfl)
-        steps += 1
-    el = time.perf_counter() - t0
-    val = case.ncell * 3 * steps / el
-    return {"value": val, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{case.spec.name}: {case.ncell} cells, {steps} SSP-RK3 steps ({3 * steps} stages) in "
-                      f"{el:.1f}s, oracle/ucns_oracle.cpp with OpenMP"}
+    t_n1 = None
+    for k in range(1, n2 + 1):
+        u, _ = orc.rk3(u, 1, dt_fixed=dt)
+        if k == n1:
+            t_n1 = time.perf_counter() - t0
+    t_n2 = time.perf_counter() - t0
+    per_stage = (t_n2 - t_n1) / (3 * (n2 - n1))
+    return {"value": case.ncell / per_stage, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"{case.spec.name}: {case.ncell} cells (the GPU run's mesh and state), steps {n1 + 1}-{n2} "
+                      f"of a {n2}-step march timed ({t_n2 - t_n1:.1f} s of {t_n2:.1f} s), oracle/ucns_oracle.cpp "
+                      f"with OpenMP"}
 
 
 def rel_errors(a: np.ndarray, b: np.ndarray) -> dict:
@@ -194,10 +252,9 @@ def accuracy(case, u0: np.ndarray, steps: list) -> dict:
     fixed dt (the FP64 CFL step of the initial state), compared at each
     requested step count; the CPU oracle joins at step 1."""
     from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
-    from ucns3d_amd.solver import Solver
 
-    fp = Solver(case, UCNS_FP64)
-    mx = Solver(case, UCNS_MIXED)
+    fp = make_solver(case, UCNS_FP64)
+    mx = make_solver(case, UCNS_MIXED)
     fp.set_state(u0)
     mx.set_state(u0)
     dt = fp.compute_dt(case.spec.cfl)
@@ -210,8 +267,12 @@ def accuracy(case, u0: np.ndarray, steps: list) -> dict:
         a, b = mx.get_state(), fp.get_state()
         rec = {"mixed_vs_fp64": rel_errors(a, b)}
         if n == 1:
+            from oracle import oracle
             from oracle.oracle import Oracle
             t0 = time.time()
+            if case.spec.mach > 0:
+                fs = case.spec.freestream()
+                oracle.set_boundary(1, fs, fs)
             ref, _ = Oracle(case).rk3(u0, 1, dt_fixed=dt)
             rec["fp64_vs_oracle"] = rel_errors(b, ref)
             rec["mixed_vs_oracle"] = rel_errors(a, ref)
@@ -223,17 +284,48 @@ def accuracy(case, u0: np.ndarray, steps: list) -> dict:
     return out
 
 
+def config_block(key: str, steps: int, warmup: int, adjoint_steps: int) -> dict:
+    """FP64 and mixed throughput on one configuration, the mixed path's error
+    after the march, and the adjoint's cost (FP64)"""
+    from ucns3d_amd.case import PRESETS, Case
+    from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
+
+    spec = PRESETS[key]
+    t0 = time.time()
+    case = Case(spec)
+    u0 = case.init_state()
+    log(f"{spec.name}: {case.ncell} cells, preprocessing {time.time() - t0:.1f}s")
+    fp = run_path(case, UCNS_FP64, steps, warmup, spec.cfl, u0)
+    mx = run_path(case, UCNS_MIXED, steps, warmup, spec.cfl, u0)
+    b64 = path_block(case, key, UCNS_FP64, fp, steps)
+    bmx = path_block(case, key, UCNS_MIXED, mx, steps)
+    a, b = mx["u"], fp["u"]
+    err = (np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))).tolist()
+    out = {"workload": spec.name, "ncell": case.ncell, "steps": steps, "warmup": warmup, "fp64": b64, "mixed": bmx,
+           "mixed_speedup_vs_fp64": bmx["value"] / b64["value"],
+           "mixed_rel_l2_vs_fp64": err, "steps_compared": warmup + steps}
+    log(f"{key}: fp64 {b64['value'] / 1e6:.1f} M/s ({b64['roofline']['frac']:.3f} of HBM in the reconstruction), "
+        f"mixed {bmx['value'] / 1e6:.1f} M/s ({bmx['roofline']['frac']:.3f}), x{out['mixed_speedup_vs_fp64']:.2f}")
+    if adjoint_steps > 0:
+        out["adjoint"] = run_adjoint(case, UCNS_FP64, adjoint_steps, spec.cfl, u0)
+        log(f"{key} adjoint: {out['adjoint']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--extra", default="c3,c4",
+                    help="comma-separated further configurations reported in blocks of their own ('' for none)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--adjoint-steps", type=int, default=2,
-                    help="steps of the adjoint / primal cost measurement on the FP64 path (0: skip)")
+    ap.add_argument("--cpu-steps", default="1,3",
+                    help="N1,N2: the CPU oracle's per-stage time is (t(N2) - t(N1)) / (3 (N2 - N1)) steps")
+    ap.add_argument("--adjoint-steps", type=int, default=10,
+                    help="steps of the adjoint / primal cost measurement (0: skip); the channel uses its 50")
     ap.add_argument("--accuracy", default="",
                     help="comma-separated step counts: report the mixed path's per-variable relative L2 and Linf "
                          "against the FP64 path (and, at step 1, the CPU oracle) at each, then exit")
@@ -274,52 +366,46 @@ def main() -> int:
         return 0
 
     fp = run_path(case, UCNS_FP64, args.steps, args.warmup, spec.cfl, u0)
-    ms_step = fp["elapsed"] / args.steps * 1e3
-    value = case.ncell * 3 * args.steps / fp["elapsed"]
-    st = fp["stats"]
-    rk_ms = st["ms_recon"] / max(st["n_recon"], 1)
-    bytes_cell = recon_bytes_per_cell(case, UCNS_FP64)
-    achieved = bytes_cell * case.ncell / (rk_ms * 1e-3) / 1e9
-    log(f"fp64: {ms_step:.2f} ms/step, {value / 1e6:.1f} M cell-updates/s/stage, recon kernel {rk_ms:.3f} ms "
-        f"({achieved:.0f} GB/s), flux {st['ms_flux'] / max(st['n_flux'], 1):.3f} ms, "
-        f"update {st['ms_update'] / max(st['n_update'], 1):.3f} ms")
+    blk = path_block(case, args.config, UCNS_FP64, fp, args.steps)
+    log(f"fp64: {blk['ms_per_step']:.2f} ms/step, {blk['value'] / 1e6:.1f} M cell-updates/s/stage, recon kernel "
+        f"{blk['kernels_ms']['recon']:.3f} ms ({blk['roofline']['achieved']:.0f} GB/s), "
+        f"flux {blk['kernels_ms']['flux']:.3f} ms, update {blk['kernels_ms']['update']:.3f} ms")
     out = {
-        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": blk["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": blk["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": spec.name, "ncell": case.ncell, "cfl": spec.cfl, "stages_per_step": 3,
                    "wenocentralweight": spec.lwci1, "seed": spec.seed, "jitter": spec.jitter},
-        "roofline": {"bound": "hbm", "kernel": recon_kernel_name(case, UCNS_FP64),
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": committed_traffic(recon_kernel_name(case, UCNS_FP64), args.config),
-                     "algorithmic_bytes": bytes_cell * case.ncell, "bytes_per_cell": bytes_cell,
-                     "kernel_ms": rk_ms},
-        "kernels_ms": {"recon": rk_ms, "flux": st["ms_flux"] / max(st["n_flux"], 1),
-                       "update": st["ms_update"] / max(st["n_update"], 1)},
+        "roofline": blk["roofline"],
+        "kernels_ms": blk["kernels_ms"],
+        "step_roofline_frac": blk["step_roofline_frac"],
         "conservation_fp64": fp["conservation"],
     }
     if not args.no_mixed:
         mx = run_path(case, UCNS_MIXED, args.steps, args.warmup, spec.cfl, u0)
-        v_mx = case.ncell * 3 * args.steps / mx["elapsed"]
-        stm = mx["stats"]
-        rk_mx = stm["ms_recon"] / max(stm["n_recon"], 1)
-        bytes_mx = recon_bytes_per_cell(case, UCNS_MIXED)
-        ach_mx = bytes_mx * case.ncell / (rk_mx * 1e-3) / 1e9
+        bmx = path_block(case, args.config, UCNS_MIXED, mx, args.steps)
         a, b = mx["u"], fp["u"]
         err = (np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))).tolist()
-        log(f"mixed: {mx['elapsed'] / args.steps * 1e3:.2f} ms/step, {v_mx / 1e6:.1f} M/s, recon {rk_mx:.3f} ms "
-            f"({ach_mx:.0f} GB/s), rel L2 vs fp64 after {args.warmup + args.steps} steps {err}")
-        out["mixed"] = {"value": v_mx, "ms_per_step": mx["elapsed"] / args.steps * 1e3, "speedup_vs_fp64": v_mx / value,
-                        "roofline_frac": ach_mx / HBM_PEAK_GBS, "recon_ms": rk_mx, "bytes_per_cell": bytes_mx,
-                        "kernel": recon_kernel_name(case, UCNS_MIXED),
-                        "traffic": committed_traffic(recon_kernel_name(case, UCNS_MIXED), args.config),
+        log(f"mixed: {bmx['ms_per_step']:.2f} ms/step, {bmx['value'] / 1e6:.1f} M/s, recon "
+            f"{bmx['kernels_ms']['recon']:.3f} ms ({bmx['roofline']['achieved']:.0f} GB/s), rel L2 vs fp64 after "
+            f"{args.warmup + args.steps} steps {err}")
+        out["mixed"] = {"value": bmx["value"], "ms_per_step": bmx["ms_per_step"],
+                        "speedup_vs_fp64": bmx["value"] / blk["value"], "roofline_frac": bmx["roofline"]["frac"],
+                        "recon_ms": bmx["kernels_ms"]["recon"], "bytes_per_cell": bmx["roofline"]["bytes_per_cell"],
+                        "kernel": bmx["roofline"]["kernel"], "traffic": bmx["roofline"]["traffic"],
+                        "step_roofline_frac": bmx["step_roofline_frac"],
                         "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps}
     if args.adjoint_steps > 0:
         out["adjoint"] = run_adjoint(case, UCNS_FP64, args.adjoint_steps, spec.cfl, u0)
         log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
+        n1, n2 = (int(x) for x in args.cpu_steps.split(","))
+        out["cpu_baseline"] = cpu_baseline(case, u0, n1, n2)
         log(f"cpu baseline: {out['cpu_baseline']}")
+    del case
+    for key in [k for k in args.extra.split(",") if k and k != args.config]:
+        adj = 0 if args.adjoint_steps <= 0 else (PRESETS[key].steps if key == "c4" else args.adjoint_steps)
+        out[key] = config_block(key, min(args.steps, 10), 1, adj)
     print(json.dumps(out), flush=True)
     return 0
 
