@@ -2,6 +2,9 @@
 
 libucnsprep.so  host FP64 preprocessor (g++ -fopenmp)
 libucnsgpu.so   HIP residual/RK3 library for gfx950 (hipcc)
+ucns_fortran_host  the Fortran binding module (fortran/ucns_gpu.f90) compiled
+                with amdflang into a test host that drives the library the
+                way UCNS3D's TIME_MARCHING would (tests/fortran/host_march.f90)
 
 The built .so files live in ucns3d_amd/lib/ so they travel with the source
 tree; nothing is installed into site-packages.
@@ -20,6 +23,7 @@ INCLUDE = os.path.join(REPO, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 
 # packed-FP32 instructions are disabled for every HIP translation unit
 HIP_FLAGS = [
@@ -54,8 +58,24 @@ def build_gpu() -> str:
     return out
 
 
+def build_fortran() -> str | None:
+    """the Fortran binding module and the Fortran test host linked against
+    both libraries (None when amdflang is not installed)"""
+    if not os.path.exists(FC):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    mod_dir = os.path.join(LIBDIR, "fortran_mod")
+    os.makedirs(mod_dir, exist_ok=True)
+    out = os.path.join(LIBDIR, "ucns_fortran_host")
+    srcs = [os.path.join(HERE, "fortran", "ucns_gpu.f90"), os.path.join(REPO, "tests", "fortran", "host_march.f90")]
+    _run([FC, "-O2", "-J", mod_dir, *srcs, "-o", out + ".tmp", "-L" + LIBDIR, "-lucnsgpu", "-lucnsprep",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> list[str]:
-    return [build_prep(), build_gpu()]
+    return [p for p in (build_prep(), build_gpu(), build_fortran()) if p]
 
 
 if __name__ == "__main__":
@@ -65,6 +85,8 @@ if __name__ == "__main__":
             print(build_prep())
         elif w == "gpu":
             print(build_gpu())
+        elif w == "fortran":
+            print(build_fortran())
         else:
             for p in build_all():
                 print(p)

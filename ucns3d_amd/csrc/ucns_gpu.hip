@@ -4189,12 +4189,24 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
         for (int i = 0; i < m->ncell; ++i)
             if (m->ifca[i] < 4 || m->ifca[i] > m->nface_max)
                 return bad(UCNS_EBADARG, "ifca out of range at cell " + std::to_string(i));
-    HIP_TRY(hipGetDevice(&ctx->device));
-    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+    // the caller's face links, checked before anything reads through them:
+    // codes, neighbour ranges and the symmetry of (cell, slot) pairs
     {
-        const int rc0 = scratch_on(ctx->device, &ctx->scratch, ctx->err);  // with the device current
-        if (rc0) return rc0;
+        const int nc0 = m->ncell, nfa0 = m->nface_max, b0 = m->index_base;
+        for (int i = 0; i < nc0; ++i)
+            for (int f = 0; f < m->ifca[i]; ++f) {
+                const size_t sl = (size_t)i * nfa0 + f;
+                const int bc = m->bcode[sl];
+                if (bc == 1 || bc == 2 || bc == 3 || bc == 4 || bc == 6) continue;
+                if (bc != 0 && bc != 5 && bc != 50)
+                    return bad(UCNS_EUNSUPPORTED, "face codes: 0 interior, 5/50 periodic, 1/2/3/4/6 boundary");
+                const int j = m->ineigh[sl] - b0, fj = m->ineighn[sl] - b0;
+                if (j < 0 || j >= nc0 || fj < 0 || fj >= m->ifca[j])
+                    return bad(UCNS_EBADARG, "face neighbour out of range at cell " + std::to_string(i));
+                const size_t slj = (size_t)j * nfa0 + fj;
+                if (m->ineigh[slj] - b0 != i || m->ineighn[slj] - b0 != f)
+                    return bad(UCNS_EBADARG, "face neighbours are not symmetric at cell " + std::to_string(i));
+            }
     }
     if (ctx->cell_order != UCNS_ORDER_CALLER) {
         if (ctx->cell_order == UCNS_ORDER_RCM)
@@ -4295,6 +4307,14 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
         if (!(m->totvolume[i] > 0)) return bad(UCNS_EBADARG, "non-positive volume");
         ovol[i] = 1.0 / m->totvolume[i];
         minedge[i] = m->minedge[i];
+    }
+    // the device resources, once the host data has been checked
+    HIP_TRY(hipGetDevice(&ctx->device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+    {
+        const int rc0 = scratch_on(ctx->device, &ctx->scratch, ctx->err);  // with the device current
+        if (rc0) return rc0;
     }
     int rc;
     if ((rc = dalloc(ctx, &ctx->cinfo, nc)) || (rc = dalloc(ctx, &ctx->cnb, cnb.size())) ||
