@@ -120,7 +120,11 @@ def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.n
     st = s.stats()
     rhs = s.residual()
     s.close()
-    cons = float(np.abs(rhs.sum(axis=0)).max() / max(np.abs(rhs).sum(axis=0).max(), 1e-300))
+    # sum of the residual over all cells relative to its absolute sum: round-off on a closed (periodic)
+    # domain; with inflow/outflow faces it is the net boundary flux, so it is not reported there
+    cons = None
+    if case.spec.mach <= 0:
+        cons = float(np.abs(rhs.sum(axis=0)).max() / max(np.abs(rhs).sum(axis=0).max(), 1e-300))
     return {"elapsed": elapsed, "t_sim": t_sim, "u": u, "stats": st, "conservation": cons}
 
 
