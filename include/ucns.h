@@ -162,6 +162,30 @@ typedef struct ucns_boundary_v1 {
     double outflow[5];          /* conserved */
 } ucns_boundary_v1;
 
+/* Cotangents of an objective with respect to the data a march reads from
+ * the mesh and PRESTORE arrays (ucns_rk3_adjoint_geometry), in the layouts of
+ * ucns_prestore_v1 and ucns_mesh_v2 and the caller's cell order.  Any pointer
+ * may be NULL (not returned).  A host chains them with the derivatives of
+ * its own geometry (PRESTORE_RECONSTRUCTION3, LOCALISE_STEN2, the face
+ * angles and areas of GEOMETRY_CALC) to reach mesh-shape parameters. */
+typedef struct ucns_geometry_bar_v1 {
+    int32_t abi_version;
+    int32_t reserved;
+    double *invmat;       /* [ncell][typesten][imax][idegfree] */
+    double *indicator;    /* [ncell][idegfree][idegfree]: entries k <= l (the upper triangle the
+                             library reads; the matrix is symmetric), zero below */
+    double *integ_basis;  /* [ncell][idegfree] */
+    double *volume11;     /* [ncell] */
+    double *qpoints;      /* [ncell][nface_max][nqp_max][3] */
+    double *surf;         /* [ncell][nface_max]: on each face's owner record (the lower cell index,
+                             or the only one on boundary faces), zero on the other */
+    double *normal;       /* [ncell][nface_max][3]: the unit outward normal
+                             (cos a1 sin a2, sin a1 sin a2, cos a2) of FACEANGLEX/Y = (a1, a2), on the
+                             owner record; tangential to the unit sphere (the normal's own direction
+                             has no effect) */
+    double *totvolume;    /* [ncell] */
+} ucns_geometry_bar_v1;
+
 typedef struct ucns_ctx ucns_ctx;
 
 /* Set up one partition on the current HIP device (topology, faces, scheme). */
@@ -223,6 +247,16 @@ int  ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed
 /* Same with device pointers, AOS layout (bc_bar_dev: 10 doubles or NULL). */
 int  ucns_rk3_adjoint_device(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed, const double *uN_bar_dev,
                              double *u0_bar_dev, double *bc_bar_dev, double *t_io);
+/* ucns_rk3_adjoint with a fixed step size that also returns, in *gbar, the
+ * cotangents of the data the march reads from the mesh and PRESTORE arrays
+ * (no counterpart in the reference, which has no adjoint): the stencil
+ * pseudo-inverses, indicators, basis integrals, VOLUME(1,1), face Gauss
+ * points, face areas and normals and the cell volumes, summed over every
+ * stage of the march.  FP64 contexts on the register kernels only
+ * (UCNS_EUNSUPPORTED otherwise); the same discrete decisions are held as in
+ * ucns_residual_vjp.  gbar->abi_version must be UCNS_ABI_VERSION. */
+int  ucns_rk3_adjoint_geometry(ucns_ctx *ctx, int32_t nsteps, double dt_fixed, const double *uN_bar, double *u0_bar,
+                               int32_t layout, double *bc_bar, const ucns_geometry_bar_v1 *gbar, double *t_io);
 int  ucns_stats(ucns_ctx *ctx, ucns_stats_v1 *out);
 /* Fault injection for tests of the error paths (no counterpart in the
  * reference): the nth device allocation the context makes from now on

@@ -419,6 +419,7 @@ struct ucnsp_case {
     static const int NF = 6;  // nface_max
     double per[3] = {0, 0, 0};
     std::vector<double> xyz;
+    std::vector<double> xyz_box;  // node coordinates before the bump
     std::vector<int32_t> canon;
     std::vector<int32_t> cnodes;  // ncell*8
     std::vector<int32_t> cshape;
@@ -1550,6 +1551,23 @@ extern "C" {
 
 const char *ucnsp_last_error(void) { return g_err.c_str(); }
 
+int ucnsp_shape_table(int32_t shape, int32_t *out) {
+    if (!out) return fail(UCNS_EBADARG, "null argument");
+    if (shape != UCNSP_HEX && shape != UCNSP_TET && shape != UCNSP_PRISM && shape != UCNSP_PYRAMID)
+        return fail(UCNS_EBADARG, "shape must be a cell shape (hex, tet, prism, pyramid)");
+    for (int k = 0; k < 64; ++k) out[k] = -1;
+    out[0] = nnodes_of(shape);
+    out[1] = nfaces_of(shape);
+    for (int f = 0; f < nfaces_of(shape); ++f) {
+        out[2 + f] = face_nv(shape, f);
+        for (int k = 0; k < face_nv(shape, f); ++k) out[8 + f * 4 + k] = face_node(shape, f, k);
+    }
+    out[32] = nsubtets(shape);
+    for (int t = 0; t < nsubtets(shape); ++t)
+        for (int k = 0; k < 4; ++k) out[33 + t * 4 + k] = subtet(shape, t)[k];
+    return UCNS_OK;
+}
+
 int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n) {
     std::vector<double> a, b;
     int m = tri ? tri_rule(igq, a, b) : quad_rule(igq, a, b);
@@ -1614,6 +1632,7 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
             }
     std::vector<double> orient;
     if (bumped) orient = subtet_orientation(*c);
+    c->xyz_box = c->xyz;
     apply_bump(*c);
     if (bumped) {
         const int64_t bad = first_inverted_cell(*c, orient);
@@ -1758,6 +1777,12 @@ int ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *coun
         return UCNS_OK;
     };
     if (n == "xyz") return D(c->xyz);
+    if (n == "xyz_box") return D(c->xyz_box);
+    if (n == "anglex") return D(c->angx);
+    if (n == "angley") return D(c->angy);
+    if (n == "reorient") return I(c->reorient);
+    if (n == "quad_rs") return D(c->quad_rs);
+    if (n == "tri_bary") return D(c->tri_bary);
     if (n == "cell_nodes") return I(c->cnodes);
     if (n == "cell_shape") return I(c->cshape);
     if (n == "centre") return D(c->centre);

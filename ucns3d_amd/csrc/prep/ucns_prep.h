@@ -68,8 +68,9 @@ int  ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out);
  * until ucnsp_free).  precision/gamma/lwci1 are the caller's scheme choices. */
 int  ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1,
                 ucns_mesh_v2 *mesh, ucns_scheme_v2 *scheme, ucns_prestore_v1 *pre);
-/* Named internal arrays for tests and drivers: "xyz", "cell_nodes",
- * "centre", "ihexl", ... ; dtype 0=int32, 1=float64. */
+/* Named internal arrays for tests and drivers: "xyz", "xyz_box" (before
+ * the bump), "cell_nodes", "centre", "ihexl", "anglex", "angley", ... ;
+ * dtype 0=int32, 1=float64. */
 int  ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *count, int32_t *dtype);
 /* Integer / real metadata: ints[0..15], dbls[0..7] (see ucns_prep.cpp). */
 int  ucnsp_info(const ucnsp_case *c, int64_t *ints, double *dbls);
@@ -78,6 +79,11 @@ int  ucnsp_info(const ucnsp_case *c, int64_t *ints, double *dbls);
  * p=100/gamma+(rho/16)(cos2x+cos2y)(cos2z+2).  noise_amp*N(0,1) is added to
  * each cell's density (seeded). */
 int  ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint64_t seed, double *u_aos);
+/* Element tables of one cell shape (UCNSP_HEX/TET/PRISM/PYRAMID) into
+ * out[64]: [0] nodes, [1] faces, [2+f] nodes of face f, [8+4f+k] its local
+ * node k, [32] DECOMPOSE3 sub-tetrahedra, [33+4t+k] their local nodes; -1
+ * where unused. */
+int  ucnsp_shape_table(int32_t shape, int32_t *out);
 /* Gauss-point tables used by the face integration (weights sum to 1). */
 int  ucnsp_face_rule(int32_t igqrules, int32_t tri, double *pts /*[n][3] barycentric or (r,s,0)*/, double *w, int32_t *n);
 void ucnsp_free(ucnsp_case *c);

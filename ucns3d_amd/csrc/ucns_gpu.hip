@@ -1525,8 +1525,8 @@ __device__ inline void bc_prim2cons(double gamma, const T *s3, T *cr) {
 }
 
 // subsonic inflow (src/flow_operations.f90:2907-2915): s1 free stream, s2 interior
-template <class T, class F>
-__device__ inline void bc_sub_in(double gamma, const F *s1, const T *s2, T sps, double nx, double ny, double nz, T *cr) {
+template <class T, class F, class N>
+__device__ inline void bc_sub_in(double gamma, const F *s1, const T *s2, T sps, N nx, N ny, N nz, T *cr) {
     T s3[NV];
     s3[4] = 0.5 * ((s1[4]) + (s2[4]) - (s2[0] * sps * ((nx * (s1[1] - s2[1])) + (ny * (s1[2] - s2[2])) + (nz * (s1[3] - s2[3])))));
     s3[0] = s1[0] + (s3[4] - s1[4]) / (sps * sps);
@@ -1537,9 +1537,8 @@ __device__ inline void bc_sub_in(double gamma, const F *s1, const T *s2, T sps, 
 }
 
 // subsonic outflow (src/flow_operations.f90:2995-3010): s1 free stream, s2 interior
-template <class T, class F>
-__device__ inline void bc_sub_out(double gamma, const F *s1, const T *s2, T sps, double nx, double ny, double nz,
-                                  T *cr) {
+template <class T, class F, class N>
+__device__ inline void bc_sub_out(double gamma, const F *s1, const T *s2, T sps, N nx, N ny, N nz, T *cr) {
     T s3[NV];
     s3[4] = T{s1[4]};
     s3[0] = s2[0] + (s3[4] - s2[4]) / (sps * sps);
@@ -1567,12 +1566,50 @@ __device__ inline void rotb(const T *h, double c1, double s1, double c2, double 
     r[4] = h[4];
 }
 
+// The face frame of the rotations: the angles' cosines and sines as the
+// primal kernels use them, or the frame vectors themselves (normal n and
+// tangents t1, t2, rows of the rotation) in a scalar type C that may carry a
+// tangent.  Both give the same rotation: n = (c1 s2, s1 s2, c2),
+// t1 = (c1 c2, s1 c2, -s2), t2 = (-s1, c1, 0).
+struct AngleFrame {
+    double c1, s1, c2, s2;
+    template <class T>
+    __device__ void fwd(const T *a, T *r) const { rotf(a, c1, s1, c2, s2, r); }
+    template <class T>
+    __device__ void back(const T *h, T *r) const { rotb(h, c1, s1, c2, s2, r); }
+    __device__ double nx() const { return c1 * s2; }
+    __device__ double ny() const { return s1 * s2; }
+    __device__ double nz() const { return c2; }
+};
+template <class C>
+struct VectorFrame {
+    C n[3], t1[3], t2[3];
+    template <class T>
+    __device__ void fwd(const T *a, T *r) const {
+        r[0] = a[0];
+        r[1] = n[0] * a[1] + n[1] * a[2] + n[2] * a[3];
+        r[2] = t1[0] * a[1] + t1[1] * a[2] + t1[2] * a[3];
+        r[3] = t2[0] * a[1] + t2[1] * a[2] + t2[2] * a[3];
+        r[4] = a[4];
+    }
+    template <class T>
+    __device__ void back(const T *h, T *r) const {
+        r[0] = h[0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r[1 + d] = n[d] * h[1] + t1[d] * h[2] + t2[d] * h[3];
+        r[4] = h[4];
+    }
+    __device__ C nx() const { return n[0]; }
+    __device__ C ny() const { return n[1]; }
+    __device__ C nz() const { return n[2]; }
+};
+
 // fin/fout: the conserved free-stream states (inflow, outflow) of the
 // boundary data, of the scalar type F (Dual when their tangent is carried)
-template <class T, class F>
-__device__ inline void ghost_state(int code, int boundtype, double gamma, const T *cl, double c1, double s1, double c2,
-                                   double s2, const F *fin, const F *fout, T *cr) {
-    const double nx = c1 * s2, ny = s1 * s2, nz = c2;
+template <class T, class F, class Frame>
+__device__ inline void ghost_state_fr(int code, int boundtype, double gamma, const T *cl, const Frame &fr,
+                                      const F *fin, const F *fout, T *cr) {
+    const auto nx = fr.nx(), ny = fr.ny(), nz = fr.nz();
     F f[NV];
     T w[NV];
     switch (code) {
@@ -1610,15 +1647,15 @@ __device__ inline void ghost_state(int code, int boundtype, double gamma, const 
     case 4:  // wall, inviscid branch (src/flow_operations.f90:3132-3160)
     {
         T rot[NV];
-        rotf(cl, c1, s1, c2, s2, rot);
+        fr.fwd(cl, rot);
         rot[1] = -rot[1];
-        rotb(rot, c1, s1, c2, s2, cr);
+        fr.back(rot, cr);
         break;
     }
     default:  // 6, farfield (src/flow_operations.f90:3203-3296)
     {
         T rot[NV];
-        rotf(cl, c1, s1, c2, s2, rot);
+        fr.fwd(cl, rot);
         const T vnb = rot[1];
         cons2prim_t(gamma, cl, w);
         const T sps = sqrt((gamma * w[4]) / (w[0]));
@@ -1639,6 +1676,12 @@ __device__ inline void ghost_state(int code, int boundtype, double gamma, const 
         }
     }
     }
+}
+
+template <class T, class F>
+__device__ inline void ghost_state(int code, int boundtype, double gamma, const T *cl, double c1, double s1, double c2,
+                                   double s2, const F *fin, const F *fout, T *cr) {
+    ghost_state_fr(code, boundtype, gamma, cl, AngleFrame{c1, s1, c2, s2}, fin, fout, cr);
 }
 
 template <typename R>
@@ -1902,6 +1945,11 @@ struct VjpParams {
     double *gbout;         // row stream: each stencil's g_bar [ntile][ideg*5][8 ts], for the scatter
     const int64_t *toff;   // first row of each tile's member table in tab
     double *tab;           // [rows][5] the member tables, summed per cell by k_adj_gather
+    // geometry cotangents (k_recon_vjp with GEO, FP64), accumulated: pseudo-
+    // inverses [ncell][lsq ts][imax][ideg], indicators [ncell][ideg][ideg]
+    // (upper triangle, the part the library reads), basis means [ncell][ideg]
+    // and face points [ncell][nf][nqm][3]
+    double *gP, *gB, *gM, *gQ;
 };
 
 __device__ inline void atomic_add_f64(double *a, double v) { unsafeAtomicAdd(a, v); }
@@ -1983,7 +2031,7 @@ __device__ inline R pw(R x, R x2, R x3, int e) {
 // Reconstruction product on the tiled and per-cell layouts: the forward is
 // repeated with its own arithmetic (the same bits, so the same decisions),
 // then g_bar of each stencil is scattered through its pseudo-inverse rows.
-template <typename R, int IDEG, int IMAX, int NQL>
+template <typename R, int IDEG, int IMAX, int NQL, bool GEO = false>
 __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
     constexpr int KL = (IDEG + 7) / 8;    // coefficients per lane in the face-point pass
@@ -2292,9 +2340,10 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
         for (int v = 0; v < NV; ++v) gb[k][v] = 0.0;
     auto put = [&](int k, int v, double x) { gb[k][v] = x; };
     double ubi[NV];  // extra cotangent of U_i from the limiter
+    double cq[NV], bb[NV];  // limiter correction at the arg-min point; indicator cotangents
     bool scatter = false;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) ubi[v] = 0.0;
+    for (int v = 0; v < NV; ++v) ubi[v] = cq[v] = bb[v] = 0.0;
     if (cvalid && !first && reduce) {
         // ULEFT(q) = slope * d2_q: slope_bar = sum_q ubar_q d2_q; through an
         // active limiter slope = (u_ext - u_i) / d2_{q*}
@@ -2307,7 +2356,6 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
                 for (int v = 0; v < NV; ++v) sb[v] += ubc[q * NV + v] * (double)res0[r][v];
             }
         }
-        double cq[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             sb[v] = grp_sum8(sb[v]);
@@ -2334,7 +2382,6 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
     } else if (cvalid && !first) {
         // omega_s = lam_s (eps+beta_s)^-4 / sum_t (...):
         // beta_bar_s = -4 omega_s/(eps+beta_s) (obar_s - sum_t omega_t obar_t)
-        double bb[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double so = grp_sum8(act ? (double)om[v] * obar[v] : 0.0);
@@ -2357,6 +2404,76 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
         for (int v = 0; v < NV; ++v)
 #pragma unroll
             for (int k = 0; k < IDEG; ++k) put(k, v, 0.0);
+    }
+
+    if constexpr (GEO) {
+        // ---- geometry: face points and basis means.  A face value is
+        // U_i + sum_k phi_k(x_q) c_k with phi_k = m_k - mbar_k and c the
+        // combined WENO coefficients (sGh) or, on the MUSCL path, slope times
+        // the central ones (sG0, with the limiter's own dependence on the
+        // arg-min point's value); phi_bar_k(q) = sum_v r_bar_v(q) c_kv ----
+        const bool weno = cvalid && !first && !reduce && p.iweno == 1;
+        const bool muscl = cvalid && !first && reduce;
+        double mba[IDEG];
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k) mba[k] = 0.0;
+#pragma unroll
+        for (int r = 0; r < NQL; ++r) {
+            const int q = s + 8 * r;
+            if (!(weno || muscl) || !qvalid(q)) continue;
+            const double x[3] = {(double)qpc[q * 3], (double)qpc[q * 3 + 1], (double)qpc[q * 3 + 2]};
+            double rb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double ub = ubc[q * NV + v];
+                rb[v] = weno ? ub : slope[v] * ub - (q == qs[v] ? cq[v] : 0.0);
+            }
+            double xb[3] = {0.0, 0.0, 0.0};
+            double pw3[3][4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                pw3[d][0] = 1.0;
+#pragma unroll
+                for (int e = 1; e < 4; ++e) pw3[d][e] = pw3[d][e - 1] * x[d];
+            }
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                double pb = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    pb += rb[v] * (double)(weno ? sGh[lc][k * NV + v] : sG0[lc][k * NV + v]);
+                mba[k] -= pb;
+                const int e0 = c_mono[k][0], e1 = c_mono[k][1], e2 = c_mono[k][2];
+                if (e0 > 0) xb[0] += pb * e0 * pw3[0][e0 - 1] * pw3[1][e1] * pw3[2][e2];
+                if (e1 > 0) xb[1] += pb * e1 * pw3[0][e0] * pw3[1][e1 - 1] * pw3[2][e2];
+                if (e2 > 0) xb[2] += pb * e2 * pw3[0][e0] * pw3[1][e1] * pw3[2][e2 - 1];
+            }
+            double *gq = p.gQ + ((size_t)cell * p.nf * p.nqm + q) * 3;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) gq[d] += xb[d];
+        }
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k) {
+            const double t = grp_sum8(mba[k]);
+            if (s == 0 && cvalid) p.gM[(size_t)cell * IDEG + k] += t;
+        }
+        // ---- indicators: beta_s,v = sum_kl B_kl g_k g_l over the full square
+        // of the packed upper triangle ----
+        if (p.iweno == 1) {
+            const bool on = weno && act;
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+                for (int l = k; l < IDEG; ++l) {
+                    double c = 0.0;
+                    if (on) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) c += bb[v] * (double)g[k][v] * (double)g[l][v];
+                    }
+                    const double t = grp_sum8(k == l ? c : 2.0 * c);
+                    if (s == 0 && cvalid) p.gB[((size_t)cell * IDEG + k) * IDEG + l] += t;
+                }
+        }
     }
 
     // ---- G: dU_bar_j = P_j^T g_bar to the members, minus the same to U_i ----
@@ -2385,6 +2502,20 @@ __global__ __launch_bounds__(WAVE) void k_recon_vjp(VjpParams<R> p) {
                 const double pk = (double)pc[k];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) d[v] += pk * gb[k][v];
+            }
+            if constexpr (GEO) {
+                // G_s = P_s dU: P_bar(j, k) = sum_v g_bar(k, v) dU_j(v)
+                double du[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) du[v] = p.U[(size_t)n * NV + v] - ui[v];
+                double *gp = p.gP + (((size_t)cell * p.lsq.ts + s) * IMAX + j) * IDEG;
+#pragma unroll
+                for (int k = 0; k < IDEG; ++k) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) t += gb[k][v] * du[v];
+                    gp[k] += t;
+                }
             }
             const unsigned sl = p.slot ? p.slot[((size_t)tile * IMAX + j) * p.lsq.tp + pos] : UCNS_NOSLOT;
             if (sl < UCNS_TILE_UQ) {
@@ -2887,6 +3018,105 @@ __global__ __launch_bounds__(256) void k_flux_vjp(FluxParams<R> p, const double 
     }
 }
 
+// Geometry cotangents of one residual product lam . R(U) on the face side
+// (FP64): per face the area (garea), the unit outward normal (gnrm) and the
+// cells' 1/TOTVOLUME (govol, through the update term dt R / V that pairs
+// with lam).  The face sum S = area sum_q w_q <rotf(Fb), H_q> is linear in
+// the area.  It depends on the rotation only through the normal (HLLC and
+// the ghost states are invariant under turning the tangents), so its
+// normal derivative is taken along two rotations of the whole frame: about
+// t1 (n -> n + phi t2, t2 -> t2 - phi n) and about t2 (n -> n + psi t1,
+// t1 -> t1 - psi n), each with a dual number; gnrm = area (dS/dphi t2 +
+// dS/dpsi t1), the tangential gradient, free of the angles' singularity at
+// n = +-z.  Every output is accumulated (+=) over the calls.
+template <bool BND>
+__global__ __launch_bounds__(256) void k_flux_geom(FluxParams<double> p, const double *lam, const double *ovol,
+                                                  double *garea, double *gnrm, double *govol) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= p.nface) return;
+    const int cL = p.fcell[2 * f], cR = p.fcell[2 * f + 1];
+    if ((cR < 0) != BND) return;
+    const int s0 = p.fslot[2 * f], sR = p.fslot[2 * f + 1];
+    const int sL = s0 & 255;
+    const bool tri = (s0 >> 8) & 1;
+    const int nq = tri ? p.qpt : p.qpq;
+    const double *w = tri ? p.wt : p.wq;
+    const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
+    const double area = p.farea[f];
+    const double nv[3] = {c1 * s2, s1 * s2, c2}, t1[3] = {c1 * c2, s1 * c2, -s2}, t2[3] = {-s1, c1, 0.0};
+    double uL[NV], uR[NV], Fb[NV], fbr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        uL[v] = p.U[(size_t)cL * NV + v];
+        uR[v] = BND ? 0.0 : p.U[(size_t)cR * NV + v];
+        Fb[v] = lam[(size_t)cL * NV + v] - (BND ? 0.0 : lam[(size_t)cR * NV + v]);
+    }
+    rotf(Fb, c1, s1, c2, s2, fbr);
+    const double *ulL = p.uleft + ((size_t)cL * p.nf + sL) * p.nqm * NV;
+    const double *ulR = p.uleft + ((size_t)(BND ? cL : cR) * p.nf + (BND ? sL : sR)) * p.nqm * NV;
+    double S = 0.0, dphi = 0.0, dpsi = 0.0, F[NV] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < nq; ++q) {
+        double a[NV], b[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            a[v] = uL[v] + ulL[q * NV + v];
+            b[v] = uR[v] + ulR[q * NV + v];
+        }
+        if (BND) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, b);
+        double la[NV], ra[NV], h[NV], hb[NV];
+        rotf(a, c1, s1, c2, s2, la);
+        rotf(b, c1, s1, c2, s2, ra);
+        hllc64(p.gamma, la, ra, h);
+        rotb(h, c1, s1, c2, s2, hb);
+        double sq = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sq += fbr[v] * h[v];
+            F[v] += w[q] * hb[v];
+        }
+        S += w[q] * sq;
+#pragma unroll 1
+        for (int seed = 0; seed < 2; ++seed) {
+            // seed 0: about t1 (n moves along t2); seed 1: about t2 (n moves along t1)
+            VectorFrame<Dual> fr;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                fr.n[d] = Dual{nv[d], seed == 0 ? t2[d] : t1[d]};
+                fr.t1[d] = Dual{t1[d], seed == 0 ? 0.0 : -nv[d]};
+                fr.t2[d] = Dual{t2[d], seed == 0 ? -nv[d] : 0.0};
+            }
+            Dual A[NV], B[NV], FB[NV], LA[NV], RA[NV], H[NV], FR[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                A[v] = Dual{a[v], 0.0};
+                B[v] = Dual{b[v], 0.0};
+                FB[v] = Dual{Fb[v], 0.0};
+            }
+            if (BND) ghost_state_fr(-cR, p.bc.boundtype, p.gamma, A, fr, p.bc.inflow, p.bc.outflow, B);
+            fr.fwd(A, LA);
+            fr.fwd(B, RA);
+            hllc_t<Dual>(p.gamma, LA, RA, H);
+            fr.fwd(FB, FR);
+            double d = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) d += (FR[v] * H[v]).d;
+            (seed == 0 ? dphi : dpsi) += w[q] * d;
+        }
+    }
+    garea[f] += S;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gnrm[(size_t)f * 3 + d] += area * (dphi * t2[d] + dpsi * t1[d]);
+    // R_L holds +area F, R_R -area F
+    double eL = 0.0, eR = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        eL += lam[(size_t)cL * NV + v] * F[v];
+        if (!BND) eR += lam[(size_t)cR * NV + v] * F[v];
+    }
+    atomic_add_f64(&govol[cL], area * eL / ovol[cL]);
+    if (!BND) atomic_add_f64(&govol[cR], -area * eR / ovol[cR]);
+}
+
 // the state [ncell][5] as 64-byte records [ncell][8] (the last three zero)
 __global__ __launch_bounds__(256) void k_pack64(size_t n64, const double *U, double *U64) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3093,6 +3323,16 @@ struct ucns_ctx {
     // coefficients, indicators, weights (as adj_g) and flags
     std::vector<void *> keep_ul, keep_g;
     std::vector<int *> keep_f;
+    // geometry cotangents of ucns_rk3_adjoint_geometry (device, FP64, live
+    // during that call only): pseudo-inverses, indicators, basis means, face
+    // points (k_recon_vjp), face areas, face normals, 1/TOTVOLUME (k_flux_geom)
+    double *geo_P = nullptr, *geo_B = nullptr, *geo_M = nullptr, *geo_Q = nullptr;
+    double *geo_area = nullptr, *geo_nrm = nullptr, *geo_ovol = nullptr;
+    bool geo_on = false;
+    // host copies for mapping them back to the caller's arrays: the faces'
+    // owner cell and slot, 1/TOTVOLUME and VOLUME(1,1) by library cell
+    std::vector<int32_t> h_fcell, h_fslot;
+    std::vector<double> h_ovol, h_vol11;
     // stats
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -3636,6 +3876,15 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
                        ctx->adj_ubf, bcbar);
     tstop(ctx, 1);
     tcollect(ctx, 1, &ctx->st.ms_flux_vjp, &ctx->st.n_flux_vjp);
+    if constexpr (sizeof(R) == 8) {
+        if (ctx->geo_on) {
+            hipLaunchKernelGGL(k_flux_geom<false>, dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp, lam,
+                               ctx->ovol, ctx->geo_area, ctx->geo_nrm, ctx->geo_ovol);
+            if (ctx->has_bnd_faces)
+                hipLaunchKernelGGL(k_flux_geom<true>, dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, fp,
+                                   lam, ctx->ovol, ctx->geo_area, ctx->geo_nrm, ctx->geo_ovol);
+        }
+    }
     VjpParams<R> vp{};
     vp.ncell = ctx->ncell;
     vp.ntile = ctx->ntile;
@@ -3666,10 +3915,40 @@ int vjp_products_t(ucns_ctx *ctx, const double *lam, double *ubar, double *bcbar
     vp.gbout = ctx->smode ? ctx->adj_gb : nullptr;
     vp.toff = ctx->adj_toff;
     vp.tab = ctx->adj_tab;
+    vp.gP = ctx->geo_P;
+    vp.gB = ctx->geo_B;
+    vp.gM = ctx->geo_M;
+    vp.gQ = ctx->geo_Q;
     const dim3 grid(vp.tiles_per_xcd * NXCD), block(WAVE);
     const bool q3 = nfq <= 24;
     tstart(ctx, 0);
-    if (ctx->smode) {
+    bool geo_done = false;
+    if constexpr (sizeof(R) == 8) {
+        if (ctx->geo_on) {
+            geo_done = true;
+            if (ctx->ideg == 19 && ctx->imax == 39) {
+                if (q3)
+                    hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 3, true>), grid, block, 0, ctx->stream, vp);
+                else
+                    hipLaunchKernelGGL((k_recon_vjp<R, 19, 39, 7, true>), grid, block, 0, ctx->stream, vp);
+            } else if (ctx->ideg == 9 && ctx->imax == 19) {
+                if (q3)
+                    hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 3, true>), grid, block, 0, ctx->stream, vp);
+                else
+                    hipLaunchKernelGGL((k_recon_vjp<R, 9, 19, 7, true>), grid, block, 0, ctx->stream, vp);
+            } else if (ctx->ideg == 3 && ctx->imax == 8) {
+                if (q3)
+                    hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 3, true>), grid, block, 0, ctx->stream, vp);
+                else
+                    hipLaunchKernelGGL((k_recon_vjp<R, 3, 8, 7, true>), grid, block, 0, ctx->stream, vp);
+            } else {
+                ctx->err = "no geometry product for idegfree=" + std::to_string(ctx->ideg);
+                return UCNS_EUNSUPPORTED;
+            }
+        }
+    }
+    if (geo_done) {
+    } else if (ctx->smode) {
         // row-stream shapes: from the forward's stored coefficients; the
         // scatter follows on the row stream
         if (!vp.gload || !vp.fload) {
@@ -4308,6 +4587,9 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
         ovol[i] = 1.0 / m->totvolume[i];
         minedge[i] = m->minedge[i];
     }
+    ctx->h_fcell = fcell;
+    ctx->h_fslot = fslot;
+    ctx->h_ovol = ovol;
     // the device resources, once the host data has been checked
     HIP_TRY(hipGetDevice(&ctx->device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -4548,8 +4830,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     // basis means
     {
         std::vector<double> mb((size_t)nc * ID);
+        ctx->h_vol11.resize(nc);
         for (int i = 0; i < nc; ++i) {
             const size_t o = src_cell(ctx, i);
+            ctx->h_vol11[i] = pre->volume11[o];
             const double oov = 1.0 / pre->volume11[o];
             for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[o * ID + d] * oov;
         }
@@ -4920,6 +5204,136 @@ int ucns_rk3_adjoint(ucns_ctx *ctx, int32_t nsteps, double cfl, double dt_fixed,
     double bc[2 * NV];
     HIP_TRY(hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = cells_out(ctx, ctx->adj_a, layout, u0_bar, true))) return rc;
+    if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
+    return UCNS_OK;
+}
+
+namespace {
+void release_geometry(ucns_ctx *ctx) {
+    dfree(ctx->geo_P);
+    dfree(ctx->geo_B);
+    dfree(ctx->geo_M);
+    dfree(ctx->geo_Q);
+    dfree(ctx->geo_area);
+    dfree(ctx->geo_nrm);
+    dfree(ctx->geo_ovol);
+    ctx->geo_on = false;
+}
+
+int geo_download(ucns_ctx *ctx, const double *dev, std::vector<double> &host, size_t n) {
+    host.resize(n);
+    HIP_TRY(hipMemcpyAsync(host.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return UCNS_OK;
+}
+
+// the accumulated device cotangents in the caller's arrays and cell order
+int geo_to_caller(ucns_ctx *ctx, const ucns_geometry_bar_v1 *g) {
+    const int nc = ctx->ncell, ID = ctx->ideg, IM = ctx->imax, ts = ctx->ts, TSA = ctx->ts_abi;
+    const int nf = ctx->nf, nqm = ctx->nqm, nfa = ctx->nf_abi, nqa = ctx->nqm_abi;
+    std::vector<double> P, B, M, Q, A, N, V, mb;
+    int rc;
+    if ((g->invmat && (rc = geo_download(ctx, ctx->geo_P, P, (size_t)nc * ts * IM * ID))) ||
+        (g->indicator && ctx->iweno == 1 && (rc = geo_download(ctx, ctx->geo_B, B, (size_t)nc * ID * ID))) ||
+        ((g->integ_basis || g->volume11) && (rc = geo_download(ctx, ctx->geo_M, M, (size_t)nc * ID))) ||
+        (g->volume11 && (rc = geo_download(ctx, (const double *)ctx->mbar, mb, (size_t)nc * ID))) ||
+        (g->qpoints && (rc = geo_download(ctx, ctx->geo_Q, Q, (size_t)nc * nf * nqm * 3))) ||
+        (g->surf && (rc = geo_download(ctx, ctx->geo_area, A, (size_t)ctx->nface))) ||
+        (g->normal && (rc = geo_download(ctx, ctx->geo_nrm, N, (size_t)ctx->nface * 3))) ||
+        (g->totvolume && (rc = geo_download(ctx, ctx->geo_ovol, V, (size_t)nc))))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (g->invmat) std::memset(g->invmat, 0, sizeof(double) * (size_t)nc * TSA * IM * ID);
+    if (g->indicator) std::memset(g->indicator, 0, sizeof(double) * (size_t)nc * ID * ID);
+    if (g->qpoints) std::memset(g->qpoints, 0, sizeof(double) * (size_t)nc * nfa * nqa * 3);
+    if (g->surf) std::memset(g->surf, 0, sizeof(double) * (size_t)nc * nfa);
+    if (g->normal) std::memset(g->normal, 0, sizeof(double) * (size_t)nc * nfa * 3);
+    for (int i = 0; i < nc; ++i) {
+        const size_t o = src_cell(ctx, i);
+        if (g->invmat)
+            for (int st = 0; st < std::min(ts, TSA); ++st)
+                std::memcpy(g->invmat + ((o * TSA + st) * IM) * ID, P.data() + (((size_t)i * ts + st) * IM) * ID,
+                            sizeof(double) * IM * ID);
+        if (g->indicator && !B.empty())
+            std::memcpy(g->indicator + o * ID * ID, B.data() + (size_t)i * ID * ID, sizeof(double) * ID * ID);
+        // mbar = INTEG_BASIS / VOLUME(1,1)
+        const double v11 = ctx->h_vol11[i];
+        double vb = 0.0;
+        for (int d = 0; d < ID; ++d) {
+            const double m = M.empty() ? 0.0 : M[(size_t)i * ID + d];
+            if (g->integ_basis) g->integ_basis[o * ID + d] = m / v11;
+            if (g->volume11) vb -= m * mb[(size_t)i * ID + d] / v11;
+        }
+        if (g->volume11) g->volume11[o] = vb;
+        if (g->qpoints)
+            for (int f = 0; f < std::min(nf, nfa); ++f)
+                for (int q = 0; q < std::min(nqm, nqa); ++q)
+                    for (int d = 0; d < 3; ++d)
+                        g->qpoints[((o * nfa + f) * nqa + q) * 3 + d] = Q[(((size_t)i * nf + f) * nqm + q) * 3 + d];
+        // 1/TOTVOLUME
+        if (g->totvolume) g->totvolume[o] = -V[i] * ctx->h_ovol[i] * ctx->h_ovol[i];
+    }
+    for (int f = 0; f < ctx->nface; ++f) {
+        const size_t o = src_cell(ctx, ctx->h_fcell[2 * f]);
+        const int sl = ctx->h_fslot[2 * f] & 255;
+        if (g->surf) g->surf[o * nfa + sl] = A[f];
+        if (g->normal)
+            for (int d = 0; d < 3; ++d) g->normal[(o * nfa + sl) * 3 + d] = N[(size_t)f * 3 + d];
+    }
+    return UCNS_OK;
+}
+}  // namespace
+
+int ucns_rk3_adjoint_geometry(ucns_ctx *ctx, int32_t nsteps, double dt_fixed, const double *uN_bar, double *u0_bar,
+                              int32_t layout, double *bc_bar, const ucns_geometry_bar_v1 *gbar, double *t_io) {
+    if (!ctx || !uN_bar || !u0_bar || !gbar) return UCNS_EBADARG;
+    if (layout != UCNS_LAYOUT_AOS && layout != UCNS_LAYOUT_SOA) return UCNS_EBADARG;
+    if (gbar->abi_version != UCNS_ABI_VERSION) {
+        ctx->err = "ucns_geometry_bar_v1: abi_version mismatch";
+        return UCNS_EBADARG;
+    }
+    if (!(dt_fixed > 0)) {
+        ctx->err = "ucns_rk3_adjoint_geometry: needs dt_fixed > 0 (the step size is held fixed under the geometry)";
+        return UCNS_EBADARG;
+    }
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (ctx->precision != UCNS_FP64 || ctx->smode) {
+        ctx->err = "ucns_rk3_adjoint_geometry: needs an FP64 context on the register kernels "
+                   "(precision UCNS_FP64; kernels UCNS_KERNELS_REGISTER where the row-stream kernels would run)";
+        return UCNS_EUNSUPPORTED;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((rc = ensure_adjoint(ctx))) return rc;
+    const size_t nc = ctx->ncell;
+    release_geometry(ctx);
+    if ((rc = dalloc(ctx, &ctx->geo_P, nc * ctx->ts * ctx->imax * ctx->ideg)) ||
+        (rc = dalloc(ctx, &ctx->geo_B, nc * ctx->ideg * ctx->ideg)) || (rc = dalloc(ctx, &ctx->geo_M, nc * ctx->ideg)) ||
+        (rc = dalloc(ctx, &ctx->geo_Q, nc * ctx->nf * ctx->nqm * 3)) ||
+        (rc = dalloc(ctx, &ctx->geo_area, (size_t)ctx->nface)) ||
+        (rc = dalloc(ctx, &ctx->geo_nrm, (size_t)ctx->nface * 3)) || (rc = dalloc(ctx, &ctx->geo_ovol, nc))) {
+        release_geometry(ctx);
+        return rc;
+    }
+    zero_dev(ctx, ctx->geo_P, nc * ctx->ts * ctx->imax * ctx->ideg);
+    zero_dev(ctx, ctx->geo_B, nc * ctx->ideg * ctx->ideg);
+    zero_dev(ctx, ctx->geo_M, nc * ctx->ideg);
+    zero_dev(ctx, ctx->geo_Q, nc * ctx->nf * ctx->nqm * 3);
+    zero_dev(ctx, ctx->geo_area, (size_t)ctx->nface);
+    zero_dev(ctx, ctx->geo_nrm, (size_t)ctx->nface * 3);
+    zero_dev(ctx, ctx->geo_ovol, nc);
+    if ((rc = cells_in(ctx, uN_bar, true, layout, ctx->adj_a)) == UCNS_OK) {
+        ctx->geo_on = true;
+        rc = rk3_adjoint_impl(ctx, nsteps, 0.0, dt_fixed, ctx->adj_a, t_io);
+        ctx->geo_on = false;
+    }
+    double bc[2 * NV];
+    if (rc == UCNS_OK && hipMemcpyAsync(bc, ctx->adj_bc, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = UCNS_EHIP;
+    if (rc == UCNS_OK) rc = cells_out(ctx, ctx->adj_a, layout, u0_bar, true);
+    if (rc == UCNS_OK) rc = geo_to_caller(ctx, gbar);
+    release_geometry(ctx);
+    if (rc == UCNS_EHIP && ctx->err.empty()) ctx->err = "ucns_rk3_adjoint_geometry: HIP error";
+    if (rc) return rc;
     if (bc_bar) std::memcpy(bc_bar, bc, sizeof(bc));
     return UCNS_OK;
 }

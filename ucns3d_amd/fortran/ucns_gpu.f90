@@ -48,6 +48,16 @@ module ucns_gpu
     real(c_double) :: inflow(5) = 0.0d0, outflow(5) = 0.0d0
   end type
 
+  ! cotangents of the mesh/PRESTORE data (ucns_rk3_adjoint_geometry): c_loc()
+  ! of the host's arrays in the layouts of ucns_prestore_v1 / ucns_mesh_v2, or
+  ! c_null_ptr for one not wanted
+  type, bind(c) :: ucns_geometry_bar_v1
+    integer(c_int32_t) :: abi_version = UCNS_ABI_VERSION
+    integer(c_int32_t) :: reserved = 0
+    type(c_ptr) :: invmat = c_null_ptr, indicator = c_null_ptr, integ_basis = c_null_ptr, volume11 = c_null_ptr
+    type(c_ptr) :: qpoints = c_null_ptr, surf = c_null_ptr, normal = c_null_ptr, totvolume = c_null_ptr
+  end type
+
   type, bind(c) :: ucns_stats_v1
     integer(c_int64_t) :: n_recon, n_flux, n_update, n_dt
     real(c_double) :: ms_recon, ms_flux, ms_update, ms_dt, bytes_recon_per_cell
@@ -158,6 +168,19 @@ module ucns_gpu
       real(c_double), intent(out) :: u0bar(*)
       integer(c_int32_t), value :: layout
       real(c_double), intent(out) :: bcbar(10)
+      real(c_double), intent(inout) :: t
+    end function
+    integer(c_int) function ucns_rk3_adjoint_geometry(ctx, nsteps, dt_fixed, unbar, u0bar, layout, bcbar, gbar, t) &
+        bind(c, name="ucns_rk3_adjoint_geometry")
+      import :: c_int, c_int32_t, c_ptr, c_double, ucns_geometry_bar_v1
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps
+      real(c_double), value :: dt_fixed
+      real(c_double), intent(in) :: unbar(*)
+      real(c_double), intent(out) :: u0bar(*)
+      integer(c_int32_t), value :: layout
+      real(c_double), intent(out) :: bcbar(10)
+      type(ucns_geometry_bar_v1), intent(in) :: gbar
       real(c_double), intent(inout) :: t
     end function
     integer(c_int) function ucns_rk3_adjoint_device(ctx, nsteps, cfl, dt_fixed, unbar_dev, u0bar_dev, bcbar_dev, t) &

@@ -63,6 +63,14 @@ class BoundaryV1(C.Structure):
     ]
 
 
+class GeometryBarV1(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("reserved", C.c_int32),
+        ("invmat", _f64p), ("indicator", _f64p), ("integ_basis", _f64p), ("volume11", _f64p),
+        ("qpoints", _f64p), ("surf", _f64p), ("normal", _f64p), ("totvolume", _f64p),
+    ]
+
+
 class StatsV1(C.Structure):
     _fields_ = [
         ("n_recon", C.c_int64), ("n_flux", C.c_int64), ("n_update", C.c_int64), ("n_dt", C.c_int64),
@@ -107,6 +115,9 @@ GPU_API = {
                                    C.POINTER(C.c_double)]),
     "ucns_rk3_adjoint_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.POINTER(C.c_double)]),
+    "ucns_rk3_adjoint_geometry": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double),
+                                            C.POINTER(GeometryBarV1), C.POINTER(C.c_double)]),
     "ucns_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsV1)]),
     "ucns_reset_stats": (C.c_int, [C.c_void_p]),
     "ucns_debug_fail_alloc": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -124,6 +135,7 @@ PREP_API = {
     "ucnsp_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "ucnsp_init_vortex": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_uint64, _f64p]),
     "ucnsp_face_rule": (C.c_int, [C.c_int32, C.c_int32, _f64p, _f64p, C.POINTER(C.c_int32)]),
+    "ucnsp_shape_table": (C.c_int, [C.c_int32, C.POINTER(C.c_int32)]),
     "ucnsp_free": (None, [C.c_void_p]),
     "ucnsp_last_error": (C.c_char_p, []),
 }

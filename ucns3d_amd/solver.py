@@ -139,6 +139,32 @@ class Solver:
                                                                  native.f64p(bc), C.byref(t)))
         return u0b, bc, t.value
 
+    # shapes of the geometry cotangents, in the layouts of ucns_prestore_v1 / ucns_mesh_v2
+    def geometry_shapes(self) -> dict:
+        c = self.case
+        nqm = max(c.qp_quad, c.qp_tri)
+        return {"invmat": (c.ncell, c.typesten, c.imax, c.ideg), "indicator": (c.ncell, c.ideg, c.ideg),
+                "integ_basis": (c.ncell, c.ideg), "volume11": (c.ncell,),
+                "qpoints": (c.ncell, c.nface_max, nqm, 3), "surf": (c.ncell, c.nface_max),
+                "normal": (c.ncell, c.nface_max, 3), "totvolume": (c.ncell,)}
+
+    def rk3_adjoint_geometry(self, nsteps: int, uN_bar: np.ndarray, dt_fixed: float, t0: float = 0.0):
+        """ucns_rk3_adjoint_geometry: nsteps RK3 steps of dt_fixed from the
+        current state (left at U_N) and the cotangents of uN_bar . U_N with
+        respect to U_0, the boundary data and the mesh/PRESTORE data, the
+        last as a dict of arrays named after ucns_geometry_bar_v1's fields"""
+        a = np.ascontiguousarray(uN_bar, dtype=np.float64).reshape(self.ncell, 5)
+        u0b = np.empty((self.ncell, 5))
+        bc = np.zeros(10)
+        out = {k: np.empty(shp) for k, shp in self.geometry_shapes().items()}
+        g = native.GeometryBarV1(abi_version=native.UCNS_ABI_VERSION, reserved=0,
+                                 **{k: native.f64p(v) for k, v in out.items()})
+        t = C.c_double(t0)
+        self._check("ucns_rk3_adjoint_geometry",
+                    self.L.ucns_rk3_adjoint_geometry(self._h, nsteps, dt_fixed, native.f64p(a), native.f64p(u0b),
+                                                     UCNS_LAYOUT_AOS, native.f64p(bc), C.byref(g), C.byref(t)))
+        return u0b, bc, out, t.value
+
     def rk3_adjoint_device(self, nsteps: int, uN_bar_ptr: int, u0_bar_ptr: int, bc_ptr: int | None = None,
                            cfl: float = 0.5, dt_fixed: float = 0.0, t0: float = 0.0) -> float:
         t = C.c_double(t0)
