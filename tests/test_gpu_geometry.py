@@ -6,7 +6,11 @@ random direction (the same discrete decisions on both sides at h small).
 
 Face data (areas, normals) are perturbed on both records of a face alike, so
 that the face keeps one geometry; the library reads one record per face and
-returns its cotangent there.  A normal is moved tangentially and its angles
+returns its cotangent there.  The march starts from a smooth non-uniform
+state: from a uniform one, an area changed alone opens the cells (the face
+normals times areas no longer sum to zero), every MUSCL cell then limits a
+ratio of two differences of the size of the change, and J is not
+differentiable there (its one-sided differences disagree at every h).  A normal is moved tangentially and its angles
 (FACEANGLEX/Y) recomputed as the preprocessor forms them."""
 import numpy as np
 import pytest
@@ -19,14 +23,15 @@ from ucns3d_amd.solver import Solver
 
 pytestmark = pytest.mark.gpu
 
+# (spec, steps, initial state: "smooth" perturbed or the case's own)
 CASES = {
     # prisms, pyramids and tetrahedra; inflow, outflow and slip walls; WENO-3
     # with MUSCL cells at the walls
-    "c4-reduced": lambda: reduced("c4", 3),
+    "c4-reduced": (lambda: reduced("c4", 3), 4, "smooth"),
     # jittered periodic tetrahedra, WENO-4
-    "tet-weno4": lambda: CaseSpec("tet-weno4", TET, 4, "weno4", jitter=0.1, seed=7),
+    "tet-weno4": (lambda: CaseSpec("tet-weno4", TET, 4, "weno4", jitter=0.1, seed=7), 4, "smooth"),
     # periodic hexahedra, MUSCL + Barth (the limiter's active points)
-    "hex-muscl": lambda: CaseSpec("hex-muscl", HEX, 6, "muscl", seed=5),
+    "hex-muscl": (lambda: CaseSpec("hex-muscl", HEX, 6, "muscl", seed=5), 4, "smooth"),
 }
 
 
@@ -73,9 +78,9 @@ def smooth_state(case):
 
 class Setup:
     def __init__(self, key):
-        spec = CASES[key]()
+        make, self.nsteps, state = CASES[key]
+        spec = make()
         self.case = case = Case(spec)
-        self.nsteps = 4
         self.fs = spec.freestream() if spec.mach > 0 else None
         if self.fs is not None:
             oracle.set_boundary(1, self.fs, self.fs)
@@ -85,7 +90,7 @@ class Setup:
         # neighbours all hold its own value (a ratio of vanishing differences)
         # or where two face points tie for its minimum, and a random field
         # would put such switches everywhere
-        self.u0 = smooth_state(case)
+        self.u0 = smooth_state(case) if state == "smooth" else case.init_state()
         self.orc = Oracle(case)
         self.dt = 0.5 * self.orc.dt(self.u0, spec.cfl)
         self.c = rng.standard_normal((case.ncell, 5))

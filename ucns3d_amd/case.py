@@ -150,6 +150,15 @@ class Case:
             raise RuntimeError(self._lib.ucnsp_last_error().decode())
         return m, s, p
 
+    def move_nodes(self, xyz: np.ndarray) -> None:
+        """the same mesh with its nodes at xyz [nnode, 3]: geometry and
+        PRESTORE data rebuilt, topology and stencils kept (ucnsp_move_nodes);
+        the arrays and structs handed out before now hold the new data"""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.nnode, 3)
+        rc = self._lib.ucnsp_move_nodes(self._h, native.f64p(x))
+        if rc != 0:
+            raise RuntimeError(f"ucnsp_move_nodes failed ({rc}): {self._lib.ucnsp_last_error().decode()}")
+
     def init_state(self) -> np.ndarray:
         if self.spec.mach > 0:
             return np.tile(self.spec.freestream(), (self.ncell, 1))

@@ -437,6 +437,7 @@ struct ucnsp_case {
     std::vector<double> indicator, integ_basis, volume11, qpoints, frame;
     std::vector<double> framedet;
     int64_t n_solved = 0;  // stencils whose pseudo-inverse was solved (others copied from a congruent one)
+    bool congruence_cache = true;  // copy data between congruent cells (off once nodes have been moved)
 };
 
 namespace {
@@ -552,19 +553,30 @@ void normal_angles(double nx, double ny, double nz, double &a1, double &a2) {
     a2 = std::acos(nz == 0.0 ? 0.0 : nz);
 }
 
+// Size a vector and fill it, keeping its storage when the size is unchanged
+// (arrays handed out by ucnsp_array / ucnsp_fill stay valid across
+// ucnsp_move_nodes).
+template <class T>
+void keep_assign(std::vector<T> &v, size_t n, T x) {
+    if (v.size() == n)
+        std::fill(v.begin(), v.end(), x);
+    else
+        v.assign(n, x);
+}
+
 // GEOMETRY_CALC: volumes (DECOMPOSE3 sums), centres (vertex average),
 // face areas (QUADAREA / TRIANGLEAREA), Newell normals and their angles
 // (FIND_ROT_ANGLES), MINEDGE = 3V/sum(S) (EDGE_CALCULATOR3d).
 void build_geometry(ucnsp_case &c) {
     const int64_t nc = c.ncell;
     const int NF = ucnsp_case::NF;
-    c.vol.assign(nc, 0.0);
-    c.centre.assign(nc * 3, 0.0);
-    c.minedge.assign(nc, 0.0);
-    c.surf.assign(nc * NF, 0.0);
-    c.angx.assign(nc * NF, 0.0);
-    c.angy.assign(nc * NF, 0.0);
-    c.normal.assign(nc * NF * 3, 0.0);
+    keep_assign(c.vol, nc, 0.0);
+    keep_assign(c.centre, nc * 3, 0.0);
+    keep_assign(c.minedge, nc, 0.0);
+    keep_assign(c.surf, nc * NF, 0.0);
+    keep_assign(c.angx, nc * NF, 0.0);
+    keep_assign(c.angy, nc * NF, 0.0);
+    keep_assign(c.normal, nc * NF * 3, 0.0);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nc; ++i) {
         int sh = c.cshape[i];
@@ -645,8 +657,8 @@ void build_geometry(ucnsp_case &c) {
     // CHECKGRADS face-vertex adoption (src/local_pt.f90:2235-2268): on a
     // shared face the lower-numbered cell takes the higher-numbered cell's
     // vertex order, so both sides build their Gauss points in the same order.
-    c.fnodes.assign(nc * NF * 4, -1);
-    c.reorient.assign(nc * NF, 0);
+    keep_assign(c.fnodes, nc * NF * 4, -1);
+    keep_assign(c.reorient, nc * NF, 0);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nc; ++i) {
         int sh = c.cshape[i];
@@ -1192,13 +1204,13 @@ void build_prestore(ucnsp_case &c) {
     const int64_t nc = c.ncell;
     const int NF = ucnsp_case::NF;
     const int TS = c.typesten, IM = c.imax, ID = c.ideg, NQ = c.nqpmax;
-    c.invmat.alloc((size_t)nc * TS * IM * ID);
-    c.indicator.assign(c.o.iweno == 1 ? (size_t)nc * ID * ID : 0, 0.0);
-    c.integ_basis.assign((size_t)nc * ID, 0.0);
-    c.volume11.assign(nc, 0.0);
-    c.qpoints.assign((size_t)nc * NF * NQ * 3, 0.0);
-    c.frame.assign((size_t)nc * 12, 0.0);
-    c.framedet.assign(nc, 0.0);
+    if (c.invmat.n != (size_t)nc * TS * IM * ID) c.invmat.alloc((size_t)nc * TS * IM * ID);
+    keep_assign(c.indicator, c.o.iweno == 1 ? (size_t)nc * ID * ID : 0, 0.0);
+    keep_assign(c.integ_basis, (size_t)nc * ID, 0.0);
+    keep_assign(c.volume11, nc, 0.0);
+    keep_assign(c.qpoints, (size_t)nc * NF * NQ * 3, 0.0);
+    keep_assign(c.frame, (size_t)nc * 12, 0.0);
+    keep_assign(c.framedet, nc, 0.0);
     const DerivTable dt = make_derivs(c.iorder);
     const bool tim = std::getenv("UCNSP_TIMING") != nullptr;
     double t0 = omp_get_wtime();
@@ -1216,7 +1228,7 @@ void build_prestore(ucnsp_case &c) {
         for (int k = 0; k < 3; ++k) c.frame[i * 12 + 9 + k] = o[k];
         c.framedet[i] = det;
     }
-    const bool cache = (c.o.jitter == 0.0);
+    const bool cache = c.congruence_cache && c.o.jitter == 0.0;
     lap("frames");
     // 1. frame-relative per-cell data
     std::vector<int64_t> brep;
@@ -1852,6 +1864,19 @@ int ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint6
             u[i * 5 + 4] = sp / vol / (gamma - 1.0) + sk / vol;
         }
     }
+    return UCNS_OK;
+}
+
+int ucnsp_move_nodes(ucnsp_case *c, const double *xyz) {
+    if (!c || !xyz) return fail(UCNS_EBADARG, "null argument");
+    for (int64_t k = 0; k < c->nnode * 3; ++k)
+        if (!std::isfinite(xyz[k])) return fail(UCNS_EBADARG, "non-finite node coordinate");
+    std::copy(xyz, xyz + c->nnode * 3, c->xyz.begin());
+    c->congruence_cache = false;
+    build_geometry(*c);
+    for (int64_t i = 0; i < c->ncell; ++i)
+        if (!(c->vol[i] > 0)) return fail(UCNS_EBADARG, "moved nodes give cell " + std::to_string(i) + " no volume");
+    build_prestore(*c);
     return UCNS_OK;
 }
 

@@ -86,6 +86,12 @@ int  ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint
 int  ucnsp_shape_table(int32_t shape, int32_t *out);
 /* Gauss-point tables used by the face integration (weights sum to 1). */
 int  ucnsp_face_rule(int32_t igqrules, int32_t tri, double *pts /*[n][3] barycentric or (r,s,0)*/, double *w, int32_t *n);
+/* Move the nodes to xyz [nnode][3] and rebuild the geometry and PRESTORE
+ * data for them with the topology, stencils and FULL flags unchanged (a
+ * mesh deformation, as a shape study or its finite differences need: the
+ * stencil selection's cone tests are not continuous in the nodes).  Data are
+ * then solved cell by cell, never copied between congruent cells. */
+int  ucnsp_move_nodes(ucnsp_case *c, const double *xyz);
 void ucnsp_free(ucnsp_case *c);
 const char *ucnsp_last_error(void);
 

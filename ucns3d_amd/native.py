@@ -136,6 +136,7 @@ PREP_API = {
     "ucnsp_init_vortex": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_uint64, _f64p]),
     "ucnsp_face_rule": (C.c_int, [C.c_int32, C.c_int32, _f64p, _f64p, C.POINTER(C.c_int32)]),
     "ucnsp_shape_table": (C.c_int, [C.c_int32, C.POINTER(C.c_int32)]),
+    "ucnsp_move_nodes": (C.c_int, [C.c_void_p, _f64p]),
     "ucnsp_free": (None, [C.c_void_p]),
     "ucnsp_last_error": (C.c_char_p, []),
 }
