@@ -92,10 +92,11 @@ CASES = {
                                   (1, prim2cons(1.0, 0.6 * math.sqrt(1.4 * 100 / 1.4), 0.0, 0.0, 100 / 1.4),
                                    prim2cons(1.0, 0.6 * math.sqrt(1.4 * 100 / 1.4), 0.0, 0.0, 100 / 1.4))),
 }
-# cases kept out of the difference test: on the bumped channel 473 of 648
-# cells take the Barth limiter, whose arg-min face point moves between
-# points at kinks of the primal
-FD_KINKED = {"hybrid-weno3-bump"}
+# cases whose state carries no cell noise: on the bumped channel 473 of 648
+# cells take the Barth limiter, and with noise from cell to cell its
+# arg-min face point and its min(1, r) switch at kinks of the primal closer
+# together than any usable h; the smooth modulation alone leaves no tie
+SMOOTH_ONLY = {"hybrid-weno3-bump"}
 _cache = {}
 
 
@@ -119,7 +120,18 @@ def setup(name):
         # values
         x = case.array("centre").reshape(-1, 3)
         u = u * (1 + 0.02 * np.sin(0.37 * x[:, :1] + 0.71 * x[:, 1:2] + 0.53 * x[:, 2:3] + 0.2))
-        u = u * (1 + 1e-3 * rng.uniform(-1, 1, u.shape))
+        if name not in SMOOTH_ONLY:
+            u = u * (1 + 1e-3 * rng.uniform(-1, 1, u.shape))
+        else:
+            # and a smooth velocity field: the vortex's velocity components
+            # vanish on whole planes (w everywhere), where the limiter would
+            # take the ratio of two differences of size h (a kink at h = 0)
+            r = u[:, 0]
+            p = (G - 1) * (u[:, 4] - 0.5 * (u[:, 1:4] ** 2).sum(1) / r)
+            vel = u[:, 1:4] / r[:, None] + np.stack([np.sin(0.9 * x[:, 0] + 0.4 * x[:, 1] + 0.7 * x[:, 2] + 0.5),
+                                                     np.cos(0.5 * x[:, 0] + 0.8 * x[:, 1] + 0.3 * x[:, 2] + 0.1),
+                                                     np.sin(0.6 * x[:, 0] + 0.2 * x[:, 1] + 0.9 * x[:, 2] + 1.1)], 1)
+            u = np.c_[r, r[:, None] * vel, p / (G - 1) + 0.5 * r * (vel ** 2).sum(1)]
         # directions scaled with the variation of each variable, so that the
         # differences stay in the range where the smoothness indicators
         # (quadratic in the coefficients) are resolved to O(h^2).  On the
@@ -143,9 +155,6 @@ def setup(name):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_residual_tangent_matches_differences(name):
-    if name in FD_KINKED:
-        pytest.skip("most cells are MUSCL cells: central differences straddle limiter switches "
-                    "(the GPU product is checked against the tangent model by the dot-product test)")
     case, orc, u, du, bc = setup(name)
     r, dr = orc.residual_tl(u, du)
     r0, _ = orc.residual(u)
