@@ -98,8 +98,9 @@ typedef struct ucns_scheme_v2 {
     int32_t idegfree;           /* coefficients without the mean */
     int32_t inumneighbours;     /* stencil size including the cell */
     int32_t typesten;           /* stencil slots per cell (1 MUSCL, 7 WENO 3D) */
-    int32_t ees;                /* 0 (WENO) ; 5 (CWENO) not supported yet */
-    int32_t wenoz;              /* 0 */
+    int32_t ees;                /* 0 (WENO) or 5 (CWENO: the central polynomial reconstituted from the
+                                   sector polynomials, src/reconstruct_p.f90:955-966) */
+    int32_t wenoz;              /* 0, or 1 with ees = 5 (CWENO-Z weights, src/reconstruct_p.f90:1019-1029) */
     int32_t wenwrt;             /* 1 conserved-variable reconstruction */
     int32_t limiter;            /* 1 Barth-Jespersen */
     int32_t riemann;            /* 1 HLLC */
@@ -135,6 +136,22 @@ typedef struct ucns_prestore_v1 {
     const double *volume11;     /* [ncell] ILOCAL_RECON3%VOLUME(1,1) */
     const double *qpoints;      /* [ncell][nface_max][nqp_max][3] QPOINTS(L,NGP,1:3), local frame */
 } ucns_prestore_v1;
+
+/* CWENO data (EES=5): the sector stencils s >= 2 (1-based) of a cell take a
+ * polynomial of degree IORDER2 (idegfree2 coefficients: 3 for IORDER <= 3, 9
+ * above) over their first inumneighbours2 members (12 or 20, the cell first),
+ * with the means of INTEG_BASIS%VALUEC (src/grid_p.f90:6151-6163,
+ * src/prestore.f90:244-263, 296-337, 520-535).  imax2 = inumneighbours2-1. */
+typedef struct ucns_cweno_v1 {
+    int32_t abi_version;
+    int32_t idegfree2;
+    int32_t inumneighbours2;
+    int32_t reserved;
+    const double *invmat_c;      /* [ncell][typesten][imax2][idegfree2] INVMAT_STENCILTC(ideg2,imax2,s);
+                                    the slot of stencil 1 is not read */
+    const double *indicator_c;   /* [ncell][idegfree2][idegfree2] INDICATORC */
+    const double *integ_basis_c; /* [ncell][idegfree2] INTEG_BASIS%VALUEC */
+} ucns_cweno_v1;
 
 /* Accumulated device-side timings and traffic model (ucns_stats). */
 typedef struct ucns_stats_v1 {
@@ -190,8 +207,12 @@ typedef struct ucns_ctx ucns_ctx;
 
 /* Set up one partition on the current HIP device (topology, faces, scheme). */
 int  ucns_create(const ucns_mesh_v2 *mesh, const ucns_scheme_v2 *scheme, ucns_ctx **out);
-/* Upload PRESTORE data (pseudo-inverses, indicators, qp coordinates ...). */
+/* Upload PRESTORE data (pseudo-inverses, indicators, qp coordinates ...).
+ * With ees = 5 (CWENO) the CWENO data must have been given first. */
 int  ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre);
+/* CWENO data for a context created with ees = 5 (CP_RECONSTRUCTION's EES=5
+ * branch, src/reconstruct_p.f90:955-1014), before ucns_upload_prestore. */
+int  ucns_upload_cweno(ucns_ctx *ctx, const ucns_cweno_v1 *cw);
 /* Free-stream data of the boundary codes 1, 2 and 6 (needed before the first
  * residual when the mesh has such faces; codes 3 and 4 need none). */
 int  ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc);

@@ -47,6 +47,10 @@ def lib() -> C.CDLL:
         L.oracle_barth.restype = C.c_double
         L.oracle_set_boundary.argtypes = [vp]
         L.oracle_set_boundary.restype = None
+        L.oracle_set_cweno.argtypes = [vp]
+        L.oracle_set_cweno.restype = None
+        L.oracle_cweno_weights.argtypes = [C.c_int, f64, C.c_double, C.c_double, C.c_int, f64]
+        L.oracle_cweno_weights.restype = None
         L.oracle_set_face_order.argtypes = [C.c_int]
         L.oracle_set_face_order.restype = None
         L.oracle_set_uleft_form.argtypes = [C.c_int]
@@ -75,11 +79,14 @@ class Oracle:
     def __init__(self, case, precision=0):
         self.case = case
         self.m, self.s, self.p = case.structs(precision)
+        self.cw = case.cweno() if hasattr(case, "cweno") else None
         self.L = lib()
         nqm = max(case.qp_quad, case.qp_tri)
         self.uleft_shape = (case.ncell, case.nface_max, nqm, 5)
 
     def _refs(self):
+        # the CWENO data of this case (or none) for the calls that follow
+        self.L.oracle_set_cweno(C.byref(self.cw) if self.cw is not None else None)
         return C.byref(self.m), C.byref(self.s), C.byref(self.p)
 
     def reconstruct(self, u):
@@ -169,6 +176,14 @@ def weno_weights(beta, lwci1, eps=1e-11):
 
 def barth(ui, umin, umax, usol):
     return lib().oracle_barth(ui, umin, umax, usol)
+
+
+def cweno_weights(beta, lwci1, eps=1e-11, wenoz=0):
+    """CWENO weights of one variable for the stencils' indicators beta (central first)"""
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    out = np.zeros_like(b)
+    lib().oracle_cweno_weights(len(b), _p(b), lwci1, eps, wenoz, _p(out))
+    return out
 
 
 def set_boundary(boundtype, inflow, outflow):

@@ -339,6 +339,37 @@ static void weno_weights(int admis, const T *beta, double lwci1, double eps, T *
     for (int ll = 0; ll < admis; ++ll) omega[ll] = wt[ll] / sum;
 }
 
+/* CWENO data of the scheme's ees = 5 (oracle_set_cweno; NULL: none) */
+static const ucns_cweno_v1 *g_cw = nullptr;
+
+/* CWENO weights (reconstruct_p.f90:955-958, 1010-1049): lambda_1 = 1 - 1/LWCI1, lambda_l =
+ * (1 - lambda_1)/(ADMIS-1); WENO: lambda/(eps+beta)^4; WENO-Z: lambda (1 + (tau/(eps+beta))^4) with
+ * tau = sum_l |beta_1 - beta_l| / (ADMIS-1); normalised */
+template <class T>
+static void cweno_weights(int admis, const T *beta, double lwci1, double eps, int wenoz, T *omega) {
+    const double lam1 = 1.0 - (1.0 / lwci1);
+    const double laml = admis > 1 ? (1.0 - lam1) / (admis - 1) : 0.0;
+    T wt[8], sum = T(0.0), tau = T(0.0);
+    if (wenoz == 1 && admis > 1) {
+        for (int ll = 0; ll < admis; ++ll) tau = tau + fabs(beta[0] - beta[ll]);
+        tau = tau / (double)(admis - 1);
+    }
+    for (int ll = 0; ll < admis; ++ll) {
+        const double lam = ll == 0 ? lam1 : laml;
+        const T x = eps + beta[ll];
+        if (wenoz == 1) {
+            const T r = tau / x;
+            const T r2 = r * r;
+            wt[ll] = lam * (1.0 + r2 * r2);
+        } else {
+            const T x2 = x * x;
+            wt[ll] = lam / (x2 * x2);
+        }
+    }
+    for (int ll = 0; ll < admis; ++ll) sum = sum + wt[ll];
+    for (int ll = 0; ll < admis; ++ll) omega[ll] = wt[ll] / sum;
+}
+
 /* per-thread scratch of the cell routines */
 template <class T>
 struct Work {
@@ -347,6 +378,123 @@ struct Work {
     Work(const ctx_t &c)
         : grad((size_t)c.ts * c.ideg * NV), mat1((size_t)c.imax * NV), bg(c.ideg), phi(c.ideg) {}
 };
+
+/* one cell of ARBITRARY_ORDER with EES=5 (CP_RECONSTRUCTION, reconstruct_p.f90:955-1014, 1062-1199):
+ * central LSQ of degree IORDER, sector LSQ of degree IORDER2 over the first numneighbours2 members
+ * (INVMAT_STENCILTC, gradients.f90:362), the zero polynomial (G_1 - sum_l lambda_l G_l) / lambda_1
+ * with lambda_1 = 1 - 1/LWCI1 and lambda_l = (1 - lambda_1)/(ADMIS-1), its indicator with INDICATOR
+ * and the sectors' with INDICATORC, WENO or WENO-Z weights (power 4), and the face values
+ * sum_l omega_l (U + res_l), res_1 on the high-order basis, res_l on the low-order one (VALUEC). */
+template <class T>
+static void cweno_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) {
+    const int ideg = c->ideg, imax = c->imax, nnb = c->nnb, ts = c->ts;
+    const int id2 = g_cw->idegfree2, im2 = g_cw->inumneighbours2 - 1;
+    const int admis = c->p->admis[i];
+    const T *ui = u + i * NV;
+    T *mat1 = w.mat1.data();
+    double *phi = w.phi.data();
+    std::vector<T> g0((size_t)ideg * NV), gc((size_t)ts * id2 * NV), zp((size_t)ideg * NV);
+    T omega[8][NV];
+    /* LSQ: the central stencil (degree IORDER) */
+    {
+        const int32_t *ih = c->p->ihexl + (size_t)i * ts * nnb;
+        const T *s1 = u + (size_t)(ih[0] - c->base) * NV;
+        for (int iq = 0; iq < imax; ++iq) {
+            const T *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
+            for (int v = 0; v < NV; ++v) mat1[iq * NV + v] = s2[v] - s1[v];
+        }
+        const double *P = c->p->invmat + (size_t)i * ts * imax * ideg;
+        for (int k = 0; k < ideg; ++k)
+            for (int v = 0; v < NV; ++v) {
+                T acc = T(0.0);
+                for (int iq = 0; iq < imax; ++iq) acc = acc + P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
+                g0[(size_t)k * NV + v] = acc;
+            }
+    }
+    /* the sectors (degree IORDER2, first numneighbours2 members) */
+    for (int ll = 1; ll < admis; ++ll) {
+        const int32_t *ih = c->p->ihexl + ((size_t)i * ts + ll) * nnb;
+        const T *s1 = u + (size_t)(ih[0] - c->base) * NV;
+        for (int iq = 0; iq < im2; ++iq) {
+            const T *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
+            for (int v = 0; v < NV; ++v) mat1[iq * NV + v] = s2[v] - s1[v];
+        }
+        const double *P = g_cw->invmat_c + ((size_t)i * ts + ll) * im2 * id2;
+        for (int k = 0; k < id2; ++k)
+            for (int v = 0; v < NV; ++v) {
+                T acc = T(0.0);
+                for (int iq = 0; iq < im2; ++iq) acc = acc + P[(size_t)iq * id2 + k] * mat1[iq * NV + v];
+                gc[((size_t)ll * id2 + k) * NV + v] = acc;
+            }
+    }
+    const double lwc = c->s->lwci1;
+    const double lam1 = 1.0 - (1.0 / lwc);
+    const double laml = admis > 1 ? (1.0 - lam1) / (admis - 1) : 0.0;
+    const double *B = c->p->indicator + (size_t)i * ideg * ideg;
+    const double *Bc = g_cw->indicator_c + (size_t)i * id2 * id2;
+    const double eps = c->s->weno_eps;
+    for (int v = 0; v < NV; ++v) {
+        /* GRAD3AL = sum of the sector polynomials, then the zero polynomial GRAD1AL */
+        std::vector<T> g3((size_t)ideg, T(0.0));
+        for (int ll = 1; ll < admis; ++ll)
+            for (int k = 0; k < id2; ++k) g3[k] = g3[k] + (laml * gc[((size_t)ll * id2 + k) * NV + v]);
+        for (int k = 0; k < ideg; ++k) zp[(size_t)k * NV + v] = (1.0 / lam1) * (g0[(size_t)k * NV + v] - g3[k]);
+        T beta[8], wt[8];
+        for (int ll = 0; ll < admis; ++ll) {
+            const int n = ll == 0 ? ideg : id2;
+            const double *BB = ll == 0 ? B : Bc;
+            T *bg = w.bg.data();
+            for (int a = 0; a < n; ++a) {
+                T acc = T(0.0);
+                for (int b = 0; b < n; ++b) {
+                    const T gb = ll == 0 ? zp[(size_t)b * NV + v] : gc[((size_t)ll * id2 + b) * NV + v];
+                    acc = acc + BB[(size_t)b * n + a] * gb;
+                }
+                bg[a] = acc;
+            }
+            T dot = T(0.0);
+            for (int a = 0; a < n; ++a) {
+                const T ga = ll == 0 ? zp[(size_t)a * NV + v] : gc[((size_t)ll * id2 + a) * NV + v];
+                dot = dot + ga * bg[a];
+            }
+            beta[ll] = dot;
+        }
+        cweno_weights(admis, beta, lwc, eps, c->s->wenoz, wt);
+        for (int ll = 0; ll < admis; ++ll) omega[ll][v] = wt[ll];
+    }
+    /* extrapolation, stencil by stencil */
+    const double *integ = c->p->integ_basis + (size_t)i * ideg;
+    const double *integc = g_cw->integ_basis_c + (size_t)i * id2;
+    const double vol11 = c->p->volume11[i];
+    for (int f = 0; f < c->m->ifca[i]; ++f)
+        for (int q = 0; q < nqp_of(c, i, f); ++q) {
+            T *o = uleft_at(c, ul, i, f, q);
+            for (int v = 0; v < NV; ++v) o[v] = T(0.0);
+        }
+    for (int ll = 0; ll < admis; ++ll)
+        for (int f = 0; f < c->m->ifca[i]; ++f)
+            for (int q = 0; q < nqp_of(c, i, f); ++q) {
+                const double *x = c->p->qpoints + (((size_t)i * c->nf + f) * c->nqm + q) * 3;
+                const int n = ll == 0 ? ideg : id2;
+                basis_rec(x, n, ll == 0 ? integ : integc, vol11, phi);
+                T *o = uleft_at(c, ul, i, f, q);
+                for (int v = 0; v < NV; ++v) {
+                    T res = T(0.0);
+                    for (int k = 0; k < n; ++k)
+                        res = res + phi[k] * (ll == 0 ? zp[(size_t)k * NV + v] : gc[((size_t)ll * id2 + k) * NV + v]);
+                    if (g_uleft_mean_first)
+                        o[v] = o[v] + res * omega[ll][v];
+                    else
+                        o[v] = o[v] + (ui[v] + res) * omega[ll][v];
+                }
+            }
+    if (g_uleft_mean_first)
+        for (int f = 0; f < c->m->ifca[i]; ++f)
+            for (int q = 0; q < nqp_of(c, i, f); ++q) {
+                T *o = uleft_at(c, ul, i, f, q);
+                for (int v = 0; v < NV; ++v) o[v] = ui[v] + o[v];
+            }
+}
 
 /* one cell of ARBITRARY_ORDER: LEAST_SQUARES + WENOWEIGHTS (CP_RECONSTRUCTION) */
 template <class T>
@@ -547,7 +695,10 @@ static int reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucn
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             if (s->iweno == 1) {
-                weno_cell(&c, u, i, ul, w);
+                if (s->ees == 5)
+                    cweno_cell(&c, u, i, ul, w);
+                else
+                    weno_cell(&c, u, i, ul, w);
                 int red = 0;
                 if (p->full[i] == 0) red = 1;
                 else red = trouble_test(&c, u, ul, i, 0.85);
@@ -773,6 +924,12 @@ void oracle_set_boundary(const ucns_boundary_v1 *bc) { g_bc = *bc; }
 void oracle_set_face_order(int reversed) { g_faces_reversed = reversed ? 1 : 0; }
 void oracle_set_uleft_form(int mean_first) { g_uleft_mean_first = mean_first ? 1 : 0; }
 void oracle_set_face_once(int once) { g_face_once = once ? 1 : 0; }
+/* the CWENO data the scheme's ees = 5 reads (NULL: none); the struct and its
+ * arrays must outlive the calls that use them */
+void oracle_set_cweno(const ucns_cweno_v1 *cw) { g_cw = cw; }
+void oracle_cweno_weights(int admis, const double *beta, double lwci1, double eps, int wenoz, double *omega) {
+    cweno_weights(admis, beta, lwci1, eps, wenoz, omega);
+}
 void oracle_ghost(int code, double gamma, const double *cl, double a1, double a2, double *cr) {
     ghost(code, gamma, cl, a1, a2, cr);
 }

@@ -23,6 +23,7 @@ program host_march
     integer(c_int32_t) :: iorder, iweno, extf, weight_lsqr, nthreads
     integer(c_int32_t) :: bc(6)
     real(c_double) :: bump(8)
+    integer(c_int32_t) :: ees
   end type
 
   interface
@@ -97,7 +98,7 @@ contains
     real(c_double), parameter :: two_pi = 6.283185307179586d0
     o%nx = n; o%ny = n; o%nz = n
     o%lx = two_pi; o%ly = two_pi; o%lz = two_pi
-    o%weight_lsqr = 0; o%nthreads = 0; o%bc = 0; o%bump = 0.0d0
+    o%weight_lsqr = 0; o%nthreads = 0; o%bc = 0; o%bump = 0.0d0; o%ees = 0
     if (shape == "tet") then       ! 6 Kuhn tetrahedra per hex, jittered, WENO-4
       o%shape = 2; o%jitter = 0.1d0; o%seed = 2; o%iorder = 3; o%iweno = 1; o%extf = 2
     else                           ! hexahedra, MUSCL + Barth
@@ -198,8 +199,8 @@ contains
     call expect_create_error(m2, scheme, UCNS_EBADARG, "abi_version")
     m2 = mesh; m2%surf = c_null_ptr
     call expect_create_error(m2, scheme, UCNS_EBADARG, "null mesh array")
-    s2 = scheme; s2%ees = 5
-    call expect_create_error(mesh, s2, UCNS_EUNSUPPORTED, "EES")
+    s2 = scheme; s2%ees = 3
+    call expect_create_error(mesh, s2, UCNS_EUNSUPPORTED, "ees must be 0")
     s2 = scheme; s2%cell_order = 7
     call expect_create_error(mesh, s2, UCNS_EBADARG, "cell_order")
     s2 = scheme; s2%cell_order = UCNS_ORDER_MORTON

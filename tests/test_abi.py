@@ -46,7 +46,7 @@ def test_header_lists_the_path():
                "ucns_reconstruct", "ucns_rk3_steps", "ucns_compute_dt", "ucns_destroy", "ucns_last_error",
                "ucns_residual_vjp", "ucns_residual_vjp_device", "ucns_rk3_adjoint", "ucns_rk3_adjoint_device"]:
         assert fn in fns
-    assert len(fns) == 24
+    assert len(fns) == 25
 
 
 def test_every_declared_symbol_is_exported(glib):
@@ -116,8 +116,9 @@ def test_create_rejects_bad_arguments(glib, small_case):
         (lambda m: setattr(m, "surf", None), None, -1, "null mesh array"),
         (None, lambda s: setattr(s, "precision", 5), -1, "precision"),
         (None, lambda s: setattr(s, "iweno", 0), -2, "iweno"),
-        (None, lambda s: setattr(s, "ees", 5), -2, "EES"),
-        (None, lambda s: setattr(s, "wenoz", 1), -2, "wenoz"),
+        (None, lambda s: setattr(s, "ees", 3), -2, "ees must be 0"),
+        (None, lambda s: setattr(s, "wenoz", 1), -2, "wenoz 1 is a CWENO option"),
+        (None, lambda s: setattr(s, "wenoz", 2), -1, "wenoz must be"),
         (None, lambda s: setattr(s, "riemann", 2), -2, "HLLC"),
         (None, lambda s: setattr(s, "limiter", 2), -2, "Barth"),
         (None, lambda s: setattr(s, "qp_quad", 0), -1, "quadrature"),

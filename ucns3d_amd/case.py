@@ -42,6 +42,8 @@ class CaseSpec:
     bump: tuple = ()      # lower-wall bump coefficients a_1..a_8 (channel configuration)
     box: tuple = ()       # (nx, ny, nz, lx, ly, lz) for a box other than n^3 cubes of side `length`
     mach: float = 0.0     # > 0: the initial state is a uniform flow along x at this Mach number, rho = p = 1
+    ees: int = 0          # 5: CWENO (central polynomial reconstituted from low-order sector polynomials)
+    wenoz: int = 0        # 1: CWENO-Z weights (with ees = 5)
 
     @property
     def iorder(self) -> int:
@@ -102,7 +104,7 @@ def reduced(key: str, n: int) -> CaseSpec:
     if s.box:  # the same aspect: 4n x n x n cells
         box = (4 * n, n, n) + tuple(s.box[3:])
     return CaseSpec(f"{key}-reduced-{n}", s.shape, n, s.scheme, s.jitter, s.seed, s.noise, s.cfl, s.steps,
-                    s.length, s.lwci1, s.gamma, s.bc, s.bump, box, s.mach)
+                    s.length, s.lwci1, s.gamma, s.bc, s.bump, box, s.mach, s.ees, s.wenoz)
 
 
 class Case:
@@ -114,7 +116,8 @@ class Case:
                             lx=lx, ly=ly, lz=lz, jitter=spec.jitter, seed=spec.seed,
                             iorder=spec.iorder, iweno=spec.iweno, extf=spec.extf, weight_lsqr=0, nthreads=nthreads,
                             bc=(C.c_int32 * 6)(*spec.bc),
-                            bump=(C.c_double * 8)(*(tuple(spec.bump) + (0.0,) * (8 - len(spec.bump)))))
+                            bump=(C.c_double * 8)(*(tuple(spec.bump) + (0.0,) * (8 - len(spec.bump)))),
+                            ees=spec.ees)
         h = C.c_void_p()
         rc = lib.ucnsp_build(C.byref(o), C.byref(h))
         if rc != 0:
@@ -148,7 +151,18 @@ class Case:
                                   C.byref(p))
         if rc != 0:
             raise RuntimeError(self._lib.ucnsp_last_error().decode())
+        s.wenoz = self.spec.wenoz
         return m, s, p
+
+    def cweno(self):
+        """the ucns_cweno_v1 struct of a CWENO case (ees 5), else None"""
+        if self.spec.ees != 5:
+            return None
+        cw = native.CwenoV1()
+        rc = self._lib.ucnsp_fill_cweno(self._h, C.byref(cw))
+        if rc != 0:
+            raise RuntimeError(self._lib.ucnsp_last_error().decode())
+        return cw
 
     def move_nodes(self, xyz: np.ndarray) -> None:
         """the same mesh with its nodes at xyz [nnode, 3]: geometry and

@@ -437,6 +437,12 @@ struct ucnsp_case {
     std::vector<double> indicator, integ_basis, volume11, qpoints, frame;
     std::vector<double> framedet;
     int64_t n_solved = 0;  // stencils whose pseudo-inverse was solved (others copied from a congruent one)
+    // CWENO (EES=5): the sector stencils s >= 1 take a polynomial of degree
+    // iorder2 (ideg2 coefficients) over their first imax2 members
+    // (src/grid_p.f90:6151-6163, src/prestore.f90:244-263, 296-337, 520-535)
+    int iorder2 = 0, ideg2 = 0, imax2 = 0;
+    std::vector<double> integ_basis_c, indicator_c;
+    Buf<double> invmat_c;
     bool congruence_cache = true;  // copy data between congruent cells (off once nodes have been moved)
 };
 
@@ -1200,6 +1206,105 @@ void classes(int64_t n, std::vector<int64_t> &rep, Hash hash, Full full) {
     }
 }
 
+// CWENO data of one mesh: INTEG_BASIS%VALUEC and INDICATORC (the low-order
+// polynomial's moments and indicator in the cell's frame, CALINTBASIS with
+// ICOMPWRT=1 and INDICATORMATRIX2, src/prestore.f90:244-263, 1700-1800), and
+// INVMAT_STENCILTC of every sector stencil s >= 1 (its first imax2 members,
+// the low-order basis with the VALUEC means; src/prestore.f90:296-337,
+// 520-535).  Stencil 0 keeps the high-order data; its slot here is zero.
+void build_cweno(ucnsp_case &c) {
+    const int64_t nc = c.ncell;
+    const int ID2 = c.ideg2, IM2 = c.imax2, TS = c.typesten, NNB = c.nnb;
+    keep_assign(c.integ_basis_c, (size_t)nc * ID2, 0.0);
+    keep_assign(c.indicator_c, (size_t)nc * ID2 * ID2, 0.0);
+    if (c.invmat_c.n != (size_t)nc * TS * IM2 * ID2) c.invmat_c.alloc((size_t)nc * TS * IM2 * ID2);
+    const DerivTable dt = make_derivs(c.iorder2);
+#pragma omp parallel
+    {
+        QPts q;
+        std::vector<double> mono(ID2), intg(ID2), mbar(ID2), A((size_t)IM2 * ID2), M((size_t)ID2 * ID2),
+            inv((size_t)ID2 * ID2), work(3 * ID2 * ID2 + ID2 + 8), volj(IM2);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nc; ++i) {
+            const double *J9 = &c.frame[i * 12], *o = &c.frame[i * 12 + 9];
+            const double adet = std::fabs(c.framedet[i]);
+            double vi[8][3];
+            member_local_nodes(c, i, i, J9, o, vi);
+            cell_rule(c.cshape[i], vi, c.iorder2, q);
+            std::fill(intg.begin(), intg.end(), 0.0);
+            for (size_t p = 0; p < q.w.size(); ++p) {
+                monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID2, mono.data());
+                for (int d = 0; d < ID2; ++d) intg[d] += q.w[p] * mono[d];
+            }
+            for (int d = 0; d < ID2; ++d) c.integ_basis_c[i * ID2 + d] = intg[d];
+            // INDICATORC: sum over 1 <= |a| <= iorder2 of the Gram matrix of D^a m
+            cell_rule(c.cshape[i], vi, 2 * (c.iorder2 - 1), q);
+            double *B = &c.indicator_c[(size_t)i * ID2 * ID2];
+            for (size_t p = 0; p < q.w.size(); ++p)
+                for (int a = 0; a < dt.na; ++a) {
+                    const int ax = dt.alpha[a * 3], ay = dt.alpha[a * 3 + 1], az = dt.alpha[a * 3 + 2];
+                    double dv[34];
+                    for (int d = 0; d < ID2; ++d) {
+                        const int ex = MONO[d][0], ey = MONO[d][1], ez = MONO[d][2];
+                        if (ex < ax || ey < ay || ez < az) {
+                            dv[d] = 0.0;
+                            continue;
+                        }
+                        dv[d] = falling(ex, ax) * falling(ey, ay) * falling(ez, az) *
+                                std::pow(q.x[p * 3], ex - ax) * std::pow(q.x[p * 3 + 1], ey - ay) *
+                                std::pow(q.x[p * 3 + 2], ez - az);
+                    }
+                    for (int u = 0; u < ID2; ++u)
+                        for (int t = 0; t < ID2; ++t) B[u * ID2 + t] += q.w[p] * dv[u] * dv[t];
+                }
+            const double vol11 = c.volume11[i];
+            for (int d = 0; d < ID2; ++d) mbar[d] = intg[d] / vol11;
+            double *Pz = c.invmat_c.p + (size_t)i * TS * IM2 * ID2;
+            std::fill(Pz, Pz + (size_t)IM2 * ID2, 0.0);  // stencil 0
+            for (int st = 1; st < TS; ++st) {
+                double *P = Pz + (size_t)st * IM2 * ID2;
+                if (st >= c.admis[i]) {
+                    std::fill(P, P + (size_t)IM2 * ID2, 0.0);
+                    continue;
+                }
+                const int32_t *mem = &c.ihexl[((size_t)i * TS + st) * NNB];
+                for (int k = 0; k < IM2; ++k) {
+                    const int64_t j = mem[k + 1];
+                    double vj[8][3];
+                    member_local_nodes(c, i, j, J9, o, vj);
+                    cell_rule(c.cshape[j], vj, c.iorder2, q);
+                    double vq = 0.0;
+                    std::fill(intg.begin(), intg.end(), 0.0);
+                    for (size_t p = 0; p < q.w.size(); ++p) {
+                        monomials(q.x[p * 3], q.x[p * 3 + 1], q.x[p * 3 + 2], ID2, mono.data());
+                        for (int d = 0; d < ID2; ++d) intg[d] += q.w[p] * mono[d];
+                        vq += q.w[p];
+                    }
+                    for (int d = 0; d < ID2; ++d) A[(size_t)k * ID2 + d] = intg[d] - mbar[d] * vq;
+                    volj[k] = c.vol[j] / adet;
+                }
+                for (int a2 = 0; a2 < ID2; ++a2)
+                    for (int b2 = a2; b2 < ID2; ++b2) {
+                        double s2 = 0.0;
+                        for (int k = 0; k < IM2; ++k) s2 += A[(size_t)k * ID2 + a2] * A[(size_t)k * ID2 + b2];
+                        M[a2 * ID2 + b2] = s2;
+                        M[b2 * ID2 + a2] = s2;
+                    }
+                qr_inverse(ID2, M.data(), inv.data(), work.data());
+                bool bad = false;
+                for (int k = 0; k < IM2; ++k)
+                    for (int a2 = 0; a2 < ID2; ++a2) {
+                        double s2 = 0.0;
+                        for (int b2 = 0; b2 < ID2; ++b2) s2 += inv[a2 * ID2 + b2] * A[(size_t)k * ID2 + b2];
+                        P[(size_t)k * ID2 + a2] = s2 * volj[k];
+                        if (P[(size_t)k * ID2 + a2] != P[(size_t)k * ID2 + a2]) bad = true;
+                    }
+                if (bad) c.full[i] = 0;  // NaN stencils disable FULL (src/prestore.f90:541-544)
+            }
+        }
+    }
+}
+
 void build_prestore(ucnsp_case &c) {
     const int64_t nc = c.ncell;
     const int NF = ucnsp_case::NF;
@@ -1305,7 +1410,9 @@ void build_prestore(ucnsp_case &c) {
         for (int64_t k = 0; k < ns; ++k) {
             const int64_t i = k / TS;
             const int s = (int)(k % TS);
-            if (s >= c.admis[i]) {
+            // CWENO: the sector stencils' high-order pseudo-inverses are not
+            // formed (their low-order ones are, by build_cweno)
+            if (s >= c.admis[i] || (c.ideg2 > 0 && s > 0)) {
                 double *P = c.invmat.p + (size_t)k * IM * ID;
                 std::fill(P, P + (size_t)IM * ID, 0.0);
                 continue;
@@ -1318,13 +1425,14 @@ void build_prestore(ucnsp_case &c) {
         for (int64_t k = 0; k < ns; ++k) {
             const int64_t r = srep[k];
             const int64_t i = k / TS;
-            if (r == k || (int)(k % TS) >= c.admis[i]) continue;
+            if (r == k || (int)(k % TS) >= c.admis[i] || (c.ideg2 > 0 && k % TS > 0)) continue;
             std::memcpy(c.invmat.p + (size_t)k * IM * ID, c.invmat.p + (size_t)r * IM * ID, sizeof(double) * IM * ID);
             ok[k] = ok[r];
         }
     }
     lap("pseudo-inverses");
     c.n_solved = solved;
+    if (c.ideg2 > 0) build_cweno(c);
     for (int64_t k = 0; k < ns; ++k)
         if (!ok[k]) c.full[k / TS] = 0;  // NaN stencils disable FULL (src/prestore.f90:526-529)
 }
@@ -1678,6 +1786,22 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
         return fail(UCNS_EBADARG, "mesh too small for the stencil size");
     }
     c->imax = c->nnb - 1;
+    if (o.ees == 5) {  // CWENO sectors (DETERMINE_SIZE, src/grid_p.f90:6151-6163)
+        if (o.iweno != 1) {
+            delete c;
+            return fail(UCNS_EBADARG, "CWENO (ees 5) needs the WENO path");
+        }
+        c->iorder2 = o.iorder <= 3 ? 1 : 2;
+        c->ideg2 = ndeg_of(c->iorder2);
+        c->imax2 = (c->iorder2 == 1 ? 12 : 20) - 1;
+        if (c->imax2 > c->imax) {
+            delete c;
+            return fail(UCNS_EUNSUPPORTED, "the sector stencils are shorter than the low-order CWENO stencils");
+        }
+    } else if (o.ees != 0) {
+        delete c;
+        return fail(UCNS_EUNSUPPORTED, "ees must be 0 (WENO) or 5 (CWENO)");
+    }
     c->igq = std::min(o.iorder, 6);
     c->qp_quad = quad_rule(c->igq, c->quad_rs, c->quad_w);
     c->qp_tri = tri_rule(c->igq, c->tri_bary, c->tri_w);
@@ -1736,7 +1860,7 @@ int ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci
     s->idegfree = c->ideg;
     s->inumneighbours = c->nnb;
     s->typesten = c->typesten;
-    s->ees = 0;
+    s->ees = c->ideg2 > 0 ? 5 : 0;
     s->wenoz = 0;
     s->wenwrt = 1;
     s->limiter = 1;
@@ -1816,6 +1940,14 @@ int ucnsp_array(const ucnsp_case *c, const char *name, void **ptr, int64_t *coun
     if (n == "frame") return D(c->frame);
     if (n == "quad_w") return D(c->quad_w);
     if (n == "tri_w") return D(c->tri_w);
+    if (n == "integ_basis_c") return D(c->integ_basis_c);
+    if (n == "indicator_c") return D(c->indicator_c);
+    if (n == "invmat_c") {
+        *ptr = (void *)c->invmat_c.p;
+        *count = (int64_t)c->invmat_c.n;
+        *dtype = 1;
+        return UCNS_OK;
+    }
     if (n == "invmat") {
         *ptr = (void *)c->invmat.p;
         *count = (int64_t)c->invmat.n;
@@ -1864,6 +1996,19 @@ int ucnsp_init_vortex(const ucnsp_case *c, double gamma, double noise_amp, uint6
             u[i * 5 + 4] = sp / vol / (gamma - 1.0) + sk / vol;
         }
     }
+    return UCNS_OK;
+}
+
+int ucnsp_fill_cweno(const ucnsp_case *c, ucns_cweno_v1 *cw) {
+    if (!c || !cw) return fail(UCNS_EBADARG, "null argument");
+    if (c->ideg2 == 0) return fail(UCNS_EBADARG, "the case was built without CWENO (ees 5)");
+    cw->abi_version = UCNS_ABI_VERSION;
+    cw->idegfree2 = c->ideg2;
+    cw->inumneighbours2 = c->imax2 + 1;
+    cw->reserved = 0;
+    cw->invmat_c = c->invmat_c.p;
+    cw->indicator_c = c->indicator_c.data();
+    cw->integ_basis_c = c->integ_basis_c.data();
     return UCNS_OK;
 }
 

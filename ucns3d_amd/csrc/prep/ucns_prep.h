@@ -59,6 +59,7 @@ typedef struct ucnsp_opts {
                                1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield */
     double  bump[8];        /* lower-wall bump a_k: with x^ = 4x/lx, z^ = z/lz, the nodes move by
                                dy = ly (1 - y/ly) sum_k a_k sin(k pi (x^-1)/2) sin(pi z^) for 1 <= x^ <= 3 */
+    int32_t ees;            /* 0 WENO; 5 CWENO: also the low-order sector data (ucnsp_fill_cweno) */
 } ucnsp_opts;
 
 typedef struct ucnsp_case ucnsp_case;
@@ -68,6 +69,8 @@ int  ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out);
  * until ucnsp_free).  precision/gamma/lwci1 are the caller's scheme choices. */
 int  ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1,
                 ucns_mesh_v2 *mesh, ucns_scheme_v2 *scheme, ucns_prestore_v1 *pre);
+/* The CWENO arrays of a case built with ees 5 (pointers into its storage). */
+int  ucnsp_fill_cweno(const ucnsp_case *c, ucns_cweno_v1 *cw);
 /* Named internal arrays for tests and drivers: "xyz", "xyz_box" (before
  * the bump), "cell_nodes", "centre", "ihexl", "anglex", "angley", ... ;
  * dtype 0=int32, 1=float64. */

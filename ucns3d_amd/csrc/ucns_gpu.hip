@@ -504,7 +504,19 @@ struct ReconWParams {
     const int32_t *uq, *ucount;
     const int64_t *toff;      // first row of each tile's member table in tab
     double *tab;              // [rows][5] the member tables, summed per cell by k_adj_gather
+    // CWENO (ees 5): the sectors' low-order indicators (packed, [ncell][6]) and
+    // the differences of the basis means mbar - mbar_c over their 3
+    // coefficients ([ncell][3]); the sectors' P rows hold their low-order
+    // pseudo-inverses (zero past 3 coefficients and imax2 members)
+    const R *Bc;
+    const R *dmb;
+    int wenoz;
 };
+
+template <typename T>
+__device__ inline T lane_bcast(T v, int src) {
+    return __shfl(v, src, WAVE);
+}
 
 template <typename T>
 __device__ inline T grp_sum8(T v) {
@@ -551,7 +563,7 @@ __device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int la
 }
 
 // STORE: also write every stencil's coefficients to p.gstore (the adjoint's forward pass)
-template <typename R, int IDEG, int NQL, bool STORE = false>
+template <typename R, int IDEG, int NQL, bool STORE = false, bool CW = false>
 __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG][NV], const double (&ui)[NV],
                                            const R (*sB)[IDEG * (IDEG + 1) / 2], R (*sG0)[IDEG * NV], int lc, int s,
                                            int cell, bool cvalid, int info, bool act) {
@@ -574,6 +586,85 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     }
     __syncthreads();
 
+    R om[NV];
+    R corr[NV] = {R(0), R(0), R(0), R(0), R(0)};  // CWENO: the sectors' low-order means
+    if constexpr (CW) {
+        // ---- CWENO (src/reconstruct_p.f90:955-1029): lambda_1 = 1 - 1/LWCI1 for
+        // the central stencil, (1 - lambda_1)/(ADMIS-1) for each sector; lane 0
+        // turns its central polynomial into the zero polynomial
+        // (G_1 - sum_l lambda_l G_l) / lambda_1 (the sectors' coefficients are
+        // zero past the first three) ----
+        const int admis = ci_admis(info);
+        const double lam0 = 1.0 - (1.0 / p.lwci1);
+        const double laml = admis > 1 ? (1.0 - lam0) / (admis - 1) : 0.0;
+#pragma unroll
+        for (int k = 0; k < IDEG; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const R sl = grp_sum8((s >= 1 && act) ? (R)laml * g[k][v] : R(0));
+                if (s == 0) g[k][v] = (R)(1.0 / lam0) * (g[k][v] - sl);
+            }
+        // indicators: the zero polynomial's with INDICATOR, a sector's with INDICATORC
+        double bd[NV] = {0, 0, 0, 0, 0};
+        if (act && s == 0) {
+#pragma unroll
+            for (int k = 0; k < IDEG; ++k) {
+                R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+                for (int l = k + 1; l < IDEG; ++l) {
+                    const R bkl = sB[lc][pidx(IDEG, k, l)];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
+                }
+                const R bkk = sB[lc][pidx(IDEG, k, k)];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) bd[v] += (double)(g[k][v] * (bkk * g[k][v] + R(2) * bg[v]));
+            }
+        } else if (act) {
+            const R *bc = p.Bc + (size_t)cell * 6;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R bg[NV] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+                for (int l = k + 1; l < 3; ++l) {
+                    const R bkl = bc[pidx(3, k, l)];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) bg[v] += bkl * g[l][v];
+                }
+                const R bkk = bc[pidx(3, k, k)];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) bd[v] += (double)(g[k][v] * (bkk * g[k][v] + R(2) * bg[v]));
+            }
+        }
+        // weights in FP64 in both precisions: WENO (power 4) or WENO-Z with
+        // tau = sum_l |beta_1 - beta_l| / (ADMIS-1)
+        const double lam = s == 0 ? lam0 : laml;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double b0 = lane_bcast(bd[v], lc * 8);
+            double w = 0.0;
+            if (p.wenoz) {
+                const double tau = admis > 1 ? grp_sum8((s >= 1 && act) ? fabs(b0 - bd[v]) : 0.0) / (admis - 1) : 0.0;
+                const double r = tau / (p.eps + bd[v]);
+                const double r2 = r * r;
+                w = lam * (1.0 + r2 * r2);
+            } else {
+                const double x = p.eps + bd[v];
+                const double x2 = x * x;
+                w = lam / (x2 * x2);
+            }
+            w = act ? w : 0.0;
+            const double sum = grp_sum8(w);
+            om[v] = act ? (R)(w / sum) : R(0);
+        }
+        // the sectors' polynomials use the means of VALUEC: their weighted
+        // coefficients times (mbar - mbar_c) shift every face value alike
+        const R *dm = p.dmb + (size_t)(cvalid ? cell : 0) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) corr[v] += dm[k] * grp_sum8((s >= 1 && act) ? om[v] * g[k][v] : R(0));
+    } else {
     // ---- smoothness indicators beta = g^T (B g) (src/reconstruct_p.f90:995-1004),
     // over the packed triangle of the symmetric B:
     //   beta = sum_k g_k (B_kk g_k + 2 sum_{l>k} B_kl g_l) ----
@@ -596,7 +687,6 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     // ---- weights (src/reconstruct_p.f90:1031-1049) and the weighted sum ----
     const R lam = (s == 0) ? (R)p.lwci1 : R(1);
     const R eps = (R)p.eps;
-    R om[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         R w;
@@ -623,6 +713,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
             gs[(size_t)v * TP] = beta[v];
             gs[(size_t)(NV + v) * TP] = om[v];
         }
+    }
     }
 #pragma unroll
     for (int k = 0; k < IDEG; ++k)
@@ -657,7 +748,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
         double uq[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            R res = R(0);
+            R res = corr[v];
 #pragma unroll
             for (int k = 0; k < IDEG; ++k) res += phi[k] * g[k][v];
             out[q * NV + v] = res;
@@ -752,7 +843,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     if (STORE && s == 0 && cvalid) p.fstore[cell] = (reduce ? 1 : 0) | (first ? 2 : 0);
 }
 
-template <typename R, int IDEG, int IMAX, int NQL, int D>
+template <typename R, int IDEG, int IMAX, int NQL, int D, bool CW = false>
 __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
     constexpr int NP = IDEG * (IDEG + 1) / 2;
     __shared__ R sB[8][NP];
@@ -833,7 +924,7 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
             }
         }
     }
-    recon_late<R, IDEG, NQL>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+    recon_late<R, IDEG, NQL, false, CW>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -1221,6 +1312,14 @@ __global__ void k_stream_idx(const int32_t *idx, unsigned char *stream, int ntil
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE) void k_recon_w64(ReconWParams<double> p) {
     recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64>(p);
+}
+template <int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE) void k_recon_cw64(ReconWParams<double> p) {
+    recon_w_body<double, IDEG, IMAX, NQL, UCNS_DEPTH64, true>(p);
+}
+template <int IDEG, int IMAX, int NQL>
+__global__ __launch_bounds__(WAVE, UCNS_WAVES32) void k_recon_cw32(ReconWParams<float> p) {
+    recon_w_body<float, IDEG, IMAX, NQL, UCNS_DEPTH32, true>(p);
 }
 template <int IDEG, int IMAX, int NQL>
 __global__ __launch_bounds__(WAVE, UCNS_WAVES32) void k_recon_w32(ReconWParams<float> p) {
@@ -1954,10 +2053,6 @@ struct VjpParams {
 
 __device__ inline void atomic_add_f64(double *a, double v) { unsafeAtomicAdd(a, v); }
 
-template <typename T>
-__device__ inline T lane_bcast(T v, int src) {
-    return __shfl(v, src, WAVE);
-}
 
 // Distinct members per tile: a wave's scatter goes through an LDS table of
 // the tile's distinct member cells (a tile of 8 neighbouring cells shares
@@ -3287,6 +3382,13 @@ struct ucns_ctx {
     int32_t *perm = nullptr;    // device copy of h_perm, or null
     double *stage = nullptr;    // [ncell][5] staging buffer of host transfers
     void *scratch = nullptr;    // this device's address of the static scratch array
+    // CWENO (ees 5): scheme switches, the caller's low-order data by library
+    // cell (ucns_upload_cweno), and on the device the sectors' packed
+    // indicators and the means' differences mbar - mbar_c
+    int ees = 0, wenoz = 0, ideg2 = 0, imax2 = 0;
+    bool have_cweno = false;
+    std::vector<double> h_cwP, h_cwB, h_cwIB;
+    void *Bc = nullptr, *dmb = nullptr;
     // library options of the scheme
     int kernels = UCNS_KERNELS_AUTO;
     int adj_keep_stages = 0;
@@ -3470,6 +3572,28 @@ int launch_recon_t(ucns_ctx *ctx) {
                     hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 5>), grid, block, 0, ctx->stream, w);
                 else
                     hipLaunchKernelGGL((k_recon_s32<9, 19, 3, 7>), grid, block, 0, ctx->stream, w);
+            }
+            return UCNS_OK;
+        }
+        if (ctx->ees == 5) {
+            // CWENO on the register kernels
+            w.Bc = (const R *)ctx->Bc;
+            w.dmb = (const R *)ctx->dmb;
+            w.wenoz = ctx->wenoz;
+            if constexpr (sizeof(R) == 8) {
+                if (ctx->ideg == 9)
+                    hipLaunchKernelGGL((k_recon_cw64<9, 19, 3>), grid, block, 0, ctx->stream, w);
+                else if (ctx->nf * ctx->nqm <= 24)
+                    hipLaunchKernelGGL((k_recon_cw64<19, 39, 3>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_cw64<19, 39, 7>), grid, block, 0, ctx->stream, w);
+            } else {
+                if (ctx->ideg == 9)
+                    hipLaunchKernelGGL((k_recon_cw32<9, 19, 3>), grid, block, 0, ctx->stream, w);
+                else if (ctx->nf * ctx->nqm <= 24)
+                    hipLaunchKernelGGL((k_recon_cw32<19, 39, 3>), grid, block, 0, ctx->stream, w);
+                else
+                    hipLaunchKernelGGL((k_recon_cw32<19, 39, 7>), grid, block, 0, ctx->stream, w);
             }
             return UCNS_OK;
         }
@@ -3761,6 +3885,10 @@ int ensure_adjoint_steps(ucns_ctx *ctx) {
 // what the earlier ones allocated, so the next call starts over and returns
 // the error again instead of running kernels on missing buffers.
 int ensure_adjoint(ucns_ctx *ctx) {
+    if (ctx->ees == 5) {
+        ctx->err = "the adjoint does not cover CWENO (ees 5)";
+        return UCNS_EUNSUPPORTED;
+    }
     if (ctx->adj_ready) return UCNS_OK;
     release_adjoint(ctx);
     const int rc = ensure_adjoint_steps(ctx);
@@ -4170,6 +4298,9 @@ void release_prestore(ucns_ctx *ctx) {
     dfree(ctx->u64);
     dfree(ctx->B);
     dfree(ctx->mbar);
+    if (ctx->Bc) (void)hipFree(ctx->Bc);
+    if (ctx->dmb) (void)hipFree(ctx->dmb);
+    ctx->Bc = ctx->dmb = nullptr;
     dfree(ctx->qp);
     ctx->have_prestore = false;
 }
@@ -4424,8 +4555,10 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
         return bad(UCNS_EBADARG, "null mesh array");
     if (s->precision != UCNS_FP64 && s->precision != UCNS_MIXED) return bad(UCNS_EBADARG, "bad precision");
     if (s->iweno != 1 && s->iweno != -1) return bad(UCNS_EUNSUPPORTED, "iweno must be 1 (WENO) or -1 (MUSCL)");
-    if (s->ees != 0) return bad(UCNS_EUNSUPPORTED, "only EES=0 stencils are supported");
-    if (s->wenoz != 0) return bad(UCNS_EUNSUPPORTED, "wenoz=1 is not supported");
+    if (s->ees != 0 && s->ees != 5) return bad(UCNS_EUNSUPPORTED, "ees must be 0 (WENO) or 5 (CWENO)");
+    if (s->ees == 5 && s->iweno != 1) return bad(UCNS_EBADARG, "CWENO (ees 5) needs iweno 1");
+    if (s->wenoz != 0 && s->wenoz != 1) return bad(UCNS_EBADARG, "wenoz must be 0 or 1");
+    if (s->wenoz == 1 && s->ees != 5) return bad(UCNS_EUNSUPPORTED, "wenoz 1 is a CWENO option (ees 5)");
     if (s->wenwrt != 1) return bad(UCNS_EUNSUPPORTED, "only conserved-variable reconstruction (wenwrt=1)");
     if (s->limiter != 1) return bad(UCNS_EUNSUPPORTED, "only the Barth-Jespersen limiter (1)");
     if (s->riemann != 1) return bad(UCNS_EUNSUPPORTED, "only the HLLC solver (1)");
@@ -4450,6 +4583,8 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     ctx->index_base = m->index_base;
     ctx->precision = s->precision;
     ctx->iweno = s->iweno;
+    ctx->ees = s->ees;
+    ctx->wenoz = s->wenoz;
     ctx->ideg = s->idegfree;
     ctx->nnb = s->inumneighbours;
     ctx->imax = s->inumneighbours - 1;
@@ -4649,6 +4784,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         ctx->err = "WENO needs the indicator matrices";
         return UCNS_EBADARG;
     }
+    if (ctx->ees == 5 && !ctx->have_cweno) {
+        ctx->err = "CWENO (ees 5): call ucns_upload_cweno before ucns_upload_prestore";
+        return UCNS_ESTATE;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     release_prestore(ctx);
     release_adjoint(ctx);  // its buffers are sized by the stencil count
@@ -4675,6 +4814,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     const bool mixed = ctx->precision == UCNS_MIXED;
     ctx->tiled = (ctx->iweno == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) && ts <= 8 &&
                   ctx->nf * ctx->nqm <= 56);
+    if (ctx->ees == 5 && !ctx->tiled) {
+        ctx->err = "CWENO (ees 5) runs on the WENO-3 / WENO-4 register kernels (idegfree 9 or 19)";
+        return UCNS_EUNSUPPORTED;
+    }
     {
         // row-stream kernels: tetrahedra (5 stencils, 24 face points) and
         // hexahedra (7 stencils, 54 face points), unless the scheme asks for
@@ -4682,7 +4825,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         const int nfq = ctx->nf * ctx->nqm;
         const bool shape_ok = ID == 19 ? ((ts == 5 && nfq <= 24) || (ts == 7 && nfq > 24))
                                        : ((ts == 5 || ts == 7) && nfq <= 24);
-        const bool stream_ok = ctx->kernels == UCNS_KERNELS_AUTO && ctx->tiled && shape_ok;
+        const bool stream_ok = ctx->kernels == UCNS_KERNELS_AUTO && ctx->tiled && shape_ok && ctx->ees != 5;
         ctx->smode = stream_ok ? ts : 0;
     }
     ctx->ntile = (nc + 7) / 8;
@@ -4761,16 +4904,29 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             hipSuccess)
             host_pin = nullptr;
         std::vector<double> host_buf;  // pageable staging when pinned memory is refused and cells move
-        if (!host_pin && !ctx->h_perm.empty()) host_buf.resize(chunk_cells * per_cell_abi);
+        if (!host_pin && (!ctx->h_perm.empty() || ctx->ees == 5)) host_buf.resize(chunk_cells * per_cell_abi);
         double *gather = host_pin ? host_pin : host_buf.data();
         for (size_t c0 = 0; c0 < (size_t)nc; c0 += chunk_cells) {
             const size_t cn = std::min(chunk_cells, (size_t)nc - c0);
             const size_t bytes = cn * per_cell_abi * sizeof(double);
             const double *src = pre->invmat + c0 * per_cell_abi;
-            if (!ctx->h_perm.empty()) {  // the chunk's cells from their places in the caller's order
+            if (!ctx->h_perm.empty() || ctx->ees == 5) {  // the chunk's cells from their places in the caller's order
                 for (size_t c = 0; c < cn; ++c)
                     std::memcpy(gather + c * per_cell_abi, pre->invmat + src_cell(ctx, (int)(c0 + c)) * per_cell_abi,
                                 per_cell_abi * sizeof(double));
+                if (ctx->ees == 5) {
+                    // CWENO: a sector's rows are its low-order pseudo-inverse, zero
+                    // past idegfree2 coefficients and imax2 members
+                    const int ID2 = ctx->ideg2, IM2 = ctx->imax2;
+                    for (size_t c = 0; c < cn; ++c)
+                        for (int st = 1; st < TSA; ++st) {
+                            double *Pd = gather + c * per_cell_abi + (size_t)st * IM * ID;
+                            const double *Pc = ctx->h_cwP.data() + (((c0 + c) * TSA + st) * IM2) * ID2;
+                            std::fill(Pd, Pd + (size_t)IM * ID, 0.0);
+                            for (int j = 0; j < IM2; ++j)
+                                for (int k = 0; k < ID2; ++k) Pd[(size_t)j * ID + k] = Pc[(size_t)j * ID2 + k];
+                        }
+                }
                 src = gather;
             } else if (host_pin) {
                 std::memcpy(host_pin, src, bytes);
@@ -4838,6 +4994,22 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[o * ID + d] * oov;
         }
         if ((rc = upload_real_any(ctx, &ctx->mbar, mb))) return rc;
+        if (ctx->ees == 5) {
+            // the sectors' indicators (packed) and mbar - mbar_c over their coefficients
+            const int ID2 = ctx->ideg2;
+            std::vector<double> bc((size_t)nc * 6), dm((size_t)nc * 3);
+            for (int i = 0; i < nc; ++i) {
+                const double *Bi = ctx->h_cwB.data() + (size_t)i * ID2 * ID2;
+                size_t e = (size_t)i * 6;
+                for (int k = 0; k < 3; ++k)
+                    for (int l = k; l < 3; ++l) bc[e++] = Bi[(size_t)k * ID2 + l];
+                const size_t o = src_cell(ctx, i);
+                const double oov = 1.0 / pre->volume11[o];
+                for (int d = 0; d < 3; ++d)
+                    dm[(size_t)i * 3 + d] = mb[(size_t)i * ID + d] - ctx->h_cwIB[(size_t)i * ID2 + d] * oov;
+            }
+            if ((rc = upload_real_any(ctx, &ctx->Bc, bc)) || (rc = upload_real_any(ctx, &ctx->dmb, dm))) return rc;
+        }
     }
     // Gauss points in the local frames: [cell][nf][nqm][3]
     {
@@ -4855,6 +5027,44 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     ctx->st.bytes_recon_per_cell =
         (double)((size_t)IM * ts * ID * (mixed ? 4 : 8)) + (ctx->iweno == 1 ? (double)ID * (ID + 1) / 2 * (mixed ? 4 : 8) : 0) +
         (double)ts * IM * 4 + (double)ctx->nf * ctx->nqm * 3 * (mixed ? 4 : 8) + (double)ID * (mixed ? 4 : 8);
+    return UCNS_OK;
+}
+
+int ucns_upload_cweno(ucns_ctx *ctx, const ucns_cweno_v1 *cw) {
+    if (!ctx || !cw) return UCNS_EBADARG;
+    if (cw->abi_version != UCNS_ABI_VERSION) {
+        ctx->err = "ucns_cweno_v1: abi_version mismatch";
+        return UCNS_EBADARG;
+    }
+    if (ctx->ees != 5) {
+        ctx->err = "ucns_upload_cweno: the scheme is not CWENO (ees 5)";
+        return UCNS_ESTATE;
+    }
+    if (!cw->invmat_c || !cw->indicator_c || !cw->integ_basis_c) {
+        ctx->err = "null CWENO array";
+        return UCNS_EBADARG;
+    }
+    // IORDER <= 3 (the kernels' idegfree 9 and 19) take linear sectors
+    if (cw->idegfree2 != 3 || cw->inumneighbours2 < 4 || cw->inumneighbours2 > ctx->nnb) {
+        ctx->err = "CWENO sectors: idegfree2 must be 3 (IORDER2 = 1) and inumneighbours2 in [4, inumneighbours]";
+        return UCNS_EUNSUPPORTED;
+    }
+    const int nc = ctx->ncell, TSA = ctx->ts_abi, ID2 = cw->idegfree2, IM2 = cw->inumneighbours2 - 1;
+    const size_t pc = (size_t)TSA * IM2 * ID2;
+    ctx->h_cwP.resize((size_t)nc * pc);
+    ctx->h_cwB.resize((size_t)nc * ID2 * ID2);
+    ctx->h_cwIB.resize((size_t)nc * ID2);
+    for (int i = 0; i < nc; ++i) {
+        const size_t o = src_cell(ctx, i);
+        std::memcpy(ctx->h_cwP.data() + (size_t)i * pc, cw->invmat_c + o * pc, pc * sizeof(double));
+        std::memcpy(ctx->h_cwB.data() + (size_t)i * ID2 * ID2, cw->indicator_c + o * ID2 * ID2,
+                    (size_t)ID2 * ID2 * sizeof(double));
+        std::memcpy(ctx->h_cwIB.data() + (size_t)i * ID2, cw->integ_basis_c + o * ID2, (size_t)ID2 * sizeof(double));
+    }
+    ctx->ideg2 = ID2;
+    ctx->imax2 = IM2;
+    ctx->have_cweno = true;
+    ctx->have_prestore = false;  // the pseudo-inverses are laid out again by ucns_upload_prestore
     return UCNS_OK;
 }
 

@@ -48,6 +48,14 @@ module ucns_gpu
     real(c_double) :: inflow(5) = 0.0d0, outflow(5) = 0.0d0
   end type
 
+  ! CWENO data (ees = 5): the sectors' low-order pseudo-inverses, indicators
+  ! and basis integrals, c_loc() of the host's packed arrays
+  type, bind(c) :: ucns_cweno_v1
+    integer(c_int32_t) :: abi_version = UCNS_ABI_VERSION
+    integer(c_int32_t) :: idegfree2 = 3, inumneighbours2 = 12, reserved = 0
+    type(c_ptr) :: invmat_c = c_null_ptr, indicator_c = c_null_ptr, integ_basis_c = c_null_ptr
+  end type
+
   ! cotangents of the mesh/PRESTORE data (ucns_rk3_adjoint_geometry): c_loc()
   ! of the host's arrays in the layouts of ucns_prestore_v1 / ucns_mesh_v2, or
   ! c_null_ptr for one not wanted
@@ -83,6 +91,11 @@ module ucns_gpu
       import :: ucns_prestore_v1, c_int, c_ptr
       type(c_ptr), value :: ctx
       type(ucns_prestore_v1), intent(in) :: pre
+    end function
+    integer(c_int) function ucns_upload_cweno(ctx, cw) bind(c, name="ucns_upload_cweno")
+      import :: c_int, c_ptr, ucns_cweno_v1
+      type(c_ptr), value :: ctx
+      type(ucns_cweno_v1), intent(in) :: cw
     end function
     integer(c_int) function ucns_set_boundary(ctx, bc) bind(c, name="ucns_set_boundary")
       import :: ucns_boundary_v1, c_int, c_ptr

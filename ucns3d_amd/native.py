@@ -63,6 +63,13 @@ class BoundaryV1(C.Structure):
     ]
 
 
+class CwenoV1(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("idegfree2", C.c_int32), ("inumneighbours2", C.c_int32),
+        ("reserved", C.c_int32), ("invmat_c", _f64p), ("indicator_c", _f64p), ("integ_basis_c", _f64p),
+    ]
+
+
 class GeometryBarV1(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("reserved", C.c_int32),
@@ -89,6 +96,7 @@ class PrepOpts(C.Structure):
         ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double), ("jitter", C.c_double),
         ("seed", C.c_uint64), ("iorder", C.c_int32), ("iweno", C.c_int32), ("extf", C.c_int32),
         ("weight_lsqr", C.c_int32), ("nthreads", C.c_int32), ("bc", C.c_int32 * 6), ("bump", C.c_double * 8),
+        ("ees", C.c_int32),
     ]
 
 
@@ -96,6 +104,7 @@ class PrepOpts(C.Structure):
 GPU_API = {
     "ucns_create": (C.c_int, [C.POINTER(MeshV2), C.POINTER(SchemeV2), C.POINTER(C.c_void_p)]),
     "ucns_upload_prestore": (C.c_int, [C.c_void_p, C.POINTER(PrestoreV1)]),
+    "ucns_upload_cweno": (C.c_int, [C.c_void_p, C.POINTER(CwenoV1)]),
     "ucns_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(BoundaryV1)]),
     "ucns_set_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_get_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
@@ -136,6 +145,7 @@ PREP_API = {
     "ucnsp_init_vortex": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_uint64, _f64p]),
     "ucnsp_face_rule": (C.c_int, [C.c_int32, C.c_int32, _f64p, _f64p, C.POINTER(C.c_int32)]),
     "ucnsp_shape_table": (C.c_int, [C.c_int32, C.POINTER(C.c_int32)]),
+    "ucnsp_fill_cweno": (C.c_int, [C.c_void_p, C.POINTER(CwenoV1)]),
     "ucnsp_move_nodes": (C.c_int, [C.c_void_p, _f64p]),
     "ucnsp_free": (None, [C.c_void_p]),
     "ucnsp_last_error": (C.c_char_p, []),

@@ -37,7 +37,8 @@ class Solver:
     a caller order of the cells of one's own."""
 
     def __init__(self, case, precision: int = UCNS_FP64, lib_path: str | None = None, cell_order: int = 0,
-                 kernels: int = 0, adj_keep_stages: int = 0, adj_keep_gib: float = 0.0, structs=None):
+                 kernels: int = 0, adj_keep_stages: int = 0, adj_keep_gib: float = 0.0, structs=None,
+                 cweno=None):
         self.case = case
         self.precision = precision
         self.L = native.gpu_lib(lib_path)
@@ -50,6 +51,9 @@ class Solver:
         rc = self.L.ucns_create(C.byref(self._m), C.byref(self._s), C.byref(h))
         self._h = h
         self._check("ucns_create", rc)
+        self._cw = cweno if cweno is not None else (case.cweno() if structs is None else None)
+        if self._cw is not None:
+            self._check("ucns_upload_cweno", self.L.ucns_upload_cweno(self._h, C.byref(self._cw)))
         rc = self.L.ucns_upload_prestore(self._h, C.byref(self._p))
         self._check("ucns_upload_prestore", rc)
         self.ncell = case.ncell
