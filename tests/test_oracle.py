@@ -261,18 +261,22 @@ def test_rk3_is_the_shu_osher_composition(co):
 
 # ---------------------------------------------------------------- polynomial reproduction
 
-def interior_cells(case, m, exps):
+def interior_cells(case, m, exps, candidates=None, limit=None):
     """cells whose every stencil member sits less than half a period away
     (there a global polynomial and its cell averages are single-valued) and
     whose every stencil has a unique least-squares fit.  Rank-deficient
     directional stencils occur on unjittered hexahedra, where the cone test
-    settles ties by rounding; the WENO weights do not fully suppress them."""
+    settles ties by rounding; the WENO weights do not fully suppress them.
+    candidates: the cells to consider (all by default); limit: stop after
+    that many accepted cells"""
     ts, nnb = case.typesten, case.nnb
     ih = case.array("ihexl").reshape(case.ncell, ts, nnb)
     adm = case.array("admis")
     out, deficient = [], 0
     per = m.L > 0  # periodic axes
-    for i in range(case.ncell):
+    for i in (range(case.ncell) if candidates is None else candidates):
+        if limit is not None and len(out) >= limit:
+            break
         mem = np.unique(ih[i, :adm[i]].ravel())
         d = np.abs(m.centre[mem] - m.centre[i])
         if not np.all(d[:, per] < 0.45 * m.L[per]):
@@ -302,6 +306,16 @@ def interior_cells(case, m, exps):
     ("hybrid-weno4-cubic", lambda: CaseSpec("hybrid", HYBRID, 9, "weno4", jitter=0.1, seed=21), 3),
     # the adjoint channel: prism wall layers, pyramids and tetrahedra, the bumped wall
     ("channel-weno3-quadratic", lambda: reduced("c4", 4), 2),
+    # WENO-5: quartic bases (34 coefficients, 70-cell stencils), 16-point
+    # quadrilateral and 10-point triangle face rules
+    # (walled boxes where a periodic one would put every stencil across the seam)
+    ("hex-weno5-quartic", lambda: CaseSpec("hex-weno5", HEX, 16, "weno5", seed=5), 4),
+    ("tet-weno5-quartic", lambda: CaseSpec("tet-weno5", TET, 6, "weno5", jitter=0.1, seed=5, bc=(4,) * 6), 4),
+    ("prism-weno5-quartic", lambda: CaseSpec("prism-weno5", PRISM, 12, "weno5", jitter=0.1, seed=13, bc=(4,) * 6), 4),
+    ("pyramid-weno5-quartic", lambda: CaseSpec("pyramid-weno5", PYRAMID, 6, "weno5", jitter=0.1, seed=17,
+                                               bc=(4,) * 6), 4),
+    ("hybrid-weno5-quartic", lambda: CaseSpec("hybrid-weno5", HYBRID, 12, "weno5", jitter=0.1, seed=21,
+                                              bc=(4,) * 6), 4),
 ])
 def test_reconstruction_reproduces_polynomials(name, spec, deg):
     """Cell averages of a polynomial of the scheme's degree are reconstructed
@@ -324,12 +338,20 @@ def test_reconstruction_reproduces_polynomials(name, spec, deg):
             u[i, v] = base[v] + amp[v] * (polys[v](x) * w).sum() / w.sum()
     ul, cnt = orc.reconstruct(u)
     assert cnt[1] == 0
-    cells, deficient = interior_cells(case, m, monomial_exponents(case.ideg))
+    cand = np.arange(case.ncell)
     if deg >= 2:
         # cells with fewer complete stencils than faces + 1 (FULL = 0) take the
         # MUSCL path, which is exact for linears only
-        full = case.array("full")
-        cells = [i for i in cells if full[i] == 1]
+        cand = cand[case.array("full")[cand] == 1]
+    if deg >= 4:
+        # quartic stencils (70 cells): 48 cells accepted from a spread sample
+        # keep the least-squares rank checks to seconds
+        ih = case.array("ihexl").reshape(case.ncell, case.typesten, case.nnb)
+        per = m.L > 0
+        d = np.abs(m.centre[ih[cand]] - m.centre[cand][:, None, None])  # unused stencil slots repeat members
+        cand = cand[np.all(d[..., per] < 0.45 * m.L[per], axis=(1, 2, 3))]
+        cand = np.random.default_rng(3).permutation(cand)
+    cells, deficient = interior_cells(case, m, monomial_exponents(case.ideg), cand, 48 if deg >= 4 else None)
     assert len(cells) >= 8, (len(cells), deficient)
     if case.spec.jitter > 0:
         assert deficient == 0
@@ -354,7 +376,10 @@ def test_reconstruction_reproduces_polynomials(name, spec, deg):
                 psi = (got - u[i, v]) @ dev / (dev @ dev)
                 assert -1e-9 <= psi <= 1 + 1e-9, psi
                 worst = max(worst, np.abs(got - u[i, v] - psi * dev).max() / amp[v])
-    assert worst < 1e-9, worst
+    # relative to the polynomial's amplitude (2e-3 of the state): 1e-9 is
+    # about 1e3 ulps of the state; the 70-member quartic fits on the jittered
+    # hybrid mesh are worse conditioned and reach 1.1e-9
+    assert worst < (1e-9 if deg <= 3 else 3e-9), worst
 
 
 @pytest.mark.parametrize("name", list(SPECS))
@@ -546,6 +571,10 @@ def wave_averages(m, k, u0):
     (TET, "weno4", (4, 8), 2.7),
     (HEX, "muscl", (8, 16), 1.4),  # Barth clips the slope at every extremum
     (HEX, "weno3", (12, 24), 2.4),
+    # WENO-5 (quartic): 4.86 on hexahedra; tetrahedra at 6 and 12 per side are
+    # short of the asymptotic range (3.95)
+    (HEX, "weno5", (12, 24), 4.3),
+    (TET, "weno5", (6, 12), 3.5),
 ])
 def test_residual_converges_at_the_scheme_order(shape, scheme, ns, order):
     """-RHS/V approximates d/dt of the exact cell averages of a smooth

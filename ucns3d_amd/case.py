@@ -29,7 +29,7 @@ class CaseSpec:
     name: str
     shape: int            # HEX, TET (6 Kuhn tets per hex), PRISM (2 per hex), PYRAMID (6 per hex) or HYBRID
     n: int                # hexes per direction
-    scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2) or "weno4" (IORDER 3)
+    scheme: str           # "muscl" (2nd order, Barth), "weno3" (IORDER 2), "weno4" (IORDER 3) or "weno5" (IORDER 4)
     jitter: float = 0.0   # interior vertex jitter, fraction of the spacing
     seed: int = 1
     noise: float = 0.0    # N(0,1) density perturbation amplitude per cell
@@ -47,7 +47,7 @@ class CaseSpec:
 
     @property
     def iorder(self) -> int:
-        return {"muscl": 1, "weno3": 2, "weno4": 3}[self.scheme]
+        return {"muscl": 1, "weno3": 2, "weno4": 3, "weno5": 4}[self.scheme]
 
     @property
     def iweno(self) -> int:

@@ -174,12 +174,66 @@ int quad_rule(int n, std::vector<double> &rs, std::vector<double> &w) {
     return n * n;
 }
 
+// Newton on a small square system f(v) = 0 (forward-difference Jacobian,
+// Gaussian elimination with partial pivoting), for the orbit parameters of
+// the symmetric triangle rules below.
+template <int N, class Fn>
+void newton_solve(double (&v)[N], Fn F) {
+    for (int it = 0; it < 60; ++it) {
+        double f0[N];
+        F(v, f0);
+        double M[N][N + 1];
+        for (int j = 0; j < N; ++j) {
+            double vp[N];
+            for (int k = 0; k < N; ++k) vp[k] = v[k];
+            const double h = 1e-7 * std::max(1.0, std::fabs(v[j]));
+            vp[j] += h;
+            double f1[N];
+            F(vp, f1);
+            for (int i = 0; i < N; ++i) M[i][j] = (f1[i] - f0[i]) / h;
+        }
+        for (int i = 0; i < N; ++i) M[i][N] = -f0[i];
+        for (int c = 0; c < N; ++c) {
+            int p = c;
+            for (int r = c + 1; r < N; ++r)
+                if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+            for (int j = 0; j <= N; ++j) std::swap(M[c][j], M[p][j]);
+            for (int r = c + 1; r < N; ++r) {
+                const double m = M[r][c] / M[c][c];
+                for (int j = c; j <= N; ++j) M[r][j] -= m * M[c][j];
+            }
+        }
+        double dx[N], nrm = 0.0;
+        for (int r = N - 1; r >= 0; --r) {
+            double s = M[r][N];
+            for (int j = r + 1; j < N; ++j) s -= M[r][j] * dx[j];
+            dx[r] = s / M[r][r];
+        }
+        for (int j = 0; j < N; ++j) {
+            v[j] += dx[j];
+            nrm += std::fabs(dx[j]);
+        }
+        if (nrm < 1e-17) break;
+    }
+}
+
+// S3-invariants of a barycentric point: e2 = l1 l2 + l2 l3 + l3 l1, e3 = l1 l2 l3
+inline void tri_invariants(double l1, double l2, double l3, double &e2, double &e3) {
+    e2 = l1 * l2 + l2 * l3 + l3 * l1;
+    e3 = l1 * l2 * l3;
+}
+
 // Triangle face rule (QUADRATURETRIANG, src/grid_t.f90:2054-2348): the
-// symmetric rules with 1, 3 and 6 points (degrees 1, 2, 4) in the reference's
-// point order.  The 6-point orbit parameters are solved here to full double
-// precision from the moment equations of the S3-invariants 1, e2, e3, e2^2
-// (means 1, 1/4, 1/60, 1/15 over the triangle) instead of copying 15-digit
-// constants.
+// symmetric rules with 1, 3, 6 and 10 points (degrees 1, 2, 4, 5) in the
+// reference's point order.  Orbit parameters are solved to full double
+// precision from the moment equations of the S3-invariants over the triangle
+// (means: 1 -> 1, e2 -> 1/4, e3 -> 1/60, e2^2 -> 1/15, e2 e3 -> 1/210, from
+// E[l1^a l2^b l3^c] = 2 a! b! c! / (a+b+c+2)!) instead of copying 15-digit
+// constants.  The 6-point rule is fixed by its degree; the 10-point rule
+// (orbits (a,b,b), the 6 permutations of (p,q,r), the centroid) is a
+// one-parameter family at degree 5, and takes its vertex-orbit coordinate
+// a = 0.888871894660413 from the reference (grid_t.f90:2099) with p, q and
+// the weights solved from the moments.
 int tri_rule(int n, std::vector<double> &bary, std::vector<double> &w) {
     bary.clear();
     w.clear();
@@ -248,6 +302,28 @@ int tri_rule(int n, std::vector<double> &bary, std::vector<double> &w) {
         bary = {A, a, a, a, A, a, a, a, A, b, b, B, b, B, b, B, b, b};
         w = {v[2], v[2], v[2], v[3], v[3], v[3]};
         return 6;
+    }
+    if (n == 4) {
+        const double a = 0.888871894660413, b = 0.5 * (1.0 - a);
+        double ea2, ea3, ec2, ec3;
+        tri_invariants(a, b, b, ea2, ea3);
+        tri_invariants(1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, ec2, ec3);
+        // unknowns: p, q (r = 1 - p - q), wa (vertex orbit), wp (6-orbit); the
+        // centroid takes 1 - 3 wa - 6 wp
+        double v[4] = {0.2955, 0.6342, 0.042, 0.112};
+        newton_solve(v, [&](const double *x, double *f) {
+            double e2, e3;
+            tri_invariants(x[0], x[1], 1.0 - x[0] - x[1], e2, e3);
+            const double wa = 3.0 * x[2], wp = 6.0 * x[3], wc = 1.0 - wa - wp;
+            f[0] = wa * ea2 + wp * e2 + wc * ec2 - 0.25;
+            f[1] = wa * ea3 + wp * e3 + wc * ec3 - 1.0 / 60.0;
+            f[2] = wa * ea2 * ea2 + wp * e2 * e2 + wc * ec2 * ec2 - 1.0 / 15.0;
+            f[3] = wa * ea2 * ea3 + wp * e2 * e3 + wc * ec2 * ec3 - 1.0 / 210.0;
+        });
+        const double p = v[0], q = v[1], r = 1.0 - p - q, c = 1.0 / 3.0;
+        bary = {a, b, b, b, a, b, b, b, a, p, q, r, p, r, q, r, p, q, q, p, r, q, r, p, r, q, p, c, c, c};
+        w = {v[2], v[2], v[2], v[3], v[3], v[3], v[3], v[3], v[3], 1.0 - 3.0 * v[2] - 6.0 * v[3]};
+        return 10;
     }
     return -1;
 }
@@ -1721,7 +1797,7 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
             if (v != 0 && v != 1 && v != 2 && v != 3 && v != 4 && v != 6)
                 return fail(UCNS_EUNSUPPORTED, "side codes: 0 periodic, 1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield");
     }
-    if (o.iorder < 1 || o.iorder > 3) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..3");
+    if (o.iorder < 1 || o.iorder > 4) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..4");
     if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
     if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
     if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");

@@ -166,10 +166,10 @@ __device__ inline bool jump_test(double gamma, const double *wi, const double *u
 
 template <typename R, int IDEG>
 __device__ inline void basis_at(const R *x, const R *mbar, R *phi) {
-    R px[4], py[4], pz[4];
+    R px[5], py[5], pz[5];
     px[0] = py[0] = pz[0] = R(1);
 #pragma unroll
-    for (int k = 1; k < 4; ++k) {
+    for (int k = 1; k < 5; ++k) {
         px[k] = px[k - 1] * x[0];
         py[k] = py[k - 1] * x[1];
         pz[k] = pz[k - 1] * x[2];
@@ -3617,6 +3617,8 @@ int launch_recon_t(ucns_ctx *ctx) {
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);
     if (ctx->ideg == 19 && ctx->imax == 39)
         hipLaunchKernelGGL((k_recon<R, 19, 39, 7>), grid, block, 0, ctx->stream, p);
+    else if (ctx->ideg == 34 && ctx->imax == 69)
+        hipLaunchKernelGGL((k_recon<R, 34, 69, 7>), grid, block, 0, ctx->stream, p);
     else if (ctx->ideg == 9 && ctx->imax == 19)
         hipLaunchKernelGGL((k_recon<R, 9, 19, 7>), grid, block, 0, ctx->stream, p);
     else if (ctx->ideg == 3 && ctx->imax == 8)
@@ -3887,6 +3889,10 @@ int ensure_adjoint_steps(ucns_ctx *ctx) {
 int ensure_adjoint(ucns_ctx *ctx) {
     if (ctx->ees == 5) {
         ctx->err = "the adjoint does not cover CWENO (ees 5)";
+        return UCNS_EUNSUPPORTED;
+    }
+    if (ctx->ideg == 34) {
+        ctx->err = "the adjoint covers idegfree 3, 9 and 19 (not WENO-5, idegfree 34)";
         return UCNS_EUNSUPPORTED;
     }
     if (ctx->adj_ready) return UCNS_OK;
@@ -4595,9 +4601,10 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     ctx->gamma = s->gamma;
     ctx->lwci1 = s->lwci1;
     ctx->eps = s->weno_eps;
-    if (!((ctx->ideg == 19 && ctx->imax == 39) || (ctx->ideg == 9 && ctx->imax == 19) || (ctx->ideg == 3 && ctx->imax == 8)))
-        return bad(UCNS_EUNSUPPORTED,
-                   "supported (idegfree, inumneighbours): (19, 40) WENO-4, (9, 20) WENO-3 and (3, 9) MUSCL");
+    if (!((ctx->ideg == 34 && ctx->imax == 69) || (ctx->ideg == 19 && ctx->imax == 39) ||
+          (ctx->ideg == 9 && ctx->imax == 19) || (ctx->ideg == 3 && ctx->imax == 8)))
+        return bad(UCNS_EUNSUPPORTED, "supported (idegfree, inumneighbours): (34, 70) WENO-5, (19, 40) WENO-4, "
+                                      "(9, 20) WENO-3 and (3, 9) MUSCL");
     if (ctx->iweno == -1 && ctx->ideg != 3) return bad(UCNS_EUNSUPPORTED, "MUSCL path is linear (idegfree 3)");
     if (m->ncell > 0 && m->ifca)
         for (int i = 0; i < m->ncell; ++i)
