@@ -35,6 +35,9 @@ CASES = {
     "hex-weno4": lambda **k: CaseSpec("cw-hex4", HEX, 8, "weno4", jitter=0.1, seed=5, ees=5, **k),
     "hybrid-weno3": lambda **k: CaseSpec("cw-hyb3", HYBRID, 6, "weno3", jitter=0.05, seed=2, ees=5, **k),
     "channel-weno3": lambda **k: CaseSpec(**{**reduced("c4", 3).__dict__, "name": "cw-channel", "ees": 5, **k}),
+    # IORDER 4: quadratic sectors on their first 20 members
+    "tet-weno5": lambda **k: CaseSpec("cw-tet5", TET, 6, "weno5", jitter=0.1, seed=3, ees=5, **k),
+    "hex-weno5": lambda **k: CaseSpec("cw-hex5", HEX, 10, "weno5", jitter=0.1, seed=5, ees=5, **k),
 }
 
 
@@ -67,15 +70,22 @@ def test_low_order_data(name):
     nc, ts, nnb = case.ncell, case.typesten, case.nnb
     cw = case.cweno()
     id2, im2 = cw.idegfree2, cw.inumneighbours2 - 1
-    assert (id2, im2) == (3, 11)  # IORDER <= 3: linear sectors on 12 members
+    # IORDER <= 3: linear sectors on 12 members; IORDER 4: quadratic on 20
+    assert (id2, im2) == ((3, 11) if case.iorder <= 3 else (9, 19))
     ib = case.array("integ_basis").reshape(nc, -1)
     ibc = case.array("integ_basis_c").reshape(nc, id2)
-    np.testing.assert_allclose(ibc, ib[:, :3], rtol=1e-12, atol=1e-14 * np.abs(ib[:, :3]).max())
+    np.testing.assert_allclose(ibc, ib[:, :id2], rtol=1e-12, atol=1e-14 * np.abs(ib[:, :id2]).max())
     v11 = case.array("volume11")
     Bc = case.array("indicator_c").reshape(nc, id2, id2)
-    np.testing.assert_allclose(Bc, v11[:, None, None] * np.eye(3)[None], rtol=1e-12, atol=1e-15)
+    # the derivative Gram matrix of the low-order monomials: INDICATOR's
+    # leading block (the high-order derivatives of low-order monomials vanish);
+    # for linear sectors VOLUME(1,1) times the identity
+    B = case.array("indicator").reshape(nc, case.ideg, case.ideg)
+    np.testing.assert_allclose(Bc, B[:, :id2, :id2], rtol=1e-12, atol=1e-14 * np.abs(B[:, :id2, :id2]).max())
+    if id2 == 3:
+        np.testing.assert_allclose(Bc, v11[:, None, None] * np.eye(3)[None], rtol=1e-12, atol=1e-15)
     # the sectors' pseudo-inverses: pinv(diag(V) A) diag(V) over the first im2 members
-    exps = monomial_exponents(3)
+    exps = monomial_exponents(id2)
     ih = case.array("ihexl").reshape(nc, ts, nnb)
     admis = case.array("admis")
     Pc = case.array("invmat_c").reshape(nc, ts, im2, id2)
@@ -116,20 +126,24 @@ BOXES = {
                                      bc=(4, 4, 4, 4, 4, 4)),
     "hybrid-weno3": lambda wz: CaseSpec("cw-hyb-box", HYBRID, 9, "weno3", jitter=0.05, seed=2, ees=5, wenoz=wz,
                                         bc=(4, 4, 4, 4, 4, 4)),
+    "tet-weno5": lambda wz: CaseSpec("cw-tet5-box", TET, 6, "weno5", jitter=0.1, seed=3, ees=5, wenoz=wz,
+                                     bc=(4, 4, 4, 4, 4, 4)),
 }
 
 
 @pytest.mark.parametrize("name", list(BOXES))
 @pytest.mark.parametrize("wenoz", [0, 1])
-def test_linear_fields_are_reconstructed_exactly(name, wenoz):
+def test_low_order_fields_are_reconstructed_exactly(name, wenoz):
+    """fields of the sectors' degree (linear; quadratic for IORDER 4)"""
     case = Case(BOXES[name](wenoz))
     m = Mesh(case)
     rng = np.random.default_rng(4)
-    a = rng.standard_normal((5, 3)) * 0.05
     base = np.array([1.0, 0.3, -0.2, 0.1, 3.0])
     L = np.array(case.spec.dims[3:]) if case.spec.box else np.full(3, case.spec.length)
     c0 = L / 2
-    fn = lambda x, v: base[v] + (np.atleast_2d(x) - c0) @ a[v]  # noqa: E731
+    deg = 1 if case.iorder <= 3 else 2
+    polys = [Cubic(rng, deg, c0, L.max()) for _ in range(5)]
+    fn = lambda x, v: base[v] + 0.05 * polys[v](x)  # noqa: E731
     # periodic meshes: only cells whose stencils do not reach across the seam
     reach = np.full(case.ncell, True)
     ih = case.array("ihexl").reshape(case.ncell, case.typesten, case.nnb)
@@ -152,19 +166,29 @@ def test_linear_fields_are_reconstructed_exactly(name, wenoz):
     assert worst <= 1e-10, worst
 
 
-@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4"])
+CENTRAL = {
+    "tet-weno4": lambda: CASES["tet-weno4"](lwci1=1e15),
+    "hex-weno4": lambda: CASES["hex-weno4"](lwci1=1e15),
+    # quartic: a walled box (a periodic one this small puts the stencils across the seam)
+    "tet-weno5": lambda: CaseSpec("cw-tet5-box", TET, 6, "weno5", jitter=0.1, seed=3, ees=5, lwci1=1e15,
+                                  bc=(4, 4, 4, 4, 4, 4)),
+}
+
+
+@pytest.mark.parametrize("name", list(CENTRAL))
 def test_central_polynomial_when_the_sectors_carry_no_weight(name):
     """LWCI1 = 1e15: lambda_l ~ 1e-16, so the face values are the central
-    cubic fit, exact for cubic data"""
-    case = Case(CASES[name](lwci1=1e15))
+    fit, exact for data of the scheme's degree (cubic; quartic for IORDER 4)"""
+    case = Case(CENTRAL[name]())
     m = Mesh(case)
     rng = np.random.default_rng(6)
-    L = np.full(3, case.spec.length)
-    cubics = [Cubic(rng, 3, L / 2, L[0]) for _ in range(5)]
+    L = np.array(case.spec.dims[3:])
+    cubics = [Cubic(rng, case.iorder, L / 2, L.max()) for _ in range(5)]
     reach = np.full(case.ncell, True)
     ih = case.array("ihexl").reshape(case.ncell, case.typesten, case.nnb)
+    per = m.L > 0
     for i in range(case.ncell):
-        if np.any(np.abs(m.centre[ih[i, 0]] - m.centre[i]) > 0.45 * L):
+        if np.any(np.abs(m.centre[ih[i, 0]] - m.centre[i])[:, per] > 0.45 * L[per]):
             reach[i] = False
     u = np.array([[cell_average(m, i, cubics[v]) + (3.0 if v in (0, 4) else 0.0) for v in range(5)]
                   for i in range(case.ncell)])

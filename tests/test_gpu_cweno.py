@@ -1,5 +1,6 @@
 """GPU parity of CWENO (EES=5) and CWENO-Z: the register kernels' central +
-linear-sector reconstruction (src/reconstruct_p.f90:955-1049, 1062-1199)
+linear-sector reconstruction (IORDER 2, 3) and the per-cell kernel's quartic
+central + quadratic-sector one (IORDER 4) (src/reconstruct_p.f90:955-1049, 1062-1199)
 through the C-ABI against the oracle.
 
 Bars as for WENO: FP64 reconstruction, residual and one SSP-RK3 step <= 1e-12
@@ -29,6 +30,10 @@ CASES = {
     "channel-weno3": lambda: CaseSpec(**{**reduced("c4", 5).__dict__, "name": "cw-channel", "ees": 5}),
     "channel-weno3-z": lambda: CaseSpec(**{**reduced("c4", 5).__dict__, "name": "cw-channel-z", "ees": 5,
                                            "wenoz": 1}),
+    # IORDER 4 (WENO-5 central, quadratic sectors on 20 members): the per-cell kernel
+    "hex-weno5": lambda: CaseSpec("cw-hex5", HEX, 12, "weno5", seed=5, ees=5),
+    "tet-weno5-z": lambda: CaseSpec("cw-tet5z", TET, 6, "weno5", jitter=0.1, seed=3, ees=5, wenoz=1),
+    "hybrid-weno5": lambda: CaseSpec("cw-hyb5", HYBRID, 9, "weno5", jitter=0.1, seed=6, ees=5),
 }
 
 _cache = {}
@@ -121,7 +126,7 @@ def test_one_step_fp64(name):
     assert np.all(err <= 1e-12), err
 
 
-@pytest.mark.parametrize("name", ["hex-weno4", "channel-weno3-z"])
+@pytest.mark.parametrize("name", ["hex-weno4", "channel-weno3-z", "hex-weno5"])
 def test_100_steps_fp64(name):
     case, orc, u, _ = get(name)
     print(f"[cweno rk3 x100] {name}", flush=True)
@@ -136,7 +141,8 @@ def test_100_steps_fp64(name):
     assert np.all(err <= 1e-9), err
 
 
-@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4-z", "hybrid-weno3", "channel-weno3"])
+@pytest.mark.parametrize("name", ["tet-weno4", "hex-weno4-z", "hybrid-weno3", "channel-weno3", "tet-weno5-z",
+                                  "hybrid-weno5"])
 def test_mixed_vs_fp64(name):
     case, orc, u, _ = get(name)
     print(f"[cweno mixed] {name}", flush=True)

@@ -89,6 +89,13 @@ struct ReconParams {
     const int32_t *cnb;    // [ncell][nf] face neighbours
     R *uleft;              // [ncell][nf][nqm][5]
     int *counters;         // [0] MUSCL cells, [1] first-order cells
+    // CWENO (ees 5) in the per-cell kernel: P holds the central stencil only
+    // ([ncell][imax][ideg]); the sectors' low-order pseudo-inverses, their
+    // indicators (packed) and mbar - mbar_c over their coefficients
+    const R *Pc;           // [ncell][ts-1][imax2][ideg2]
+    const R *Bc;           // [ncell][ideg2*(ideg2+1)/2]
+    const R *dmb;          // [ncell][ideg2]
+    int wenoz;
 };
 
 __device__ inline void cons2prim_d(double gamma, const double *u, double *w) {
@@ -200,11 +207,18 @@ struct ReconShared {
     R b[IDEG * IDEG];                // indicator, unpacked
     R mb[IDEG];
     double ui[NV];
+    // CWENO: the sectors' indicator (unpacked), the indicators in FP64 and
+    // the face-value shift of the sectors' low-order means
+    R bc[9 * 9];
+    double betad[TSMAX * NV];
+    R corr[NV];
 };
 
-// Fused reconstruction, one wavefront per cell.
-template <typename R, int IDEG, int IMAX, int TSMAX>
+// Fused reconstruction, one wavefront per cell.  CW: CWENO with sectors of
+// ID2 coefficients over their first IM2 members (src/reconstruct_p.f90:955-1049).
+template <typename R, int IDEG, int IMAX, int TSMAX, bool CW = false, int ID2 = 9, int IM2 = 19>
 __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
+    static_assert(!CW || (ID2 <= 9 && IM2 <= IMAX), "CWENO sectors: at most 9 coefficients");
     constexpr int NROW = (TSMAX * IDEG + WAVE - 1) / WAVE;
     __shared__ ReconShared<R, IDEG, IMAX, TSMAX> sh;
     const int lane = threadIdx.x;
@@ -226,10 +240,31 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
 
     // ---- phase 0: dU of every stencil member (FP64, then rounded to R) ----
     const int32_t *nb = p.nbr + (size_t)cell * ts * IMAX;
-    for (int e = lane; e < admis * IMAX; e += WAVE) {
-        const double *Un = p.U + (size_t)nb[e] * NV;
+    if constexpr (CW) {
+        // the central stencil's IMAX members, then each sector's first IM2
+        for (int e = lane; e < IMAX + (admis - 1) * IM2; e += WAVE) {
+            const int m = e < IMAX ? e : ((e - IMAX) / IM2 + 1) * IMAX + (e - IMAX) % IM2;
+            const double *Un = p.U + (size_t)nb[m] * NV;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sh.u.du[e * NV + v] = (R)(Un[v] - ui[v]);
+            for (int v = 0; v < NV; ++v) sh.u.du[e * NV + v] = (R)(Un[v] - ui[v]);
+        }
+        const R *bcp = p.Bc + (size_t)cell * (ID2 * (ID2 + 1) / 2);
+        for (int e = lane; e < ID2 * (ID2 + 1) / 2; e += WAVE) {
+            int k = 0, base = 0;
+            while (e >= base + (ID2 - k)) {
+                base += ID2 - k;
+                ++k;
+            }
+            const int l = k + (e - base);
+            sh.bc[k * ID2 + l] = bcp[e];
+            sh.bc[l * ID2 + k] = bcp[e];
+        }
+    } else {
+        for (int e = lane; e < admis * IMAX; e += WAVE) {
+            const double *Un = p.U + (size_t)nb[e] * NV;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sh.u.du[e * NV + v] = (R)(Un[v] - ui[v]);
+        }
     }
     if (p.iweno == 1) {
         const R *Bc = p.B + (size_t)cell * (IDEG * (IDEG + 1) / 2);
@@ -250,7 +285,40 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
     __syncthreads();
 
     // ---- phase 1: G_s = P_s dU_s, one (stencil,coefficient) row per lane ----
-    {
+    if constexpr (CW) {
+        // rows 0..IDEG-1: the central stencil; then ID2 rows per sector
+        constexpr int NROWC = (IDEG + (TSMAX - 1) * ID2 + WAVE - 1) / WAVE;
+        const int nrows = IDEG + (admis - 1) * ID2;
+        const R *Pcen = p.P + (size_t)cell * IMAX * IDEG;
+        const R *Psec = p.Pc + (size_t)cell * (ts - 1) * IM2 * ID2;
+#pragma unroll
+        for (int t = 0; t < NROWC; ++t) {
+            const int r = lane + t * WAVE;
+            if (r >= nrows) continue;
+            R acc[NV] = {R(0), R(0), R(0), R(0), R(0)};
+            if (r < IDEG) {
+                for (int j = 0; j < IMAX; ++j) {
+                    const R pv = Pcen[(size_t)j * IDEG + r];
+                    const R *du = &sh.u.du[j * NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] += pv * du[v];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sh.g[r * NV + v] = acc[v];
+            } else {
+                const int sec = (r - IDEG) / ID2, k = (r - IDEG) % ID2;
+                const R *Ps = Psec + (size_t)sec * IM2 * ID2;
+                for (int j = 0; j < IM2; ++j) {
+                    const R pv = Ps[(size_t)j * ID2 + k];
+                    const R *du = &sh.u.du[(IMAX + sec * IM2 + j) * NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] += pv * du[v];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sh.g[((sec + 1) * IDEG + k) * NV + v] = acc[v];
+            }
+        }
+    } else {
         R acc[NROW][NV];
         int srow[NROW];
 #pragma unroll
@@ -288,7 +356,93 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
 
     bool reduce = (full == 0);
     const int nslot = ifca * p.nqm;
+    if constexpr (CW) {
+        // ---- CWENO: the zero polynomial (G_1 - sum_l lambda_l G_l) / lambda_1,
+        // lambda_1 = 1 - 1/LWCI1, lambda_l = (1 - lambda_1)/(ADMIS-1) ----
+        const double lam0 = 1.0 - (1.0 / p.lwci1);
+        const double laml = admis > 1 ? (1.0 - lam0) / (admis - 1) : 0.0;
+        for (int e = lane; e < IDEG * NV; e += WAVE) {
+            const int k = e / NV, v = e % NV;
+            R sl = R(0);
+            if (k < ID2)
+                for (int s = 1; s < admis; ++s) sl += (R)laml * sh.g[(s * IDEG + k) * NV + v];
+            sh.u.late.gh[e] = (R)(1.0 / lam0) * (sh.g[e] - sl);
+        }
+        __syncthreads();
+        // indicators: the zero polynomial's with INDICATOR, a sector's with INDICATORC
+        if (lane < admis * NV) {
+            const int s = lane / NV, v = lane % NV;
+            double beta = 0.0;
+            if (s == 0) {
+                for (int k = 0; k < IDEG; ++k) {
+                    double bg = 0.0;
+                    for (int l = 0; l < IDEG; ++l) bg += (double)sh.b[k * IDEG + l] * (double)sh.u.late.gh[l * NV + v];
+                    beta += (double)sh.u.late.gh[k * NV + v] * bg;
+                }
+            } else {
+                const R *gs = &sh.g[s * IDEG * NV + v];
+#pragma unroll
+                for (int k = 0; k < ID2; ++k) {
+                    double bg = 0.0;
+#pragma unroll
+                    for (int l = 0; l < ID2; ++l) bg += (double)sh.bc[k * ID2 + l] * (double)gs[l * NV];
+                    beta += (double)gs[k * NV] * bg;
+                }
+            }
+            sh.betad[lane] = beta;
+        }
+        __syncthreads();
+        // weights in FP64: WENO (power 4) or WENO-Z, tau = sum_l |beta_1 - beta_l| / (ADMIS-1)
+        if (lane < admis * NV) {
+            const int s = lane / NV, v = lane % NV;
+            double tau = 0.0;
+            if (p.wenoz && admis > 1) {
+                for (int t = 1; t < admis; ++t) tau += fabs(sh.betad[v] - sh.betad[t * NV + v]);
+                tau /= (double)(admis - 1);
+            }
+            double sum = 0.0, mine = 0.0;
+            for (int t = 0; t < admis; ++t) {
+                const double lam = t == 0 ? lam0 : laml;
+                const double x = p.eps + sh.betad[t * NV + v];
+                double w;
+                if (p.wenoz) {
+                    const double r = tau / x, r2 = r * r;
+                    w = lam * (1.0 + r2 * r2);
+                } else {
+                    const double x2 = x * x;
+                    w = lam / (x2 * x2);
+                }
+                sum += w;
+                if (t == s) mine = w;
+            }
+            sh.u.late.omega[lane] = (R)(mine / sum);
+        }
+        __syncthreads();
+        // weighted polynomial on the high-order basis; the sectors' polynomials
+        // use the means of VALUEC: their weighted coefficients times
+        // (mbar - mbar_c) shift every face value alike
+        for (int e = lane; e < IDEG * NV; e += WAVE) {
+            const int k = e / NV, v = e % NV;
+            R acc = sh.u.late.omega[v] * sh.u.late.gh[e];
+            if (k < ID2)
+                for (int s = 1; s < admis; ++s) acc += sh.u.late.omega[s * NV + v] * sh.g[(s * IDEG + k) * NV + v];
+            sh.u.late.gh[e] = acc;
+        }
+        if (lane < NV) {
+            const R *dm = p.dmb + (size_t)cell * ID2;
+            R c = R(0);
+#pragma unroll
+            for (int k = 0; k < ID2; ++k) {
+                R wg = R(0);
+                for (int s = 1; s < admis; ++s) wg += sh.u.late.omega[s * NV + lane] * sh.g[(s * IDEG + k) * NV + lane];
+                c += dm[k] * wg;
+            }
+            sh.corr[lane] = c;
+        }
+        __syncthreads();
+    }
     if (p.iweno == 1) {
+        if constexpr (!CW) {
         // ---- phase 2: beta_{s,v} = g^T (B g) (src/reconstruct_p.f90:995-1004) ----
         if (lane < admis * NV) {
             const int s = lane / NV, v = lane % NV;
@@ -345,6 +499,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             sh.u.late.gh[e] = acc;
         }
         __syncthreads();
+        }
         // ---- phase 5: ULEFT at the face points + CHECKSOL (0.85) ----
         double wi[NV];
         cons2prim_d(p.gamma, ui, wi);
@@ -363,6 +518,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
                 R res = R(0);
 #pragma unroll
                 for (int k = 0; k < IDEG; ++k) res += phi[k] * sh.u.late.gh[k * NV + v];
+                if constexpr (CW) res += sh.corr[v];
                 // ULEFT is kept as the deviation from the FP64 cell average
                 uq[v] = ui[v] + (double)res;
                 sh.u.late.ul[e * NV + v] = res;
@@ -3389,6 +3545,8 @@ struct ucns_ctx {
     bool have_cweno = false;
     std::vector<double> h_cwP, h_cwB, h_cwIB;
     void *Bc = nullptr, *dmb = nullptr;
+    void *Pc = nullptr;     // per-cell kernel: the sectors' low-order pseudo-inverses [ncell][ts-1][imax2][ideg2]
+    bool cw_cell = false;   // CWENO on the per-cell kernel (WENO-5)
     // library options of the scheme
     int kernels = UCNS_KERNELS_AUTO;
     int adj_keep_stages = 0;
@@ -3491,6 +3649,10 @@ int launch_recon_t(ucns_ctx *ctx) {
     p.cnb = ctx->cnb;
     p.uleft = (R *)ctx->uleft;
     p.counters = ctx->counters;
+    p.Pc = (const R *)ctx->Pc;
+    p.Bc = (const R *)ctx->Bc;
+    p.dmb = (const R *)ctx->dmb;
+    p.wenoz = ctx->wenoz;
     if (ctx->tiled) {
         ReconWParams<R> w{};
         w.ncell = ctx->ncell;
@@ -3615,6 +3777,15 @@ int launch_recon_t(ucns_ctx *ctx) {
         return UCNS_OK;
     }
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);
+    if (ctx->cw_cell) {
+        if (ctx->ideg == 34 && ctx->imax == 69 && ctx->ideg2 == 9 && ctx->imax2 == 19) {
+            hipLaunchKernelGGL((k_recon<R, 34, 69, 7, true, 9, 19>), grid, block, 0, ctx->stream, p);
+            return UCNS_OK;
+        }
+        ctx->err = "no CWENO kernel for idegfree=" + std::to_string(ctx->ideg) + " idegfree2=" +
+                   std::to_string(ctx->ideg2) + " imax2=" + std::to_string(ctx->imax2);
+        return UCNS_EUNSUPPORTED;
+    }
     if (ctx->ideg == 19 && ctx->imax == 39)
         hipLaunchKernelGGL((k_recon<R, 19, 39, 7>), grid, block, 0, ctx->stream, p);
     else if (ctx->ideg == 34 && ctx->imax == 69)
@@ -4306,7 +4477,8 @@ void release_prestore(ucns_ctx *ctx) {
     dfree(ctx->mbar);
     if (ctx->Bc) (void)hipFree(ctx->Bc);
     if (ctx->dmb) (void)hipFree(ctx->dmb);
-    ctx->Bc = ctx->dmb = nullptr;
+    if (ctx->Pc) (void)hipFree(ctx->Pc);
+    ctx->Bc = ctx->dmb = ctx->Pc = nullptr;
     dfree(ctx->qp);
     ctx->have_prestore = false;
 }
@@ -4821,8 +4993,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     const bool mixed = ctx->precision == UCNS_MIXED;
     ctx->tiled = (ctx->iweno == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) && ts <= 8 &&
                   ctx->nf * ctx->nqm <= 56);
-    if (ctx->ees == 5 && !ctx->tiled) {
-        ctx->err = "CWENO (ees 5) runs on the WENO-3 / WENO-4 register kernels (idegfree 9 or 19)";
+    ctx->cw_cell = ctx->ees == 5 && !ctx->tiled;
+    if (ctx->cw_cell && ID != 34) {
+        ctx->err = "CWENO (ees 5) runs on the WENO-3 / WENO-4 register kernels (idegfree 9 or 19) or the WENO-5 "
+                   "per-cell kernel";
         return UCNS_EUNSUPPORTED;
     }
     {
@@ -4871,8 +5045,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     if (!nbr.empty())
         HIP_TRY(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
-    // pseudo-inverses: staged in chunks, re-laid out on the device
-    const size_t per_cell_abi = (size_t)TSA * IM * ID, per_cell_dev = (size_t)IM * ts * ID;
+    // pseudo-inverses: staged in chunks, re-laid out on the device (CWENO on
+    // the per-cell kernel: the central stencil's only)
+    const int tsP = ctx->cw_cell ? 1 : ts;
+    const size_t per_cell_abi = (size_t)TSA * IM * ID, per_cell_dev = (size_t)IM * tsP * ID;
     const size_t dev_elems = ctx->smode ? 1 : ctx->tiled ? (size_t)ctx->ntile * 8 * per_cell_dev : per_cell_dev * nc;
     rc = mixed ? dalloc(ctx, (float **)&ctx->P, dev_elems) : dalloc(ctx, (double **)&ctx->P, dev_elems);
     if (rc) return rc;
@@ -4921,7 +5097,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                 for (size_t c = 0; c < cn; ++c)
                     std::memcpy(gather + c * per_cell_abi, pre->invmat + src_cell(ctx, (int)(c0 + c)) * per_cell_abi,
                                 per_cell_abi * sizeof(double));
-                if (ctx->ees == 5) {
+                if (ctx->ees == 5 && ctx->tiled) {
                     // CWENO: a sector's rows are its low-order pseudo-inverse, zero
                     // past idegfree2 coefficients and imax2 members
                     const int ID2 = ctx->ideg2, IM2 = ctx->imax2;
@@ -4961,10 +5137,10 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
                 const int blocks = (int)std::min<size_t>(4096, (cn * per_cell_dev + 255) / 256);
                 if (mixed)
                     hipLaunchKernelGGL(k_relayout_p<float>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
-                                       (float *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+                                       (float *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, tsP, IM, ID);
                 else
                     hipLaunchKernelGGL(k_relayout_p<double>, dim3(blocks), dim3(256), 0, ctx->stream, stage,
-                                       (double *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, ts, IM, ID);
+                                       (double *)ctx->P + c0 * per_cell_dev, (int)cn, TSA, tsP, IM, ID);
             }
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -5003,19 +5179,30 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
         if ((rc = upload_real_any(ctx, &ctx->mbar, mb))) return rc;
         if (ctx->ees == 5) {
             // the sectors' indicators (packed) and mbar - mbar_c over their coefficients
-            const int ID2 = ctx->ideg2;
-            std::vector<double> bc((size_t)nc * 6), dm((size_t)nc * 3);
+            const int ID2 = ctx->ideg2, NB2 = ID2 * (ID2 + 1) / 2;
+            std::vector<double> bc((size_t)nc * NB2), dm((size_t)nc * ID2);
             for (int i = 0; i < nc; ++i) {
                 const double *Bi = ctx->h_cwB.data() + (size_t)i * ID2 * ID2;
-                size_t e = (size_t)i * 6;
-                for (int k = 0; k < 3; ++k)
-                    for (int l = k; l < 3; ++l) bc[e++] = Bi[(size_t)k * ID2 + l];
+                size_t e = (size_t)i * NB2;
+                for (int k = 0; k < ID2; ++k)
+                    for (int l = k; l < ID2; ++l) bc[e++] = Bi[(size_t)k * ID2 + l];
                 const size_t o = src_cell(ctx, i);
                 const double oov = 1.0 / pre->volume11[o];
-                for (int d = 0; d < 3; ++d)
-                    dm[(size_t)i * 3 + d] = mb[(size_t)i * ID + d] - ctx->h_cwIB[(size_t)i * ID2 + d] * oov;
+                for (int d = 0; d < ID2; ++d)
+                    dm[(size_t)i * ID2 + d] = mb[(size_t)i * ID + d] - ctx->h_cwIB[(size_t)i * ID2 + d] * oov;
             }
             if ((rc = upload_real_any(ctx, &ctx->Bc, bc)) || (rc = upload_real_any(ctx, &ctx->dmb, dm))) return rc;
+            if (ctx->cw_cell) {
+                // the sectors' low-order pseudo-inverses, stencils 2..ts
+                const int IM2 = ctx->imax2;
+                const size_t pc = (size_t)IM2 * ID2;
+                std::vector<double> P2((size_t)nc * std::max(ts - 1, 1) * pc, 0.0);
+                for (int i = 0; i < nc; ++i)
+                    for (int st = 1; st < ts; ++st)
+                        std::memcpy(P2.data() + ((size_t)i * (ts - 1) + st - 1) * pc,
+                                    ctx->h_cwP.data() + ((size_t)i * TSA + st) * pc, pc * sizeof(double));
+                if ((rc = upload_real_any(ctx, &ctx->Pc, P2))) return rc;
+            }
         }
     }
     // Gauss points in the local frames: [cell][nf][nqm][3]
@@ -5034,6 +5221,14 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     ctx->st.bytes_recon_per_cell =
         (double)((size_t)IM * ts * ID * (mixed ? 4 : 8)) + (ctx->iweno == 1 ? (double)ID * (ID + 1) / 2 * (mixed ? 4 : 8) : 0) +
         (double)ts * IM * 4 + (double)ctx->nf * ctx->nqm * 3 * (mixed ? 4 : 8) + (double)ID * (mixed ? 4 : 8);
+    if (ctx->cw_cell) {
+        // central P, the sectors' low-order P, both indicators, the members read, mbar - mbar_c
+        const int ID2 = ctx->ideg2, IM2 = ctx->imax2, r = mixed ? 4 : 8;
+        ctx->st.bytes_recon_per_cell = (double)IM * ID * r + (double)(ts - 1) * IM2 * ID2 * r +
+                                       (double)ID * (ID + 1) / 2 * r + (double)ID2 * (ID2 + 1) / 2 * r +
+                                       (double)(IM + (ts - 1) * IM2) * 4 + (double)ctx->nf * ctx->nqm * 3 * r +
+                                       (double)(ID + ID2) * r;
+    }
     return UCNS_OK;
 }
 
@@ -5051,9 +5246,12 @@ int ucns_upload_cweno(ucns_ctx *ctx, const ucns_cweno_v1 *cw) {
         ctx->err = "null CWENO array";
         return UCNS_EBADARG;
     }
-    // IORDER <= 3 (the kernels' idegfree 9 and 19) take linear sectors
-    if (cw->idegfree2 != 3 || cw->inumneighbours2 < 4 || cw->inumneighbours2 > ctx->nnb) {
-        ctx->err = "CWENO sectors: idegfree2 must be 3 (IORDER2 = 1) and inumneighbours2 in [4, inumneighbours]";
+    // IORDER <= 3 (idegfree 9 and 19) take linear sectors, IORDER 4 (idegfree
+    // 34) quadratic ones (DETERMINE_SIZE, src/grid_p.f90:6151-6161)
+    const int want2 = ctx->ideg == 34 ? 9 : 3;
+    if (cw->idegfree2 != want2 || cw->inumneighbours2 < want2 + 1 || cw->inumneighbours2 > ctx->nnb) {
+        ctx->err = "CWENO sectors: idegfree2 must be " + std::to_string(want2) +
+                   " for this idegfree and inumneighbours2 in [idegfree2 + 1, inumneighbours]";
         return UCNS_EUNSUPPORTED;
     }
     const int nc = ctx->ncell, TSA = ctx->ts_abi, ID2 = cw->idegfree2, IM2 = cw->inumneighbours2 - 1;
