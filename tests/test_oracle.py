@@ -5,6 +5,7 @@ conservation, the Shu-Osher SSP-RK3 composition, exact reproduction of
 polynomials by the reconstruction, and the convergence order of the
 residual on a smooth advected wave.  The reference ships no golden vectors
 for this path, so these checks are what makes the oracle trustworthy."""
+import dataclasses
 import math
 
 import numpy as np
@@ -316,6 +317,8 @@ def interior_cells(case, m, exps, candidates=None, limit=None):
                                                bc=(4,) * 6), 4),
     ("hybrid-weno5-quartic", lambda: CaseSpec("hybrid-weno5", HYBRID, 12, "weno5", jitter=0.1, seed=21,
                                               bc=(4,) * 6), 4),
+    # the blast box (C5's construction) with WENO-5
+    ("blast-weno5-quartic", lambda: dataclasses.replace(reduced("c5", 12), ees=0), 4),
 ])
 def test_reconstruction_reproduces_polynomials(name, spec, deg):
     """Cell averages of a polynomial of the scheme's degree are reconstructed

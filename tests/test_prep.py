@@ -3,6 +3,7 @@ volumes, face areas and normals, neighbour pairing, the face quadrature
 rules, Gauss points, stencils, the local frame data and the least-squares
 pseudo-inverses.  Everything here is recomputed in numpy from the mesh
 vertices (tests/meshcheck.py), not from the preprocessor's own formulas."""
+import dataclasses
 import math
 
 import numpy as np
@@ -36,6 +37,9 @@ CASES = {
     # triangle face rules, derivatives up to fourth order in the indicator
     "hex-weno5": lambda: CaseSpec("hex-weno5", HEX, 8, "weno5", jitter=0.1, seed=9),
     "tet-weno5": lambda: CaseSpec("tet-weno5", TET, 4, "weno5", jitter=0.1, seed=10),
+    # the blast box: hexahedra at the centre, a shell of pyramids and
+    # tetrahedron pairs, Kuhn tetrahedra, prism layers at the z ends; outflow sides
+    "blast-weno3": lambda: dataclasses.replace(reduced("c5", 8), scheme="weno3", ees=0),
 }
 _cache = {}
 

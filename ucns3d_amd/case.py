@@ -8,6 +8,7 @@ Fortran host would pass.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import math
 from dataclasses import dataclass
 
@@ -21,6 +22,7 @@ PRISM = 3
 PYRAMID = 4
 HYBRID = 5  # hexes, pyramids and prisms in thirds of x
 CHANNEL = 6  # prism wall layers, a pyramid/tetrahedron interface layer, tetrahedra above
+BLAST = 7  # a central box of hexahedra, a pyramid/tetrahedron shell, tetrahedra, prism layers at the z ends
 
 
 @dataclass(frozen=True)
@@ -44,6 +46,8 @@ class CaseSpec:
     mach: float = 0.0     # > 0: the initial state is a uniform flow along x at this Mach number, rho = p = 1
     ees: int = 0          # 5: CWENO (central polynomial reconstituted from low-order sector polynomials)
     wenoz: int = 0        # 1: CWENO-Z weights (with ees = 5)
+    init: str = ""        # "blast": rho = 8, p = 10 for cell centres within 0.2 of the box centre, rho = p = 1
+                          # elsewhere, at rest; otherwise the vortex (or the free stream when mach > 0)
 
     @property
     def iorder(self) -> int:
@@ -71,6 +75,16 @@ class CaseSpec:
         if self.shape == CHANNEL:
             layers = max(1, min(10, ny // 3))
             return nx * nz * (2 * layers + 7 + 6 * (ny - layers - 1))
+        if self.shape == BLAST:
+            core = [n - 2 * (n // 4) for n in (nx, ny, nz)]
+            box = [c + 2 for c in core]
+            ncore, nbox = math.prod(core), math.prod(box)
+            # shell hexes: 6 pyramids, one more cell per face on the shell box's outside
+            shell = 6 * (nbox - ncore) + 2 * (box[0] * box[1] + box[1] * box[2] + box[0] * box[2])
+            pl = max(1, nz // 10)
+            nprism = nx * ny * 2 * pl
+            ntet = nx * ny * nz - nbox - nx * ny * 2 * pl
+            return ncore + shell + 2 * nprism + 6 * ntet
         return nx * ny * nz * {HEX: 1, TET: 6, PRISM: 2, PYRAMID: 6}[self.shape]
 
     def freestream(self):
@@ -95,16 +109,23 @@ PRESETS = {
                    CHANNEL, 30, "weno3", seed=4, steps=50, bc=(1, 2, 4, 4, 4, 4),
                    bump=tuple(float(a) for a in np.random.default_rng(4).uniform(0.0, 0.01, 8)),
                    box=(120, 30, 30, 4.0, 1.0, 1.0), mach=0.6),
+    # the unit cube, 121^3 hexes: a 61^3 box of hexahedra at the centre, a
+    # one-hex shell of pyramids and tetrahedra around it, prism layers (12
+    # hexes deep) at z = 0 and z = 1, Kuhn tetrahedra elsewhere; 5% vertex
+    # jitter; 8,112,739 cells.  Blast at the centre, outflow (copy) on every
+    # side; WENO-5 central polynomial with quadratic CWENO sectors
+    "c5": CaseSpec("c5: 121^3 blast box (hex core, pyramid/tet shell, tets, prism layers), WENO-5 CWENO, mixed",
+                   BLAST, 121, "weno5", jitter=0.05, seed=5, steps=300, bc=(2, 2, 2, 2, 2, 2),
+                   box=(121, 121, 121, 1.0, 1.0, 1.0), ees=5, init="blast"),
 }
 
 
 def reduced(key: str, n: int) -> CaseSpec:
     s = PRESETS[key]
     box = ()
-    if s.box:  # the same aspect: 4n x n x n cells
-        box = (4 * n, n, n) + tuple(s.box[3:])
-    return CaseSpec(f"{key}-reduced-{n}", s.shape, n, s.scheme, s.jitter, s.seed, s.noise, s.cfl, s.steps,
-                    s.length, s.lwci1, s.gamma, s.bc, s.bump, box, s.mach, s.ees, s.wenoz)
+    if s.box:  # the same aspect (c4: 4n x n x n cells; c5: n^3)
+        box = tuple(n * b // s.n for b in s.box[:3]) + tuple(s.box[3:])
+    return dataclasses.replace(s, name=f"{key}-reduced-{n}", n=n, box=box)
 
 
 class Case:
@@ -176,6 +197,14 @@ class Case:
     def init_state(self) -> np.ndarray:
         if self.spec.mach > 0:
             return np.tile(self.spec.freestream(), (self.ncell, 1))
+        if self.spec.init == "blast":
+            x = self.array("centre").reshape(-1, 3)
+            c = 0.5 * np.array(self.spec.dims[3:])
+            inside = np.linalg.norm(x - c, axis=1) < 0.2
+            u = np.zeros((self.ncell, 5))
+            u[:, 0] = np.where(inside, 8.0, 1.0)
+            u[:, 4] = np.where(inside, 10.0, 1.0) / (self.spec.gamma - 1.0)
+            return u
         u = np.empty((self.ncell, 5), dtype=np.float64)
         rc = self._lib.ucnsp_init_vortex(self._h, self.spec.gamma, self.spec.noise, self.spec.seed,
                                          native.f64p(u))

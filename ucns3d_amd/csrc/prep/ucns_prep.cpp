@@ -46,6 +46,15 @@ const int PYR_TETS[2][4] = {{0, 1, 2, 4}, {0, 2, 3, 4}};
 // hexes of the channel's interface layer (pyramids and tetrahedra), an
 // element kind of the generator only: its cells are UCNSP_PYRAMID and UCNSP_TET
 constexpr int UCNSP_INTERFACE = 100;
+// hexes of the blast box's shell around its hexahedra: a pyramid on each face
+// towards a quadrilateral-faced neighbour (hexahedra, shell hexes, the box
+// side), two tetrahedra on each face towards a triangle-faced one (Kuhn
+// tetrahedra, prisms), all with the hex centre as apex
+constexpr int UCNSP_SHELL = 101;
+// the diagonal of each hex face (HEX_FACES order) that the Kuhn split and the
+// prisms' split take: from the face's lowest corner to its highest, as local
+// hex nodes (0 = (i,j,k), 6 = (i+1,j+1,k+1))
+const int FACE_DIAG[6][2] = {{0, 2}, {4, 6}, {1, 6}, {0, 7}, {3, 6}, {0, 5}};
 // prism layers of the channel: 10 (the configuration's wall region), fewer
 // on coarse boxes so that tetrahedra fill at least two thirds of y
 inline int channel_layers(int ny) { return std::max(1, std::min(10, ny / 3)); }
@@ -1599,18 +1608,56 @@ void gen_box(ucnsp_case &c) {
     // the wall layers j < channel_layers(ny), the interface layer of
     // pyramids and tetrahedra (UCNSP_INTERFACE) and Kuhn tetrahedra above
     const int layers = channel_layers(ny);
-    auto etype = [&](int i, int j) {
+    // blast box: hexahedra in [c0, c1) on every axis, the shell one hex
+    // around them, prism layers at the z ends
+    const int blast_pl = std::max(1, nz / 10);
+    const int bc0[3] = {nx / 4, ny / 4, nz / 4}, bc1[3] = {nx - nx / 4, ny - ny / 4, nz - nz / 4};
+    auto etype = [&](int i, int j, int k) {
+        if (o.shape == UCNSP_BLAST) {
+            const int q[3] = {i, j, k};
+            bool core = true, near = true;
+            for (int d = 0; d < 3; ++d) {
+                core = core && q[d] >= bc0[d] && q[d] < bc1[d];
+                near = near && q[d] >= bc0[d] - 1 && q[d] <= bc1[d];
+            }
+            if (core) return UCNSP_HEX;
+            if (near) return UCNSP_SHELL;
+            return k < blast_pl || k >= nz - blast_pl ? UCNSP_PRISM : UCNSP_TET;
+        }
         if (o.shape == UCNSP_CHANNEL) return j < layers ? UCNSP_PRISM : j == layers ? UCNSP_INTERFACE : UCNSP_TET;
         return o.shape != UCNSP_HYBRID ? o.shape
                : i < nx / 3            ? UCNSP_HEX
                : i < 2 * nx / 3        ? UCNSP_PYRAMID
                                        : UCNSP_PRISM;
     };
-    auto per_hex = [](int t) {
+    // shell hexes: which faces (HEX_FACES order) meet triangle-faced neighbours
+    auto shell_tri = [&](int i, int j, int k) {
+        const int d[6][3] = {{0, 0, -1}, {0, 0, 1}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+        const int n3[3] = {nx, ny, nz};
+        int mask = 0;
+        for (int f = 0; f < 6; ++f) {
+            int q[3] = {i + d[f][0], j + d[f][1], k + d[f][2]};
+            bool out = false;
+            for (int a = 0; a < 3; ++a) {
+                if (q[a] >= 0 && q[a] < n3[a]) continue;
+                if (pax[a]) q[a] = (q[a] + n3[a]) % n3[a];
+                else out = true;
+            }
+            if (out) continue;
+            const int t = etype(q[0], q[1], q[2]);
+            if (t == UCNSP_TET || t == UCNSP_PRISM) mask |= 1 << f;
+        }
+        return mask;
+    };
+    auto per_hex = [&](int t, int i, int j, int k) {
+        if (t == UCNSP_SHELL) return 6 + __builtin_popcount(shell_tri(i, j, k));
         return t == UCNSP_HEX ? 1 : t == UCNSP_PRISM ? 2 : t == UCNSP_INTERFACE ? 7 : 6;
     };
-    auto has_apex = [&](int i, int j) { const int t = etype(i, j); return t == UCNSP_PYRAMID || t == UCNSP_INTERFACE; };
-    if (o.shape == UCNSP_PYRAMID || o.shape == UCNSP_HYBRID || o.shape == UCNSP_CHANNEL) {
+    auto has_apex = [&](int i, int j, int k) {
+        const int t = etype(i, j, k);
+        return t == UCNSP_PYRAMID || t == UCNSP_INTERFACE || t == UCNSP_SHELL;
+    };
+    if (o.shape == UCNSP_PYRAMID || o.shape == UCNSP_HYBRID || o.shape == UCNSP_CHANNEL || o.shape == UCNSP_BLAST) {
         // one apex node per hex that holds pyramids (the node slot of every
         // hex is reserved), at the mean of its corners
         c.nnode += nhex;
@@ -1621,7 +1668,7 @@ void gen_box(ucnsp_case &c) {
                 for (int i = 0; i < nx; ++i) {
                     const int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k), n = ngrid + h;
                     c.canon[n] = (int32_t)n;
-                    if (!has_apex(i, j)) {
+                    if (!has_apex(i, j, k)) {
                         for (int d = 0; d < 3; ++d) c.xyz[n * 3 + d] = 0.0;
                         continue;
                     }
@@ -1637,7 +1684,7 @@ void gen_box(ucnsp_case &c) {
     c.ncell = 0;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) c.ncell += per_hex(etype(i, j));
+            for (int i = 0; i < nx; ++i) c.ncell += per_hex(etype(i, j, k), i, j, k);
     c.cnodes.assign(c.ncell * 8, -1);
     c.cshape.assign(c.ncell, o.shape);
     int64_t cell0 = 0;  // first cell of the current hex
@@ -1647,8 +1694,9 @@ void gen_box(ucnsp_case &c) {
                 int64_t h = i + (int64_t)nx * (j + (int64_t)ny * k);
                 int64_t v[8] = {nid(i, j, k),     nid(i + 1, j, k),     nid(i + 1, j + 1, k),     nid(i, j + 1, k),
                                 nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
-                const int et = etype(i, j);
-                for (int q = 0; q < per_hex(et); ++q) c.cshape[cell0 + q] = et;
+                const int et = etype(i, j, k);
+                const int nper = per_hex(et, i, j, k);
+                for (int q = 0; q < nper; ++q) c.cshape[cell0 + q] = et;
                 // a tetrahedron from four nodes, numbered with positive orientation
                 auto add_tet = [&](int64_t cell, int64_t a, int64_t b, int64_t cc, int64_t d) {
                     double x[4][3];
@@ -1668,7 +1716,28 @@ void gen_box(ucnsp_case &c) {
                     c.cshape[cell] = UCNSP_TET;
                     for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)tv[q];
                 };
-                if (et == UCNSP_INTERFACE) {
+                if (et == UCNSP_SHELL) {
+                    // a pyramid per quadrilateral-faced neighbour, two
+                    // tetrahedra along the Kuhn diagonal per triangle-faced one
+                    const int mask = shell_tri(i, j, k);
+                    int64_t cell = cell0;
+                    for (int f = 0; f < 6; ++f) {
+                        if (!(mask & (1 << f))) {
+                            c.cshape[cell] = UCNSP_PYRAMID;
+                            for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)v[HEX_FACES[f][3 - q]];
+                            c.cnodes[cell * 8 + 4] = (int32_t)(ngrid + h);
+                            ++cell;
+                            continue;
+                        }
+                        // the face's corners in cyclic order, the diagonal's ends a and c
+                        const int *fc = HEX_FACES[f];
+                        int ia = 0;
+                        while (fc[ia] != FACE_DIAG[f][0]) ++ia;
+                        const int a = fc[ia], b = fc[(ia + 1) % 4], cc = fc[(ia + 2) % 4], d = fc[(ia + 3) % 4];
+                        add_tet(cell++, v[a], v[b], v[cc], ngrid + h);
+                        add_tet(cell++, v[a], v[cc], v[d], ngrid + h);
+                    }
+                } else if (et == UCNSP_INTERFACE) {
                     // between the prism layers (quadrilateral faces at y = j)
                     // and the Kuhn tetrahedra (triangles at y = j + 1): five
                     // pyramids on the hex faces other than y = j + 1, whose
@@ -1737,7 +1806,7 @@ void gen_box(ucnsp_case &c) {
                         for (int q = 0; q < 4; ++q) c.cnodes[cell * 8 + q] = (int32_t)tv[q];
                     }
                 }
-                cell0 += per_hex(et);
+                cell0 += nper;
             }
 }
 
@@ -1787,8 +1856,11 @@ int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n
 int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
     const ucnsp_opts &o = *opts;
-    if (o.shape < UCNSP_HEX || o.shape > UCNSP_CHANNEL)
-        return fail(UCNS_EUNSUPPORTED, "shape must be hex(1), tet(2), prism(3), pyramid(4), hybrid(5) or channel(6)");
+    if (o.shape < UCNSP_HEX || o.shape > UCNSP_BLAST)
+        return fail(UCNS_EUNSUPPORTED,
+                    "shape must be hex(1), tet(2), prism(3), pyramid(4), hybrid(5), channel(6) or blast(7)");
+    if (o.shape == UCNSP_BLAST && (o.nx < 8 || o.ny < 8 || o.nz < 8))
+        return fail(UCNS_EBADARG, "the blast box needs at least 8 hexes per axis (core, shell, tetrahedra, prisms)");
     if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
     for (int d = 0; d < 3; ++d) {
         const int a = o.bc[2 * d], b = o.bc[2 * d + 1];

@@ -36,6 +36,7 @@ extern "C" {
 #define UCNSP_PYRAMID 4
 #define UCNSP_HYBRID 5
 #define UCNSP_CHANNEL 6
+#define UCNSP_BLAST  7
 
 typedef struct ucnsp_opts {
     int32_t shape;          /* UCNSP_HEX: nx*ny*nz hexahedra; UCNSP_TET: 6 Kuhn tets per hex;
@@ -44,7 +45,13 @@ typedef struct ucnsp_opts {
                                UCNSP_HYBRID: hexes, pyramids, prisms in thirds of x;
                                UCNSP_CHANNEL: 2 prisms per hex in the wall layers
                                j < min(10, ny/3), then one layer of 5 pyramids + 2 tets
-                               per hex, then 6 Kuhn tets per hex */
+                               per hex, then 6 Kuhn tets per hex;
+                               UCNSP_BLAST: a central box of hexahedra (the middle half of
+                               each axis), a one-hex shell around it of pyramids (faces
+                               towards hexahedra and shell hexes) and tetrahedron pairs
+                               (faces towards tetrahedra and prisms), 2 prisms per hex in
+                               the layers k < max(1, nz/10) and k >= nz - max(1, nz/10),
+                               6 Kuhn tets per hex elsewhere */
     int32_t nx, ny, nz;
     double  lx, ly, lz;     /* periodic box [0,lx]x[0,ly]x[0,lz] */
     double  jitter;         /* interior vertices moved by U(-j,j)*spacing per coordinate */
