@@ -11,9 +11,11 @@ is reported beside it.  The same line carries a block for the 128^3-hex
 WENO-4 mesh (FP64, mixed, and the adjoint over a march long enough that the
 kept reconstructions run out of their memory budget) and one for the
 507,600-cell mixed-element channel (WENO-3, FP64 and mixed, the adjoint over
-its 50-step march).
+its 50-step march).  `--config c5` or `--extra c5` measures the WENO-5 CWENO
+blast box (FP64 and mixed) at `--c5-n` hexes per axis.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1|c4] [--extra c3,c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1|c4|c5] [--extra c3,c4]
+                       [--c5-n N]
 """
 from __future__ import annotations
 
@@ -52,6 +54,15 @@ def recon_bytes_per_cell(case, precision: int) -> float:
     b = 4 if precision == 1 else 8
     adm = case.array("admis").astype(np.float64).mean()
     nqp = face_points(case).mean()
+    if case.spec.ees == 5:
+        # CWENO: the central stencil's pseudo-inverse, the sectors' low-order
+        # ones over their first imax2 members, both indicators, mbar and mbar - mbar_c
+        cw = case.cweno()
+        id2, im2 = cw.idegfree2, cw.inumneighbours2 - 1
+        by = case.imax * case.ideg * b + (adm - 1) * im2 * id2 * b
+        by += case.ideg * (case.ideg + 1) / 2 * b + id2 * (id2 + 1) / 2 * b
+        by += (case.imax + (adm - 1) * im2) * 4 + nqp * 3 * b + (case.ideg + id2) * b + 5 * 8
+        return float(by)
     by = adm * case.imax * case.ideg * b
     if case.iweno == 1:
         by += case.ideg * (case.ideg + 1) / 2 * b
@@ -68,6 +79,9 @@ def recon_kernel_name(case, precision: int) -> str:
     nf = int((nv > 0).sum(1).max())
     nqm = max(case.qp_quad if (nv == 4).any() else 0, case.qp_tri if (nv == 3).any() else 0)
     nfq = nf * nqm
+    if case.ideg == 34:  # WENO-5: the per-cell kernel (CWENO: its compact-layout variant)
+        r = "float" if precision else "double"
+        return f"k_recon<{r},34,69,7{',true,9,19' if case.spec.ees == 5 else ''}>"
     stream = ((ts == 5 and nfq <= 24) or (ts == 7 and nfq > 24)) if case.ideg == 19 else (ts in (5, 7) and nfq <= 24)
     nql = 3 if nfq <= 24 else 7
     if stream:
@@ -96,6 +110,9 @@ def make_solver(case, precision: int, **opts):
     if case.spec.mach > 0:  # the channel: characteristic inflow/outflow at the free stream
         fs = case.spec.freestream()
         s.set_boundary(1, fs, fs)
+    elif any(b != 0 for b in case.spec.bc):  # outflow sides with BOUNDTYPE 0: the ghosts copy the cell
+        amb = np.array([1.0, 0.0, 0.0, 0.0, 1.0 / (case.spec.gamma - 1.0)])
+        s.set_boundary(0, amb, amb)
     return s
 
 
@@ -123,7 +140,7 @@ def run_path(case, precision: int, steps: int, warmup: int, cfl: float, u0: np.n
     # sum of the residual over all cells relative to its absolute sum: round-off on a closed (periodic)
     # domain; with inflow/outflow faces it is the net boundary flux, so it is not reported there
     cons = None
-    if case.spec.mach <= 0:
+    if all(b == 0 for b in case.spec.bc):
         cons = float(np.abs(rhs.sum(axis=0)).max() / max(np.abs(rhs).sum(axis=0).max(), 1e-300))
     return {"elapsed": elapsed, "t_sim": t_sim, "u": u, "stats": st, "conservation": cons}
 
@@ -294,7 +311,7 @@ def config_block(key: str, steps: int, warmup: int, adjoint_steps: int) -> dict:
     from ucns3d_amd.case import PRESETS, Case
     from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 
-    spec = PRESETS[key]
+    spec = spec_of(key)
     t0 = time.time()
     case = Case(spec)
     u0 = case.init_state()
@@ -316,12 +333,28 @@ def config_block(key: str, steps: int, warmup: int, adjoint_steps: int) -> dict:
     return out
 
 
+C5_N = 121
+
+
+def spec_of(key: str):
+    """the configuration's spec; C5 at --c5-n hexes per axis (the blast box's
+    WENO-5 preprocessing runs at about 0.3 ms per cell on 8 cores, so its full
+    121^3 takes most of an hour)"""
+    from ucns3d_amd.case import PRESETS, reduced
+    if key == "c5" and C5_N != PRESETS["c5"].n:
+        return reduced("c5", C5_N)
+    return PRESETS[key]
+
+
 def main() -> int:
+    global C5_N
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-n", type=int, default=40,
+                    help="hexes per axis of the C5 blast box (121: the configuration's 8.1M cells)")
     ap.add_argument("--extra", default="c3,c4",
                     help="comma-separated further configurations reported in blocks of their own ('' for none)")
     ap.add_argument("--no-mixed", action="store_true")
@@ -336,6 +369,7 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference point for HBM byte counters)")
     args = ap.parse_args()
+    C5_N = args.c5_n
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the whole mesh as one partition on one GPU; "
                                    "--gpus must be 1 (the multi-GPU halo design is documented, not built)"}))
@@ -358,7 +392,7 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
         return 0
-    spec = PRESETS[args.config]
+    spec = spec_of(args.config)
     t0 = time.time()
     case = Case(spec)
     u0 = case.init_state()
@@ -399,7 +433,7 @@ def main() -> int:
                         "kernel": bmx["roofline"]["kernel"], "traffic": bmx["roofline"]["traffic"],
                         "step_roofline_frac": bmx["step_roofline_frac"],
                         "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps}
-    if args.adjoint_steps > 0:
+    if args.adjoint_steps > 0 and args.config != "c5":
         out["adjoint"] = run_adjoint(case, UCNS_FP64, args.adjoint_steps, spec.cfl, u0)
         log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu:
@@ -408,7 +442,8 @@ def main() -> int:
         log(f"cpu baseline: {out['cpu_baseline']}")
     del case
     for key in [k for k in args.extra.split(",") if k and k != args.config]:
-        adj = 0 if args.adjoint_steps <= 0 else (PRESETS[key].steps if key == "c4" else args.adjoint_steps)
+        # (the adjoint does not cover WENO-5 / CWENO: none for c5)
+        adj = 0 if args.adjoint_steps <= 0 or key == "c5" else (PRESETS[key].steps if key == "c4" else args.adjoint_steps)
         out[key] = config_block(key, min(args.steps, 10), 1, adj)
     print(json.dumps(out), flush=True)
     return 0
