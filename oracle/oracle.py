@@ -57,6 +57,10 @@ def lib() -> C.CDLL:
         L.oracle_set_uleft_form.restype = None
         L.oracle_set_face_once.argtypes = [C.c_int]
         L.oracle_set_face_once.restype = None
+        L.oracle_set_raw.argtypes = [C.c_int]
+        L.oracle_set_raw.restype = None
+        L.oracle_char_eigen.argtypes = [C.c_double, f64, f64, f64, f64]
+        L.oracle_char_eigen.restype = None
         L.oracle_ghost.argtypes = [C.c_int, C.c_double, f64, C.c_double, C.c_double, f64]
         L.oracle_ghost.restype = None
         L.oracle_residual_tl.argtypes = [vp, vp, vp, f64, f64, f64, f64, f64]
@@ -212,6 +216,22 @@ def set_face_once(once: bool) -> None:
     neighbour, instead of from both sides with each side's own face angles
     and area (process-wide): equal in exact arithmetic, rounded differently"""
     lib().oracle_set_face_once(1 if once else 0)
+
+
+def set_raw(on: bool) -> None:
+    """reconstruct without the troubled-cell logic (CHECKSOL, MUSCL,
+    CHECKSOLX): the WENO face values alone (process-wide)"""
+    lib().oracle_set_raw(1 if on else 0)
+
+
+def char_eigen(gamma, rl, rr):
+    """(R, L): the eigenvectors of characteristic WENO at the average of the
+    two rotated states rl, rr"""
+    rl = np.ascontiguousarray(rl, dtype=np.float64)
+    rr = np.ascontiguousarray(rr, dtype=np.float64)
+    R, L = np.zeros(25), np.zeros(25)
+    lib().oracle_char_eigen(gamma, _p(rl), _p(rr), _p(R), _p(L))
+    return R.reshape(5, 5), L.reshape(5, 5)
 
 
 def ghost(code, gamma, cl, a1, a2):

@@ -570,6 +570,151 @@ static void weno_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) 
             }
 }
 
+/* Eigenvectors of the x-split Euler flux Jacobian at the face's averaged
+ * state (COMPUTE_EIGENVECTORS, flow_operations.f90:3934-3995): density,
+ * velocity and total energy averaged arithmetically between the two rotated
+ * states rl, rr, pressure and speed of sound from those; R columns (u-a, u,
+ * v-shear, w-shear, u+a), L = R^-1 in the textbook form (Toro, Riemann
+ * Solvers, 3rd ed., sec. 3.2.2) with b = (gamma-1)/a^2.  R and L are [5][5]
+ * row-major. */
+template <class T>
+static void char_eigen(double gamma, const T *rl, const T *rr, T *R, T *L) {
+    const T rs = 0.5 * (rl[0] + rr[0]);
+    const T us = 0.5 * (rl[1] / rl[0] + rr[1] / rr[0]);
+    const T vs = 0.5 * (rl[2] / rl[0] + rr[2] / rr[0]);
+    const T ws = 0.5 * (rl[3] / rl[0] + rr[3] / rr[0]);
+    const T es = 0.5 * (rl[4] + rr[4]);
+    const T q2 = us * us + vs * vs + ws * ws;
+    const T ps = (gamma - 1.0) * (es - 0.5 * rs * q2);
+    const T as = sqrt(gamma * ps / rs);
+    const T hs = 0.5 * q2 + as * as / (gamma - 1.0);
+    const T Rm[5][5] = {{T(1.0), T(1.0), T(0.0), T(0.0), T(1.0)},
+                        {us - as, us, T(0.0), T(0.0), us + as},
+                        {vs, vs, T(1.0), T(0.0), vs},
+                        {ws, ws, T(0.0), T(1.0), ws},
+                        {hs - us * as, 0.5 * q2, vs, ws, hs + us * as}};
+    const T b = (gamma - 1.0) / (as * as), bq = 0.5 * b * q2, oa = 1.0 / as;
+    const T Lm[5][5] = {{0.5 * (bq + us * oa), -0.5 * (b * us + oa), -0.5 * b * vs, -0.5 * b * ws, 0.5 * b},
+                        {1.0 - bq, b * us, b * vs, b * ws, -b},
+                        {-vs, T(0.0), T(1.0), T(0.0), T(0.0)},
+                        {-ws, T(0.0), T(0.0), T(1.0), T(0.0)},
+                        {0.5 * (bq - us * oa), -0.5 * (b * us - oa), -0.5 * b * vs, -0.5 * b * ws, 0.5 * b}};
+    for (int a = 0; a < 5; ++a)
+        for (int c = 0; c < 5; ++c) {
+            R[a * 5 + c] = Rm[a][c];
+            L[a * 5 + c] = Lm[a][c];
+        }
+}
+
+/* Skip the troubled-cell logic (CHECKSOL, MUSCL, CHECKSOLX) in reconstruct
+ * (oracle_set_raw): the WENO face values alone.  Test hook. */
+static int g_raw = 0;
+
+/* One cell of CHARACTERISTIC_RECONSTRUCTION (reconstruct_p.f90:497-758, EES=0):
+ * per face, the neighbour's cell average (WENO_NEIGHBOUR, 761-917: the
+ * boundary ghost of the cell's own average on boundary faces), both rotated
+ * into the face frame, the eigenvectors of their average (char_eigen), every
+ * stencil's coefficients and the cell average projected with L
+ * (compute_gradcharv_smoothindicator, 1690-1818), indicators per
+ * characteristic variable with INDICATOR (diag_At_B_A, 1608-1688), WENO
+ * weights per characteristic variable, the weighted characteristic
+ * coefficients projected back with R, and that face's values
+ * FINDW_0 + sum_k phi_k FINDW_k. */
+template <class T>
+static void char_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) {
+    const int ideg = c->ideg, imax = c->imax, nnb = c->nnb, ts = c->ts;
+    const int admis = c->p->admis[i];
+    const T *ui = u + i * NV;
+    T *grad = w.grad.data(), *mat1 = w.mat1.data();
+    double *phi = w.phi.data();
+    const double eps = c->s->weno_eps;
+    /* LSQ (COMPUTE_GRADIENTS_MEAN_LSQ) */
+    for (int ll = 0; ll < admis; ++ll) {
+        const int32_t *ih = c->p->ihexl + ((size_t)i * ts + ll) * nnb;
+        const T *s1 = u + (size_t)(ih[0] - c->base) * NV;
+        for (int iq = 0; iq < imax; ++iq) {
+            const T *s2 = u + (size_t)(ih[iq + 1] - c->base) * NV;
+            for (int v = 0; v < NV; ++v) mat1[iq * NV + v] = s2[v] - s1[v];
+        }
+        const double *P = c->p->invmat + ((size_t)i * ts + ll) * imax * ideg;
+        for (int k = 0; k < ideg; ++k)
+            for (int v = 0; v < NV; ++v) {
+                T acc = T(0.0);
+                for (int iq = 0; iq < imax; ++iq) acc = acc + P[(size_t)iq * ideg + k] * mat1[iq * NV + v];
+                grad[((size_t)ll * ideg + k) * NV + v] = acc;
+            }
+    }
+    const double *B = c->p->indicator + (size_t)i * ideg * ideg;
+    const double *integ = c->p->integ_basis + (size_t)i * ideg;
+    const double vol11 = c->p->volume11[i];
+    std::vector<T> W((size_t)admis * (ideg + 1) * NV), F((size_t)(ideg + 1) * NV), ba(ideg);
+    for (int f = 0; f < c->m->ifca[i]; ++f) {
+        const int64_t slot = i * c->nf + f;
+        const double a1 = c->m->faceanglex[slot], a2 = c->m->faceangley[slot];
+        const int bcode = c->m->bcode[slot];
+        const int bnd = bcode != 0 && bcode != 5 && bcode != 50;
+        T ur[NV], rl[NV], rr[NV], R[25], L[25];
+        if (bnd)
+            ghost(bcode, c->s->gamma, ui, a1, a2, ur);
+        else
+            for (int v = 0; v < NV; ++v) ur[v] = u[(size_t)(c->m->ineigh[slot] - c->base) * NV + v];
+        rotatef(rl, ui, a1, a2);
+        rotatef(rr, ur, a1, a2);
+        char_eigen(c->s->gamma, rl, rr, R, L);
+        /* characteristic coefficients W[ll][k][ch], k = 0 the cell average */
+        for (int ll = 0; ll < admis; ++ll)
+            for (int k = 0; k <= ideg; ++k)
+                for (int ch = 0; ch < NV; ++ch) {
+                    T acc = T(0.0);
+                    for (int v = 0; v < NV; ++v)
+                        acc = acc + (k == 0 ? ui[v] : grad[((size_t)ll * ideg + k - 1) * NV + v]) * L[ch * 5 + v];
+                    W[((size_t)ll * (ideg + 1) + k) * NV + ch] = acc;
+                }
+        /* indicators and weights per characteristic variable */
+        T omega[8][NV];
+        for (int ch = 0; ch < NV; ++ch) {
+            T beta[8], wt[8];
+            for (int ll = 0; ll < admis; ++ll) {
+                for (int a = 0; a < ideg; ++a) {
+                    T acc = T(0.0);
+                    for (int b = 0; b < ideg; ++b)
+                        acc = acc + B[(size_t)a * ideg + b] * W[((size_t)ll * (ideg + 1) + b + 1) * NV + ch];
+                    ba[a] = acc;
+                }
+                T dot = T(0.0);
+                for (int a = 0; a < ideg; ++a) dot = dot + W[((size_t)ll * (ideg + 1) + a + 1) * NV + ch] * ba[a];
+                beta[ll] = dot;
+            }
+            weno_weights(admis, beta, c->s->lwci1, eps, wt);
+            for (int ll = 0; ll < admis; ++ll) omega[ll][ch] = wt[ll];
+        }
+        /* weighted characteristic coefficients, back to conserved variables */
+        for (int k = 0; k <= ideg; ++k) {
+            T lim[NV];
+            for (int ch = 0; ch < NV; ++ch) {
+                T acc = T(0.0);
+                for (int ll = 0; ll < admis; ++ll) acc = acc + W[((size_t)ll * (ideg + 1) + k) * NV + ch] * omega[ll][ch];
+                lim[ch] = acc;
+            }
+            for (int v = 0; v < NV; ++v) {
+                T acc = T(0.0);
+                for (int ch = 0; ch < NV; ++ch) acc = acc + R[v * 5 + ch] * lim[ch];
+                F[(size_t)k * NV + v] = acc;
+            }
+        }
+        for (int q = 0; q < nqp_of(c, i, f); ++q) {
+            const double *x = c->p->qpoints + (((size_t)i * c->nf + f) * c->nqm + q) * 3;
+            basis_rec(x, ideg, integ, vol11, phi);
+            T *o = uleft_at(c, ul, i, f, q);
+            for (int v = 0; v < NV; ++v) {
+                T res = T(0.0);
+                for (int k = 0; k < ideg; ++k) res = res + phi[k] * F[(size_t)(k + 1) * NV + v];
+                o[v] = (g_uleft_mean_first ? ui[v] : F[v]) + res;
+            }
+        }
+    }
+}
+
 /* CHECKSOL / CHECKSOLX face-point test (reconstruct_p.f90:3183-3196, 3294-3310) */
 template <class T>
 static int trouble_test(const ctx_t *c, const T *u, const T *ul, int64_t i, double jump) {
@@ -695,12 +840,15 @@ static int reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucn
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < nc; ++i) {
             if (s->iweno == 1) {
-                if (s->ees == 5)
+                if (s->wenwrt == 2)
+                    char_cell(&c, u, i, ul, w);
+                else if (s->ees == 5)
                     cweno_cell(&c, u, i, ul, w);
                 else
                     weno_cell(&c, u, i, ul, w);
                 int red = 0;
-                if (p->full[i] == 0) red = 1;
+                if (g_raw) red = 0;
+                else if (p->full[i] == 0) red = 1;
                 else red = trouble_test(&c, u, ul, i, 0.85);
                 reduce[i] = red;
             } else {
@@ -710,6 +858,7 @@ static int reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucn
         /* MUSCL for (TROUBLED & REDUCE) | (TROUBLED & ~FULL); TROUBLED=1 for every FV cell */
 #pragma omp for schedule(static) reduction(+ : n_red)
         for (int64_t i = 0; i < nc; ++i) {
+            if (g_raw) continue;
             if (reduce[i] == 1 || p->full[i] == 0) {
                 lsq_central(&c, u, i, w);
                 muscl_cell(&c, u, i, ul, w);
@@ -719,7 +868,7 @@ static int reconstruct(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucn
         /* CHECKSOLX: first order where the limited values still jump */
 #pragma omp for schedule(static) reduction(+ : n_first)
         for (int64_t i = 0; i < nc; ++i) {
-            if (trouble_test(&c, u, ul, i, 0.95)) {
+            if (!g_raw && trouble_test(&c, u, ul, i, 0.95)) {
                 for (int f = 0; f < m->ifca[i]; ++f)
                     for (int q = 0; q < c.nqm; ++q) {
                         T *o = uleft_at(&c, ul, i, f, q);
@@ -927,6 +1076,10 @@ void oracle_set_face_once(int once) { g_face_once = once ? 1 : 0; }
 /* the CWENO data the scheme's ees = 5 reads (NULL: none); the struct and its
  * arrays must outlive the calls that use them */
 void oracle_set_cweno(const ucns_cweno_v1 *cw) { g_cw = cw; }
+void oracle_set_raw(int on) { g_raw = on; }
+void oracle_char_eigen(double gamma, const double *rl, const double *rr, double *R, double *L) {
+    char_eigen(gamma, rl, rr, R, L);
+}
 void oracle_cweno_weights(int admis, const double *beta, double lwci1, double eps, int wenoz, double *omega) {
     cweno_weights(admis, beta, lwci1, eps, wenoz, omega);
 }

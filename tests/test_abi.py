@@ -119,6 +119,9 @@ def test_create_rejects_bad_arguments(glib, small_case):
         (None, lambda s: setattr(s, "ees", 3), -2, "ees must be 0"),
         (None, lambda s: setattr(s, "wenoz", 1), -2, "wenoz 1 is a CWENO option"),
         (None, lambda s: setattr(s, "wenoz", 2), -1, "wenoz must be"),
+        (None, lambda s: setattr(s, "wenwrt", 3), -2, "wenwrt must be 1"),
+        # (the small case is MUSCL: characteristic reconstruction is a WENO option)
+        (None, lambda s: setattr(s, "wenwrt", 2), -2, "characteristic reconstruction"),
         (None, lambda s: setattr(s, "riemann", 2), -2, "HLLC"),
         (None, lambda s: setattr(s, "limiter", 2), -2, "Barth"),
         (None, lambda s: setattr(s, "qp_quad", 0), -1, "quadrature"),

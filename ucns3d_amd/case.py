@@ -46,6 +46,7 @@ class CaseSpec:
     mach: float = 0.0     # > 0: the initial state is a uniform flow along x at this Mach number, rho = p = 1
     ees: int = 0          # 5: CWENO (central polynomial reconstituted from low-order sector polynomials)
     wenoz: int = 0        # 1: CWENO-Z weights (with ees = 5)
+    wenwrt: int = 1       # 2: WENO in characteristic variables, face by face (WENWRT 2)
     init: str = ""        # "blast": rho = 8, p = 10 for cell centres within 0.2 of the box centre, rho = p = 1
                           # elsewhere, at rest; otherwise the vortex (or the free stream when mach > 0)
 
@@ -173,6 +174,7 @@ class Case:
         if rc != 0:
             raise RuntimeError(self._lib.ucnsp_last_error().decode())
         s.wenoz = self.spec.wenoz
+        s.wenwrt = self.spec.wenwrt
         return m, s, p
 
     def cweno(self):

@@ -96,6 +96,12 @@ struct ReconParams {
     const R *Bc;           // [ncell][ideg2*(ideg2+1)/2]
     const R *dmb;          // [ncell][ideg2]
     int wenoz;
+    // characteristic WENO (wenwrt 2): per cell face the neighbour (negative:
+    // the boundary code) and the cell's own face angles (cos a1, sin a1, cos
+    // a2, sin a2); the free-stream data of the boundary ghosts
+    const int32_t *chnb;   // [ncell][nf]
+    const double *chang;   // [ncell][nf][4]
+    ucns_boundary_v1 bc;
 };
 
 __device__ inline void cons2prim_d(double gamma, const double *u, double *w) {
@@ -191,6 +197,45 @@ __device__ inline double wave_min(double v) {
     return v;
 }
 
+// face rotation and boundary ghost states (defined with the flux kernels)
+template <class T>
+__device__ inline void rotf(const T *a, double c1, double s1, double c2, double s2, T *r);
+template <class T, class F>
+__device__ inline void ghost_state(int code, int boundtype, double gamma, const T *cl, double c1, double s1, double c2,
+                                   double s2, const F *fin, const F *fout, T *cr);
+
+// Eigenvectors of the x-split Euler flux Jacobian at the average of two
+// rotated states (COMPUTE_EIGENVECTORS, src/flow_operations.f90:3934-3995):
+// density, velocity and total energy averaged arithmetically; R columns
+// (u-a, u, v-shear, w-shear, u+a), L = R^-1 in the textbook form with
+// b = (gamma-1)/a^2.  Row-major [5][5].
+__device__ inline void char_eigen_d(double gamma, const double *rl, const double *rr, double *R, double *L) {
+    const double rs = 0.5 * (rl[0] + rr[0]);
+    const double us = 0.5 * (rl[1] / rl[0] + rr[1] / rr[0]);
+    const double vs = 0.5 * (rl[2] / rl[0] + rr[2] / rr[0]);
+    const double ws = 0.5 * (rl[3] / rl[0] + rr[3] / rr[0]);
+    const double es = 0.5 * (rl[4] + rr[4]);
+    const double q2 = us * us + vs * vs + ws * ws;
+    const double ps = (gamma - 1.0) * (es - 0.5 * rs * q2);
+    const double as = sqrt(gamma * ps / rs);
+    const double hs = 0.5 * q2 + as * as / (gamma - 1.0);
+    const double b = (gamma - 1.0) / (as * as), bq = 0.5 * b * q2, oa = 1.0 / as;
+    const double Rm[25] = {1.0, 1.0, 0.0, 0.0, 1.0,
+                           us - as, us, 0.0, 0.0, us + as,
+                           vs, vs, 1.0, 0.0, vs,
+                           ws, ws, 0.0, 1.0, ws,
+                           hs - us * as, 0.5 * q2, vs, ws, hs + us * as};
+    const double Lm[25] = {0.5 * (bq + us * oa), -0.5 * (b * us + oa), -0.5 * b * vs, -0.5 * b * ws, 0.5 * b,
+                           1.0 - bq, b * us, b * vs, b * ws, -b,
+                           -vs, 0.0, 1.0, 0.0, 0.0,
+                           -ws, 0.0, 0.0, 1.0, 0.0,
+                           0.5 * (bq - us * oa), -0.5 * (b * us - oa), -0.5 * b * vs, -0.5 * b * ws, 0.5 * b};
+    for (int e = 0; e < 25; ++e) {
+        R[e] = Rm[e];
+        L[e] = Lm[e];
+    }
+}
+
 template <typename R, int IDEG, int IMAX, int TSMAX>
 struct ReconShared {
     static constexpr int NQMAX = 6 * 16;
@@ -216,7 +261,8 @@ struct ReconShared {
 
 // Fused reconstruction, one wavefront per cell.  CW: CWENO with sectors of
 // ID2 coefficients over their first IM2 members (src/reconstruct_p.f90:955-1049).
-template <typename R, int IDEG, int IMAX, int TSMAX, bool CW = false, int ID2 = 9, int IM2 = 19>
+// CH: characteristic-variable WENO, face by face (src/reconstruct_p.f90:497-758).
+template <typename R, int IDEG, int IMAX, int TSMAX, bool CW = false, int ID2 = 9, int IM2 = 19, bool CH = false>
 __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
     static_assert(!CW || (ID2 <= 9 && IM2 <= IMAX), "CWENO sectors: at most 9 coefficients");
     constexpr int NROW = (TSMAX * IDEG + WAVE - 1) / WAVE;
@@ -441,7 +487,104 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
         }
         __syncthreads();
     }
-    if (p.iweno == 1) {
+    if constexpr (CH) {
+        // ---- characteristic WENO: per face, the eigenvectors of the averaged
+        // cell / neighbour state in the face frame (WENO_NEIGHBOUR: the ghost of
+        // the cell's own average on boundary faces), every stencil's
+        // coefficients projected with L, indicators and weights per
+        // characteristic variable, the weighted coefficients projected back
+        // with R, and that face's values (compute_gradcharv_smoothindicator,
+        // diag_At_B_A, src/reconstruct_p.f90:1608-1818) ----
+        __shared__ double chR[25], chL[25];
+        __shared__ double cbeta[TSMAX * NV];
+        __shared__ R wch[TSMAX * IDEG * NV];
+        double wi[NV];
+        cons2prim_d(p.gamma, ui, wi);
+        int trouble = 0;
+        const R *qpc = p.qp + (size_t)cell * p.nf * p.nqm * 3;
+        for (int f = 0; f < ifca; ++f) {
+            if (lane == 0) {
+                const int nbf = p.chnb[(size_t)cell * p.nf + f];
+                const double *an = p.chang + ((size_t)cell * p.nf + f) * 4;
+                double ur[NV], rl[NV], rr[NV];
+                if (nbf < 0)
+                    ghost_state(-nbf, p.bc.boundtype, p.gamma, ui, an[0], an[1], an[2], an[3], p.bc.inflow,
+                                p.bc.outflow, ur);
+                else
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) ur[v] = p.U[(size_t)nbf * NV + v];
+                rotf(ui, an[0], an[1], an[2], an[3], rl);
+                rotf(ur, an[0], an[1], an[2], an[3], rr);
+                char_eigen_d(p.gamma, rl, rr, chR, chL);
+            }
+            __syncthreads();
+            for (int e = lane; e < admis * IDEG * NV; e += WAVE) {
+                const int c = e % NV, sk = e / NV;
+                double acc = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc += (double)sh.g[sk * NV + v] * chL[c * 5 + v];
+                wch[e] = (R)acc;
+            }
+            __syncthreads();
+            if (lane < admis * NV) {
+                const int s = lane / NV, c = lane % NV;
+                const R *ws = &wch[s * IDEG * NV + c];
+                double beta = 0.0;
+                for (int a = 0; a < IDEG; ++a) {
+                    double ba = 0.0;
+                    for (int b2 = 0; b2 < IDEG; ++b2) ba += (double)sh.b[a * IDEG + b2] * (double)ws[b2 * NV];
+                    beta += (double)ws[a * NV] * ba;
+                }
+                cbeta[lane] = beta;
+            }
+            __syncthreads();
+            if (lane < admis * NV) {
+                // lambda/(eps+beta)^4 per characteristic variable, in FP64 in both precisions
+                const int s = lane / NV, c = lane % NV;
+                double sum = 0.0, mine = 0.0;
+                for (int t = 0; t < admis; ++t) {
+                    const double x = p.eps + cbeta[t * NV + c];
+                    const double x2 = x * x;
+                    const double w = (t == 0 ? p.lwci1 : 1.0) / (x2 * x2);
+                    sum += w;
+                    if (t == s) mine = w;
+                }
+                sh.u.late.omega[lane] = (R)(mine / sum);
+            }
+            __syncthreads();
+            for (int e = lane; e < IDEG * NV; e += WAVE) {
+                const int k = e / NV, v = e % NV;
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NV; ++c) {
+                    double lim = 0.0;
+                    for (int s = 0; s < admis; ++s)
+                        lim += (double)wch[(s * IDEG + k) * NV + c] * (double)sh.u.late.omega[s * NV + c];
+                    acc += chR[v * 5 + c] * lim;
+                }
+                sh.u.late.gh[e] = (R)acc;
+            }
+            __syncthreads();
+            const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+            for (int q = lane; q < nq; q += WAVE) {
+                const int e = f * p.nqm + q;
+                R x[3] = {qpc[e * 3], qpc[e * 3 + 1], qpc[e * 3 + 2]};
+                R phi[IDEG];
+                basis_at<R, IDEG>(x, sh.mb, phi);
+                double uq[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    R res = R(0);
+                    for (int k = 0; k < IDEG; ++k) res += phi[k] * sh.u.late.gh[k * NV + v];
+                    uq[v] = ui[v] + (double)res;
+                    sh.u.late.ul[e * NV + v] = res;
+                }
+                if (full && jump_test(p.gamma, wi, uq, 0.85)) trouble = 1;
+            }
+            __syncthreads();
+        }
+        reduce = reduce || (__syncthreads_or(trouble) != 0);
+    } else if (p.iweno == 1) {
         if constexpr (!CW) {
         // ---- phase 2: beta_{s,v} = g^T (B g) (src/reconstruct_p.f90:995-1004) ----
         if (lane < admis * NV) {
@@ -3542,6 +3685,11 @@ struct ucns_ctx {
     // cell (ucns_upload_cweno), and on the device the sectors' packed
     // indicators and the means' differences mbar - mbar_c
     int ees = 0, wenoz = 0, ideg2 = 0, imax2 = 0;
+    // characteristic WENO (wenwrt 2): per cell face the neighbour or -(boundary
+    // code), and the cell's own face angles' cosines and sines
+    int wenwrt = 1;
+    int32_t *chnb = nullptr;
+    double *chang = nullptr;
     bool have_cweno = false;
     std::vector<double> h_cwP, h_cwB, h_cwIB;
     void *Bc = nullptr, *dmb = nullptr;
@@ -3653,6 +3801,9 @@ int launch_recon_t(ucns_ctx *ctx) {
     p.Bc = (const R *)ctx->Bc;
     p.dmb = (const R *)ctx->dmb;
     p.wenoz = ctx->wenoz;
+    p.chnb = ctx->chnb;
+    p.chang = ctx->chang;
+    p.bc = ctx->bc;
     if (ctx->tiled) {
         ReconWParams<R> w{};
         w.ncell = ctx->ncell;
@@ -3777,6 +3928,19 @@ int launch_recon_t(ucns_ctx *ctx) {
         return UCNS_OK;
     }
     const dim3 grid(p.per_xcd * NXCD), block(WAVE);
+    if (ctx->wenwrt == 2) {
+        if (ctx->ideg == 19 && ctx->imax == 39)
+            hipLaunchKernelGGL((k_recon<R, 19, 39, 7, false, 9, 19, true>), grid, block, 0, ctx->stream, p);
+        else if (ctx->ideg == 9 && ctx->imax == 19)
+            hipLaunchKernelGGL((k_recon<R, 9, 19, 7, false, 9, 19, true>), grid, block, 0, ctx->stream, p);
+        else if (ctx->ideg == 34 && ctx->imax == 69)
+            hipLaunchKernelGGL((k_recon<R, 34, 69, 7, false, 9, 19, true>), grid, block, 0, ctx->stream, p);
+        else {
+            ctx->err = "no characteristic reconstruction kernel for idegfree=" + std::to_string(ctx->ideg);
+            return UCNS_EUNSUPPORTED;
+        }
+        return UCNS_OK;
+    }
     if (ctx->cw_cell) {
         if (ctx->ideg == 34 && ctx->imax == 69 && ctx->ideg2 == 9 && ctx->imax2 == 19) {
             hipLaunchKernelGGL((k_recon<R, 34, 69, 7, true, 9, 19>), grid, block, 0, ctx->stream, p);
@@ -4064,6 +4228,10 @@ int ensure_adjoint(ucns_ctx *ctx) {
     }
     if (ctx->ideg == 34) {
         ctx->err = "the adjoint covers idegfree 3, 9 and 19 (not WENO-5, idegfree 34)";
+        return UCNS_EUNSUPPORTED;
+    }
+    if (ctx->wenwrt == 2) {
+        ctx->err = "the adjoint does not cover characteristic reconstruction (wenwrt 2)";
         return UCNS_EUNSUPPORTED;
     }
     if (ctx->adj_ready) return UCNS_OK;
@@ -4737,7 +4905,10 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     if (s->ees == 5 && s->iweno != 1) return bad(UCNS_EBADARG, "CWENO (ees 5) needs iweno 1");
     if (s->wenoz != 0 && s->wenoz != 1) return bad(UCNS_EBADARG, "wenoz must be 0 or 1");
     if (s->wenoz == 1 && s->ees != 5) return bad(UCNS_EUNSUPPORTED, "wenoz 1 is a CWENO option (ees 5)");
-    if (s->wenwrt != 1) return bad(UCNS_EUNSUPPORTED, "only conserved-variable reconstruction (wenwrt=1)");
+    if (s->wenwrt != 1 && s->wenwrt != 2)
+        return bad(UCNS_EUNSUPPORTED, "wenwrt must be 1 (conserved) or 2 (characteristic variables)");
+    if (s->wenwrt == 2 && (s->iweno != 1 || s->ees != 0))
+        return bad(UCNS_EUNSUPPORTED, "characteristic reconstruction (wenwrt 2) is built for WENO with ees 0");
     if (s->limiter != 1) return bad(UCNS_EUNSUPPORTED, "only the Barth-Jespersen limiter (1)");
     if (s->riemann != 1) return bad(UCNS_EUNSUPPORTED, "only the HLLC solver (1)");
     if (s->typesten < 1 || s->typesten > 7) return bad(UCNS_EUNSUPPORTED, "typesten must be 1..7");
@@ -4762,6 +4933,7 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     ctx->precision = s->precision;
     ctx->iweno = s->iweno;
     ctx->ees = s->ees;
+    ctx->wenwrt = s->wenwrt;
     ctx->wenoz = s->wenoz;
     ctx->ideg = s->idegfree;
     ctx->nnb = s->inumneighbours;
@@ -4835,6 +5007,8 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     if (nf * nqm > 6 * 16) return bad(UCNS_EUNSUPPORTED, "too many face Gauss points per cell");
     // unique faces: owner = lower cell id (equal ids: lower slot)
     std::vector<int32_t> cinfo(nc, 0), cnb((size_t)nc * nf, -1), cface((size_t)nc * nf, 0);
+    std::vector<int32_t> chnb(ctx->wenwrt == 2 ? (size_t)nc * nf : 0, 0);
+    std::vector<double> chang(ctx->wenwrt == 2 ? (size_t)nc * nf * 4 : 0, 0.0);
     std::vector<int32_t> fcell, fslot;
     std::vector<double> frot, farea;
     for (int i = 0; i < nc; ++i) {
@@ -4846,6 +5020,16 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
         for (int f = 0; f < ifca; ++f) {
             const size_t sl = (size_t)i * nfa + f;
             const int bc = m->bcode[sl];
+            if (!chnb.empty()) {
+                const bool bnd = bc == 1 || bc == 2 || bc == 3 || bc == 4 || bc == 6;
+                chnb[(size_t)i * nf + f] = bnd ? -bc : m->ineigh[sl] - base;
+                const double a1 = m->faceanglex[sl], a2 = m->faceangley[sl];
+                double *an = &chang[((size_t)i * nf + f) * 4];
+                an[0] = std::cos(a1);
+                an[1] = std::sin(a1);
+                an[2] = std::cos(a2);
+                an[3] = std::sin(a2);
+            }
             if (bc == 1 || bc == 2 || bc == 3 || bc == 4 || bc == 6) {
                 // boundary face: owner only, the right state is a ghost (BOUNDARYS)
                 if (bc != 3 && bc != 4) ctx->needs_bc = true;
@@ -4935,6 +5119,11 @@ int ucns_create(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, ucns_ctx **out) 
     HIP_TRY(hipMemset(ctx->uleft, 0, ul_elems * (ctx->precision == UCNS_MIXED ? 4 : 8)));
     HIP_TRY(hipMemcpy(ctx->cinfo, cinfo.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->cnb, cnb.data(), cnb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!chnb.empty()) {
+        if ((rc = dalloc(ctx, &ctx->chnb, chnb.size())) || (rc = dalloc(ctx, &ctx->chang, chang.size()))) return rc;
+        HIP_TRY(hipMemcpy(ctx->chnb, chnb.data(), chnb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctx->chang, chang.data(), chang.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(ctx->cface, cface.data(), cface.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->fcell, fcell.data(), fcell.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->fslot, fslot.data(), fslot.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -4991,8 +5180,8 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     }
     ctx->ts = ts;
     const bool mixed = ctx->precision == UCNS_MIXED;
-    ctx->tiled = (ctx->iweno == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) && ts <= 8 &&
-                  ctx->nf * ctx->nqm <= 56);
+    ctx->tiled = (ctx->iweno == 1 && ctx->wenwrt == 1 && ((ID == 19 && IM == 39) || (ID == 9 && IM == 19)) &&
+                  ts <= 8 && ctx->nf * ctx->nqm <= 56);
     ctx->cw_cell = ctx->ees == 5 && !ctx->tiled;
     if (ctx->cw_cell && ID != 34) {
         ctx->err = "CWENO (ees 5) runs on the WENO-3 / WENO-4 register kernels (idegfree 9 or 19) or the WENO-5 "
@@ -5786,6 +5975,8 @@ void ucns_destroy(ucns_ctx *ctx) {
     release_adjoint(ctx);
     dfree(ctx->cinfo);
     dfree(ctx->cnb);
+    dfree(ctx->chnb);
+    dfree(ctx->chang);
     dfree(ctx->cface);
     dfree(ctx->fcell);
     dfree(ctx->fslot);
