@@ -216,6 +216,16 @@ int  ucns_upload_cweno(ucns_ctx *ctx, const ucns_cweno_v1 *cw);
 /* Free-stream data of the boundary codes 1, 2 and 6 (needed before the first
  * residual when the mesh has such faces; codes 3 and 4 need none). */
 int  ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc);
+/* Position-dependent INFLOW() / OUTFLOW() (src/flow_operations.f90:1161-1302,
+ * evaluated by the host at each boundary face's centre, the mean of its nodes
+ * as GET_STATES_BOUNDS takes it, src/dg_functions.f90:1818-1829): conserved
+ * states per face record, [ncell][nface_max][5] in the mesh's cell order,
+ * read at the faces of codes 1, 2 and 6 (the inflow array for 1 and 6, the
+ * outflow array for 2 and 6).  A null array keeps the uniform state of
+ * ucns_set_boundary for its side; both null return to uniform states.  Call
+ * after ucns_set_boundary, which still gives BOUNDTYPE.  The adjoint entry
+ * points refuse a context that holds per-face states. */
+int  ucns_set_boundary_faces(ucns_ctx *ctx, const double *inflow, const double *outflow);
 /* Cell averages U_C(i)%VAL(1,1:5) in / out. */
 int  ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout);
 int  ucns_get_state(ucns_ctx *ctx, double *u, int32_t layout);

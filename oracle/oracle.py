@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
         L.oracle_set_face_once.restype = None
         L.oracle_set_raw.argtypes = [C.c_int]
         L.oracle_set_raw.restype = None
+        L.oracle_set_boundary_faces.argtypes = [vp, vp]
+        L.oracle_set_boundary_faces.restype = None
         L.oracle_char_eigen.argtypes = [C.c_double, f64, f64, f64, f64]
         L.oracle_char_eigen.restype = None
         L.oracle_ghost.argtypes = [C.c_int, C.c_double, f64, C.c_double, C.c_double, f64]
@@ -216,6 +218,20 @@ def set_face_once(once: bool) -> None:
     neighbour, instead of from both sides with each side's own face angles
     and area (process-wide): equal in exact arithmetic, rounded differently"""
     lib().oracle_set_face_once(1 if once else 0)
+
+
+_BFACES = None
+
+
+def set_boundary_faces(inflow=None, outflow=None) -> None:
+    """per-face INFLOW() / OUTFLOW() states [ncell][nface_max][5] for the
+    ghosts of codes 1, 2, 6 (process-wide; the arrays are held until the next
+    call); None: the uniform state of set_boundary for that side"""
+    global _BFACES
+    fi = None if inflow is None else np.ascontiguousarray(inflow, dtype=np.float64)
+    fo = None if outflow is None else np.ascontiguousarray(outflow, dtype=np.float64)
+    _BFACES = (fi, fo)
+    lib().oracle_set_boundary_faces(None if fi is None else fi.ctypes.data, None if fo is None else fo.ctypes.data)
 
 
 def set_raw(on: bool) -> None:

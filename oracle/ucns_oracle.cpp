@@ -172,13 +172,16 @@ static void bc_sub_out(double gamma, const T *s1, const T *s2, T sps, double nx,
 
 /* BOUNDARYS (flow_operations.f90:2860-3375), Euler 3D without a rotating
  * frame: ghost state cr of a boundary face from the face-point state cl */
+/* per-face INFLOW() / OUTFLOW() states, [ncell][nf][5] (oracle_set_boundary_faces; null: g_bc's) */
+static const double *g_bfin = nullptr, *g_bfout = nullptr;
+
 template <class T>
-static void ghost(int code, double gamma, const T *cl, double a1, double a2, T *cr) {
+static void ghost(int code, double gamma, const T *cl, double a1, double a2, T *cr, int64_t slot = -1) {
     const double nx = (cos(a1) * sin(a2)), ny = (sin(a1) * sin(a2)), nz = (cos(a2));
     T s1[NV], s2[NV], fin[NV], fout[NV];
     for (int v = 0; v < NV; ++v) {
-        fin[v] = bc_at<T>(g_bc.inflow, v);
-        fout[v] = bc_at<T>(g_bc.outflow, v);
+        fin[v] = slot >= 0 && g_bfin ? T(g_bfin[slot * NV + v]) : bc_at<T>(g_bc.inflow, v);
+        fout[v] = slot >= 0 && g_bfout ? T(g_bfout[slot * NV + v]) : bc_at<T>(g_bc.outflow, v);
     }
     switch (code) {
     case 1: /* inflow (2880-2925) */
@@ -655,7 +658,7 @@ static void char_cell(const ctx_t *c, const T *u, int64_t i, T *ul, Work<T> &w) 
         const int bnd = bcode != 0 && bcode != 5 && bcode != 50;
         T ur[NV], rl[NV], rr[NV], R[25], L[25];
         if (bnd)
-            ghost(bcode, c->s->gamma, ui, a1, a2, ur);
+            ghost(bcode, c->s->gamma, ui, a1, a2, ur, slot);
         else
             for (int v = 0; v < NV; ++v) ur[v] = u[(size_t)(c->m->ineigh[slot] - c->base) * NV + v];
         rotatef(rl, ui, a1, a2);
@@ -939,7 +942,7 @@ static int fluxes(const ucns_mesh_v2 *m, const ucns_scheme_v2 *s, const ucns_pre
                 T gst[NV];
                 const T *cr;
                 if (bnd) { /* GET_STATES_BOUNDS -> BOUNDARYS (dg_functions.f90:1808-1828) */
-                    ghost(bcode, s->gamma, cl, a1, a2, gst);
+                    ghost(bcode, s->gamma, cl, a1, a2, gst, slot);
                     cr = gst;
                 } else {
                     cr = uleft_at(&c, (T *)ul, j, fj, q);
@@ -1077,6 +1080,10 @@ void oracle_set_face_once(int once) { g_face_once = once ? 1 : 0; }
  * arrays must outlive the calls that use them */
 void oracle_set_cweno(const ucns_cweno_v1 *cw) { g_cw = cw; }
 void oracle_set_raw(int on) { g_raw = on; }
+void oracle_set_boundary_faces(const double *inflow, const double *outflow) {
+    g_bfin = inflow;
+    g_bfout = outflow;
+}
 void oracle_char_eigen(double gamma, const double *rl, const double *rr, double *R, double *L) {
     char_eigen(gamma, rl, rr, R, L);
 }

@@ -239,3 +239,51 @@ def test_code50_neighbours_stay_out_of_the_muscl_bounds():
     assert not np.array_equal(ul5[has50], ul50[has50])
     r50, _ = orc50.residual(u)
     assert np.all(np.abs(r50.sum(0)) <= 1e-12 * np.abs(r50).sum(0))
+
+
+def face_centres(case):
+    """mean of each face record's nodes (CORDINATES3, src/grid_t.f90:953-967),
+    [ncell][nface_max][3]; NaN where a cell has no such face"""
+    fn = case.array("face_nodes").reshape(case.ncell, case.nface_max, 4)
+    xyz = case.array("xyz").reshape(-1, 3)
+    pts = np.where((fn >= 0)[..., None], xyz[np.clip(fn, 0, None)], 0.0)
+    cnt = (fn >= 0).sum(2)[..., None]
+    return np.where(cnt > 0, pts.sum(2) / np.maximum(cnt, 1), np.nan)
+
+
+@pytest.mark.parametrize("code", [1, 2])
+def test_per_face_boundary_states_reach_their_own_faces(code):
+    """per-face INFLOW()/OUTFLOW() states (position-dependent, evaluated at the
+    face centres): every cell's residual equals the one computed with the
+    uniform state of its own boundary faces' region, bit for bit, and cells
+    without such faces are untouched"""
+    case = Case(CaseSpec("jet", HEX, 8, "weno3", jitter=0.1, seed=6, bc=(1, 2, 4, 4, 3, 3)))
+    orc = Oracle(case)
+    u = case.init_state()
+    SA = prim2cons(1.0, 0.5, 0.0, 0.0, 100.0 / 1.4)
+    SB = prim2cons(1.3, 0.8, 0.1, -0.1, 90.0 / 1.4)
+    c = face_centres(case)
+    inside = np.linalg.norm(c[..., 1:] - 0.5 * case.spec.length, axis=-1) < 0.3 * case.spec.length
+    per_face = np.where(inside[..., None], SB, SA)
+    bcode = case.array("bcode").reshape(case.ncell, case.nface_max)
+    on = bcode == code
+    assert on[inside].sum() > 0 and on[~inside & (bcode >= 0)].sum() > 0
+    oracle.set_boundary(1, SA, SA)
+    r_a, _ = orc.residual(u)
+    oracle.set_boundary(1, SB, SB)
+    r_b, _ = orc.residual(u)
+    oracle.set_boundary(1, SA, SA)
+    try:
+        if code == 1:
+            oracle.set_boundary_faces(per_face, None)
+        else:
+            oracle.set_boundary_faces(None, per_face)
+        r, _ = orc.residual(u)
+    finally:
+        oracle.set_boundary_faces(None, None)
+    in_a = (on & ~inside).any(1)
+    in_b = (on & inside).any(1)
+    assert np.array_equal(r[in_b & ~in_a], r_b[in_b & ~in_a])
+    assert np.array_equal(r[~in_b], r_a[~in_b])
+    assert (in_b & ~in_a).sum() > 0 and (in_a & ~in_b).sum() > 0
+    assert not np.array_equal(r_a[in_b], r_b[in_b])  # the regions' states matter

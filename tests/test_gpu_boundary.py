@@ -162,3 +162,62 @@ def test_code50_periodic_faces(scheme):
     m = np.isfinite(ul_ref)
     assert np.abs(ul[m] - ul_ref[m]).max() <= 1e-12 * np.abs(ul_ref[m]).max()
     s.close()
+
+
+def jet_case(wenwrt=1):
+    """subsonic box with a jet in the inflow side: the INFLOW() state differs on
+    the faces whose centres lie within 0.3 L of the side's centre"""
+    from test_boundary import face_centres
+    case = Case(CaseSpec("jet", HEX, 12, "weno4", jitter=0.1, seed=9, bc=(1, 2, 4, 4, 3, 3), wenwrt=wenwrt))
+    SA = prim2cons(1.0, 0.5, 0.0, 0.0, P0)
+    SB = prim2cons(1.4, 2.0, 0.2, -0.1, 0.9 * P0)
+    c = face_centres(case)
+    inside = np.linalg.norm(c[..., 1:] - 0.5 * case.spec.length, axis=-1) < 0.3 * case.spec.length
+    return case, SA, np.where(inside[..., None], SB, SA)
+
+
+def test_uniform_per_face_states_give_the_same_bits():
+    case, SA, _ = jet_case()
+    u = case.init_state()
+    print(f"[per-face boundary] uniform arrays ncell={case.ncell}", flush=True)
+    s = Solver(case, UCNS_FP64)
+    s.set_boundary(1, SA, SA)
+    s.set_state(u)
+    r0 = s.residual()
+    s.set_boundary_faces(np.broadcast_to(SA, (case.ncell, case.nface_max, 5)),
+                         np.broadcast_to(SA, (case.ncell, case.nface_max, 5)))
+    r1 = s.residual()
+    s.close()
+    assert np.array_equal(r0, r1)
+
+
+@pytest.mark.parametrize("wenwrt", [1, 2])
+def test_jet_inflow_parity(wenwrt):
+    case, SA, per_face = jet_case(wenwrt)
+    u = case.init_state()
+    print(f"[per-face boundary] jet wenwrt={wenwrt} ncell={case.ncell}", flush=True)
+    orc = Oracle(case)
+    oracle.set_boundary(1, SA, SA)
+    oracle.set_boundary_faces(per_face, None)
+    try:
+        s = Solver(case, UCNS_FP64)
+        s.set_boundary(1, SA, SA)
+        s.set_boundary_faces(per_face, None)
+        s.set_state(u)
+        r = s.residual()
+        r_ref, _ = orc.residual(u)
+        err = rel_l2(r, r_ref)
+        s.rk3_steps(10, cfl=0.5)
+        u10 = s.get_state()
+        s.close()
+        u10_ref, _ = orc.rk3(u, 10, cfl=0.5)
+        err10 = rel_l2(u10, u10_ref)
+        # the jet changed the flow: against the uniform inflow the states differ
+        oracle.set_boundary_faces(None, None)
+        u10_uni, _ = orc.rk3(u, 10, cfl=0.5)
+    finally:
+        oracle.set_boundary_faces(None, None)
+    print("  residual rel L2", err, "10 steps", err10, "jet effect", rel_l2(u10_uni, u10_ref))
+    assert np.all(err <= 1e-11), err
+    assert np.all(err10 <= 1e-9), err10
+    assert rel_l2(u10_uni, u10_ref).max() > 1e-4

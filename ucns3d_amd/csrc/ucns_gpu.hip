@@ -102,6 +102,8 @@ struct ReconParams {
     const int32_t *chnb;   // [ncell][nf]
     const double *chang;   // [ncell][nf][4]
     ucns_boundary_v1 bc;
+    const int32_t *cface;  // [ncell][nf] +(face+1) where the cell owns the face
+    const double *bface;   // [nface][10] per-face INFLOW(), OUTFLOW() states, or null: bc's
 };
 
 __device__ inline void cons2prim_d(double gamma, const double *u, double *w) {
@@ -507,9 +509,11 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
                 const int nbf = p.chnb[(size_t)cell * p.nf + f];
                 const double *an = p.chang + ((size_t)cell * p.nf + f) * 4;
                 double ur[NV], rl[NV], rr[NV];
-                if (nbf < 0)
-                    ghost_state(-nbf, p.bc.boundtype, p.gamma, ui, an[0], an[1], an[2], an[3], p.bc.inflow,
-                                p.bc.outflow, ur);
+                if (nbf < 0) {
+                    const double *bf = p.bface ? p.bface + (size_t)(p.cface[(size_t)cell * p.nf + f] - 1) * 10 : nullptr;
+                    ghost_state(-nbf, p.bc.boundtype, p.gamma, ui, an[0], an[1], an[2], an[3],
+                                bf ? bf : p.bc.inflow, bf ? bf + 5 : p.bc.outflow, ur);
+                }
                 else
 #pragma unroll
                     for (int v = 0; v < NV; ++v) ur[v] = p.U[(size_t)nbf * NV + v];
@@ -1904,6 +1908,7 @@ struct FluxParams {
     const double *wq;       // [qpq] quadrilateral-face weights of this context, sum 1
     const double *wt;       // [qpt] triangular-face weights, sum 1
     ucns_boundary_v1 bc;    // free-stream data of the boundary codes
+    const double *bface;    // [nface][10] per-face INFLOW(), OUTFLOW() states, or null: bc's
 };
 
 // ---------------------------------------------------------------------------
@@ -2095,6 +2100,8 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
     const double c1 = p.frot[4 * f], s1 = p.frot[4 * f + 1], c2 = p.frot[4 * f + 2], s2 = p.frot[4 * f + 3];
     const double area = p.farea[f];
     const bool bnd = cR < 0;  // boundary face: -code
+    const double *fin = p.bface ? p.bface + (size_t)f * 10 : p.bc.inflow;
+    const double *fout = p.bface ? p.bface + (size_t)f * 10 + 5 : p.bc.outflow;
     double uL[NV], uR[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -2134,7 +2141,7 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams<R> p) {
             a[v] = uL[v] + (double)xl;
             bq[v] = uR[v] + (double)xr;
         }
-        if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, p.bc.inflow, p.bc.outflow, bq);  // GET_STATES_BOUNDS
+        if (bnd) ghost_state(-cR, p.bc.boundtype, p.gamma, a, c1, s1, c2, s2, fin, fout, bq);  // GET_STATES_BOUNDS
         // ROTATEF (src/grid_t.f90:4217-4258)
         const double la[NV] = {a[0], c1 * s2 * a[1] + s1 * s2 * a[2] + c2 * a[3],
                                c1 * c2 * a[1] + s1 * c2 * a[2] + (-s2) * a[3], (-s1) * a[1] + c1 * a[2], a[4]};
@@ -3666,6 +3673,7 @@ struct ucns_ctx {
     bool tiled = false;  // WENO-4 lane-per-stencil kernel with the tiled layout
     int smode = 0;      // stencils per cell of the row-stream kernel in use (0: none)
     bool needs_bc = false, have_bc = false;  // boundary codes 1/2/6 present; ucns_set_boundary called
+    double *bface = nullptr;  // [nface][10] per-face INFLOW(), OUTFLOW() states (ucns_set_boundary_faces)
     bool has_bnd_faces = false;              // any face with a ghost state (codes 1/2/3/4/6)
     ucns_boundary_v1 bc{};
     int ntile = 0;
@@ -3804,6 +3812,8 @@ int launch_recon_t(ucns_ctx *ctx) {
     p.chnb = ctx->chnb;
     p.chang = ctx->chang;
     p.bc = ctx->bc;
+    p.cface = ctx->cface;
+    p.bface = ctx->bface;
     if (ctx->tiled) {
         ReconWParams<R> w{};
         w.ncell = ctx->ncell;
@@ -3985,6 +3995,7 @@ void launch_flux_t(ucns_ctx *ctx) {
     p.wq = ctx->wquad;
     p.wt = ctx->wquad + 64;
     p.bc = ctx->bc;
+    p.bface = ctx->bface;
     hipLaunchKernelGGL((k_flux<R>), dim3((ctx->nface + 255) / 256), dim3(256), 0, ctx->stream, p);
 }
 
@@ -4232,6 +4243,10 @@ int ensure_adjoint(ucns_ctx *ctx) {
     }
     if (ctx->wenwrt == 2) {
         ctx->err = "the adjoint does not cover characteristic reconstruction (wenwrt 2)";
+        return UCNS_EUNSUPPORTED;
+    }
+    if (ctx->bface) {
+        ctx->err = "the adjoint takes uniform boundary states (ucns_set_boundary), not per-face ones";
         return UCNS_EUNSUPPORTED;
     }
     if (ctx->adj_ready) return UCNS_OK;
@@ -5486,6 +5501,49 @@ int ucns_set_boundary(ucns_ctx *ctx, const ucns_boundary_v1 *bc) {
     return UCNS_OK;
 }
 
+int ucns_set_boundary_faces(ucns_ctx *ctx, const double *inflow, const double *outflow) {
+    if (!ctx) return UCNS_EBADARG;
+    if (!ctx->have_bc) {
+        ctx->err = "call ucns_set_boundary first (BOUNDTYPE and the states of the faces not given here)";
+        return UCNS_ESTATE;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    dfree(ctx->bface);
+    ctx->bface = nullptr;
+    if (!inflow && !outflow) return UCNS_OK;
+    const int nfa = ctx->nf_abi;
+    std::vector<double> h((size_t)ctx->nface * 10);
+    for (int f = 0; f < ctx->nface; ++f) {
+        const int code = -ctx->h_fcell[2 * f + 1];
+        const int ci = ctx->h_fcell[2 * f], sl = ctx->h_fslot[2 * f] & 255;
+        const size_t rec = (src_cell(ctx, ci) * nfa + sl) * NV;
+        for (int v = 0; v < NV; ++v) {
+            h[(size_t)f * 10 + v] = ctx->bc.inflow[v];
+            h[(size_t)f * 10 + 5 + v] = ctx->bc.outflow[v];
+        }
+        if (code <= 0) continue;  // interior and periodic faces: no ghost
+        for (int v = 0; v < NV; ++v) {
+            if (inflow) h[(size_t)f * 10 + v] = inflow[rec + v];
+            if (outflow) h[(size_t)f * 10 + 5 + v] = outflow[rec + v];
+        }
+        for (int v = 0; v < 10; ++v)
+            if (!std::isfinite(h[(size_t)f * 10 + v])) {
+                ctx->err = "non-finite boundary state at cell " + std::to_string(src_cell(ctx, ci)) + " face " +
+                           std::to_string(sl);
+                return UCNS_EBADARG;
+            }
+        if (!(h[(size_t)f * 10] > 0) || !(h[(size_t)f * 10 + 5] > 0)) {
+            ctx->err = "boundary density must be positive (cell " + std::to_string(src_cell(ctx, ci)) + " face " +
+                       std::to_string(sl) + ")";
+            return UCNS_EBADARG;
+        }
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->bface, h.size()))) return rc;
+    HIP_TRY(hipMemcpy(ctx->bface, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return UCNS_OK;
+}
+
 int ucns_set_state(ucns_ctx *ctx, const double *u, int32_t layout) {
     if (!ctx || !u) return UCNS_EBADARG;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -5977,6 +6035,7 @@ void ucns_destroy(ucns_ctx *ctx) {
     dfree(ctx->cnb);
     dfree(ctx->chnb);
     dfree(ctx->chang);
+    dfree(ctx->bface);
     dfree(ctx->cface);
     dfree(ctx->fcell);
     dfree(ctx->fslot);

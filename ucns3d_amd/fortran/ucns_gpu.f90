@@ -102,6 +102,11 @@ module ucns_gpu
       type(c_ptr), value :: ctx
       type(ucns_boundary_v1), intent(in) :: bc
     end function
+    ! per-face INFLOW()/OUTFLOW() states: c_loc of real(c_double) arrays (5, nface_max, ncell), or c_null_ptr
+    integer(c_int) function ucns_set_boundary_faces(ctx, inflow, outflow) bind(c, name="ucns_set_boundary_faces")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, inflow, outflow
+    end function
     integer(c_int) function ucns_set_state(ctx, u, layout) bind(c, name="ucns_set_state")
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -106,6 +106,7 @@ GPU_API = {
     "ucns_upload_prestore": (C.c_int, [C.c_void_p, C.POINTER(PrestoreV1)]),
     "ucns_upload_cweno": (C.c_int, [C.c_void_p, C.POINTER(CwenoV1)]),
     "ucns_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(BoundaryV1)]),
+    "ucns_set_boundary_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "ucns_set_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_get_state": (C.c_int, [C.c_void_p, _f64p, C.c_int32]),
     "ucns_set_state_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),

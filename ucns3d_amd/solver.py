@@ -72,6 +72,19 @@ class Solver:
                               inflow=(C.c_double * 5)(*inflow), outflow=(C.c_double * 5)(*outflow))
         self._check("ucns_set_boundary", self.L.ucns_set_boundary(self._h, C.byref(b)))
 
+    def set_boundary_faces(self, inflow=None, outflow=None) -> None:
+        """position-dependent INFLOW() / OUTFLOW() states per face record
+        [ncell][nface_max][5] (read at the faces of codes 1, 2, 6); None keeps
+        the uniform state of that side"""
+        def arr(a):
+            if a is None:
+                return None
+            return np.ascontiguousarray(a, dtype=np.float64).reshape(self.ncell, self.case.nface_max, 5)
+        fi, fo = arr(inflow), arr(outflow)
+        self._check("ucns_set_boundary_faces",
+                    self.L.ucns_set_boundary_faces(self._h, native.f64p(fi) if fi is not None else None,
+                                                   native.f64p(fo) if fo is not None else None))
+
     # -- state -----------------------------------------------------------------
     def set_state(self, u: np.ndarray) -> None:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.ncell, 5)
