@@ -130,7 +130,9 @@ def reduced(key: str, n: int) -> CaseSpec:
 
 
 class Case:
-    def __init__(self, spec: CaseSpec, nthreads: int = 0):
+    def __init__(self, spec: CaseSpec, nthreads: int = 0, grid_dir: str | None = None, binary: bool = False):
+        """the synthetic mesh of spec, or (grid_dir) the mesh of the GRID.cel /
+        GRID.vrt / GRID.bnd files there with spec's scheme options"""
         self.spec = spec
         lib = native.prep_lib()
         nx, ny, nz, lx, ly, lz = spec.dims
@@ -141,9 +143,13 @@ class Case:
                             bump=(C.c_double * 8)(*(tuple(spec.bump) + (0.0,) * (8 - len(spec.bump)))),
                             ees=spec.ees)
         h = C.c_void_p()
-        rc = lib.ucnsp_build(C.byref(o), C.byref(h))
+        if grid_dir is None:
+            rc = lib.ucnsp_build(C.byref(o), C.byref(h))
+        else:
+            rc = lib.ucnsp_read_grid(grid_dir.encode(), 1 if binary else 0, C.byref(o), C.byref(h))
         if rc != 0:
-            raise RuntimeError(f"ucnsp_build failed ({rc}): {lib.ucnsp_last_error().decode()}")
+            raise RuntimeError(f"ucnsp_{'build' if grid_dir is None else 'read_grid'} failed ({rc}): "
+                               f"{lib.ucnsp_last_error().decode()}")
         self._h = h
         self._lib = lib
         ints = (C.c_int64 * 16)()
@@ -186,6 +192,13 @@ class Case:
         if rc != 0:
             raise RuntimeError(self._lib.ucnsp_last_error().decode())
         return cw
+
+    def write_grid(self, grid_dir: str, binary: bool = False) -> None:
+        """the mesh as GRID.cel / GRID.vrt / GRID.bnd in grid_dir (text, or
+        Fortran unformatted records)"""
+        rc = self._lib.ucnsp_write_grid(self._h, grid_dir.encode(), 1 if binary else 0)
+        if rc != 0:
+            raise RuntimeError(f"ucnsp_write_grid failed ({rc}): {self._lib.ucnsp_last_error().decode()}")
 
     def move_nodes(self, xyz: np.ndarray) -> None:
         """the same mesh with its nodes at xyz [nnode, 3]: geometry and
@@ -232,3 +245,23 @@ def face_rule(igq: int, tri: bool):
     if rc != 0:
         raise RuntimeError(lib.ucnsp_last_error().decode())
     return pts[: n.value].copy(), w[: n.value].copy()
+
+
+def write_restart(path: str, it: int, t: float, u: np.ndarray) -> None:
+    """RESTART.dat of an explicit march: step, time and the cell averages [ncell][5]"""
+    lib = native.prep_lib()
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 5)
+    rc = lib.ucnsp_write_restart(path.encode(), it, t, native.f64p(u), len(u))
+    if rc != 0:
+        raise RuntimeError(f"ucnsp_write_restart failed ({rc}): {lib.ucnsp_last_error().decode()}")
+
+
+def read_restart(path: str, ncell: int):
+    """(step, time, cell averages [ncell][5]) of a RESTART.dat"""
+    lib = native.prep_lib()
+    u = np.empty((ncell, 5))
+    it, t = C.c_int32(), C.c_double()
+    rc = lib.ucnsp_read_restart(path.encode(), ncell, C.byref(it), C.byref(t), native.f64p(u))
+    if rc != 0:
+        raise RuntimeError(f"ucnsp_read_restart failed ({rc}): {lib.ucnsp_last_error().decode()}")
+    return it.value, t.value, u

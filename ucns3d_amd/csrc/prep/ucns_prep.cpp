@@ -11,6 +11,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <array>
+#include <map>
+#include <memory>
 #include <vector>
 #include <omp.h>
 #include <sys/mman.h>
@@ -529,6 +532,10 @@ struct ucnsp_case {
     std::vector<double> integ_basis_c, indicator_c;
     Buf<double> invmat_c;
     bool congruence_cache = true;  // copy data between congruent cells (off once nodes have been moved)
+    // a mesh read from GRID.cel/.vrt/.bnd: the boundary codes by the sorted
+    // node ids of each GRID.bnd face (0-based; triangles end in -1)
+    bool from_file = false;
+    std::map<std::array<int32_t, 4>, int32_t> bnd_code;
 };
 
 namespace {
@@ -593,6 +600,24 @@ void build_connectivity(ucnsp_case &c) {
     }
     for (int64_t i = 0; i < nc; ++i)
         for (int f = 0; f < c.ifca[i]; ++f) c.types_faces[i * NF + f] = face_nv(c.cshape[i], f) == 4 ? 5 : 6;
+    if (c.from_file) {
+        // GRID.bnd: codes of the unmatched faces and of the periodic pairs (5 or 50)
+        for (int64_t i = 0; i < nc; ++i)
+            for (int f = 0; f < c.ifca[i]; ++f) {
+                const int64_t sl = i * NF + f;
+                if (c.ineigh[sl] >= 0 && c.bcode[sl] == 0) continue;
+                std::array<int32_t, 4> k = {-1, -1, -1, -1};
+                const int sh = c.cshape[i], nv = face_nv(sh, f);
+                for (int q = 0; q < nv; ++q) k[q] = c.cnodes[i * 8 + face_node(sh, f, q)];
+                std::sort(k.begin(), k.begin() + nv);
+                const auto it = c.bnd_code.find(k);
+                if (it != c.bnd_code.end())
+                    c.bcode[sl] = it->second;
+                else if (c.ineigh[sl] < 0)
+                    c.bcode[sl] = -1;  // a boundary face GRID.bnd does not list
+            }
+        return;
+    }
     // unmatched faces lie on a bounded side of the box: code of that side
     const double L[3] = {c.o.lx, c.o.ly, c.o.lz};
     for (int64_t i = 0; i < nc; ++i)
@@ -1812,74 +1837,10 @@ void gen_box(ucnsp_case &c) {
 
 }  // namespace
 
-extern "C" {
-
-const char *ucnsp_last_error(void) { return g_err.c_str(); }
-
-int ucnsp_shape_table(int32_t shape, int32_t *out) {
-    if (!out) return fail(UCNS_EBADARG, "null argument");
-    if (shape != UCNSP_HEX && shape != UCNSP_TET && shape != UCNSP_PRISM && shape != UCNSP_PYRAMID)
-        return fail(UCNS_EBADARG, "shape must be a cell shape (hex, tet, prism, pyramid)");
-    for (int k = 0; k < 64; ++k) out[k] = -1;
-    out[0] = nnodes_of(shape);
-    out[1] = nfaces_of(shape);
-    for (int f = 0; f < nfaces_of(shape); ++f) {
-        out[2 + f] = face_nv(shape, f);
-        for (int k = 0; k < face_nv(shape, f); ++k) out[8 + f * 4 + k] = face_node(shape, f, k);
-    }
-    out[32] = nsubtets(shape);
-    for (int t = 0; t < nsubtets(shape); ++t)
-        for (int k = 0; k < 4; ++k) out[33 + t * 4 + k] = subtet(shape, t)[k];
-    return UCNS_OK;
-}
-
-int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n) {
-    std::vector<double> a, b;
-    int m = tri ? tri_rule(igq, a, b) : quad_rule(igq, a, b);
-    if (m < 0) return fail(UCNS_EUNSUPPORTED, "face rule not available");
-    for (int g = 0; g < m; ++g) {
-        if (tri) {
-            pts[g * 3] = a[g * 3];
-            pts[g * 3 + 1] = a[g * 3 + 1];
-            pts[g * 3 + 2] = a[g * 3 + 2];
-        } else {
-            pts[g * 3] = a[g * 2];
-            pts[g * 3 + 1] = a[g * 2 + 1];
-            pts[g * 3 + 2] = 0.0;
-        }
-        w[g] = b[g];
-    }
-    *n = m;
-    return UCNS_OK;
-}
-
-int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
-    if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
-    const ucnsp_opts &o = *opts;
-    if (o.shape < UCNSP_HEX || o.shape > UCNSP_BLAST)
-        return fail(UCNS_EUNSUPPORTED,
-                    "shape must be hex(1), tet(2), prism(3), pyramid(4), hybrid(5), channel(6) or blast(7)");
-    if (o.shape == UCNSP_BLAST && (o.nx < 8 || o.ny < 8 || o.nz < 8))
-        return fail(UCNS_EBADARG, "the blast box needs at least 8 hexes per axis (core, shell, tetrahedra, prisms)");
-    if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
-    for (int d = 0; d < 3; ++d) {
-        const int a = o.bc[2 * d], b = o.bc[2 * d + 1];
-        if ((a == 0) != (b == 0)) return fail(UCNS_EBADARG, "a periodic axis needs code 0 on both sides");
-        for (int v : {a, b})
-            if (v != 0 && v != 1 && v != 2 && v != 3 && v != 4 && v != 6)
-                return fail(UCNS_EUNSUPPORTED, "side codes: 0 periodic, 1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield");
-    }
-    if (o.iorder < 1 || o.iorder > 4) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..4");
-    if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
-    if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
-    if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");
-    bool bumped = false;
-    for (int q = 0; q < 8; ++q) bumped = bumped || o.bump[q] != 0.0;
-    if (bumped && o.bc[2] == 0)
-        return fail(UCNS_EBADARG, "a lower-wall bump needs a bounded y axis (the y sides cannot be periodic)");
-    if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
-    ucnsp_case *c = new ucnsp_case();
-    c->o = o;
+// Everything after the mesh: connectivity, the bump, geometry, scheme sizes,
+// stencils and PRESTORE.  Deletes c on failure.
+static int build_after_mesh(ucnsp_case *c, bool bumped) {
+    const ucnsp_opts &o = c->o;
     const bool tim = std::getenv("UCNSP_TIMING") != nullptr;
     double t0 = omp_get_wtime();
     auto lap = [&](const char *what) {
@@ -1888,15 +1849,14 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
         std::fprintf(stderr, "[ucnsp] %-14s %8.3f s\n", what, t1 - t0);
         t0 = t1;
     };
-    gen_box(*c);
-    lap("mesh");
     build_connectivity(*c);
     lap("connectivity");
     for (int64_t i = 0; i < c->ncell; ++i)
         for (int f = 0; f < c->ifca[i]; ++f)
             if (c->ineigh[i * ucnsp_case::NF + f] < 0 && c->bcode[i * ucnsp_case::NF + f] <= 0) {
                 delete c;
-                return fail(UCNS_EBADARG, "unmatched face off the box sides");
+                return fail(UCNS_EBADARG, c->from_file ? "a boundary face of the mesh is missing from GRID.bnd"
+                                                       : "unmatched face off the box sides");
             }
     std::vector<double> orient;
     if (bumped) orient = subtet_orientation(*c);
@@ -1976,6 +1936,370 @@ int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
     lap("stencils");
     build_prestore(*c);
     lap("prestore");
+    return UCNS_OK;
+}
+
+namespace {
+
+// Fortran unformatted sequential records (BINIO = 1): a 4-byte length before
+// and after each record's payload
+void put_rec(FILE *f, const void *p, int32_t n) {
+    std::fwrite(&n, 4, 1, f);
+    std::fwrite(p, 1, (size_t)n, f);
+    std::fwrite(&n, 4, 1, f);
+}
+bool get_rec(FILE *f, void *p, int32_t n) {
+    int32_t a = 0, b = 0;
+    if (std::fread(&a, 4, 1, f) != 1 || a != n) return false;
+    if (std::fread(p, 1, (size_t)n, f) != (size_t)n) return false;
+    return std::fread(&b, 4, 1, f) == 1 && b == n;
+}
+
+// GRID.cel node lists: eight ids with repeats that say the shape (the rule
+// of src/grid_p.f90:1670-1684), and the shape's own node list in this
+// preprocessor's 8-slot order
+int cel_shape(const int32_t *I) {
+    if (I[4] != I[5] && I[5] != I[6] && I[6] != I[7] && I[2] != I[3]) return UCNSP_HEX;
+    if (I[2] == I[3] && I[4] == I[5] && I[5] == I[6] && I[6] == I[7]) return UCNSP_TET;
+    if (I[2] != I[3] && I[4] == I[5] && I[5] == I[6] && I[6] == I[7]) return UCNSP_PYRAMID;
+    if (I[2] == I[3] && I[4] != I[5] && I[6] == I[7]) return UCNSP_PRISM;
+    return 0;
+}
+// slots of I1..I8 that hold the shape's nodes, in node order
+const int CEL_SLOTS[5][8] = {{0}, {0, 1, 2, 3, 4, 5, 6, 7}, {0, 1, 2, 4}, {0, 1, 2, 4, 5, 6}, {0, 1, 2, 3, 4}};
+// I1..I8 from a shape's nodes n[]: tetrahedron n0 n1 n2 n2 n3 n3 n3 n3,
+// pyramid n0..n3 n4 n4 n4 n4, prism n0 n1 n2 n2 n3 n4 n5 n5
+void cel_line(int shape, const int32_t *n, int32_t *I) {
+    switch (shape) {
+    case UCNSP_HEX:
+        for (int k = 0; k < 8; ++k) I[k] = n[k];
+        break;
+    case UCNSP_TET: {
+        const int32_t t[8] = {n[0], n[1], n[2], n[2], n[3], n[3], n[3], n[3]};
+        std::memcpy(I, t, sizeof(t));
+        break;
+    }
+    case UCNSP_PYRAMID: {
+        const int32_t t[8] = {n[0], n[1], n[2], n[3], n[4], n[4], n[4], n[4]};
+        std::memcpy(I, t, sizeof(t));
+        break;
+    }
+    default: {
+        const int32_t t[8] = {n[0], n[1], n[2], n[2], n[3], n[4], n[5], n[5]};
+        std::memcpy(I, t, sizeof(t));
+    }
+    }
+}
+
+struct FileCloser {
+    FILE *f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ucnsp_write_grid(const ucnsp_case *c, const char *dir, int32_t binary) {
+    if (!c || !dir) return fail(UCNS_EBADARG, "null argument");
+    const std::string d(dir);
+    const char *mode = binary ? "wb" : "w";
+    FileCloser cel{std::fopen((d + "/GRID.cel").c_str(), mode)}, vrt{std::fopen((d + "/GRID.vrt").c_str(), mode)},
+        bnd{std::fopen((d + "/GRID.bnd").c_str(), mode)};
+    if (!cel.f || !vrt.f || !bnd.f) return fail(UCNS_EBADARG, "cannot create GRID.cel/.vrt/.bnd in " + d);
+    // nodes (1-based ids; %.17g keeps every double exactly)
+    for (int64_t n = 0; n < c->nnode; ++n) {
+        const int32_t id = (int32_t)(n + 1);
+        const double *x = &c->xyz[(size_t)n * 3];
+        if (binary) {
+            char buf[28];
+            std::memcpy(buf, &id, 4);
+            std::memcpy(buf + 4, x, 24);
+            put_rec(vrt.f, buf, 28);
+        } else {
+            std::fprintf(vrt.f, "%d %.17g %.17g %.17g\n", id, x[0], x[1], x[2]);
+        }
+    }
+    // cells
+    for (int64_t i = 0; i < c->ncell; ++i) {
+        int32_t n[8], I[9];
+        for (int k = 0; k < 8; ++k) n[k] = c->cnodes[i * 8 + k] + 1;
+        I[0] = (int32_t)(i + 1);
+        cel_line(c->cshape[i], n, I + 1);
+        if (binary)
+            put_rec(cel.f, I, 36);
+        else
+            std::fprintf(cel.f, "%d %d %d %d %d %d %d %d %d\n", I[0], I[1], I[2], I[3], I[4], I[5], I[6], I[7], I[8]);
+    }
+    // boundary faces (codes 1, 2, 3, 4, 6) and both sides of periodic pairs (5, 50)
+    int32_t id = 0;
+    for (int64_t i = 0; i < c->ncell; ++i)
+        for (int f = 0; f < c->ifca[i]; ++f) {
+            const int64_t sl = i * ucnsp_case::NF + f;
+            const int code = c->bcode[sl];
+            if (code == 0) continue;
+            const int sh = c->cshape[i], nv = face_nv(sh, f);
+            int32_t B[6];
+            B[0] = ++id;
+            for (int q = 0; q < 4; ++q) B[1 + q] = c->cnodes[i * 8 + face_node(sh, f, q < nv ? q : nv - 1)] + 1;
+            B[5] = code;
+            if (binary)
+                put_rec(bnd.f, B, 24);
+            else
+                std::fprintf(bnd.f, "%d %d %d %d %d %d\n", B[0], B[1], B[2], B[3], B[4], B[5]);
+        }
+    return UCNS_OK;
+}
+
+int ucnsp_read_grid(const char *dir, int32_t binary, const ucnsp_opts *opts, ucnsp_case **out) {
+    if (!dir || !opts || !out) return fail(UCNS_EBADARG, "null argument");
+    const ucnsp_opts &o = *opts;
+    if (o.iorder < 1 || o.iorder > 4) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..4");
+    if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
+    if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
+    if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");
+    const std::string d(dir);
+    const char *mode = binary ? "rb" : "r";
+    FileCloser cel{std::fopen((d + "/GRID.cel").c_str(), mode)}, vrt{std::fopen((d + "/GRID.vrt").c_str(), mode)},
+        bnd{std::fopen((d + "/GRID.bnd").c_str(), mode)};
+    if (!cel.f || !vrt.f || !bnd.f) return fail(UCNS_EBADARG, "cannot open GRID.cel/.vrt/.bnd in " + d);
+    if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
+    std::unique_ptr<ucnsp_case> c(new ucnsp_case());
+    c->o = o;
+    for (int q = 0; q < 8; ++q) c->o.bump[q] = 0.0;
+    c->o.jitter = 0.0;
+    c->from_file = true;
+    c->shape = UCNSP_HYBRID;
+    // nodes: ids 1..N in order
+    for (;;) {
+        int32_t id;
+        double x[3];
+        if (binary) {
+            char buf[28];
+            if (!get_rec(vrt.f, buf, 28)) break;
+            std::memcpy(&id, buf, 4);
+            std::memcpy(x, buf + 4, 24);
+        } else if (std::fscanf(vrt.f, "%d %lf %lf %lf", &id, &x[0], &x[1], &x[2]) != 4) {
+            break;
+        }
+        if (id != (int32_t)(c->nnode + 1)) return fail(UCNS_EBADARG, "GRID.vrt: node ids must run 1, 2, 3, ...");
+        c->xyz.insert(c->xyz.end(), x, x + 3);
+        ++c->nnode;
+    }
+    if (c->nnode < 4) return fail(UCNS_EBADARG, "GRID.vrt: no nodes");
+    // cells
+    for (;;) {
+        int32_t I[9];
+        if (binary) {
+            if (!get_rec(cel.f, I, 36)) break;
+        } else if (std::fscanf(cel.f, "%d %d %d %d %d %d %d %d %d", &I[0], &I[1], &I[2], &I[3], &I[4], &I[5], &I[6],
+                               &I[7], &I[8]) != 9) {
+            break;
+        }
+        if (I[0] != (int32_t)(c->ncell + 1)) return fail(UCNS_EBADARG, "GRID.cel: cell ids must run 1, 2, 3, ...");
+        const int sh = cel_shape(I + 1);
+        if (!sh) return fail(UCNS_EBADARG, "GRID.cel: cell " + std::to_string(I[0]) + " has no element shape");
+        int32_t n[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+        for (int k = 0; k < nnodes_of(sh); ++k) {
+            const int32_t v = I[1 + CEL_SLOTS[sh][k]];
+            if (v < 1 || v > c->nnode) return fail(UCNS_EBADARG, "GRID.cel: node id out of range");
+            n[k] = v - 1;
+        }
+        c->cnodes.insert(c->cnodes.end(), n, n + 8);
+        c->cshape.push_back(sh);
+        ++c->ncell;
+    }
+    if (c->ncell < 1) return fail(UCNS_EBADARG, "GRID.cel: no cells");
+    // boundary faces
+    std::vector<std::array<int32_t, 5>> bf;
+    for (;;) {
+        int32_t B[6];
+        if (binary) {
+            if (!get_rec(bnd.f, B, 24)) break;
+        } else if (std::fscanf(bnd.f, "%d %d %d %d %d %d", &B[0], &B[1], &B[2], &B[3], &B[4], &B[5]) != 6) {
+            break;
+        }
+        const int code = B[5];
+        if (code != 1 && code != 2 && code != 3 && code != 4 && code != 5 && code != 6 && code != 50)
+            return fail(UCNS_EUNSUPPORTED, "GRID.bnd: codes 1, 2, 3, 4, 5, 6 and 50");
+        std::array<int32_t, 4> k;
+        const int nv = B[3] == B[4] ? 3 : 4;  // triangle when IB3 = IB4 (src/bc_p.f90:100-110)
+        for (int q = 0; q < 4; ++q) {
+            if (q < nv && (B[1 + q] < 1 || B[1 + q] > c->nnode)) return fail(UCNS_EBADARG, "GRID.bnd: node id out of range");
+            k[q] = q < nv ? B[1 + q] - 1 : -1;
+        }
+        std::sort(k.begin(), k.begin() + nv);
+        c->bnd_code[k] = code;
+        bf.push_back({k[0], k[1], k[2], nv == 4 ? k[3] : -1, code});
+    }
+    // the box and its periodic axes: a periodic face lies on a side of the
+    // bounding box; its axis's period is the box's extent
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t n = 0; n < c->nnode; ++n)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], c->xyz[n * 3 + a]);
+            hi[a] = std::max(hi[a], c->xyz[n * 3 + a]);
+        }
+    c->o.lx = hi[0] - lo[0];
+    c->o.ly = hi[1] - lo[1];
+    c->o.lz = hi[2] - lo[2];
+    bool pax[3] = {false, false, false};
+    for (const auto &F : bf) {
+        if (F[4] != 5 && F[4] != 50) continue;
+        const int nv = F[3] < 0 ? 3 : 4;
+        int axis = -1;
+        for (int a = 0; a < 3 && axis < 0; ++a) {
+            const double ext = hi[a] - lo[a], tol = 1e-9 * std::max(ext, 1e-300);
+            for (const double side : {lo[a], hi[a]}) {
+                bool on = true;
+                for (int q = 0; q < nv; ++q) on = on && std::fabs(c->xyz[(size_t)F[q] * 3 + a] - side) <= tol;
+                if (on) axis = a;
+            }
+        }
+        if (axis < 0) return fail(UCNS_EUNSUPPORTED, "GRID.bnd: a periodic face off the bounding box's sides");
+        pax[axis] = true;
+    }
+    c->canon.resize(c->nnode);
+    for (int64_t n = 0; n < c->nnode; ++n) c->canon[n] = (int32_t)n;
+    for (int a = 0; a < 3; ++a) {
+        if (!pax[a]) continue;
+        c->per[a] = hi[a] - lo[a];
+        const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        const double scale = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+        const double tol = 1e-9 * scale;
+        auto key = [&](int64_t n) {
+            return std::make_pair((int64_t)std::llround(c->xyz[n * 3 + b1] / tol), (int64_t)std::llround(c->xyz[n * 3 + b2] / tol));
+        };
+        std::map<std::pair<int64_t, int64_t>, int32_t> lower;
+        for (int64_t n = 0; n < c->nnode; ++n)
+            if (std::fabs(c->xyz[n * 3 + a] - lo[a]) <= tol) lower[key(n)] = (int32_t)n;
+        for (int64_t n = 0; n < c->nnode; ++n) {
+            if (std::fabs(c->xyz[n * 3 + a] - hi[a]) > tol) continue;
+            const auto it = lower.find(key(n));
+            if (it == lower.end()) return fail(UCNS_EBADARG, "periodic node without a partner on the opposite side");
+            c->canon[n] = c->canon[it->second];
+        }
+    }
+    ucnsp_case *raw = c.release();
+    const int rc = build_after_mesh(raw, false);
+    if (rc != UCNS_OK) return rc;
+    *out = raw;
+    return UCNS_OK;
+}
+
+/* RESTART.dat of an explicit march (CHECKPOINT, src/io.f90:13457-13639): the
+ * step count (int32), the time (float64) and the cell averages U_C(i)%VAL(1,1:5)
+ * in global cell order (float64), native byte order, no record markers */
+int ucnsp_write_restart(const char *path, int32_t it, double t, const double *u, int64_t ncell) {
+    if (!path || !u || ncell < 1) return fail(UCNS_EBADARG, "null argument");
+    FileCloser f{std::fopen(path, "wb")};
+    if (!f.f) return fail(UCNS_EBADARG, std::string("cannot create ") + path);
+    if (std::fwrite(&it, 4, 1, f.f) != 1 || std::fwrite(&t, 8, 1, f.f) != 1 ||
+        std::fwrite(u, 8, (size_t)ncell * 5, f.f) != (size_t)ncell * 5)
+        return fail(UCNS_EBADARG, std::string("short write to ") + path);
+    return UCNS_OK;
+}
+
+int ucnsp_read_restart(const char *path, int64_t ncell, int32_t *it, double *t, double *u) {
+    if (!path || !it || !t || !u || ncell < 1) return fail(UCNS_EBADARG, "null argument");
+    FileCloser f{std::fopen(path, "rb")};
+    if (!f.f) return fail(UCNS_EBADARG, std::string("cannot open ") + path);
+    std::fseek(f.f, 0, SEEK_END);
+    const long size = std::ftell(f.f);
+    std::fseek(f.f, 0, SEEK_SET);
+    if (size != 12 + 40 * (long)ncell)
+        return fail(UCNS_EBADARG, "RESTART.dat holds " + std::to_string(size) + " bytes, not 12 + 40 x " +
+                                      std::to_string(ncell) + " (step, time, 5 doubles per cell)");
+    if (std::fread(it, 4, 1, f.f) != 1 || std::fread(t, 8, 1, f.f) != 1 ||
+        std::fread(u, 8, (size_t)ncell * 5, f.f) != (size_t)ncell * 5)
+        return fail(UCNS_EBADARG, std::string("short read from ") + path);
+    return UCNS_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+const char *ucnsp_last_error(void) { return g_err.c_str(); }
+
+int ucnsp_shape_table(int32_t shape, int32_t *out) {
+    if (!out) return fail(UCNS_EBADARG, "null argument");
+    if (shape != UCNSP_HEX && shape != UCNSP_TET && shape != UCNSP_PRISM && shape != UCNSP_PYRAMID)
+        return fail(UCNS_EBADARG, "shape must be a cell shape (hex, tet, prism, pyramid)");
+    for (int k = 0; k < 64; ++k) out[k] = -1;
+    out[0] = nnodes_of(shape);
+    out[1] = nfaces_of(shape);
+    for (int f = 0; f < nfaces_of(shape); ++f) {
+        out[2 + f] = face_nv(shape, f);
+        for (int k = 0; k < face_nv(shape, f); ++k) out[8 + f * 4 + k] = face_node(shape, f, k);
+    }
+    out[32] = nsubtets(shape);
+    for (int t = 0; t < nsubtets(shape); ++t)
+        for (int k = 0; k < 4; ++k) out[33 + t * 4 + k] = subtet(shape, t)[k];
+    return UCNS_OK;
+}
+
+int ucnsp_face_rule(int32_t igq, int32_t tri, double *pts, double *w, int32_t *n) {
+    std::vector<double> a, b;
+    int m = tri ? tri_rule(igq, a, b) : quad_rule(igq, a, b);
+    if (m < 0) return fail(UCNS_EUNSUPPORTED, "face rule not available");
+    for (int g = 0; g < m; ++g) {
+        if (tri) {
+            pts[g * 3] = a[g * 3];
+            pts[g * 3 + 1] = a[g * 3 + 1];
+            pts[g * 3 + 2] = a[g * 3 + 2];
+        } else {
+            pts[g * 3] = a[g * 2];
+            pts[g * 3 + 1] = a[g * 2 + 1];
+            pts[g * 3 + 2] = 0.0;
+        }
+        w[g] = b[g];
+    }
+    *n = m;
+    return UCNS_OK;
+}
+
+int ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out) {
+    if (!opts || !out) return fail(UCNS_EBADARG, "null argument");
+    const ucnsp_opts &o = *opts;
+    if (o.shape < UCNSP_HEX || o.shape > UCNSP_BLAST)
+        return fail(UCNS_EUNSUPPORTED,
+                    "shape must be hex(1), tet(2), prism(3), pyramid(4), hybrid(5), channel(6) or blast(7)");
+    if (o.shape == UCNSP_BLAST && (o.nx < 8 || o.ny < 8 || o.nz < 8))
+        return fail(UCNS_EBADARG, "the blast box needs at least 8 hexes per axis (core, shell, tetrahedra, prisms)");
+    if (o.nx < 3 || o.ny < 3 || o.nz < 3) return fail(UCNS_EBADARG, "the box needs at least 3 cells per direction");
+    for (int d = 0; d < 3; ++d) {
+        const int a = o.bc[2 * d], b = o.bc[2 * d + 1];
+        if ((a == 0) != (b == 0)) return fail(UCNS_EBADARG, "a periodic axis needs code 0 on both sides");
+        for (int v : {a, b})
+            if (v != 0 && v != 1 && v != 2 && v != 3 && v != 4 && v != 6)
+                return fail(UCNS_EUNSUPPORTED, "side codes: 0 periodic, 1 inflow, 2 outflow, 3 symmetry, 4 wall, 6 farfield");
+    }
+    if (o.iorder < 1 || o.iorder > 4) return fail(UCNS_EUNSUPPORTED, "iorder must be 1..4");
+    if (o.iweno != 1 && o.iweno != -1) return fail(UCNS_EBADARG, "iweno must be 1 or -1");
+    if (o.iweno == -1 && o.iorder != 1) return fail(UCNS_EUNSUPPORTED, "MUSCL path supports iorder 1");
+    if (o.iweno == 1 && o.iorder < 2) return fail(UCNS_EUNSUPPORTED, "WENO needs iorder >= 2");
+    bool bumped = false;
+    for (int q = 0; q < 8; ++q) bumped = bumped || o.bump[q] != 0.0;
+    if (bumped && o.bc[2] == 0)
+        return fail(UCNS_EBADARG, "a lower-wall bump needs a bounded y axis (the y sides cannot be periodic)");
+    if (o.nthreads > 0) omp_set_num_threads(o.nthreads);
+    ucnsp_case *c = new ucnsp_case();
+    c->o = o;
+    const bool tim = std::getenv("UCNSP_TIMING") != nullptr;
+    double t0 = omp_get_wtime();
+    auto lap = [&](const char *what) {
+        if (!tim) return;
+        double t1 = omp_get_wtime();
+        std::fprintf(stderr, "[ucnsp] %-14s %8.3f s\n", what, t1 - t0);
+        t0 = t1;
+    };
+    gen_box(*c);
+    lap("mesh");
+    const int rc = build_after_mesh(c, bumped);
+    if (rc != UCNS_OK) return rc;
     *out = c;
     return UCNS_OK;
 }

@@ -72,6 +72,22 @@ typedef struct ucnsp_opts {
 typedef struct ucnsp_case ucnsp_case;
 
 int  ucnsp_build(const ucnsp_opts *opts, ucnsp_case **out);
+/* GRID.cel / GRID.vrt / GRID.bnd (OPEN_ARBITRARY and the readers of
+ * src/io.f90:4157-4301, src/grid_p.f90:1655-1760, src/bc_p.f90:30-140): text
+ * (binary = 0) or Fortran unformatted sequential records (binary = 1, BINIO);
+ * cells as 8 node ids whose repeats give the shape, nodes as id x y z,
+ * boundary faces as id + 4 node ids (a triangle repeats its third) + code.
+ * ucnsp_write_grid writes a case's mesh (periodic pairs on both sides, code 5);
+ * ucnsp_read_grid builds a case from the files with the scheme options of
+ * opts (its shape, box, jitter, bc and bump are ignored): periodic axes from
+ * the faces of codes 5/50, paired with their translates across the bounding
+ * box. */
+int  ucnsp_write_grid(const ucnsp_case *c, const char *dir, int32_t binary);
+int  ucnsp_read_grid(const char *dir, int32_t binary, const ucnsp_opts *opts, ucnsp_case **out);
+/* RESTART.dat of an explicit march (CHECKPOINT, src/io.f90:13457-13639): step
+ * (int32), time (float64), ncell x 5 cell averages (float64) in global cell order */
+int  ucnsp_write_restart(const char *path, int32_t it, double t, const double *u, int64_t ncell);
+int  ucnsp_read_restart(const char *path, int64_t ncell, int32_t *it, double *t, double *u);
 /* Fill the boundary structs with pointers into the case's storage (valid
  * until ucnsp_free).  precision/gamma/lwci1 are the caller's scheme choices. */
 int  ucnsp_fill(const ucnsp_case *c, int32_t precision, double gamma, double lwci1,

@@ -138,6 +138,11 @@ GPU_API = {
 
 PREP_API = {
     "ucnsp_build": (C.c_int, [C.POINTER(PrepOpts), C.POINTER(C.c_void_p)]),
+    "ucnsp_write_grid": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "ucnsp_read_grid": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(PrepOpts), C.POINTER(C.c_void_p)]),
+    "ucnsp_write_restart": (C.c_int, [C.c_char_p, C.c_int32, C.c_double, C.POINTER(C.c_double), C.c_int64]),
+    "ucnsp_read_restart": (C.c_int, [C.c_char_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "ucnsp_fill": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double,
                              C.POINTER(MeshV2), C.POINTER(SchemeV2), C.POINTER(PrestoreV1)]),
     "ucnsp_array": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
