@@ -238,11 +238,16 @@ __device__ inline void char_eigen_d(double gamma, const double *rl, const double
     }
 }
 
-template <typename R, int IDEG, int IMAX, int TSMAX>
+// CWS: the CWENO layout, the sectors' rows compact (IM2 members, ID2
+// coefficients each) and the indicator packed (upper triangle by rows)
+template <typename R, int IDEG, int IMAX, int TSMAX, bool CWS = false, int ID2 = 9, int IM2 = 19>
 struct ReconShared {
     static constexpr int NQMAX = 6 * 16;
+    static constexpr int DU = CWS ? IMAX + (TSMAX - 1) * IM2 : TSMAX * IMAX;
+    static constexpr int GN = CWS ? IDEG + (TSMAX - 1) * ID2 : TSMAX * IDEG;
+    static constexpr int BN = CWS ? IDEG * (IDEG + 1) / 2 : IDEG * IDEG;
     union {
-        R du[TSMAX * IMAX * NV];  // phase 1
+        R du[DU * NV];  // phase 1
         struct {
             R gh[IDEG * NV];         // combined polynomial
             R beta[TSMAX * NV];
@@ -250,8 +255,8 @@ struct ReconShared {
             R ul[NQMAX * NV];        // face-point values
         } late;
     } u;
-    R g[TSMAX * IDEG * NV];          // per-stencil coefficients
-    R b[IDEG * IDEG];                // indicator, unpacked
+    R g[GN * NV];                    // per-stencil coefficients
+    R b[BN];                         // indicator, unpacked (CWS: packed)
     R mb[IDEG];
     double ui[NV];
     // CWENO: the sectors' indicator (unpacked), the indicators in FP64 and
@@ -265,10 +270,12 @@ struct ReconShared {
 // ID2 coefficients over their first IM2 members (src/reconstruct_p.f90:955-1049).
 // CH: characteristic-variable WENO, face by face (src/reconstruct_p.f90:497-758).
 template <typename R, int IDEG, int IMAX, int TSMAX, bool CW = false, int ID2 = 9, int IM2 = 19, bool CH = false>
-__global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
+__global__ __launch_bounds__(WAVE, CW ? 2 : 1) void k_recon(ReconParams<R> p) {
     static_assert(!CW || (ID2 <= 9 && IM2 <= IMAX), "CWENO sectors: at most 9 coefficients");
     constexpr int NROW = (TSMAX * IDEG + WAVE - 1) / WAVE;
-    __shared__ ReconShared<R, IDEG, IMAX, TSMAX> sh;
+    __shared__ ReconShared<R, IDEG, IMAX, TSMAX, CW, ID2, IM2> sh;
+    // CWENO: row of sector s >= 1's coefficient k in sh.g
+    auto crow = [](int s, int k) { return IDEG + (s - 1) * ID2 + k; };
     const int lane = threadIdx.x;
     // XCD-aware order: consecutive blocks go to different XCDs, so give
     // each XCD a contiguous range of cells (its L2 then holds the
@@ -314,7 +321,10 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             for (int v = 0; v < NV; ++v) sh.u.du[e * NV + v] = (R)(Un[v] - ui[v]);
         }
     }
-    if (p.iweno == 1) {
+    if (CW) {
+        const R *Bc = p.B + (size_t)cell * (IDEG * (IDEG + 1) / 2);
+        for (int e = lane; e < IDEG * (IDEG + 1) / 2; e += WAVE) sh.b[e] = Bc[e];
+    } else if (p.iweno == 1) {
         const R *Bc = p.B + (size_t)cell * (IDEG * (IDEG + 1) / 2);
         for (int e = lane; e < IDEG * (IDEG + 1) / 2; e += WAVE) {
             // packed upper triangle by rows: row k holds l = k..IDEG-1
@@ -363,7 +373,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
                     for (int v = 0; v < NV; ++v) acc[v] += pv * du[v];
                 }
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sh.g[((sec + 1) * IDEG + k) * NV + v] = acc[v];
+                for (int v = 0; v < NV; ++v) sh.g[crow(sec + 1, k) * NV + v] = acc[v];
             }
         }
     } else {
@@ -413,7 +423,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             const int k = e / NV, v = e % NV;
             R sl = R(0);
             if (k < ID2)
-                for (int s = 1; s < admis; ++s) sl += (R)laml * sh.g[(s * IDEG + k) * NV + v];
+                for (int s = 1; s < admis; ++s) sl += (R)laml * sh.g[crow(s, k) * NV + v];
             sh.u.late.gh[e] = (R)(1.0 / lam0) * (sh.g[e] - sl);
         }
         __syncthreads();
@@ -422,13 +432,17 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             const int s = lane / NV, v = lane % NV;
             double beta = 0.0;
             if (s == 0) {
+                // over the packed triangle: sum_k z_k (B_kk z_k + 2 sum_{l>k} B_kl z_l)
+                int e = 0;
                 for (int k = 0; k < IDEG; ++k) {
+                    const double zk = (double)sh.u.late.gh[k * NV + v];
                     double bg = 0.0;
-                    for (int l = 0; l < IDEG; ++l) bg += (double)sh.b[k * IDEG + l] * (double)sh.u.late.gh[l * NV + v];
-                    beta += (double)sh.u.late.gh[k * NV + v] * bg;
+                    for (int l = k + 1; l < IDEG; ++l) bg += (double)sh.b[e + l - k] * (double)sh.u.late.gh[l * NV + v];
+                    beta += zk * ((double)sh.b[e] * zk + 2.0 * bg);
+                    e += IDEG - k;
                 }
             } else {
-                const R *gs = &sh.g[s * IDEG * NV + v];
+                const R *gs = &sh.g[crow(s, 0) * NV + v];
 #pragma unroll
                 for (int k = 0; k < ID2; ++k) {
                     double bg = 0.0;
@@ -473,7 +487,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             const int k = e / NV, v = e % NV;
             R acc = sh.u.late.omega[v] * sh.u.late.gh[e];
             if (k < ID2)
-                for (int s = 1; s < admis; ++s) acc += sh.u.late.omega[s * NV + v] * sh.g[(s * IDEG + k) * NV + v];
+                for (int s = 1; s < admis; ++s) acc += sh.u.late.omega[s * NV + v] * sh.g[crow(s, k) * NV + v];
             sh.u.late.gh[e] = acc;
         }
         if (lane < NV) {
@@ -482,7 +496,7 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
 #pragma unroll
             for (int k = 0; k < ID2; ++k) {
                 R wg = R(0);
-                for (int s = 1; s < admis; ++s) wg += sh.u.late.omega[s * NV + lane] * sh.g[(s * IDEG + k) * NV + lane];
+                for (int s = 1; s < admis; ++s) wg += sh.u.late.omega[s * NV + lane] * sh.g[crow(s, k) * NV + lane];
                 c += dm[k] * wg;
             }
             sh.corr[lane] = c;
@@ -570,7 +584,9 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
             }
             __syncthreads();
             const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
+#pragma unroll 1
             for (int q = lane; q < nq; q += WAVE) {
+                asm volatile("" ::: "memory");
                 const int e = f * p.nqm + q;
                 R x[3] = {qpc[e * 3], qpc[e * 3 + 1], qpc[e * 3 + 2]};
                 R phi[IDEG];
@@ -652,7 +668,11 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
         cons2prim_d(p.gamma, ui, wi);
         int trouble = 0;
         const R *qpc = p.qp + (size_t)cell * p.nf * p.nqm * 3;
+#pragma unroll 1
         for (int e = lane; e < nslot; e += WAVE) {
+            // keep the coefficient loads inside the point loop (hoisted, they
+            // would hold IDEG x 5 values in registers across it)
+            asm volatile("" ::: "memory");
             const int f = e / p.nqm, q = e % p.nqm;
             const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
             if (q >= nq) continue;
@@ -697,7 +717,11 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
         double slope[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) slope[v] = 1.0e16;
+#pragma unroll 1
         for (int e = lane; e < nslot; e += WAVE) {
+            // keep the coefficient loads inside the point loop (hoisted, they
+            // would hold IDEG x 5 values in registers across it)
+            asm volatile("" ::: "memory");
             const int f = e / p.nqm, q = e % p.nqm;
             const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
             if (q >= nq) continue;
@@ -722,7 +746,11 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) slope[v] = wave_min(slope[v]);
+#pragma unroll 1
         for (int e = lane; e < nslot; e += WAVE) {
+            // keep the coefficient loads inside the point loop (hoisted, they
+            // would hold IDEG x 5 values in registers across it)
+            asm volatile("" ::: "memory");
             const int f = e / p.nqm, q = e % p.nqm;
             const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
             if (q >= nq) continue;
@@ -746,7 +774,11 @@ __global__ __launch_bounds__(WAVE) void k_recon(ReconParams<R> p) {
         double wi[NV];
         cons2prim_d(p.gamma, ui, wi);
         int trouble = 0;
+#pragma unroll 1
         for (int e = lane; e < nslot; e += WAVE) {
+            // keep the coefficient loads inside the point loop (hoisted, they
+            // would hold IDEG x 5 values in registers across it)
+            asm volatile("" ::: "memory");
             const int f = e / p.nqm, q = e % p.nqm;
             const int nq = ci_tri(info, f) ? p.qpt : p.qpq;
             if (q >= nq) continue;
