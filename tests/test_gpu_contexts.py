@@ -5,6 +5,10 @@
   on tetrahedra, nine per quadrilateral on hexahedra) interleave their calls, each checked
   against its oracle (the weights are w_q * SURF per scheme,
   src/flux_p.f90:669).
+- A state that is not finite after an RK3 march is reported as
+  UCNS_ENONFINITE (the march ran: the NaN reached the update through the
+  ordinary kernels), and the context steps a finite state afterwards exactly
+  as a fresh one does.
 - A failed adjoint set-up (an allocation refused by ucns_debug_fail_alloc)
   returns its error again on the next call instead of running kernels on
   missing buffers, and a later call with the allocation allowed gives the
@@ -15,12 +19,13 @@ import pytest
 
 from oracle.oracle import Oracle
 from ucns3d_amd.case import Case, reduced
-from ucns3d_amd.native import UCNS_FP64
+from ucns3d_amd.native import UCNS_FP64, UCNS_MIXED
 from ucns3d_amd.solver import Solver, UcnsError
 
 pytestmark = pytest.mark.gpu
 
 UCNS_ENOMEM = -3
+UCNS_ENONFINITE = -5
 
 
 def rel(a, b):
@@ -97,3 +102,29 @@ def test_failed_adjoint_setup_reports_again_and_recovers(name):
         k += 1
     print(f"  {k - 1} allocation points failed and recovered")
     assert k > 5
+
+
+@pytest.mark.parametrize("precision", [UCNS_FP64, UCNS_MIXED])
+def test_non_finite_state_is_reported(precision):
+    case = Case(reduced("c2", 4))
+    u = case.init_state()
+    fresh = Solver(case, precision)
+    fresh.set_state(u)
+    fresh.rk3_steps(1, dt_fixed=1e-3)
+    ref = fresh.get_state()
+    fresh.close()
+    bad = u.copy()
+    bad[case.ncell // 3, 4] = np.nan
+    print(f"[non-finite] c2 reduced ncell={case.ncell} precision={precision}", flush=True)
+    s = Solver(case, precision)
+    for kw in ({"dt_fixed": 1e-3}, {"cfl": 0.5}):
+        s.set_state(bad)
+        with pytest.raises(UcnsError) as ei:
+            s.rk3_steps(1, **kw)
+        assert ei.value.code == UCNS_ENONFINITE, ei.value
+        assert "non-finite" in str(ei.value)
+        assert not np.all(np.isfinite(s.get_state()))
+    s.set_state(u)
+    s.rk3_steps(1, dt_fixed=1e-3)
+    assert np.array_equal(s.get_state(), ref)
+    s.close()
