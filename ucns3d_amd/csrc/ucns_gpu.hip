@@ -897,11 +897,13 @@ __device__ __forceinline__ void dma_chunk(const char *src, char *lds_dst, int la
     if (lane * 16 < valid) lds_load16(src + lane * 16, lds_addr(lds_dst));
 }
 
-// STORE: also write every stencil's coefficients to p.gstore (the adjoint's forward pass)
-template <typename R, int IDEG, int NQL, bool STORE = false, bool CW = false>
+// STORE: also write every stencil's coefficients to p.gstore (the adjoint's forward pass).
+// qpc / mb: the cell's face points and basis means, in HBM or staged in LDS
+// by loads still in flight (QWAIT: wait for them before the face points).
+template <typename R, int IDEG, int NQL, bool STORE = false, bool CW = false, bool QWAIT = false>
 __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG][NV], const double (&ui)[NV],
                                            const R (*sB)[IDEG * (IDEG + 1) / 2], R (*sG0)[IDEG * NV], int lc, int s,
-                                           int cell, bool cvalid, int info, bool act) {
+                                           int cell, bool cvalid, int info, bool act, const R *qpc, const R *mb) {
     const int ifca = ci_ifca(info), full = ci_full(info);
     if (STORE && act) {
         // coalesced: for each coefficient the 8 ts stencils of the tile are contiguous
@@ -1060,8 +1062,7 @@ __device__ __forceinline__ void recon_late(const ReconWParams<R> &p, R (&g)[IDEG
     // cannot fail CHECKSOLX (its 0.95 jumps imply the 0.85 ones), so only
     // the MUSCL branch below re-tests and overwrites its face points.
     const int nslot = ifca * p.nqm;
-    const R *qpc = p.qp + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * 3;
-    const R *mb = p.mbar + (size_t)(cvalid ? cell : 0) * IDEG;
+    if constexpr (QWAIT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     R *out = p.uleft + (size_t)(cvalid ? cell : 0) * p.nf * p.nqm * NV;
     double wi[NV];
     cons2prim_d(p.gamma, ui, wi);
@@ -1259,7 +1260,9 @@ __device__ __forceinline__ void recon_w_body(const ReconWParams<R> &p) {
             }
         }
     }
-    recon_late<R, IDEG, NQL, false, CW>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+    const size_t cq = cvalid ? cell : 0;
+    recon_late<R, IDEG, NQL, false, CW>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act,
+                                        p.qp + cq * p.nf * p.nqm * 3, p.mbar + cq * IDEG);
 }
 
 // ---------------------------------------------------------------------------
@@ -1312,7 +1315,22 @@ struct StreamShape {
     static_assert(GSLOT < DEPTH && NBC * 1024 <= NB * SB, "late data must fit the ring");
     static_assert(8 * IDEG * NV * (int)sizeof(R) <= SB, "central coefficients exceed a slot");
     static_assert((DEPTH - 1) * PER + NBC <= 63, "vmcnt immediate out of range");
+    // after the stream the ring holds B ([B0, B1): every chunk issue_b
+    // writes) and the central coefficients ([G0, G1)); the rest is free
+    static constexpr int B0 = BSLOT * SB, B1 = B0 + NBC * 1024;
+    static constexpr int G0 = GSLOT * SB, G1 = G0 + 8 * IDEG * NV * (int)sizeof(R);
 };
+
+// first 1 KiB-aligned offset in [0, ring) where `need` bytes avoid the three
+// given ranges; -1 if there is none
+constexpr int free_lds(int ring, int need, int a0, int a1, int b0, int b1, int c0, int c1) {
+    for (int o = 0; o + need <= ring; o += 1024) {
+        const int e = o + need;
+        if ((e > a0 && o < a1) || (e > b0 && o < b1) || (e > c0 && o < c1)) continue;
+        return o;
+    }
+    return -1;
+}
 
 template <typename R, int IDEG, int IMAX, int NQL, int TS, int DEPTH, bool STORE = false>
 __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
@@ -1451,10 +1469,45 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         issue_b();
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // indicators landed
+    // The tile's face points and basis means follow into free LDS, so that
+    // the face-point loop reads them there instead of waiting on HBM once
+    // per point; the indicators wait for B only.  Where they do not fit
+    // (hexahedra in FP32) the loop reads them from HBM.
+    constexpr int RING = DEPTH * SS::SB;
+    constexpr int NQC = (8 * NQL * 8 * 3 * (int)sizeof(R) + 1023) / 1024;
+    constexpr int NMC = (8 * IDEG * (int)sizeof(R) + 1023) / 1024;
+    constexpr int QO = free_lds(RING, NQC * 1024, SS::B0, SS::B1, SS::G0, SS::G1, 0, 0);
+    constexpr int MO = QO < 0 ? -1 : free_lds(RING, NMC * 1024, SS::B0, SS::B1, SS::G0, SS::G1, QO, QO + NQC * 1024);
+    const size_t cq = cvalid ? cell : 0;
+    const R *qpc = p.qp + cq * p.nf * p.nqm * 3;
+    const R *mb = p.mbar + cq * IDEG;
+    if constexpr (QO >= 0) {
+        static_assert(NQC + NMC <= 63, "vmcnt immediate out of range");
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the tail's reads of these slots are done
+        auto issue_blk = [&](const R *src, int bytes, int off, int nch) {
+#pragma unroll 1
+            for (int c = 0; c < nch; ++c) {
+                const int valid = bytes - c * 1024;
+                dma_chunk(valid > 0 ? (const char *)src + c * 1024 : (const char *)src, ring + off + c * 1024, lane,
+                          valid > 0 ? valid : 16);
+            }
+        };
+        const int per = p.nf * p.nqm * 3;
+        issue_blk(p.qp + (size_t)tile * 8 * per, nvc * per * (int)sizeof(R), QO, NQC);
+        qpc = (const R *)(ring + QO) + lc * per;
+        if constexpr (MO >= 0) {
+            issue_blk(p.mbar + (size_t)tile * 8 * IDEG, nvc * IDEG * (int)sizeof(R), MO, NMC);
+            mb = (const R *)(ring + MO) + lc * IDEG;
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(NQC + NMC) : "memory");  // indicators landed
+        } else {
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(NQC) : "memory");
+        }
+    } else {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
     const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
     R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
-    recon_late<R, IDEG, NQL, STORE>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act);
+    recon_late<R, IDEG, NQL, STORE, false, (QO >= 0)>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act, qpc, mb);
 }
 
 template <int IDEG, int IMAX, int NQL, int TS>
@@ -5412,6 +5465,9 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
             const double oov = 1.0 / pre->volume11[o];
             for (int d = 0; d < ID; ++d) mb[(size_t)i * ID + d] = pre->integ_basis[o * ID + d] * oov;
         }
+        // zero pad, as B: the row-stream kernels stage a tile's means in LDS
+        // with whole 16-byte pieces
+        mb.resize(mb.size() + 512, 0.0);
         if ((rc = upload_real_any(ctx, &ctx->mbar, mb))) return rc;
         if (ctx->ees == 5) {
             // the sectors' indicators (packed) and mbar - mbar_c over their coefficients
@@ -5444,7 +5500,7 @@ int ucns_upload_prestore(ucns_ctx *ctx, const ucns_prestore_v1 *pre) {
     // Gauss points in the local frames: [cell][nf][nqm][3]
     {
         const int nf = ctx->nf, nqm = ctx->nqm, nfa = ctx->nf_abi, nqa = ctx->nqm_abi;
-        std::vector<double> q((size_t)nc * nf * nqm * 3, 0.0);
+        std::vector<double> q((size_t)nc * nf * nqm * 3 + 512, 0.0);  // zero pad, as the means
         for (int i = 0; i < nc; ++i)
             for (int f = 0; f < nf; ++f)
                 for (int g = 0; g < nqm && g < nqa; ++g)
