@@ -1537,9 +1537,10 @@ __device__ __forceinline__ void recon_s_body(const ReconWParams<R> &p) {
     }
     const R (*sB)[NP] = (const R (*)[NP])(ring + SS::BSLOT * SS::SB);
     R (*sG0)[IDEG * NV] = (R (*)[IDEG * NV])(ring + SS::GSLOT * SS::SB);
-    // FP64 face values leave through LDS as coalesced words (C3 hexahedra
-    // 20.8 -> 20.1 ms, C2 tetrahedra 10.86 -> 10.71); in FP32 the direct
-    // stores measured as fast or faster (C3 10.8 ms direct, 11.6 staged)
+    // FP64 face values leave through LDS as coalesced words (128^3 hexahedra
+    // 20.8 -> 20.1 ms, 64^3 x 6 tetrahedra 10.86 -> 10.71); in FP32 the
+    // direct stores measured as fast or faster (hexahedra 10.8 ms direct,
+    // 11.6 staged)
     constexpr bool STG = sizeof(R) == 8;
     static_assert(!STG || 8 * 8 * NV * (int)sizeof(R) <= SS::B1 - SS::B0, "face-value staging exceeds the B block");
     recon_late<R, IDEG, NQL, STORE, false, (QO >= 0), STG>(p, g, ui, sB, sG0, lc, s, cell, cvalid, info, act, qpc, mb,
